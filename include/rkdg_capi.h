@@ -1,0 +1,159 @@
+/*
+ * rkdg_capi.h — C-ABI of the MI355X-native explicit RKDG right-hand side.
+ *
+ * The entry points replace the reference's hot-path interfaces (vkorchagova/RKDG3D):
+ *
+ *   rkdg_create / rkdg_destroy   FE_Evolution ctor + UpdateInverseMassMatrix +
+ *                                UpdateAfluxPointer after a re-mesh
+ *                                (src/fe_evolution/fe_evolution.hpp:66-81,
+ *                                 main.cpp:148-164, 610-611)
+ *   rkdg_rhs                     FE_Evolution::Mult(const Vector&, Vector&)
+ *                                (src/fe_evolution/fe_evolution.hpp:75,
+ *                                 fe_evolution.cpp:102-216); the lambda_max
+ *                                out-parameter replaces the global
+ *                                max_char_speed (fe_evolution.hpp:10)
+ *   rkdg_face_lambda             A.Mult(sol, z) used for the first dt
+ *                                (main.cpp:467-484)
+ *   rkdg_limit                   Limiter::update(Vector&) (src/limiter/limiter.hpp:111,
+ *                                limiter.cpp:22-94)
+ *   rkdg_step                    ExplicitRKLimitedSolver::Step(Vector&, double&, double&)
+ *                                (src/ode_solver/rk_explicit_limited.hpp:24-60,
+ *                                 rk_explicit_limited.cpp:30-82)
+ *   rkdg_hmin                    minimum GetElementSize(i,1) (main.cpp:191-201)
+ *   rkdg_means                   Averager::computeMeanValues (src/averager/averager.cpp:84-109)
+ *
+ * Plug-in families (RiemannSolver, BoundaryIntegrator::computeRightState,
+ * Indicator, Limiter, EoS globals) are selected by the enums of
+ * rkdg_scheme_desc; see src/riemannsolvers/rs_basic.hpp:18-51,
+ * src/bc/boundary_integrator.hpp:23-64, src/indicator/indicator.hpp:19-84,
+ * src/limiter/limiter.hpp:24-116, src/physics/physics.hpp:17-60.
+ *
+ * State layout is the reference's byNODES layout of the vector space:
+ * x[v*NE*nd + e*nd + i] for conserved variable v, element e, local node i
+ * (tensor elements: lexicographic nodes, x fastest).
+ *
+ * Pointers named x, y, k are DEVICE pointers (caller-owned). Calls on one
+ * context are ordered on the context's stream and are not re-entrant.
+ * Host out-parameters (lambda_max, hmin) synchronize the stream.
+ * Return value: RKDG_OK or a negative error code; rkdg_last_error() gives text.
+ */
+#ifndef RKDG_CAPI_H
+#define RKDG_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RKDG_OK 0
+#define RKDG_E_ARG (-1)
+#define RKDG_E_UNPHYSICAL (-2) /* a Riemann solver saw an unphysical state (reference: exit(1)) */
+#define RKDG_E_HIP (-3)
+#define RKDG_E_UNSUPPORTED (-4)
+
+#define RKDG_GEOM_QUAD 0
+#define RKDG_GEOM_TRI 1
+#define RKDG_GEOM_HEX 2
+
+#define RKDG_RS_RUSANOV 0
+#define RKDG_RS_LLF 1
+#define RKDG_RS_HLL 2
+#define RKDG_RS_HLLC 3
+
+#define RKDG_WS_TORO 0
+#define RKDG_WS_EINFELDT 1
+
+#define RKDG_BC_SLIP 0
+#define RKDG_BC_OUTLET 1
+#define RKDG_BC_SUPERSONIC_INLET 2
+#define RKDG_BC_SUBSONIC_INLET_TOTAL_PRESSURE 3
+#define RKDG_BC_SUBSONIC_INLET_FIXED_PRESSURE 4
+#define RKDG_BC_FIXED_PRESSURE_OUTLET 5
+#define RKDG_BC_TOTAL_PRESSURE_OUTLET 6
+
+#define RKDG_IND_NOWHERE 0
+#define RKDG_IND_EVERYWHERE 1
+#define RKDG_IND_BJ 2
+
+#define RKDG_PHI_MICHALAK 0
+#define RKDG_PHI_BJ 1
+#define RKDG_PHI_VENKAT 2
+
+#define RKDG_LIM_NONE 0
+#define RKDG_LIM_FINDIFF 1
+#define RKDG_LIM_MULTI 2
+
+#define RKDG_MAX_BC 16
+
+typedef struct rkdg_ctx rkdg_ctx;
+
+/* Mesh after any refinement, flattened by the host (the analogue of an MFEM
+ * mesh with discontinuous nodes). All pointers are HOST pointers, copied at
+ * create time. */
+typedef struct {
+  int geom;                  /* RKDG_GEOM_* */
+  int order;                 /* polynomial order p (DG_FECollection(p, dim)) */
+  int64_t num_elements;
+  const double* elem_coords; /* [NE][nv][dim] vertex coordinates, MFEM vertex order */
+  const int64_t* elem_verts; /* [NE][nv] topological vertex ids (identified for periodic) */
+  int64_t num_bdr;
+  const int64_t* bdr_verts;  /* [num_bdr][nvf] */
+  const int32_t* bdr_attr;   /* [num_bdr] 1-based attribute */
+} rkdg_mesh_desc;
+
+/* Scheme: RiemannSolver, wave speeds, EoS, boundary integrators, indicator, limiter. */
+typedef struct {
+  int riemann;      /* RKDG_RS_* (case_manager.cpp:523-543) */
+  int wave_speeds;  /* RKDG_WS_* (physics.cpp:269-350) */
+  int hllc_abs;     /* 1: HLLC reports max(|SL|,|SR|); 0: signed max(SL,SR) (rs_hllc.cpp:185) */
+  double gamma, gas_constant, covolume;  /* specific_heat_ratio, gas_constant, covolume_constant */
+  int num_bc;
+  int bc_attr[RKDG_MAX_BC];
+  int bc_type[RKDG_MAX_BC];
+  double bc_param[RKDG_MAX_BC][8];
+  int indicator;    /* RKDG_IND_* (case_manager.cpp:574-594) */
+  int phi;          /* RKDG_PHI_* limiter function of the BJ-family indicator */
+  double venkat_k;  /* Venkatakrishnan K (eps^2 = (K h)^3); 0 gives the K-free form */
+  int limiter;      /* RKDG_LIM_* (case_manager.cpp:596-609) */
+  int last_hope;    /* haveLastHope */
+  int pow_nd;       /* 1: multiplier applied nd times (limiter_multiplier.cpp:24-38) */
+} rkdg_scheme_desc;
+
+typedef struct {
+  int dim, num_equations, nodes_per_element, faces_per_element;
+  int64_t num_elements, num_dofs;      /* num_dofs = NE*nd (scalar space) */
+  int64_t state_size;                  /* num_equations*num_dofs */
+  int64_t num_interior_faces, num_boundary_faces;
+  int affine;                          /* 1 if every element is affine */
+} rkdg_info;
+
+int rkdg_create(const rkdg_mesh_desc* mesh, const rkdg_scheme_desc* scheme, rkdg_ctx** out);
+int rkdg_destroy(rkdg_ctx* ctx);
+const char* rkdg_last_error(void);
+int rkdg_get_info(rkdg_ctx* ctx, rkdg_info* info);
+/* hipStream_t as void*; NULL selects the null stream. */
+int rkdg_set_stream(rkdg_ctx* ctx, void* stream);
+
+/* y = M^-1 [ face terms + volume terms ](x).  lambda_max: HOST pointer or NULL. */
+int rkdg_rhs(rkdg_ctx* ctx, const double* x, double* y, double* lambda_max);
+/* Max wave speed over face quadrature points only (first-dt path). */
+int rkdg_face_lambda(rkdg_ctx* ctx, const double* x, double* lambda_max);
+/* In-place limiter. troubled: DEVICE int8[NE] or NULL; bit0 = indicator < 1, bit1 = last hope. */
+int rkdg_limit(rkdg_ctx* ctx, double* x, int8_t* troubled);
+/* One RK step, Butcher tableau consumed with the packed a[l++] rule. s <= 4. */
+int rkdg_step(rkdg_ctx* ctx, double* x, double t, double dt, int s, const double* a,
+              const double* b, const double* c, double* lambda_max);
+/* Element means, layout [v*NE + e] (DEVICE pointer). */
+int rkdg_means(rkdg_ctx* ctx, const double* x, double* means);
+int rkdg_hmin(rkdg_ctx* ctx, double* hmin);
+/* Physical coordinates of the nodes, HOST [NE][nd][dim] (for initial conditions). */
+int rkdg_node_coords(rkdg_ctx* ctx, double* xyz);
+/* Synchronize and report a deferred device error (RKDG_E_UNPHYSICAL with the
+ * element pair in info[0], info[1]); clears it. */
+int rkdg_check_error(rkdg_ctx* ctx, int64_t* info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,227 @@
+"""ctypes wrapper of the CPU oracle (oracle/refcpu.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker and the timed CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "librefcpu.so")
+MAX_BC = 16
+
+
+class RefScheme(C.Structure):
+    _fields_ = [
+        ("riemann", C.c_int), ("wave_speeds", C.c_int), ("hllc_abs", C.c_int),
+        ("gamma", C.c_double), ("gas_constant", C.c_double), ("covolume", C.c_double),
+        ("num_bc", C.c_int), ("bc_attr", C.c_int * MAX_BC), ("bc_type", C.c_int * MAX_BC),
+        ("bc_param", (C.c_double * 8) * MAX_BC),
+        ("indicator", C.c_int), ("phi", C.c_int), ("venkat_k", C.c_double),
+        ("limiter", C.c_int), ("last_hope", C.c_int), ("pow_nd", C.c_int),
+    ]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        P = C.c_void_p
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int64)
+        L.ref_create.restype = P
+        L.ref_create.argtypes = [C.c_int, C.c_int, C.c_int64, dp, ip, C.c_int64, ip,
+                                 C.POINTER(C.c_int), C.POINTER(RefScheme), C.c_char_p, C.c_int]
+        L.ref_destroy.argtypes = [P]
+        L.ref_num_dofs.restype = C.c_int64
+        L.ref_num_dofs.argtypes = [P]
+        L.ref_nodes_per_element.argtypes = [P]
+        L.ref_num_interior_faces.restype = C.c_int64
+        L.ref_num_interior_faces.argtypes = [P]
+        L.ref_node_coords.argtypes = [P, dp]
+        L.ref_mult.argtypes = [P, dp, dp, dp, ip]
+        L.ref_face_mult.argtypes = [P, dp, dp, dp, ip]
+        L.ref_means.argtypes = [P, dp, dp]
+        L.ref_limiter_update.argtypes = [P, dp, dp, dp]
+        L.ref_step.argtypes = [P, dp, C.c_double, C.c_double, C.c_int, dp, dp, dp, dp, ip]
+        L.ref_hmin.restype = C.c_double
+        L.ref_hmin.argtypes = [P]
+        L.ref_element_mass_inverse.argtypes = [P, C.c_int64, dp]
+        L.ref_set_threads.argtypes = [C.c_int]
+        L.ref_riemann_eval.restype = C.c_double
+        L.ref_riemann_eval.argtypes = [C.POINTER(RefScheme), C.c_int, dp, dp, dp, dp]
+        L.ref_bc_state.argtypes = [C.POINTER(RefScheme), C.c_int, C.c_int, dp, dp, dp]
+        L.ref_gl_rule.argtypes = [C.c_int, dp, dp]
+        L.ref_shape.argtypes = [C.c_int, C.c_int, dp, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def ref_scheme(s) -> RefScheme:
+    r = RefScheme()
+    r.riemann = s.riemann
+    r.wave_speeds = s.wave_speeds
+    r.hllc_abs = s.hllc_abs
+    r.gamma = s.gamma
+    r.gas_constant = s.gas_constant
+    r.covolume = s.covolume
+    r.num_bc = len(s.bcs)
+    for i, bc in enumerate(s.bcs):
+        r.bc_attr[i] = bc.attr
+        r.bc_type[i] = bc.type
+        for j, p in enumerate(bc.params):
+            r.bc_param[i][j] = p
+    r.indicator = s.indicator
+    r.phi = s.phi
+    r.venkat_k = s.venkat_k
+    r.limiter = s.limiter
+    r.last_hope = s.last_hope
+    r.pow_nd = s.pow_nd
+    return r
+
+
+class RefSolver:
+    """The oracle for one mesh + scheme: FE_Evolution::Mult, Limiter::update, Step."""
+
+    def __init__(self, mesh, order, scheme, threads=None):
+        L = lib()
+        if threads is not None:
+            L.ref_set_threads(int(threads))
+        self.mesh = mesh
+        self.order = order
+        self.scheme = scheme
+        self._rs = ref_scheme(scheme)
+        err = C.create_string_buffer(256)
+        ba = np.ascontiguousarray(mesh.bdr_attr, dtype=np.int32)
+        self._ba = ba
+        self.h = L.ref_create(mesh.geom, order, mesh.num_elements, _dp(mesh.elem_coords),
+                              _ip(mesh.elem_verts), len(mesh.bdr_attr), _ip(mesh.bdr_verts),
+                              ba.ctypes.data_as(C.POINTER(C.c_int)), C.byref(self._rs), err, 256)
+        if not self.h:
+            raise RuntimeError("ref_create failed: " + err.value.decode())
+        self.nd = L.ref_nodes_per_element(self.h)
+        self.ndofs = L.ref_num_dofs(self.h)
+        self.neq = mesh.dim + 2
+        self.size = self.neq * self.ndofs
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ref_destroy(self.h)
+            self.h = None
+
+    def node_coords(self):
+        out = np.zeros((self.mesh.num_elements, self.nd, self.mesh.dim))
+        lib().ref_node_coords(self.h, _dp(out))
+        return out
+
+    def mult(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        m = np.zeros(1)
+        bad = np.full(2, -1, np.int64)
+        rc = lib().ref_mult(self.h, _dp(x), _dp(y), _dp(m), _ip(bad))
+        if rc:
+            raise FloatingPointError(f"unphysical state at faces between elements {bad[0]} {bad[1]}")
+        return y, float(m[0])
+
+    def face_mult(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        z = np.zeros_like(x)
+        m = np.zeros(1)
+        bad = np.full(2, -1, np.int64)
+        rc = lib().ref_face_mult(self.h, _dp(x), _dp(z), _dp(m), _ip(bad))
+        if rc:
+            raise FloatingPointError(f"unphysical state at faces between elements {bad[0]} {bad[1]}")
+        return z, float(m[0])
+
+    def means(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(self.neq * self.mesh.num_elements)
+        lib().ref_means(self.h, _dp(x), _dp(out))
+        return out
+
+    def limit(self, x):
+        x = np.array(x, dtype=np.float64, copy=True)
+        ind = np.zeros(self.neq * self.mesh.num_elements)
+        mv = np.zeros(self.mesh.num_elements)
+        lib().ref_limiter_update(self.h, _dp(x), _dp(ind), _dp(mv))
+        return x, ind, mv
+
+    def step(self, x, dt, tableau, t=0.0):
+        s, a, b, c = tableau
+        x = np.array(x, dtype=np.float64, copy=True)
+        a = np.ascontiguousarray(a + [0.0], dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        c = np.ascontiguousarray(c + [0.0], dtype=np.float64)
+        m = np.zeros(1)
+        bad = np.full(2, -1, np.int64)
+        rc = lib().ref_step(self.h, _dp(x), t, dt, s, _dp(a), _dp(b), _dp(c), _dp(m), _ip(bad))
+        if rc:
+            raise FloatingPointError(f"unphysical state at faces between elements {bad[0]} {bad[1]}")
+        return x, float(m[0])
+
+    def hmin(self):
+        return lib().ref_hmin(self.h)
+
+    def mass_inverse(self, e):
+        out = np.zeros((self.nd, self.nd))
+        lib().ref_element_mass_inverse(self.h, e, _dp(out))
+        return out
+
+
+def riemann_eval(scheme, dim, s1, s2, nor):
+    rs = ref_scheme(scheme)
+    s1 = np.ascontiguousarray(s1, np.float64)
+    s2 = np.ascontiguousarray(s2, np.float64)
+    nor = np.ascontiguousarray(nor, np.float64)
+    f = np.zeros(dim + 2)
+    lam = lib().ref_riemann_eval(C.byref(rs), dim, _dp(s1), _dp(s2), _dp(nor), _dp(f))
+    return f, lam
+
+
+def bc_state(scheme, dim, bc_index, s1, nor):
+    rs = ref_scheme(scheme)
+    s1 = np.ascontiguousarray(s1, np.float64)
+    nor = np.ascontiguousarray(nor, np.float64)
+    s2 = np.zeros(dim + 2)
+    lib().ref_bc_state(C.byref(rs), dim, bc_index, _dp(s1), _dp(nor), _dp(s2))
+    return s2
+
+
+def gl_rule(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    lib().ref_gl_rule(n, _dp(x), _dp(w))
+    return x, w
+
+
+def shape(geom, order, xi):
+    from_dim = {0: 2, 1: 2, 2: 3}[geom]
+    nd = {0: (order + 1) ** 2, 1: (order + 1) * (order + 2) // 2, 2: (order + 1) ** 3}[geom]
+    xi = np.ascontiguousarray(xi, np.float64)
+    phi = np.zeros(nd)
+    dphi = np.zeros(nd * from_dim)
+    lib().ref_shape(geom, order, _dp(xi), _dp(phi), _dp(dphi))
+    return phi, dphi.reshape(nd, from_dim)

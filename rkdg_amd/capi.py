@@ -1,0 +1,226 @@
+"""ctypes binding of the C-ABI in include/rkdg_capi.h (librkdg.so, built in-tree).
+
+This is the host-side mirror a Python caller uses; a C/C++ caller includes the
+header directly. Device buffers are passed as raw pointers (torch tensors on
+``cuda`` are accepted for convenience; torch is only used for device memory).
+The library must be present: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import scheme as S
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librkdg.so")
+
+RKDG_OK = 0
+RKDG_E_ARG = -1
+RKDG_E_UNPHYSICAL = -2
+RKDG_E_HIP = -3
+RKDG_E_UNSUPPORTED = -4
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [
+        ("geom", C.c_int), ("order", C.c_int), ("num_elements", C.c_int64),
+        ("elem_coords", C.POINTER(C.c_double)), ("elem_verts", C.POINTER(C.c_int64)),
+        ("num_bdr", C.c_int64), ("bdr_verts", C.POINTER(C.c_int64)),
+        ("bdr_attr", C.POINTER(C.c_int32)),
+    ]
+
+
+class SchemeDesc(C.Structure):
+    _fields_ = [
+        ("riemann", C.c_int), ("wave_speeds", C.c_int), ("hllc_abs", C.c_int),
+        ("gamma", C.c_double), ("gas_constant", C.c_double), ("covolume", C.c_double),
+        ("num_bc", C.c_int), ("bc_attr", C.c_int * S.MAX_BC), ("bc_type", C.c_int * S.MAX_BC),
+        ("bc_param", (C.c_double * 8) * S.MAX_BC),
+        ("indicator", C.c_int), ("phi", C.c_int), ("venkat_k", C.c_double),
+        ("limiter", C.c_int), ("last_hope", C.c_int), ("pow_nd", C.c_int),
+    ]
+
+
+class Info(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int), ("num_equations", C.c_int), ("nodes_per_element", C.c_int),
+        ("faces_per_element", C.c_int), ("num_elements", C.c_int64), ("num_dofs", C.c_int64),
+        ("state_size", C.c_int64), ("num_interior_faces", C.c_int64),
+        ("num_boundary_faces", C.c_int64), ("affine", C.c_int),
+    ]
+
+
+# every symbol declared in include/rkdg_capi.h
+EXPORTS = [
+    "rkdg_create", "rkdg_destroy", "rkdg_last_error", "rkdg_get_info", "rkdg_set_stream",
+    "rkdg_rhs", "rkdg_face_lambda", "rkdg_limit", "rkdg_step", "rkdg_means", "rkdg_hmin",
+    "rkdg_node_coords", "rkdg_check_error",
+]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python rkdg_amd/build.py` "
+                          "(there is no CPU fallback for the RKDG path)")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    dp = C.c_void_p
+    L.rkdg_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(SchemeDesc), C.POINTER(C.c_void_p)]
+    L.rkdg_destroy.argtypes = [P]
+    L.rkdg_last_error.restype = C.c_char_p
+    L.rkdg_get_info.argtypes = [P, C.POINTER(Info)]
+    L.rkdg_set_stream.argtypes = [P, C.c_void_p]
+    L.rkdg_rhs.argtypes = [P, dp, dp, C.POINTER(C.c_double)]
+    L.rkdg_face_lambda.argtypes = [P, dp, C.POINTER(C.c_double)]
+    L.rkdg_limit.argtypes = [P, dp, dp]
+    L.rkdg_step.argtypes = [P, dp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double),
+                            C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.rkdg_means.argtypes = [P, dp, dp]
+    L.rkdg_hmin.argtypes = [P, C.POINTER(C.c_double)]
+    L.rkdg_node_coords.argtypes = [P, C.POINTER(C.c_double)]
+    L.rkdg_check_error.argtypes = [P, C.POINTER(C.c_int64)]
+    _lib = L
+    return L
+
+
+class RKDGError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"rkdg error {code}: {msg}")
+        self.code = code
+
+
+class UnphysicalStateError(RKDGError):
+    pass
+
+
+def _check(rc):
+    if rc != RKDG_OK:
+        msg = lib().rkdg_last_error().decode()
+        if rc == RKDG_E_UNPHYSICAL:
+            raise UnphysicalStateError(rc, msg)
+        raise RKDGError(rc, msg)
+
+
+def scheme_desc(s: S.Scheme) -> SchemeDesc:
+    d = SchemeDesc()
+    d.riemann = s.riemann
+    d.wave_speeds = s.wave_speeds
+    d.hllc_abs = s.hllc_abs
+    d.gamma = s.gamma
+    d.gas_constant = s.gas_constant
+    d.covolume = s.covolume
+    d.num_bc = len(s.bcs)
+    for i, bc in enumerate(s.bcs):
+        d.bc_attr[i] = bc.attr
+        d.bc_type[i] = bc.type
+        for j, p in enumerate(bc.params):
+            d.bc_param[i][j] = p
+    d.indicator = s.indicator
+    d.phi = s.phi
+    d.venkat_k = s.venkat_k
+    d.limiter = s.limiter
+    d.last_hope = s.last_hope
+    d.pow_nd = s.pow_nd
+    return d
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        return C.c_void_p(t.data_ptr())
+    return C.c_void_p(int(t))
+
+
+class Context:
+    """One rkdg_ctx: a flattened mesh + scheme resident on the GPU."""
+
+    def __init__(self, mesh, order: int, scheme: S.Scheme):
+        L = lib()
+        self._keep = (np.ascontiguousarray(mesh.elem_coords), np.ascontiguousarray(mesh.elem_verts),
+                      np.ascontiguousarray(mesh.bdr_verts), np.ascontiguousarray(mesh.bdr_attr, np.int32))
+        md = MeshDesc()
+        md.geom = mesh.geom
+        md.order = order
+        md.num_elements = mesh.num_elements
+        md.elem_coords = self._keep[0].ctypes.data_as(C.POINTER(C.c_double))
+        md.elem_verts = self._keep[1].ctypes.data_as(C.POINTER(C.c_int64))
+        md.num_bdr = len(mesh.bdr_attr)
+        md.bdr_verts = self._keep[2].ctypes.data_as(C.POINTER(C.c_int64))
+        md.bdr_attr = self._keep[3].ctypes.data_as(C.POINTER(C.c_int32))
+        self._sd = scheme_desc(scheme)
+        h = C.c_void_p()
+        _check(L.rkdg_create(C.byref(md), C.byref(self._sd), C.byref(h)))
+        self.h = h
+        info = Info()
+        _check(L.rkdg_get_info(self.h, C.byref(info)))
+        self.info = info
+        self.mesh = mesh
+        self.order = order
+        self.scheme = scheme
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rkdg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def state_size(self):
+        return self.info.state_size
+
+    def set_stream(self, stream_ptr):
+        _check(lib().rkdg_set_stream(self.h, C.c_void_p(stream_ptr)))
+
+    def rhs(self, x, y, want_lambda=True):
+        lam = C.c_double(0.0)
+        _check(lib().rkdg_rhs(self.h, _ptr(x), _ptr(y), C.byref(lam) if want_lambda else None))
+        return lam.value
+
+    def face_lambda(self, x):
+        lam = C.c_double(0.0)
+        _check(lib().rkdg_face_lambda(self.h, _ptr(x), C.byref(lam)))
+        return lam.value
+
+    def limit(self, x, troubled=None):
+        _check(lib().rkdg_limit(self.h, _ptr(x), _ptr(troubled)))
+
+    def means(self, x, out):
+        _check(lib().rkdg_means(self.h, _ptr(x), _ptr(out)))
+
+    def step(self, x, dt, tableau, t=0.0, want_lambda=True):
+        s, a, b, c = tableau
+        a = (C.c_double * max(1, len(a)))(*a) if a else (C.c_double * 1)(0.0)
+        b = (C.c_double * len(b))(*b)
+        c = (C.c_double * max(1, len(c)))(*c) if c else (C.c_double * 1)(0.0)
+        lam = C.c_double(0.0)
+        _check(lib().rkdg_step(self.h, _ptr(x), t, dt, s, a, b, c,
+                               C.byref(lam) if want_lambda else None))
+        return lam.value
+
+    def hmin(self):
+        v = C.c_double(0.0)
+        _check(lib().rkdg_hmin(self.h, C.byref(v)))
+        return v.value
+
+    def node_coords(self):
+        out = np.zeros((self.info.num_elements, self.info.nodes_per_element, self.info.dim))
+        _check(lib().rkdg_node_coords(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def check_error(self):
+        info = (C.c_int64 * 2)()
+        _check(lib().rkdg_check_error(self.h, info))

@@ -1,0 +1,77 @@
+// rkdg_kernels.hpp — argument blocks and launchers shared by the host code and
+// the HIP kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rkdg_physics.hpp"
+
+namespace rkdg {
+
+constexpr int kMaxN = 6;  // up to p = 5 nodes per direction
+constexpr int kMaxQ = 8;  // face quadrature points per direction
+
+// 1D tables of a tensor-product element of order p (N = p+1 Gauss-Legendre
+// nodes) and its face rule (NQ Gauss-Legendre points).
+struct TensorTables {
+  double x[kMaxN], w[kMaxN], invw[kMaxN];
+  double D[kMaxN][kMaxN];   // D[a][b] = psi_b'(x_a)
+  double E[2][kMaxN];       // E[c][b] = psi_b(c), c in {0, 1}
+  double B[kMaxQ][kMaxN];   // B[q][b] = psi_b(s_q), face points s_q
+  double wq[kMaxQ];         // face weights
+  double BI[2][kMaxN];      // psi_b at the two indicator points (IntRules order 2)
+};
+
+// out = (base ? base : stage input) + coef * k, element-wise
+struct StageOut {
+  double* out;
+  const double* base;
+  double coef;
+};
+
+struct RhsArgs {
+  const double* x;   // stage input (limited)
+  double* k;         // optional: store the RHS itself
+  int nout;
+  StageOut outs[4];
+  double* means;     // optional: element means of outs[means_of] ([v*NE + e])
+  int means_of;
+  const double* geo;   // affine tensor geometry [g][NE]: adj(J) row-major, then det(J)
+  const int* nbr;      // [lf][NE] neighbour element, or -1-bc_slot for boundary faces
+  const int* fcode;    // [lf][NE] bits 0-2 neighbour local face, 3-5 orientation, 6 = Elem1
+  int64_t NE;
+  double* lam;         // device max wave speed (non-negative doubles compared as int64)
+  long long* err;      // device error word [4]: flag, e1, e2, local face
+  int face_only;       // first-dt path: face terms and wave speeds only, no outputs
+};
+
+struct LimitArgs {
+  double* x;           // state limited in place
+  const double* means; // [v*NE + e] of the pre-limit state
+  const double* geo;
+  const int* nbr;
+  const int* fcode;
+  int64_t NE;
+  int8_t* troubled;
+};
+
+struct MeansArgs {
+  const double* x;
+  double* means;
+  const double* geo;
+  int64_t NE;
+};
+
+// launchers (rkdg_tensor.hip)
+hipError_t launch_tensor_rhs(int dim, int p, const RhsArgs& a, const SchemeParams& g,
+                             const TensorTables& t, hipStream_t s);
+hipError_t launch_tensor_limit(int dim, int p, const LimitArgs& a, const SchemeParams& g,
+                               const TensorTables& t, hipStream_t s);
+hipError_t launch_tensor_means(int dim, int p, const MeansArgs& a, const TensorTables& t,
+                               hipStream_t s);
+hipError_t launch_fill(double* p, double v, int64_t n, hipStream_t s);
+hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
+bool tensor_supported(int dim, int p);
+
+}  // namespace rkdg

@@ -1,0 +1,387 @@
+// rkdg_physics.hpp — device (and host) physics for the RKDG kernels:
+// EoS, physical fluxes, wave-speed estimates, Riemann solvers and ghost states.
+// Formulas follow src/physics/physics.cpp, src/riemannsolvers/rs_*.cpp and
+// src/bc/boundary_integrator_*.cpp of the reference; each function cites its line.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/rkdg_capi.h"
+
+#define RKDG_HD __host__ __device__ __forceinline__
+
+namespace rkdg {
+
+struct SchemeParams {
+  int riemann, wave_speeds, hllc_abs;
+  double gamma, gas_constant, covolume;
+  int num_bc;
+  int bc_attr[RKDG_MAX_BC];
+  int bc_type[RKDG_MAX_BC];
+  double bc_param[RKDG_MAX_BC][8];
+  int indicator, phi;
+  double venkat_k;
+  int limiter, last_hope, pow_nd;
+};
+
+// physics.cpp:101-112 (covolume EoS; b == 0 skips the exact division by 1)
+template <int DIM>
+RKDG_HD double pressure(const double* s, const SchemeParams& g) {
+  double v2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) v2 += s[1 + d] * s[1 + d];
+  v2 /= s[0];
+  double p = (g.gamma - 1.0) * (s[1 + DIM] - 0.5 * v2);
+  if (g.covolume != 0.0) p /= (1.0 - s[0] * g.covolume);
+  return p;
+}
+
+// physics.cpp:134-140
+template <int DIM>
+RKDG_HD double sound_speed_p(const double* s, double p, const SchemeParams& g) {
+  double a = g.gamma * p / s[0];
+  if (g.covolume != 0.0) a /= (1.0 - s[0] * g.covolume);
+  return sqrt(a);
+}
+
+// physics.cpp:53-97 (pressure check without covolume, :84)
+template <int DIM>
+RKDG_HD bool physical_say(const double* s, const SchemeParams& g) {
+  double den = s[0], E = s[1 + DIM];
+  if (!(den >= 0.0)) return false;
+  if (!(E > 0.0)) return false;
+  double v2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < DIM; i++) v2 += s[1 + i] * s[1 + i];
+  v2 /= den;
+  double p = (g.gamma - 1.0) * (E - 0.5 * v2);
+  return p > 0.0;
+}
+
+// physics.cpp:6-50
+template <int DIM>
+RKDG_HD bool physical(const double* s, const SchemeParams& g) {
+  double den = s[0], E = s[1 + DIM];
+  if (!(den >= 0.0)) return false;
+  if (!(E > 0.0)) return false;
+  double p = pressure<DIM>(s, g);
+  return p > 0.0;
+}
+
+// physics.cpp:252-266: |u| + c
+template <int DIM>
+RKDG_HD double max_char_speed(const double* s, const SchemeParams& g) {
+  double v2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) v2 += s[1 + d] * s[1 + d];
+  v2 /= s[0];
+  double p = (g.gamma - 1.0) * (s[1 + DIM] - 0.5 * v2);
+  if (g.covolume != 0.0) p /= (1.0 - s[0] * g.covolume);
+  double c = sound_speed_p<DIM>(s, p, g);
+  return sqrt(v2 / s[0]) + c;
+}
+
+// physics.cpp:143-168; f[k*DIM + d]
+template <int DIM>
+RKDG_HD void flux_tensor(const double* s, const SchemeParams& g, double* f) {
+  const double den = s[0];
+  const double p = pressure<DIM>(s, g);
+  const double inv = 1.0 / den;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    f[0 * DIM + d] = s[1 + d];
+#pragma unroll
+    for (int i = 0; i < DIM; i++) f[(1 + i) * DIM + d] = s[1 + i] * s[1 + d] * inv;
+    f[(1 + d) * DIM + d] += p;
+  }
+  const double H = (s[1 + DIM] + p) * inv;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) f[(1 + DIM) * DIM + d] = s[1 + d] * H;
+}
+
+// physics.cpp:171-195
+template <int DIM>
+RKDG_HD void flux_dot_n(const double* s, const double* n, double p, const SchemeParams& g, double* fn) {
+  const double inv = 1.0 / s[0];
+  double vn = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) vn += s[1 + d] * n[d];
+  fn[0] = vn;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) fn[1 + d] = vn * s[1 + d] * inv + p * n[d];
+  fn[1 + DIM] = vn * ((s[1 + DIM] + p) * inv);
+}
+
+// physics.cpp:198-231 (x-flux of a rotated state)
+template <int DIM>
+RKDG_HD void flux_f(const double* s, double p, double* f) {
+  const double inv = 1.0 / s[0];
+  f[0] = s[1];
+  f[1] = s[1] * s[1] * inv + p;
+#pragma unroll
+  for (int d = 1; d < DIM; d++) f[d + 1] = s[1] * s[d + 1] * inv;
+  f[1 + DIM] = s[1] * ((s[1 + DIM] + p) * inv);
+}
+
+// rs_basic.cpp:11-38
+template <int DIM>
+RKDG_HD void rotate(double* s, const double* n) {
+  double* m = s + 1;
+  if (DIM == 2) {
+    double mx = n[0] * m[0] + n[1] * m[1];
+    double my = -n[1] * m[0] + n[0] * m[1];
+    m[0] = mx;
+    m[1] = my;
+  } else {
+    const double sp = n[2];
+    const double cp = sqrt(1 - sp * sp);
+    const double st = fabs(cp) > 1e-6 ? n[1] / cp : 1.0;
+    const double ct = fabs(cp) > 1e-6 ? n[0] / cp : 0.0;
+    double mx = cp * ct * m[0] + cp * st * m[1] + sp * m[2];
+    double my = -st * m[0] + ct * m[1];
+    double mz = -sp * ct * m[0] - sp * st * m[1] + cp * m[2];
+    m[0] = mx;
+    m[1] = my;
+    m[2] = mz;
+  }
+}
+
+// rs_basic.cpp:40-67
+template <int DIM>
+RKDG_HD void inverse_rotate(double* s, const double* n) {
+  double* m = s + 1;
+  if (DIM == 2) {
+    double mx = n[0] * m[0] - n[1] * m[1];
+    double my = n[1] * m[0] + n[0] * m[1];
+    m[0] = mx;
+    m[1] = my;
+  } else {
+    const double sp = n[2];
+    const double cp = sqrt(1 - sp * sp);
+    const double st = fabs(cp) > 1e-6 ? n[1] / cp : 1.0;
+    const double ct = fabs(cp) > 1e-6 ? n[0] / cp : 0.0;
+    double mx = cp * ct * m[0] - st * m[1] - sp * ct * m[2];
+    double my = cp * st * m[0] + ct * m[1] - sp * st * m[2];
+    double mz = sp * m[0] + cp * m[2];
+    m[0] = mx;
+    m[1] = my;
+    m[2] = mz;
+  }
+}
+
+// Toro two-rarefaction speeds (physics.cpp:309-350) or Einfeldt (:269-296) on
+// rotated states; lo/hi = lambdaF[0], lambdaF[dim+1].
+template <int DIM>
+RKDG_HD void wave_speeds(const double* s1, const double* s2, double p1, double p2,
+                         const SchemeParams& g, double& lo, double& hi) {
+  const double cL = sound_speed_p<DIM>(s1, p1, g), cR = sound_speed_p<DIM>(s2, p2, g);
+  const double uL = s1[1] / s1[0], uR = s2[1] / s2[0];
+  if (g.wave_speeds == RKDG_WS_EINFELDT) {
+    const double sL = sqrt(s1[0]), sR = sqrt(s2[0]);
+    const double sum = sL + sR;
+    const double eta2 = 0.5 * sL * sR / (sum * sum);
+    const double uav = (s1[1] / sL + s2[1] / sR) / sum;
+    const double cav = sqrt((sL * (cL * cL) + sR * (cR * cR)) / sum + eta2 * ((uR - uL) * (uR - uL)));
+    lo = uav - cav;
+    hi = uav + cav;
+    return;
+  }
+  const double gm = g.gamma;
+  const double z = 0.5 * (gm - 1.0) / gm;
+  const double pStar = pow((cL + cR - 0.5 * (gm - 1.0) * (uR - uL)) / (cL / pow(p1, z) + cR / pow(p2, z)),
+                           1.0 / z);
+  const double qL = pStar > p1 ? sqrt(1.0 + 0.5 * (gm + 1.0) * (pStar / p1 - 1.0) / gm) : 1.0;
+  const double qR = pStar > p2 ? sqrt(1.0 + 0.5 * (gm + 1.0) * (pStar / p2 - 1.0) / gm) : 1.0;
+  lo = uL - qL * cL;
+  hi = uR + qR * cR;
+}
+
+// Numerical flux through the unit normal n (rs_rusanov.cpp:6-40, rs_llf.cpp:6-37,
+// rs_hll.cpp:6-50, rs_hllc.cpp:71-161 with getFStar :39-67). The states are
+// taken by value (the reference rotates its inputs in place; nothing observes
+// it). Returns the solver's wave speed; ok=false flags an unphysical input.
+template <int DIM>
+RKDG_HD double riemann(const SchemeParams& g, const double* uL, const double* uR, const double* n,
+                       double* flux, bool& ok) {
+  constexpr int NE = DIM + 2;
+  ok = physical_say<DIM>(uL, g) && physical_say<DIM>(uR, g);
+  double s1[NE], s2[NE], f1[NE], f2[NE];
+#pragma unroll
+  for (int i = 0; i < NE; i++) {
+    s1[i] = uL[i];
+    s2[i] = uR[i];
+  }
+  if (g.riemann == RKDG_RS_RUSANOV) {
+    const double p1 = pressure<DIM>(s1, g), p2 = pressure<DIM>(s2, g);
+    double v1 = 0.0, v2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      v1 += s1[1 + d] * s1[1 + d];
+      v2 += s2[1 + d] * s2[1 + d];
+    }
+    v1 /= s1[0];
+    v2 /= s2[0];
+    const double m1 = sqrt(v1 / s1[0]) + sound_speed_p<DIM>(s1, p1, g);
+    const double m2 = sqrt(v2 / s2[0]) + sound_speed_p<DIM>(s2, p2, g);
+    const double mE = m1 > m2 ? m1 : m2;
+    flux_dot_n<DIM>(s1, n, p1, g, f1);
+    flux_dot_n<DIM>(s2, n, p2, g, f2);
+    double nm = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) nm += n[d] * n[d];
+    const double inm = 1.0 / sqrt(nm);
+#pragma unroll
+    for (int i = 0; i < NE; i++) flux[i] = 0.5 * (f1[i] + f2[i]) * inm - 0.5 * mE * (s2[i] - s1[i]);
+    return mE;
+  }
+  rotate<DIM>(s1, n);
+  rotate<DIM>(s2, n);
+  const double p1 = pressure<DIM>(s1, g), p2 = pressure<DIM>(s2, g);
+  flux_f<DIM>(s1, p1, f1);
+  flux_f<DIM>(s2, p2, f2);
+  double SL, SR;
+  wave_speeds<DIM>(s1, s2, p1, p2, g, SL, SR);
+  double mE;
+  if (g.riemann == RKDG_RS_LLF) {
+    mE = fmax(fabs(SL), fabs(SR));
+#pragma unroll
+    for (int i = 0; i < NE; i++) flux[i] = 0.5 * (f1[i] + f2[i]) - 0.5 * mE * (s2[i] - s1[i]);
+  } else if (g.riemann == RKDG_RS_HLL) {
+    mE = fmax(fabs(SL), fabs(SR));
+    if (SL >= 0) {
+#pragma unroll
+      for (int i = 0; i < NE; i++) flux[i] = f1[i];
+    } else if (SR <= 0) {
+#pragma unroll
+      for (int i = 0; i < NE; i++) flux[i] = f2[i];
+    } else {
+      const double inv = 1.0 / (SR - SL);
+#pragma unroll
+      for (int i = 0; i < NE; i++) flux[i] = (SR * f1[i] - SL * f2[i] + SL * SR * (s2[i] - s1[i])) * inv;
+    }
+  } else {  // HLLC
+    mE = g.hllc_abs ? fmax(fabs(SL), fabs(SR)) : fmax(SL, SR);
+    if (SL >= 0) {
+#pragma unroll
+      for (int i = 0; i < NE; i++) flux[i] = f1[i];
+    } else if (SR <= 0) {
+#pragma unroll
+      for (int i = 0; i < NE; i++) flux[i] = f2[i];
+    } else {
+      const double cL = SL - s1[1] / s1[0];
+      const double cR = SR - s2[1] / s2[0];
+      const double sStar = (p2 - p1 + s1[1] * cL - s2[1] * cR) / (s1[0] * cL - s2[0] * cR);
+      const bool left = sStar > 0.0;
+      const double* st = left ? s1 : s2;
+      const double* fK = left ? f1 : f2;
+      const double pK = left ? p1 : p2;
+      const double SK = left ? SL : SR;
+      const double cK = left ? cL : cR;
+      const double pterm = SK * (pK + st[0] * cK * (sStar - st[1] / st[0]));
+      const double inv = 1.0 / (SK - sStar);
+#pragma unroll
+      for (int i = 0; i < NE; i++) {
+        const double D = i == 1 ? 1.0 : (i == DIM + 1 ? sStar : 0.0);
+        flux[i] = (sStar * (st[i] * SK - fK[i]) + D * pterm) * inv;
+      }
+    }
+  }
+  inverse_rotate<DIM>(flux, n);
+  return mE;
+}
+
+// Ghost state for boundary slot bi (src/bc/boundary_integrator_*.cpp
+// computeRightState); n is the unit outward normal.
+template <int DIM>
+RKDG_HD void bc_state(const SchemeParams& g, int bi, const double* s1, const double* n, double* s2) {
+  constexpr int NE = DIM + 2;
+  const double* P = g.bc_param[bi];
+  const double gm = g.gamma, R = g.gas_constant;
+#pragma unroll
+  for (int i = 0; i < NE; i++) s2[i] = s1[i];
+  switch (g.bc_type[bi]) {
+    case RKDG_BC_SLIP:  // boundary_integrator_slip.cpp:7-17
+      rotate<DIM>(s2, n);
+      s2[1] *= -1;
+      inverse_rotate<DIM>(s2, n);
+      break;
+    case RKDG_BC_OUTLET:  // boundary_integrator_open.cpp:9-12
+      break;
+    case RKDG_BC_SUPERSONIC_INLET: {  // ..._supersonic_inlet.cpp:8-19; state: case_manager.cpp:736-750
+      const double rho = P[0], u = P[1], v = P[2], w = P[3], p = P[4];
+      s2[0] = rho;
+      s2[1] = rho * u;
+      s2[2] = rho * v;
+      if (DIM == 3) s2[3] = rho * w;
+      s2[NE - 1] = p / (gm - 1.) + 0.5 * rho * (u * u + v * v);
+      break;
+    }
+    case RKDG_BC_SUBSONIC_INLET_TOTAL_PRESSURE:    // ..._subsonic_inlet_total_pressure.cpp:8-38
+    case RKDG_BC_SUBSONIC_INLET_FIXED_PRESSURE: {  // ..._subsonic_inlet_fixed_pressure.cpp:8-38
+      const double pTot = P[0], TTot = P[1];
+      double rhoU2 = s2[1] * s2[1] + s2[2] * s2[2];
+      if (DIM == 3) rhoU2 += s2[3] * s2[3];
+      rhoU2 /= s2[0];
+      const double MTTotSq = rhoU2 / s2[0] / gm / R / TTot;
+      const double MSq = MTTotSq / (1.0 - 0.5 * (gm - 1.0) * MTTotSq);
+      const double p = g.bc_type[bi] == RKDG_BC_SUBSONIC_INLET_TOTAL_PRESSURE
+                           ? pTot * pow(1.0 + 0.5 * (gm - 1.0) * MSq, -gm / (gm - 1.0))
+                           : pTot;
+      const double T = TTot / (1.0 + 0.5 * (gm - 1.0) * MSq);
+      const double rho = p / R / T;
+      s2[NE - 1] = p / (gm - 1.0) + 0.5 * rhoU2 / s2[0] * rho;
+      s2[1] = s2[1] / s2[0] * rho;
+      s2[2] = s2[2] / s2[0] * rho;
+      if (DIM == 3) s2[3] = s2[3] / s2[0] * rho;
+      s2[0] = rho;
+      break;
+    }
+    case RKDG_BC_FIXED_PRESSURE_OUTLET: {  // ..._open_fixed_pressure.cpp:9-32
+      const double pFix = P[0];
+      double rhoU2 = s2[1] * s2[1] + s2[2] * s2[2];
+      if (DIM == 3) rhoU2 += s2[3] * s2[3];
+      const double U2 = rhoU2 / s2[0] / s2[0];
+      const double epsIn = (s2[NE - 1] - 0.5 * rhoU2) / s2[0];
+      const double rho = pFix / (gm - 1.0) / epsIn;
+      s2[NE - 1] = pFix / (gm - 1.0) + 0.5 * rho * U2;
+      s2[1] = s1[1] / s1[0] * rho;
+      s2[2] = s1[2] / s1[0] * rho;
+      if (DIM == 3) s2[3] = s2[3] / s1[0] * rho;
+      s2[0] = rho;
+      break;
+    }
+    case RKDG_BC_TOTAL_PRESSURE_OUTLET: {  // ..._open_total_pressure.cpp:9-42 (normal index fixed)
+      const double pTotal = P[0], TFix = P[1];
+      double r2U2 = s2[1] * s2[1] + s2[2] * s2[2];
+      if (DIM == 3) r2U2 += s2[3] * s2[3];
+      double rhoN = s2[1] * n[0] + s2[2] * n[1];
+      if (DIM == 3) rhoN += s2[3] * n[2];
+      const double U2 = r2U2 / s2[0] / s2[0];
+      const double epsIn = rhoN > 0 ? (s2[NE - 1] - 0.5 * r2U2) / s2[0] : R / (gm - 1.0) * TFix;
+      double v2 = r2U2 / s2[0];
+      double pin = pressure<DIM>(s2, g);
+      const double M = sqrt(v2 / s2[0]) / sound_speed_p<DIM>(s2, pin, g);
+      double pr = M > 1 ? pTotal * pow(1.0 + 0.5 * (gm - 1.0) * M * M, -gm / (gm - 1.0)) : pTotal;
+      const double rho = M > 1 ? pr / (gm - 1.0) / epsIn : pr / (0.5 * U2 + (gm - 1.0) * epsIn);
+      pr = M > 1 ? pr : pr - 0.5 * U2 * rho;
+      s2[NE - 1] = pr / (gm - 1.0) + 0.5 * rho * U2;
+      s2[1] *= rho / s2[0];
+      s2[2] *= rho / s2[0];
+      if (DIM == 3) s2[3] *= rho / s2[0];
+      s2[0] = rho;
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+// indicator_bj.cpp:49-79: ratio y from the stencil bounds, then the limiter
+// function (Michalak active in the reference; BJ and Venkatakrishnan selectable).
+RKDG_HD double indicator_y(double diff, double mean, double mI, double MI) {
+  if (diff > 1e-3) return (MI - mean) / diff;
+  if (diff < -1e-3) return (mI - mean) / diff;
+  return 1.0;
+}
+
+}  // namespace rkdg

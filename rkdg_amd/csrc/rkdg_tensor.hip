@@ -1,0 +1,743 @@
+// rkdg_tensor.hip — gfx950 kernels for tensor-product elements (Q_p quads and
+// hexes on affine geometry): the fused right-hand side FE_Evolution::Mult
+// (src/fe_evolution/fe_evolution.cpp:102-216) with its RK-stage epilogue
+// (src/ode_solver/rk_explicit_limited.cpp:51-76), element means
+// (src/averager/averager.cpp:84-109) and the limiter
+// (src/limiter/limiter.cpp:22-94, src/indicator/indicator_bj.cpp:15-200,
+// src/limiter/limiter_multiplier.cpp:12-99, src/limiter/limiter_findiff.cpp:3-33).
+//
+// Work decomposition: one team of T lanes per element (T = next power of two of
+// max(nodes, 2 * face points)), 256-lane workgroups holding EPB = 256/T
+// elements. The element's state is staged in LDS; the volume term is the
+// collocated weak derivative (Gauss-Legendre nodes are the volume quadrature
+// points, so Aflux reduces to a 1D derivative matrix per axis); faces are
+// processed one axis at a time (the -r and +r faces together): normal-direction
+// interpolation to face nodes, tangential interpolation to the face Gauss
+// points, Riemann solve, lift back. Every face flux is evaluated with the
+// arguments ordered as (Elem1 state, Elem2 state, Elem1 normal), so both
+// elements of a face see bitwise identical fluxes and the scheme is
+// conservative to round-off.
+#include "rkdg_kernels.hpp"
+
+namespace rkdg {
+
+template <int DIM, int P>
+struct TC {
+  static constexpr int N = P + 1;
+  static constexpr int ND = DIM == 3 ? N * N * N : N * N;
+  static constexpr int NEQ = DIM + 2;
+  static constexpr int NF = 2 * DIM;
+  // face rule: IntRules order OrderW + 2p (OrderW = 2 trilinear hex, 1 bilinear quad)
+  static constexpr int NQ = DIM == 3 ? P + 2 : P + 1;
+  static constexpr int NFP = DIM == 3 ? NQ * NQ : NQ;
+  static constexpr int NFN = DIM == 3 ? N * N : N;
+  static constexpr int NVERT = DIM == 3 ? 8 : 4;
+  static constexpr int NIP = DIM == 3 ? 4 : 2;  // indicator points per face (order 2 rule)
+  static constexpr int m1 = ND > 2 * NFP ? ND : 2 * NFP;
+  static constexpr int T = m1 <= 4 ? 4 : (m1 <= 8 ? 8 : (m1 <= 16 ? 16 : (m1 <= 32 ? 32 : 64)));
+  static constexpr int BLOCK = 256;
+  static constexpr int EPB = BLOCK / T;
+};
+
+// MFEM local face numbering <-> (normal axis r, side c)
+template <int DIM>
+__host__ __device__ __forceinline__ int lf_of(int r, int c) {
+  if (DIM == 3) {
+    const int t[3][2] = {{4, 2}, {1, 3}, {0, 5}};
+    return t[r][c];
+  }
+  const int t[2][2] = {{3, 1}, {0, 2}};
+  return t[r][c];
+}
+template <int DIM>
+__host__ __device__ __forceinline__ void rc_of(int lf, int& r, int& c) {
+  if (DIM == 3) {
+    const int rr[6] = {2, 1, 0, 1, 0, 2}, cc[6] = {0, 0, 1, 1, 0, 1};
+    r = rr[lf];
+    c = cc[lf];
+  } else {
+    const int rr[4] = {1, 0, 1, 0}, cc[4] = {0, 1, 1, 0};
+    r = rr[lf];
+    c = cc[lf];
+  }
+}
+
+// volume node with index k along axis r and tangential indices (a, b)
+template <int DIM, int N>
+__device__ __forceinline__ int vnode(int r, int k, int a, int b) {
+  if (DIM == 2) return r == 0 ? k + N * a : a + N * k;
+  if (r == 0) return k + N * (a + N * b);
+  if (r == 1) return a + N * (k + N * b);
+  return a + N * (b + N * k);
+}
+
+// orientation: my face-point indices (i, j) -> neighbour frame (i', j')
+template <int DIM, int NQ>
+__device__ __forceinline__ int map_point(int code, int qi, int qj) {
+  int o = (code >> 3) & 7;
+  int i = qi, j = qj;
+  if (DIM == 3 && (o & 1)) {
+    int t = i;
+    i = j;
+    j = t;
+  }
+  if (o & 2) i = NQ - 1 - i;
+  if (DIM == 3 && (o & 4)) j = NQ - 1 - j;
+  return DIM == 3 ? i + NQ * j : i;
+}
+
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nb) {
+  // workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+  // contiguous range of elements so face neighbours share its L2.
+  const int64_t q = nb / 8, r = nb % 8;
+  const int64_t x = b % 8, l = b / 8;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + l;
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
+  if (!(v > 0.0)) return;
+  atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
+}
+
+__device__ __forceinline__ void flag_error(long long* err, int64_t e1, int64_t e2, int lf) {
+  if (atomicCAS(reinterpret_cast<unsigned long long*>(err), 0ull, 1ull) == 0ull) {
+    err[1] = e1;
+    err[2] = e2;
+    err[3] = lf;
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <int DIM, int P>
+__global__ __launch_bounds__(256) void tensor_rhs_kernel(const RhsArgs A, const SchemeParams g,
+                                                         const TensorTables tb) {
+  using C = TC<DIM, P>;
+  constexpr int N = C::N, ND = C::ND, NEQ = C::NEQ, NQ = C::NQ, NFP = C::NFP, NFN = C::NFN;
+  constexpr int T = C::T, EPB = C::EPB, BLOCK = C::BLOCK;
+  constexpr int GD = DIM * DIM + 1;
+
+  __shared__ double s_u[EPB][NEQ][ND];
+  __shared__ double s_tro[EPB][2][NEQ][NFN];
+  __shared__ double s_trn[EPB][2][NEQ][NFN];
+  constexpr int WORK = NEQ * ND > 2 * NEQ * NFP ? NEQ * ND : 2 * NEQ * NFP;
+  __shared__ double s_work[EPB][WORK];
+  __shared__ double s_red[BLOCK / 64];
+
+  const int tid = threadIdx.x;
+  const int team = tid / T, lane = tid % T;
+  const int64_t NE = A.NE;
+  const int64_t NDOFS = NE * ND;
+  const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t e0 = blk * EPB;
+  const int64_t e = e0 + team;
+  const bool active = e < NE;
+
+  for (int v = 0; v < NEQ; v++) {
+    const double* xv = A.x + v * NDOFS + e0 * ND;
+    for (int idx = tid; idx < EPB * ND; idx += BLOCK) {
+      const int te = idx / ND, tn = idx - te * ND;
+      s_u[te][v][tn] = (e0 + te < NE) ? xv[idx] : 1.0;
+    }
+  }
+  double adj[DIM * DIM], detJ = 1.0;
+#pragma unroll
+  for (int i = 0; i < DIM * DIM; i++) adj[i] = active ? A.geo[i * NE + e] : 0.0;
+  if (active) detJ = A.geo[(GD - 1) * NE + e];
+  __syncthreads();
+
+  const int j = lane;
+  const bool node_lane = active && lane < ND;
+  int jj[3] = {0, 0, 0};
+  if (DIM == 2) {
+    jj[0] = j % N;
+    jj[1] = j / N;
+  } else {
+    jj[0] = j % N;
+    jj[1] = (j / N) % N;
+    jj[2] = j / (N * N);
+  }
+  double z[NEQ];
+#pragma unroll
+  for (int v = 0; v < NEQ; v++) z[v] = 0.0;
+  double lmax = 0.0;
+  double* work = s_work[team];
+
+  // ---- volume term: sum_r D^T (w * adj(J) row r . F(u)) along axis r
+  if (!A.face_only) {
+    double F[NEQ * DIM];
+    double wj = 1.0;
+    if (node_lane) {
+      double st[NEQ];
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) st[v] = s_u[team][v][j];
+      flux_tensor<DIM>(st, g, F);
+      lmax = max_char_speed<DIM>(st, g);
+      wj = tb.w[jj[0]] * tb.w[jj[1]];
+      if (DIM == 3) wj *= tb.w[jj[2]];
+    }
+#pragma unroll
+    for (int r = 0; r < DIM; r++) {
+      if (node_lane) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) {
+          double gv = 0.0;
+#pragma unroll
+          for (int d = 0; d < DIM; d++) gv += adj[r * DIM + d] * F[v * DIM + d];
+          work[v * ND + j] = wj * gv;
+        }
+      }
+      __syncthreads();
+      if (node_lane) {
+#pragma unroll
+        for (int a = 0; a < N; a++) {
+          const double dd = tb.D[a][jj[r]];
+          const int q = vnode<DIM, N>(r, a, r == 0 ? jj[1] : jj[0], r == 2 ? jj[1] : jj[2]);
+#pragma unroll
+          for (int v = 0; v < NEQ; v++) z[v] += dd * work[v * ND + q];
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  // ---- face terms, one axis at a time
+  bool bad = false;
+#pragma unroll
+  for (int r = 0; r < DIM; r++) {
+    // own normal-direction traces on faces (r,0), (r,1)
+    for (int it = lane; it < 2 * NFN; it += T) {
+      if (!active) break;
+      const int c = it / NFN, fn = it - c * NFN;
+      const int a = fn % N, b = fn / N;
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; k++) t += s_u[team][v][vnode<DIM, N>(r, k, a, b)] * tb.E[c][k];
+        s_tro[team][c][v][fn] = t;
+      }
+    }
+    // neighbour traces (neighbour's own face frame), read from global memory
+    for (int it = lane; it < 2 * NFN; it += T) {
+      if (!active) break;
+      const int c = it / NFN, fn = it - c * NFN;
+      const int lf = lf_of<DIM>(r, c);
+      const int nb = A.nbr[lf * NE + e];
+      if (nb < 0) continue;
+      const int code = A.fcode[lf * NE + e];
+      int rn, cn;
+      rc_of<DIM>(code & 7, rn, cn);
+      const int a = fn % N, b = fn / N;
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) {
+        const double* xn = A.x + v * NDOFS + (int64_t)nb * ND;
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; k++) t += xn[vnode<DIM, N>(rn, k, a, b)] * tb.E[cn][k];
+        s_trn[team][c][v][fn] = t;
+      }
+    }
+    __syncthreads();
+    // face points: traces, Riemann solve, weighted flux
+    for (int it = lane; it < 2 * NFP; it += T) {
+      if (!active) break;
+      const int c = it / NFP, q = it - c * NFP;
+      const int qi = q % NQ, qj = DIM == 3 ? q / NQ : 0;
+      const int lf = lf_of<DIM>(r, c);
+      const int nb = A.nbr[lf * NE + e];
+      const int code = nb >= 0 ? A.fcode[lf * NE + e] : 0;
+      double um[NEQ], up[NEQ];
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) {
+        double t = 0.0;
+        if (DIM == 3) {
+#pragma unroll
+          for (int b = 0; b < N; b++) {
+            double tt = 0.0;
+#pragma unroll
+            for (int a = 0; a < N; a++) tt += s_tro[team][c][v][a + N * b] * tb.B[qi][a];
+            t += tt * tb.B[qj][b];
+          }
+        } else {
+#pragma unroll
+          for (int a = 0; a < N; a++) t += s_tro[team][c][v][a] * tb.B[qi][a];
+        }
+        um[v] = t;
+      }
+      double an[DIM];
+      const double sgn = c ? 1.0 : -1.0;
+      bool elem1 = true;
+      if (nb >= 0) {
+        const int q2 = map_point<DIM, NQ>(code, qi, qj);
+        const int pi = q2 % NQ, pj = DIM == 3 ? q2 / NQ : 0;
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) {
+          double t = 0.0;
+          if (DIM == 3) {
+#pragma unroll
+            for (int b = 0; b < N; b++) {
+              double tt = 0.0;
+#pragma unroll
+              for (int a = 0; a < N; a++) tt += s_trn[team][c][v][a + N * b] * tb.B[pi][a];
+              t += tt * tb.B[pj][b];
+            }
+          } else {
+#pragma unroll
+            for (int a = 0; a < N; a++) t += s_trn[team][c][v][a] * tb.B[pi][a];
+          }
+          up[v] = t;
+        }
+        elem1 = (code >> 6) & 1;
+        if (elem1) {
+#pragma unroll
+          for (int d = 0; d < DIM; d++) an[d] = sgn * adj[r * DIM + d];
+        } else {
+          int rn, cn;
+          rc_of<DIM>(code & 7, rn, cn);
+          const double sn = cn ? 1.0 : -1.0;
+#pragma unroll
+          for (int d = 0; d < DIM; d++) an[d] = sn * A.geo[(rn * DIM + d) * NE + nb];
+        }
+      } else {
+#pragma unroll
+        for (int d = 0; d < DIM; d++) an[d] = sgn * adj[r * DIM + d];
+      }
+      double nm2 = 0.0;
+#pragma unroll
+      for (int d = 0; d < DIM; d++) nm2 += an[d] * an[d];
+      const double normag = sqrt(nm2);
+      const double inv = 1.0 / normag;
+      double nrm[DIM];
+#pragma unroll
+      for (int d = 0; d < DIM; d++) nrm[d] = an[d] * inv;
+      double fl[NEQ];
+      double factor = -1.0;
+      bool ok = true;
+      double lam = 0.0;
+      if (nb >= 0) {
+        if (elem1) {
+          lam = riemann<DIM>(g, um, up, nrm, fl, ok);
+        } else {
+          lam = riemann<DIM>(g, up, um, nrm, fl, ok);
+          factor = 1.0;
+        }
+      } else {
+        const int slot = -1 - nb;
+        if (slot < g.num_bc) {
+          bc_state<DIM>(g, slot, um, nrm, up);
+          lam = riemann<DIM>(g, um, up, nrm, fl, ok);
+        } else {
+#pragma unroll
+          for (int v = 0; v < NEQ; v++) fl[v] = 0.0;
+        }
+      }
+      if (!ok || lam < 0.0) {
+        if (!bad) flag_error(A.err, elem1 ? e : nb, elem1 ? (nb >= 0 ? nb : -1) : e, lf);
+        bad = true;
+      } else {
+        lmax = lam > lmax ? lam : lmax;
+      }
+      const double wgt = factor * (DIM == 3 ? tb.wq[qi] * tb.wq[qj] : tb.wq[qi]) * normag;
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) work[(c * NEQ + v) * NFP + q] = fl[v] * wgt;
+    }
+    __syncthreads();
+    // lift to face nodes (reuses s_tro)
+    for (int it = lane; it < 2 * NFN; it += T) {
+      if (!active) break;
+      const int c = it / NFN, fn = it - c * NFN;
+      const int a = fn % N, b = fn / N;
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) {
+        double L = 0.0;
+#pragma unroll
+        for (int q = 0; q < NFP; q++) {
+          const int qi = q % NQ, qj = DIM == 3 ? q / NQ : 0;
+          const double bb = DIM == 3 ? tb.B[qi][a] * tb.B[qj][b] : tb.B[qi][a];
+          L += work[(c * NEQ + v) * NFP + q] * bb;
+        }
+        s_tro[team][c][v][fn] = L;
+      }
+    }
+    __syncthreads();
+    if (node_lane) {
+      const int k = jj[r];
+      const int a = r == 0 ? jj[1] : jj[0];
+      const int b = DIM == 3 ? (r == 2 ? jj[1] : jj[2]) : 0;
+      const int fn = DIM == 3 ? a + N * b : a;
+#pragma unroll
+      for (int v = 0; v < NEQ; v++)
+        z[v] += s_tro[team][0][v][fn] * tb.E[0][k] + s_tro[team][1][v][fn] * tb.E[1][k];
+    }
+    __syncthreads();
+  }
+
+  // ---- inverse mass (affine: diagonal det(J) * w_j) and stage epilogue
+  if (!A.face_only) {
+    double outv[NEQ];
+    if (node_lane) {
+      double iw = tb.invw[jj[0]] * tb.invw[jj[1]];
+      if (DIM == 3) iw *= tb.invw[jj[2]];
+      const double im = iw / detJ;
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) {
+        const double yv = z[v] * im;
+        const int64_t idx = v * NDOFS + e * ND + j;
+        if (A.k) A.k[idx] = yv;
+        outv[v] = 0.0;
+        for (int m = 0; m < A.nout; m++) {
+          const double base = A.outs[m].base ? A.outs[m].base[idx] : s_u[team][v][j];
+          const double o = base + A.outs[m].coef * yv;
+          A.outs[m].out[idx] = o;
+          if (m == A.means_of) outv[v] = o;
+        }
+      }
+    }
+    if (A.means) {
+      // element means of the selected output: sum_k (w_k detJ) u_k / sum_k w_k detJ
+      if (node_lane) {
+        double wj = tb.w[jj[0]] * tb.w[jj[1]];
+        if (DIM == 3) wj *= tb.w[jj[2]];
+        const double mk = wj * detJ;
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) work[v * ND + j] = outv[v];
+        work[NEQ * ND + j] = mk;
+      }
+      __syncthreads();
+      if (active && lane < NEQ) {
+        double s = 0.0, ms = 0.0;
+        for (int k = 0; k < ND; k++) {
+          const double mk = work[NEQ * ND + k];
+          s += mk * work[lane * ND + k];
+          ms += mk;
+        }
+        A.means[lane * NE + e] = s / ms;
+      }
+    }
+  }
+
+  // ---- max wave speed: wave reduction, one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(lmax, off, 64);
+    lmax = o > lmax ? o : lmax;
+  }
+  if ((tid & 63) == 0) atomic_max_nonneg(A.lam, lmax);
+}
+
+// ---------------------------------------------------------------------------
+template <int DIM, int P>
+__global__ __launch_bounds__(256) void tensor_means_kernel(const MeansArgs A, const TensorTables tb) {
+  using C = TC<DIM, P>;
+  constexpr int N = C::N, ND = C::ND, NEQ = C::NEQ;
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= A.NE) return;
+  const int64_t NDOFS = A.NE * ND;
+  const double detJ = A.geo[DIM * DIM * A.NE + e];
+  double ms = 0.0;
+  double s[NEQ];
+#pragma unroll
+  for (int v = 0; v < NEQ; v++) s[v] = 0.0;
+  for (int k = 0; k < ND; k++) {
+    double wk = tb.w[k % N] * tb.w[(k / N) % N];
+    if (DIM == 3) wk *= tb.w[k / (N * N)];
+    const double mk = wk * detJ;
+    ms += mk;
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) s[v] += mk * A.x[v * NDOFS + e * ND + k];
+  }
+#pragma unroll
+  for (int v = 0; v < NEQ; v++) A.means[v * A.NE + e] = s[v] / ms;
+}
+
+// ---------------------------------------------------------------------------
+// Limiter::update for one element per team: stencil bounds of the means over
+// the face neighbours, the indicator at two Gauss points per face direction of
+// every interior face, the limiter function, then LimiterMultiplier (with the
+// last-hope reset at the reference vertices) or LimiterFinDiff.
+template <int DIM, int P>
+__global__ __launch_bounds__(256) void tensor_limit_kernel(const LimitArgs A, const SchemeParams g,
+                                                           const TensorTables tb) {
+  using C = TC<DIM, P>;
+  constexpr int N = C::N, ND = C::ND, NEQ = C::NEQ, NFN = C::NFN, NF = C::NF, NIP = C::NIP;
+  constexpr int T = C::T, EPB = C::EPB, BLOCK = C::BLOCK;
+  constexpr int NPT = NF * NIP;  // indicator points per element
+  constexpr int NIT = NPT > C::NVERT ? NPT : C::NVERT;
+
+  __shared__ double s_u[EPB][NEQ][ND];
+  __shared__ double s_tr[EPB][NF][NEQ][NFN];
+  __shared__ double s_y[EPB][NIT][NEQ];
+  __shared__ int s_flag[EPB];
+
+  const int tid = threadIdx.x;
+  const int team = tid / T, lane = tid % T;
+  const int64_t NE = A.NE;
+  const int64_t NDOFS = NE * ND;
+  const int64_t e0 = (int64_t)blockIdx.x * EPB;
+  const int64_t e = e0 + team;
+  const bool active = e < NE;
+
+  for (int v = 0; v < NEQ; v++) {
+    const double* xv = A.x + v * NDOFS + e0 * ND;
+    for (int idx = tid; idx < EPB * ND; idx += BLOCK) {
+      const int te = idx / ND, tn = idx - te * ND;
+      s_u[te][v][tn] = (e0 + te < NE) ? xv[idx] : 1.0;
+    }
+  }
+  if (lane == 0) s_flag[team] = 0;
+  double mean[NEQ], mI[NEQ], MI[NEQ];
+#pragma unroll
+  for (int v = 0; v < NEQ; v++) {
+    mean[v] = active ? A.means[v * NE + e] : 0.0;
+    mI[v] = 1e9;
+    MI[v] = -1e9;
+  }
+  __syncthreads();
+
+  // normal-direction traces on all faces of the element (own frame)
+  for (int it = lane; it < NF * NFN; it += T) {
+    if (!active) break;
+    const int f = it / NFN, fn = it - f * NFN;
+    const int r = f / 2, c = f % 2;
+    const int a = fn % N, b = fn / N;
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; k++) t += s_u[team][v][vnode<DIM, N>(r, k, a, b)] * tb.E[c][k];
+      s_tr[team][f][v][fn] = t;
+    }
+  }
+  // stencil bounds (cell + face neighbours), in every lane
+  const bool ind_bj = g.indicator == RKDG_IND_BJ;
+  if (active && ind_bj) {
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) {
+      mI[v] = mean[v] < mI[v] ? mean[v] : mI[v];
+      MI[v] = mean[v] > MI[v] ? mean[v] : MI[v];
+    }
+    for (int lf = 0; lf < NF; lf++) {
+      const int nb = A.nbr[lf * NE + e];
+      if (nb < 0) continue;
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) {
+        const double a = A.means[v * NE + nb];
+        mI[v] = a < mI[v] ? a : mI[v];
+        MI[v] = a > MI[v] ? a : MI[v];
+      }
+    }
+  }
+  __syncthreads();
+
+  double hK = 0.0;
+  if (active && g.phi == RKDG_PHI_VENKAT && g.venkat_k > 0.0)
+    hK = pow(A.geo[DIM * DIM * NE + e], 1.0 / DIM);
+
+  // indicator values at the points of interior faces
+  for (int it = lane; it < NPT; it += T) {
+    if (!active) break;
+    const int f = it / NIP, ip = it - f * NIP;
+    const int r = f / 2, c = f % 2;
+    const int lf = lf_of<DIM>(r, c);
+    double yv[NEQ];
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) yv[v] = 1e9;
+    if (ind_bj && A.nbr[lf * NE + e] >= 0) {
+      const int pi = ip % 2, pj = ip / 2;
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) {
+        double u = 0.0;
+        if (DIM == 3) {
+#pragma unroll
+          for (int b = 0; b < N; b++) {
+            double tt = 0.0;
+#pragma unroll
+            for (int a = 0; a < N; a++) tt += s_tr[team][f][v][a + N * b] * tb.BI[pi][a];
+            u += tt * tb.BI[pj][b];
+          }
+        } else {
+#pragma unroll
+          for (int a = 0; a < N; a++) u += s_tr[team][f][v][a] * tb.BI[pi][a];
+        }
+        const double diff = u - mean[v];
+        const double y = indicator_y(diff, mean[v], mI[v], MI[v]);
+        double yc;
+        if (g.phi == RKDG_PHI_BJ) {
+          yc = y < 1.0 ? y : 1.0;
+        } else if (g.phi == RKDG_PHI_VENKAT) {
+          if (g.venkat_k > 0.0) {
+            const double eps2 = pow(g.venkat_k * hK, 3.0);
+            const double d2 = diff * diff;
+            yc = (d2 * (y * y + 2.0 * y) + eps2) / (d2 * (y * y + y + 2.0) + eps2);
+          } else {
+            yc = (y * y + 2.0 * y) / (y * y + y + 2.0);
+          }
+        } else {
+          const double yStar = 1.5;
+          const double yRel = y / yStar;
+          yc = y < 1.0 ? y + (3.0 - 2.0 * yStar) * yRel * yRel + (yStar - 2.0) * yRel * yRel * yRel : 1.0;
+        }
+        yv[v] = yc;
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) s_y[team][it][v] = yv[v];
+  }
+  __syncthreads();
+
+  double yMin[NEQ];
+#pragma unroll
+  for (int v = 0; v < NEQ; v++) {
+    double m = 1e9;
+    if (g.indicator == RKDG_IND_NOWHERE) m = 1.0;
+    else if (g.indicator == RKDG_IND_EVERYWHERE) m = 0.0;
+    else {
+      for (int it = 0; it < NPT; it++) {
+        const double y = s_y[team][it][v];
+        m = y < m ? y : m;
+      }
+    }
+    yMin[v] = m;
+  }
+
+  // limit: each node lane owns one node
+  const int j = lane;
+  const bool node_lane = active && lane < ND;
+  double u[NEQ];
+  if (node_lane) {
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) u[v] = s_u[team][v][j];
+  }
+  int trouble = 0;
+  if (g.limiter == RKDG_LIM_MULTI) {
+    const double yr = yMin[0];
+    if (yr < 1.0) trouble = 1;
+    const int reps = g.pow_nd ? ND : 1;
+    if (node_lane) {
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) {
+        double uv = u[v];
+        for (int rr = 0; rr < reps; rr++) uv = mean[v] + yr * (uv - mean[v]);
+        u[v] = uv;
+      }
+    }
+    if (g.last_hope) {
+      __syncthreads();
+      if (node_lane) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) s_u[team][v][j] = u[v];
+      }
+      __syncthreads();
+      for (int it = lane; it < C::NVERT; it += T) {
+        if (!active) break;
+        // reference vertex (bits of it select 0/1 per axis)
+        const int cx = it & 1, cy = (it >> 1) & 1, cz = (it >> 2) & 1;
+        double vs[NEQ];
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) {
+          double s = 0.0;
+          for (int k = 0; k < ND; k++) {
+            double sh = tb.E[cx][k % N] * tb.E[cy][(k / N) % N];
+            if (DIM == 3) sh *= tb.E[cz][k / (N * N)];
+            s += s_u[team][v][k] * sh;
+          }
+          vs[v] = s;
+        }
+        if (!physical<DIM>(vs, g)) atomicOr(&s_flag[team], 1);
+      }
+      __syncthreads();
+      if (active && s_flag[team]) {
+        trouble |= 2;
+        if (node_lane) {
+#pragma unroll
+          for (int v = 0; v < NEQ; v++) u[v] = mean[v];
+        }
+      }
+    }
+  } else if (g.limiter == RKDG_LIM_FINDIFF) {
+    if (yMin[NEQ - 1] < 0.999) {
+      trouble = 1;
+      if (node_lane) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) u[v] = mean[v];
+      }
+    }
+  }
+  if (node_lane && g.limiter != RKDG_LIM_NONE) {
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) A.x[v * NDOFS + e * ND + j] = u[v];
+  }
+  if (A.troubled && active && lane == 0) A.troubled[e] = (int8_t)trouble;
+}
+
+// ---------------------------------------------------------------------------
+__global__ void fill_kernel(double* p, double v, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void copy_kernel(double* dst, const double* src, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+hipError_t launch_fill(double* p, double v, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  fill_kernel<<<(n + 255) / 256, 256, 0, s>>>(p, v, n);
+  return hipGetLastError();
+}
+hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  copy_kernel<<<(n + 255) / 256, 256, 0, s>>>(dst, src, n);
+  return hipGetLastError();
+}
+
+#define RKDG_TENSOR_CASES(X) X(2, 1) X(2, 2) X(2, 3) X(3, 1) X(3, 2) X(3, 3)
+
+bool tensor_supported(int dim, int p) {
+#define X(D, PP) \
+  if (dim == D && p == PP) return true;
+  RKDG_TENSOR_CASES(X)
+#undef X
+  return false;
+}
+
+hipError_t launch_tensor_rhs(int dim, int p, const RhsArgs& a, const SchemeParams& g,
+                             const TensorTables& t, hipStream_t s) {
+#define X(D, PP)                                                           \
+  if (dim == D && p == PP) {                                               \
+    using C = TC<D, PP>;                                                   \
+    const int64_t nb = (a.NE + C::EPB - 1) / C::EPB;                       \
+    tensor_rhs_kernel<D, PP><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t);   \
+    return hipGetLastError();                                              \
+  }
+  RKDG_TENSOR_CASES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_tensor_limit(int dim, int p, const LimitArgs& a, const SchemeParams& g,
+                               const TensorTables& t, hipStream_t s) {
+#define X(D, PP)                                                           \
+  if (dim == D && p == PP) {                                               \
+    using C = TC<D, PP>;                                                   \
+    const int64_t nb = (a.NE + C::EPB - 1) / C::EPB;                       \
+    tensor_limit_kernel<D, PP><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t); \
+    return hipGetLastError();                                              \
+  }
+  RKDG_TENSOR_CASES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_tensor_means(int dim, int p, const MeansArgs& a, const TensorTables& t,
+                               hipStream_t s) {
+#define X(D, PP)                                                              \
+  if (dim == D && p == PP) {                                                  \
+    tensor_means_kernel<D, PP><<<(unsigned)((a.NE + 255) / 256), 256, 0, s>>>(a, t); \
+    return hipGetLastError();                                                 \
+  }
+  RKDG_TENSOR_CASES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+}  // namespace rkdg

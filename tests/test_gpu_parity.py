@@ -1,0 +1,273 @@
+"""GPU parity of the HIP path (through the C-ABI) against the CPU oracle.
+
+Bar: relative L-infinity per conserved variable <= 1e-12 for the right-hand
+side, the limiter and RK steps; wave speeds to 1e-13; troubled-cell flags
+identical. Inputs are seeded smooth states with nodal noise on small meshes
+the oracle handles in seconds.
+"""
+import numpy as np
+import pytest
+
+from rkdg_amd import mesh as M
+from rkdg_amd import scheme as S
+from oracle.refcpu import RefSolver
+from tests.helpers import all_outlet, rel_linf, scramble_orientations, smooth_state
+
+TOL = 1e-12
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(mesh, p, sch):
+    from rkdg_amd.capi import Context
+
+    return Context(mesh, p, sch)
+
+
+def _dev(x, cuda):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(x)).to(cuda)
+
+
+def _host(t):
+    import torch
+
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def _rhs_pair(mesh, p, sch, cuda, seed=0, amp=1e-3):
+    import torch
+
+    ref = RefSolver(mesh, p, sch)
+    x = smooth_state(ref.node_coords(), seed=seed, amp=amp)
+    y_ref, lam_ref = ref.mult(x)
+    ctx = _ctx(mesh, p, sch)
+    xd = _dev(x, cuda)
+    yd = torch.empty_like(xd)
+    lam = ctx.rhs(xd, yd)
+    return _host(yd), y_ref, lam, lam_ref, mesh.dim + 2
+
+
+MESHES = {
+    "hex_q2_periodic": (lambda: M.structured_hex(5, 4, 3), 2),
+    "hex_q1_periodic": (lambda: M.structured_hex(4, 3, 5), 1),
+    "hex_q3_periodic": (lambda: M.structured_hex(3, 3, 3), 3),
+    "quad_q1_periodic": (lambda: M.structured_quad(9, 7, periodic=(True, True)), 1),
+    "quad_q2_periodic": (lambda: M.structured_quad(6, 5, periodic=(True, True)), 2),
+    "hex_q2_skewed": (lambda: _skew(M.structured_hex(4, 4, 3)), 2),
+}
+
+
+def _skew(m):
+    # affine shear keeps elements parallelepipeds with non-diagonal adj(J)
+    A = np.array([[1.0, 0.2, 0.1], [0.0, 0.9, -0.15], [0.05, 0.0, 1.1]])
+    m.elem_coords = np.ascontiguousarray(m.elem_coords @ A.T)
+    return m
+
+
+@pytest.mark.parametrize("name", list(MESHES))
+@pytest.mark.parametrize("rs", [S.RUSANOV, S.LLF, S.HLL, S.HLLC])
+def test_rhs_parity_periodic(name, rs, cuda):
+    mk, p = MESHES[name]
+    m = mk()
+    y, y_ref, lam, lam_ref, neq = _rhs_pair(m, p, S.Scheme(riemann=rs), cuda)
+    err = rel_linf(y, y_ref, neq)
+    print(f"[{name} rs={rs}] rel Linf {err:.3e} lam {lam} ref {lam_ref}", flush=True)
+    assert err <= TOL
+    assert abs(lam - lam_ref) <= 1e-13 * lam_ref
+
+
+@pytest.mark.parametrize("geom", ["hex", "quad"])
+def test_rhs_parity_scrambled_orientation(geom, cuda):
+    m = M.structured_hex(4, 3, 3) if geom == "hex" else M.structured_quad(7, 6, periodic=(True, True))
+    m = scramble_orientations(m, seed=3)
+    p = 2
+    for rs in (S.RUSANOV, S.HLLC):
+        y, y_ref, lam, lam_ref, neq = _rhs_pair(m, p, S.Scheme(riemann=rs), cuda, seed=4)
+        err = rel_linf(y, y_ref, neq)
+        print(f"[scrambled {geom} rs={rs}] rel Linf {err:.3e}", flush=True)
+        assert err <= TOL
+
+
+def test_rhs_parity_einfeldt_and_covolume(cuda):
+    m = M.structured_hex(4, 3, 3)
+    for sch in (S.Scheme(riemann=S.HLLC, wave_speeds=S.EINFELDT),
+                S.Scheme(riemann=S.HLL, covolume=0.05),
+                S.Scheme(riemann=S.RUSANOV, covolume=0.05)):
+        y, y_ref, lam, lam_ref, neq = _rhs_pair(m, 2, sch, cuda, seed=5)
+        assert rel_linf(y, y_ref, neq) <= TOL
+
+
+def _bc_schemes_2d():
+    inlet = S.BC(4, S.BC_SUPERSONIC_INLET, (1.2, 0.4, 0.05, 0.0, 1.1))
+    return [
+        [S.BC(a, S.BC_SLIP) for a in (1, 2, 3, 4)],
+        [S.BC(1, S.BC_SLIP), S.BC(3, S.BC_SLIP), S.BC(2, S.BC_OUTLET), inlet],
+        [S.BC(1, S.BC_SLIP), S.BC(3, S.BC_SLIP), S.BC(2, S.BC_FIXED_PRESSURE_OUTLET, (0.9,)),
+         S.BC(4, S.BC_SUBSONIC_INLET_TOTAL_PRESSURE, (1.5, 1.3))],
+        [S.BC(1, S.BC_SLIP), S.BC(2, S.BC_TOTAL_PRESSURE_OUTLET, (1.0, 1.0)),
+         S.BC(4, S.BC_SUBSONIC_INLET_FIXED_PRESSURE, (1.05, 1.3))],
+        [S.BC(1, S.BC_SLIP), S.BC(2, S.BC_OUTLET)],  # attributes 3, 4 without integrator
+    ]
+
+
+@pytest.mark.parametrize("case", range(5))
+@pytest.mark.parametrize("rs", [S.RUSANOV, S.HLL, S.HLLC])
+def test_rhs_parity_boundaries_2d(case, rs, cuda):
+    m = M.structured_quad(8, 5)
+    sch = S.Scheme(riemann=rs, bcs=_bc_schemes_2d()[case])
+    y, y_ref, lam, lam_ref, neq = _rhs_pair(m, 1, sch, cuda, seed=case)
+    err = rel_linf(y, y_ref, neq)
+    print(f"[bc2d case={case} rs={rs}] rel Linf {err:.3e}", flush=True)
+    assert err <= TOL
+    assert abs(lam - lam_ref) <= 1e-13 * lam_ref
+
+
+@pytest.mark.parametrize("rs", [S.RUSANOV, S.LLF, S.HLLC])
+def test_rhs_parity_boundaries_3d(rs, cuda):
+    m = M.structured_hex(4, 3, 3, periodic=(False, True, False))
+    bcs = [S.BC(1, S.BC_SLIP), S.BC(6, S.BC_SLIP), S.BC(5, S.BC_SUPERSONIC_INLET, (1.1, 0.5, 0.0, 0.1, 1.0)),
+           S.BC(3, S.BC_OUTLET)]
+    y, y_ref, lam, lam_ref, neq = _rhs_pair(m, 2, S.Scheme(riemann=rs, bcs=bcs), cuda, seed=7)
+    assert rel_linf(y, y_ref, neq) <= TOL
+
+
+def test_face_lambda(cuda):
+    m = M.structured_hex(4, 4, 3)
+    sch = S.Scheme(riemann=S.HLLC)
+    ref = RefSolver(m, 2, sch)
+    x = smooth_state(ref.node_coords(), seed=9)
+    _, lam_ref = ref.face_mult(x)
+    ctx = _ctx(m, 2, sch)
+    lam = ctx.face_lambda(_dev(x, cuda))
+    assert abs(lam - lam_ref) <= 1e-13 * lam_ref
+
+
+def test_conservation_periodic(cuda):
+    """sum_e sum_k m_k y_k = 0 per variable on a periodic mesh (round-off)."""
+    import torch
+
+    m = M.structured_hex(6, 5, 4)
+    sch = S.Scheme(riemann=S.HLLC)
+    ctx = _ctx(m, 2, sch)
+    ref = RefSolver(m, 2, sch)
+    x = smooth_state(ref.node_coords(), seed=11, amp=1e-2)
+    xd = _dev(x, cuda)
+    yd = torch.empty_like(xd)
+    ctx.rhs(xd, yd)
+    means = torch.empty(5 * m.num_elements, dtype=torch.float64, device=cuda)
+    ctx.means(yd, means)
+    y = _host(yd).reshape(5, -1)
+    vol = 1.0 / m.num_elements
+    tot = _host(means).reshape(5, -1).sum(axis=1) * vol
+    scale = np.abs(y).max(axis=1) * 1.0
+    print("conservation", tot / scale, flush=True)
+    assert np.all(np.abs(tot) <= 1e-13 * scale)
+
+
+LIMITERS = [
+    S.Scheme(riemann=S.HLLC, indicator=S.IND_BJ, phi=S.PHI_MICHALAK, limiter=S.LIM_MULTI),
+    S.Scheme(riemann=S.HLLC, indicator=S.IND_BJ, phi=S.PHI_BJ, limiter=S.LIM_MULTI, pow_nd=0),
+    S.Scheme(riemann=S.RUSANOV, indicator=S.IND_BJ, phi=S.PHI_VENKAT, venkat_k=5.0, limiter=S.LIM_MULTI),
+    S.Scheme(riemann=S.RUSANOV, indicator=S.IND_BJ, phi=S.PHI_VENKAT, limiter=S.LIM_MULTI),
+    S.Scheme(riemann=S.HLL, indicator=S.IND_BJ, phi=S.PHI_MICHALAK, limiter=S.LIM_FINDIFF),
+    S.Scheme(riemann=S.HLL, indicator=S.IND_NOWHERE, limiter=S.LIM_MULTI),
+    S.Scheme(riemann=S.HLL, indicator=S.IND_EVERYWHERE, limiter=S.LIM_MULTI),
+]
+
+
+def _shocky_state(xyz, seed):
+    """Smooth state plus a density/pressure jump: exercises troubled cells."""
+    x = smooth_state(xyz, seed=seed, amp=1e-2)
+    ne, nd, dim = xyz.shape
+    neq = dim + 2
+    U = x.reshape(neq, ne * nd)
+    X = xyz.reshape(-1, dim)
+    jump = (X[:, 0] > 0.5).astype(float)
+    U[0] *= 1.0 + 2.0 * jump
+    U[-1] *= 1.0 + 4.0 * jump
+    return np.ascontiguousarray(U.reshape(-1))
+
+
+@pytest.mark.parametrize("li", range(len(LIMITERS)))
+@pytest.mark.parametrize("geom", ["hex", "quad"])
+def test_limiter_parity(li, geom, cuda):
+    import torch
+
+    sch = LIMITERS[li]
+    if geom == "hex":
+        m, p = M.structured_hex(6, 4, 3), 2
+    else:
+        m, p = M.structured_quad(12, 6, periodic=(False, True)), 1
+        sch = S.Scheme(**{**sch.__dict__, "bcs": all_outlet(m)})
+    ref = RefSolver(m, p, sch)
+    x = _shocky_state(ref.node_coords(), seed=li)
+    x_ref, ind, mv = ref.limit(x)
+    ctx = _ctx(m, p, sch)
+    xd = _dev(x, cuda)
+    tr = torch.zeros(m.num_elements, dtype=torch.int8, device=cuda)
+    ctx.limit(xd, tr)
+    xl = _host(xd)
+    err = rel_linf(xl, x_ref, m.dim + 2)
+    flags = _host(tr)
+    ref_flags = (mv < 1.0).astype(np.int8) if sch.limiter == S.LIM_MULTI else None
+    print(f"[limiter {li} {geom}] rel Linf {err:.3e} troubled {int((flags & 1).sum())}", flush=True)
+    assert err <= TOL
+    if ref_flags is not None:
+        near = np.abs(mv - 1.0) < 1e-12
+        assert np.all(((flags & 1) == ref_flags) | near)
+
+
+@pytest.mark.parametrize("tab", ["ssprk3", "ref_literal_rk3", "ref_literal_rk2", "euler"])
+def test_step_parity(tab, cuda):
+    m = M.structured_quad(16, 4, periodic=(False, True))
+    sch = S.Scheme(riemann=S.HLLC, indicator=S.IND_BJ, limiter=S.LIM_MULTI, bcs=all_outlet(m))
+    ref = RefSolver(m, 1, sch)
+    x = _shocky_state(ref.node_coords(), seed=2)
+    ctx = _ctx(m, 1, sch)
+    xd = _dev(x, cuda)
+    dt = 2e-3
+    tableau = S.TABLEAUS[tab]
+    for _ in range(3):
+        x, lam_ref = ref.step(x, dt, tableau)
+        lam = ctx.step(xd, dt, tableau)
+    err = rel_linf(_host(xd), x, m.dim + 2)
+    print(f"[step {tab}] rel Linf {err:.3e} lam {lam} {lam_ref}", flush=True)
+    assert err <= TOL
+    assert abs(lam - lam_ref) <= 1e-12 * lam_ref
+
+
+def test_step_parity_hex_venkat(cuda):
+    m = M.structured_hex(5, 4, 4)
+    sch = S.Scheme(riemann=S.RUSANOV, indicator=S.IND_BJ, phi=S.PHI_VENKAT, venkat_k=5.0,
+                   limiter=S.LIM_MULTI)
+    ref = RefSolver(m, 2, sch)
+    x = smooth_state(ref.node_coords(), seed=8, amp=1e-2)
+    ctx = _ctx(m, 2, sch)
+    xd = _dev(x, cuda)
+    for _ in range(2):
+        x, lam_ref = ref.step(x, 1e-3, S.TABLEAUS["ssprk3"])
+        lam = ctx.step(xd, 1e-3, S.TABLEAUS["ssprk3"])
+    assert rel_linf(_host(xd), x, 5) <= TOL
+
+
+def test_unphysical_state_reports_error(cuda):
+    from rkdg_amd.capi import UnphysicalStateError
+    import torch
+
+    m = M.structured_hex(3, 3, 3)
+    ctx = _ctx(m, 1, S.Scheme(riemann=S.HLLC))
+    ref = RefSolver(m, 1, S.Scheme(riemann=S.HLLC))
+    x = smooth_state(ref.node_coords(), seed=1).reshape(5, -1)
+    x[0, 5 * 8:6 * 8] = -1.0  # negative density in element 5
+    xd = _dev(x.reshape(-1), cuda)
+    yd = torch.empty_like(xd)
+    with pytest.raises(UnphysicalStateError):
+        ctx.rhs(xd, yd)
+    with pytest.raises(FloatingPointError):
+        ref.mult(x.reshape(-1))
+    # the error word is cleared: a good state works again
+    x[0, 5 * 8:6 * 8] = 1.0
+    ctx.rhs(_dev(x.reshape(-1), cuda), yd)
