@@ -781,8 +781,37 @@ struct Ref {
   std::vector<double> face_buf;  // slot * 2 * nd * ne
   int64_t n_slots = 0;
 
+  std::vector<char> affine;    // element map is affine (parallelogram/parallelepiped/triangle)
+
   int ne() const { return dim + 2; }
   Trans trans(int64_t e) const { return Trans{&fe, X.data() + e * nv * dim}; }
+  // exact Jacobian of an affine element: J(:,r) = X(v_r) - X(v_0) with v_r the
+  // vertex at the unit point of reference axis r
+  void affine_jacobian(int64_t e, double* J) const {
+    const double* Xe = X.data() + e * nv * dim;
+    int cols[3] = {1, fe.geom == REF_GEOM_TRI ? 2 : 3, 4};
+    for (int i = 0; i < dim; i++)
+      for (int r = 0; r < dim; r++) J[i * dim + r] = Xe[cols[r] * dim + i] - Xe[i];
+  }
+  void detect_affine() {
+    affine.assign(NE, 1);
+    if (fe.geom == REF_GEOM_TRI) return;
+    for (int64_t e = 0; e < NE; e++) {
+      double J[9];
+      affine_jacobian(e, J);
+      const double* Xe = X.data() + e * nv * dim;
+      double scale = 0;
+      for (int i = 0; i < dim * dim; i++) scale = std::max(scale, std::fabs(J[i]));
+      for (int v = 0; v < nv; v++) {
+        const double* R = fe.vert(v);
+        for (int i = 0; i < dim; i++) {
+          double xv = Xe[i];
+          for (int r = 0; r < dim; r++) xv += J[i * dim + r] * R[r];
+          if (std::fabs(xv - Xe[v * dim + i]) > 1e-10 * scale) affine[e] = 0;
+        }
+      }
+    }
+  }
 
   // reference coordinates of face point s (face param) in element e's local face
   void face_to_elem(const Face& f, int side, const double* s, double* xi) const {
@@ -809,11 +838,18 @@ struct Ref {
     }
   }
   // CalcOrtho of the face Jacobian at face point s (normal out of Elem1, |n| = measure density)
+  // The reference faces are affine images of the unit square/segment, so the
+  // face-map derivative D is exact (edge vectors of the reference face); for
+  // affine elements the element Jacobian is the exact vertex-difference matrix.
+  // Exactness matters: the reference 3D rotation (rs_basic.cpp:21-28) loses
+  // orthogonality when n_z = 1 - ulp, so last-bit noise in an axis-aligned
+  // unit normal would perturb rotated fluxes at the 1e-8 level.
   void face_normal(const Face& f, const double* s, double* nor) const {
     double xi[3];
     face_to_elem(f, 0, s, xi);
     double J[9];
-    trans(f.e1).jacobian(xi, J);
+    if (affine[f.e1]) affine_jacobian(f.e1, J);
+    else trans(f.e1).jacobian(xi, J);
     const double* R[4];
     for (int m = 0; m < fe.nvf; m++) R[m] = fe.vert(fe.face_vert(f.lf1, m));
     if (dim == 2) {
@@ -824,8 +860,8 @@ struct Ref {
     } else {
       double Ds[3], Dt[3];
       for (int d = 0; d < 3; d++) {
-        Ds[d] = -(1 - s[1]) * R[0][d] + (1 - s[1]) * R[1][d] + s[1] * R[2][d] - s[1] * R[3][d];
-        Dt[d] = -(1 - s[0]) * R[0][d] - s[0] * R[1][d] + s[0] * R[2][d] + (1 - s[0]) * R[3][d];
+        Ds[d] = R[1][d] - R[0][d];
+        Dt[d] = R[3][d] - R[0][d];
       }
       double a[3], b[3];
       for (int i = 0; i < 3; i++) {
@@ -1382,6 +1418,7 @@ void* ref_create(int geom, int order, int64_t num_elements, const double* elem_c
     r->contrib[r->faces[f].e1].push_back({slot, 0});
     slot++;
   }
+  r->detect_affine();
   r->assemble_aflux();
   r->assemble_mass();
   return r;

@@ -376,6 +376,29 @@ RKDG_HD void bc_state(const SchemeParams& g, int bi, const double* s1, const dou
   }
 }
 
+// Euclidean norm by the scaled (hypot-style) recurrence of MFEM Vector::Norml2,
+// which the reference uses to normalise face normals (face_integrator.cpp:165).
+// A vector with one non-zero entry a gets exactly |a|.
+template <int DIM>
+RKDG_HD double norml2_scaled(const double* v) {
+  double scale = 0.0, sum = 0.0;
+#pragma unroll
+  for (int i = 0; i < DIM; i++) {
+    if (v[i] != 0.0) {
+      const double a = fabs(v[i]);
+      if (scale <= a) {
+        const double r = scale / a;
+        sum = 1.0 + sum * (r * r);
+        scale = a;
+      } else {
+        const double r = a / scale;
+        sum += r * r;
+      }
+    }
+  }
+  return scale * sqrt(sum);
+}
+
 // indicator_bj.cpp:49-79: ratio y from the stencil bounds, then the limiter
 // function (Michalak active in the reference; BJ and Venkatakrishnan selectable).
 RKDG_HD double indicator_y(double diff, double mean, double mI, double MI) {

@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void tensor_rhs_kernel(const RhsArgs A, const 
   __shared__ double s_u[EPB][NEQ][ND];
   __shared__ double s_tro[EPB][2][NEQ][NFN];
   __shared__ double s_trn[EPB][2][NEQ][NFN];
-  constexpr int WORK = NEQ * ND > 2 * NEQ * NFP ? NEQ * ND : 2 * NEQ * NFP;
+  constexpr int WORK = (NEQ + 1) * ND > 2 * NEQ * NFP ? (NEQ + 1) * ND : 2 * NEQ * NFP;
   __shared__ double s_work[EPB][WORK];
   __shared__ double s_red[BLOCK / 64];
 
@@ -306,7 +306,7 @@ __global__ __launch_bounds__(256) void tensor_rhs_kernel(const RhsArgs A, const 
 #pragma unroll
       for (int d = 0; d < DIM; d++) nm2 += an[d] * an[d];
       const double normag = sqrt(nm2);
-      const double inv = 1.0 / normag;
+      const double inv = 1.0 / norml2_scaled<DIM>(an);
       double nrm[DIM];
 #pragma unroll
       for (int d = 0; d < DIM; d++) nrm[d] = an[d] * inv;
