@@ -1,0 +1,212 @@
+"""Benchmark: DOF-stage updates/s of the RKDG step on one MI355X.
+
+Workload (one GPU): 3D periodic acoustic pulse on [0,1]^3, 96^3 hex mesh
+(884,736 elements), Q2 DG (27 nodes x 5 variables = 119,439,360 DOF), Rusanov
+flux, Venkatakrishnan limiter (K = 5) on the multiplier limiter, SSP-RK3
+(Butcher form consumed as the reference does), CFL 0.2, FP64. A "step" is one
+RK step = 3 stages of limiter + right-hand side + stage update; the metric
+counts DOF x stages per second, inputs resident in HBM.
+
+The dominant kernel (the fused right-hand-side/stage kernel) is timed with HIP
+events on the context's stream inside the timed region; its roofline uses the
+algorithmic bytes of each launch (arrays it must read and write once, plus
+geometry and connectivity). The CPU baseline is the oracle (a restatement of
+the reference path with its data structures) timed on this host on a smaller
+mesh of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 96] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+METRIC = "DOF-stage updates/s (FP64, Q2 hex) & % HBM roofline; adjoint/forward time ratio"
+
+
+def pulse_state(xyz, gamma=1.4, amp=1e-3, width=0.05):
+    """rho = 1 + A g, p = 1/gamma + A g, u = 0, g = exp(-|x - 1/2|^2 / w^2)."""
+    r2 = ((xyz - 0.5) ** 2).sum(axis=-1).ravel()
+    g = np.exp(-r2 / (width * width))
+    rho = 1.0 + amp * g
+    p = 1.0 / gamma + amp * g
+    E = p / (gamma - 1.0)
+    z = np.zeros_like(rho)
+    return np.ascontiguousarray(np.concatenate([rho, z, z, z, E]))
+
+
+def scheme():
+    from rkdg_amd import scheme as S
+
+    return S.Scheme(riemann=S.RUSANOV, indicator=S.IND_BJ, phi=S.PHI_VENKAT, venkat_k=5.0,
+                    limiter=S.LIM_MULTI, last_hope=1, pow_nd=1)
+
+
+def stage_bytes(neq, ndofs, ne, dim, s=3):
+    """Algorithmic bytes of the s stage-kernel launches of one SSP-RK3 step.
+
+    stage 0 reads x and writes W1, W2, acc; stage 1 reads W1, W2, acc and writes
+    W2, acc; stage 2 reads W2, acc and writes x. Every launch also reads the
+    per-element geometry (adj(J), det J), the neighbour and face-code tables and
+    writes the element means of its next-stage output.
+    """
+    vec = 8 * neq * ndofs
+    per_el = 8 * (dim * dim + 1) + 2 * 4 * 2 * dim + 8 * neq
+    arrays = [1 + 3, 3 + 2, 2 + 1]
+    return [a * vec + per_el * ne for a in arrays[:s]]
+
+
+def cpu_baseline(n, steps, threads):
+    from oracle.refcpu import RefSolver, lib
+    from rkdg_amd import mesh as M
+    from rkdg_amd import scheme as S
+
+    lib().ref_set_threads(threads)
+    m = M.structured_hex(n, n, n)
+    ref = RefSolver(m, 2, scheme(), threads=threads)
+    x = pulse_state(ref.node_coords())
+    _, lam = ref.face_mult(x)
+    dt = 0.2 * ref.hmin() / lam / 5.0
+    x, lam = ref.step(x, dt, S.TABLEAUS["ssprk3"])  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        x, lam = ref.step(x, dt, S.TABLEAUS["ssprk3"])
+    el = time.perf_counter() - t0
+    dofs = 5 * m.num_elements * 27
+    return dofs * 3 * steps / el, el, m.num_elements
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=96)
+    ap.add_argument("--cfl", type=float, default=0.2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=48)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the RKDG path is built and measured on a single MI355X)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from rkdg_amd import mesh as M
+    from rkdg_amd import scheme as S
+    from rkdg_amd.capi import Context
+
+    dev = torch.device("cuda:0")
+    n = args.n
+    t0 = time.perf_counter()
+    m = M.structured_hex(n, n, n)
+    ctx = Context(m, 2, scheme())
+    info = ctx.info
+    x0 = pulse_state(ctx.node_coords())
+    print(f"[bench] mesh {n}^3 Q2 hex: NE={info.num_elements} DOF={info.state_size} "
+          f"setup {time.perf_counter() - t0:.1f}s", flush=True)
+    x = torch.from_numpy(x0).to(dev)
+    tab = S.TABLEAUS["ssprk3"]
+    hmin = ctx.hmin()
+    lam = ctx.face_lambda(x)
+    ctx.limit(x)
+    dt = args.cfl * hmin / lam / (2 * 2 + 1)
+    for _ in range(args.warmup):
+        lam = ctx.step(x, dt, tab)
+        dt = args.cfl * hmin / lam / 5.0
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.step(x, dt, tab, want_lambda=False)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    rhs_ms, rhs_n, lim_ms, lim_n = ctx.profile_read()
+    ctx.profile(False)
+    ctx.check_error()
+    lam = ctx.face_lambda(x)
+
+    dofs = info.state_size
+    ms_step = el * 1e3 / args.steps
+    value = dofs * 3 * args.steps / el
+    sb = stage_bytes(5, info.num_dofs, info.num_elements, 3)
+    bytes_per_launch = sum(sb) / len(sb)
+    rhs_avg_ms = rhs_ms / max(1, rhs_n)
+    lim_avg_ms = lim_ms / max(1, lim_n)
+    achieved = bytes_per_launch / (rhs_avg_ms * 1e-3) / 1e9
+    lim_bytes = 2 * 8 * dofs + 8 * 5 * info.num_elements * 7
+    traffic = None
+    tr_file = os.path.join(ROOT, "profiles", "traffic_rhs_stage_q2hex96.json")
+    if os.path.exists(tr_file):
+        try:
+            traffic = json.load(open(tr_file)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    print(f"[bench] stage kernel avg {rhs_avg_ms:.3f} ms over {rhs_n} launches "
+          f"({achieved:.0f} GB/s algorithmic), limiter avg {lim_avg_ms:.3f} ms "
+          f"({lim_bytes / (lim_avg_ms * 1e-3) / 1e9:.0f} GB/s), step {ms_step:.3f} ms, "
+          f"lambda {lam:.6f}", flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cv, cel, cne = cpu_baseline(args.cpu_n, args.cpu_steps, threads)
+        cpu = {"value": cv, "unit": "DOF-stage updates/s", "cores": threads, "kind": "port",
+               "sample": f"oracle (CSR Aflux, dense inverse mass, per-point shape evaluation) on the "
+                         f"same workload at {args.cpu_n}^3 elements ({cne} elements), "
+                         f"{args.cpu_steps} SSP-RK3 steps after 1 warm-up step, {cel:.1f} s, "
+                         f"OpenMP {threads} threads"}
+        print(f"[bench] cpu baseline {cv:.3e} DOF-stage/s ({cel:.1f}s)", flush=True)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF-stage updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"3D periodic acoustic pulse, {n}^3 Q2 hex ({info.num_elements} elements, "
+                        f"{dofs} DOF), Rusanov, Venkatakrishnan K=5 multiplier limiter, SSP-RK3, "
+                        f"CFL {args.cfl}",
+            "elements": info.num_elements,
+            "dof": dofs,
+            "stages_per_step": 3,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "tensor_rhs_kernel<3,2> (fused RHS + SSP-RK3 stage update)",
+            "avg_launch_ms": rhs_avg_ms,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -153,6 +153,15 @@ int rkdg_node_coords(rkdg_ctx* ctx, double* xyz);
  * element pair in info[0], info[1]); clears it. */
 int rkdg_check_error(rkdg_ctx* ctx, int64_t* info);
 
+/* Instrumentation, no counterpart in the reference (which only has a wall-clock
+ * StopWatch per step, main.cpp:458-459): when enabled, HIP events on the
+ * context's stream bracket every right-hand-side kernel (plain or RK stage) and
+ * every limiter kernel launch. Enabling resets the counters; reading
+ * synchronizes and returns the summed kernel milliseconds and launch counts. */
+int rkdg_profile_enable(rkdg_ctx* ctx, int enable);
+int rkdg_profile_read(rkdg_ctx* ctx, double* rhs_ms, int64_t* rhs_launches, double* limit_ms,
+                      int64_t* limit_launches);
+
 #ifdef __cplusplus
 }
 #endif

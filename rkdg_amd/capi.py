@@ -57,7 +57,7 @@ class Info(C.Structure):
 EXPORTS = [
     "rkdg_create", "rkdg_destroy", "rkdg_last_error", "rkdg_get_info", "rkdg_set_stream",
     "rkdg_rhs", "rkdg_face_lambda", "rkdg_limit", "rkdg_step", "rkdg_means", "rkdg_hmin",
-    "rkdg_node_coords", "rkdg_check_error",
+    "rkdg_node_coords", "rkdg_check_error", "rkdg_profile_enable", "rkdg_profile_read",
 ]
 
 _lib = None
@@ -87,6 +87,9 @@ def lib():
     L.rkdg_hmin.argtypes = [P, C.POINTER(C.c_double)]
     L.rkdg_node_coords.argtypes = [P, C.POINTER(C.c_double)]
     L.rkdg_check_error.argtypes = [P, C.POINTER(C.c_int64)]
+    L.rkdg_profile_enable.argtypes = [P, C.c_int]
+    L.rkdg_profile_read.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     _lib = L
     return L
 
@@ -220,6 +223,16 @@ class Context:
         out = np.zeros((self.info.num_elements, self.info.nodes_per_element, self.info.dim))
         _check(lib().rkdg_node_coords(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
+
+    def profile(self, enable=True):
+        _check(lib().rkdg_profile_enable(self.h, 1 if enable else 0))
+
+    def profile_read(self):
+        """(rhs_ms_total, rhs_launches, limit_ms_total, limit_launches)"""
+        a, b = C.c_double(0), C.c_int64(0)
+        c_, d = C.c_double(0), C.c_int64(0)
+        _check(lib().rkdg_profile_read(self.h, C.byref(a), C.byref(b), C.byref(c_), C.byref(d)))
+        return a.value, b.value, c_.value, d.value
 
     def check_error(self):
         info = (C.c_int64 * 2)()

@@ -149,11 +149,16 @@ struct rkdg_ctx {
   double* d_lam = nullptr;    // per-stage wave speeds [8]
   long long* d_err = nullptr;
   double* d_means = nullptr;
+  TensorTables* d_tb = nullptr;
   double* d_stage[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<double> coords;  // [NE][nv][dim]
   std::vector<double> geo_h;   // [g][NE]
   int64_t n_int = 0, n_bdr = 0;
   double hmin = 0.0;
+  // instrumentation: event pairs around kernel launches
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, int>> ev_rhs, ev_lim;  // (start, stop) pool indices
 };
 
 namespace {
@@ -166,7 +171,9 @@ void free_ctx(rkdg_ctx* c) {
   (void)hipFree(c->d_lam);
   (void)hipFree(c->d_err);
   (void)hipFree(c->d_means);
+  (void)hipFree(c->d_tb);
   for (auto& p : c->d_stage) (void)hipFree(p);
+  for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
   delete c;
 }
 
@@ -408,7 +415,36 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
   HIP_TRY(hipMalloc(&c->d_err, sizeof(long long) * 4));
   HIP_TRY(hipMemset(c->d_err, 0, sizeof(long long) * 4));
   HIP_TRY(hipMalloc(&c->d_means, sizeof(double) * c->neq * NE));
+  HIP_TRY(hipMalloc(&c->d_tb, sizeof(TensorTables)));
+  HIP_TRY(hipMemcpy(c->d_tb, &c->tb, sizeof(TensorTables), hipMemcpyHostToDevice));
   return RKDG_OK;
+}
+
+constexpr size_t kMaxProfiled = 8192;
+
+// returns (start, stop) event indices or (-1, -1) when profiling is off/full
+std::pair<int, int> prof_begin(rkdg_ctx* c, std::vector<std::pair<int, int>>& list) {
+  if (!c->prof || list.size() >= kMaxProfiled) return {-1, -1};
+  size_t need = (c->ev_rhs.size() + c->ev_lim.size() + 1) * 2;
+  while (c->ev_pool.size() < need) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return {-1, -1};
+    c->ev_pool.push_back(e);
+  }
+  std::pair<int, int> pr{(int)need - 2, (int)need - 1};
+  (void)hipEventRecord(c->ev_pool[pr.first], c->stream);
+  list.push_back(pr);
+  return pr;
+}
+void prof_end(rkdg_ctx* c, std::pair<int, int> pr) {
+  if (pr.second >= 0) (void)hipEventRecord(c->ev_pool[pr.second], c->stream);
+}
+
+hipError_t rhs_launch(rkdg_ctx* c, const RhsArgs& a) {
+  auto pr = prof_begin(c, c->ev_rhs);
+  hipError_t e = launch_tensor_rhs(c->dim, c->p, a, c->sp, c->d_tb, c->n_bdr > 0, c->stream);
+  prof_end(c, pr);
+  return e;
 }
 
 RhsArgs base_args(rkdg_ctx* c, const double* x) {
@@ -453,10 +489,13 @@ int read_lambda(rkdg_ctx* c, int slot, double* lam) {
 int limit_impl(rkdg_ctx* c, double* x, int8_t* troubled, bool have_means) {
   if (!have_means) {
     MeansArgs ma{x, c->d_means, c->d_geo, c->NE};
-    HIP_TRY(launch_tensor_means(c->dim, c->p, ma, c->tb, c->stream));
+    HIP_TRY(launch_tensor_means(c->dim, c->p, ma, c->d_tb, c->stream));
   }
   LimitArgs la{x, c->d_means, c->d_geo, c->d_nbr, c->d_fcode, c->NE, troubled};
-  HIP_TRY(launch_tensor_limit(c->dim, c->p, la, c->sp, c->tb, c->stream));
+  auto pr = prof_begin(c, c->ev_lim);
+  hipError_t le = launch_tensor_limit(c->dim, c->p, la, c->sp, c->d_tb, c->stream);
+  prof_end(c, pr);
+  HIP_TRY(le);
   return RKDG_OK;
 }
 
@@ -523,7 +562,7 @@ int rkdg_rhs(rkdg_ctx* c, const double* x, double* y, double* lambda_max) {
   HIP_TRY(launch_fill(c->d_lam, 0.0, 1, c->stream));
   RhsArgs a = base_args(c, x);
   a.k = y;
-  HIP_TRY(launch_tensor_rhs(c->dim, c->p, a, c->sp, c->tb, c->stream));
+  HIP_TRY(rhs_launch(c, a));
   if (lambda_max) return read_lambda(c, 0, lambda_max);
   return RKDG_OK;
 }
@@ -533,7 +572,7 @@ int rkdg_face_lambda(rkdg_ctx* c, const double* x, double* lambda_max) {
   HIP_TRY(launch_fill(c->d_lam, 0.0, 1, c->stream));
   RhsArgs a = base_args(c, x);
   a.face_only = 1;
-  HIP_TRY(launch_tensor_rhs(c->dim, c->p, a, c->sp, c->tb, c->stream));
+  HIP_TRY(rhs_launch(c, a));
   return read_lambda(c, 0, lambda_max);
 }
 
@@ -545,7 +584,7 @@ int rkdg_limit(rkdg_ctx* c, double* x, int8_t* troubled) {
 int rkdg_means(rkdg_ctx* c, const double* x, double* means) {
   if (!c || !x || !means) return fail(RKDG_E_ARG, "null argument");
   MeansArgs ma{x, means, c->d_geo, c->NE};
-  HIP_TRY(launch_tensor_means(c->dim, c->p, ma, c->tb, c->stream));
+  HIP_TRY(launch_tensor_means(c->dim, c->p, ma, c->d_tb, c->stream));
   return RKDG_OK;
 }
 
@@ -587,7 +626,7 @@ int rkdg_step(rkdg_ctx* c, double* x, double t, double dt, int s, const double* 
     ra.nout = n;
     ra.means = c->d_means;
     ra.means_of = last ? n - 1 : 0;  // next stage input W_{i+1} is outs[0]; final is the last
-    HIP_TRY(launch_tensor_rhs(c->dim, c->p, ra, c->sp, c->tb, c->stream));
+    HIP_TRY(rhs_launch(c, ra));
     if (last && s == 1) HIP_TRY(launch_copy(x, acc, (int64_t)c->neq * c->ndofs, c->stream));
     double* lim = last ? x : W[i + 1];
     rc = limit_impl(c, lim, nullptr, true);
@@ -626,6 +665,36 @@ int rkdg_node_coords(rkdg_ctx* c, double* xyz) {
       }
     }
   }
+  return RKDG_OK;
+}
+
+int rkdg_profile_enable(rkdg_ctx* c, int enable) {
+  if (!c) return fail(RKDG_E_ARG, "null context");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->prof = enable != 0;
+  c->ev_rhs.clear();
+  c->ev_lim.clear();
+  return RKDG_OK;
+}
+
+int rkdg_profile_read(rkdg_ctx* c, double* rhs_ms, int64_t* rhs_n, double* lim_ms, int64_t* lim_n) {
+  if (!c) return fail(RKDG_E_ARG, "null context");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  double tr = 0.0, tl = 0.0;
+  for (auto& pr : c->ev_rhs) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev_pool[pr.first], c->ev_pool[pr.second]));
+    tr += ms;
+  }
+  for (auto& pr : c->ev_lim) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev_pool[pr.first], c->ev_pool[pr.second]));
+    tl += ms;
+  }
+  if (rhs_ms) *rhs_ms = tr;
+  if (rhs_n) *rhs_n = (int64_t)c->ev_rhs.size();
+  if (lim_ms) *lim_ms = tl;
+  if (lim_n) *lim_n = (int64_t)c->ev_lim.size();
   return RKDG_OK;
 }
 

@@ -64,11 +64,12 @@ struct MeansArgs {
 };
 
 // launchers (rkdg_tensor.hip)
+// t: device copy of the tables; has_bdr selects the kernel with boundary faces
 hipError_t launch_tensor_rhs(int dim, int p, const RhsArgs& a, const SchemeParams& g,
-                             const TensorTables& t, hipStream_t s);
+                             const TensorTables* t, bool has_bdr, hipStream_t s);
 hipError_t launch_tensor_limit(int dim, int p, const LimitArgs& a, const SchemeParams& g,
-                               const TensorTables& t, hipStream_t s);
-hipError_t launch_tensor_means(int dim, int p, const MeansArgs& a, const TensorTables& t,
+                               const TensorTables* t, hipStream_t s);
+hipError_t launch_tensor_means(int dim, int p, const MeansArgs& a, const TensorTables* t,
                                hipStream_t s);
 hipError_t launch_fill(double* p, double v, int64_t n, hipStream_t s);
 hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);

@@ -200,53 +200,116 @@ RKDG_HD void wave_speeds(const double* s1, const double* s2, double p1, double p
 // rs_hll.cpp:6-50, rs_hllc.cpp:71-161 with getFStar :39-67). The states are
 // taken by value (the reference rotates its inputs in place; nothing observes
 // it). Returns the solver's wave speed; ok=false flags an unphysical input.
+//
+// Per side, 1/rho is formed once and reused (the reference divides by rho in
+// each formula); on the device it is v_rcp_f64 refined by two Newton steps
+// (correct to about one ulp). The differences are rounding-level.
+RKDG_HD double recip(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return r;
+#else
+  return 1.0 / x;
+#endif
+}
+
 template <int DIM>
-RKDG_HD double riemann(const SchemeParams& g, const double* uL, const double* uR, const double* n,
-                       double* flux, bool& ok) {
+struct SidePrim {
+  double inv, m2, p, pid;  // 1/rho, |rho u|^2, pressure, covolume-free pressure
+};
+template <int DIM>
+RKDG_HD SidePrim<DIM> side_prim(const double* s, const SchemeParams& g) {
+  SidePrim<DIM> q;
+  q.inv = recip(s[0]);
+  double m2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) m2 += s[1 + d] * s[1 + d];
+  q.m2 = m2;
+  q.pid = (g.gamma - 1.0) * (s[1 + DIM] - 0.5 * (m2 * q.inv));
+  q.p = g.covolume != 0.0 ? q.pid / (1.0 - s[0] * g.covolume) : q.pid;
+  return q;
+}
+template <int DIM>
+RKDG_HD double side_sound(const double* s, const SidePrim<DIM>& q, const SchemeParams& g) {
+  double a = g.gamma * q.p * q.inv;
+  if (g.covolume != 0.0) a /= (1.0 - s[0] * g.covolume);
+  return sqrt(a);
+}
+// StateIsPhysicalSay (physics.cpp:53-97) from the side primitives
+template <int DIM>
+RKDG_HD bool side_ok(const double* s, const SidePrim<DIM>& q) {
+  return (s[0] >= 0.0) && (s[1 + DIM] > 0.0) && (q.pid > 0.0);
+}
+
+template <int DIM, int RS>
+RKDG_HD double riemann_t(const SchemeParams& g, const double* uL, const double* uR, const double* n,
+                         double* flux, bool& ok) {
   constexpr int NE = DIM + 2;
-  ok = physical_say<DIM>(uL, g) && physical_say<DIM>(uR, g);
   double s1[NE], s2[NE], f1[NE], f2[NE];
 #pragma unroll
   for (int i = 0; i < NE; i++) {
     s1[i] = uL[i];
     s2[i] = uR[i];
   }
-  if (g.riemann == RKDG_RS_RUSANOV) {
-    const double p1 = pressure<DIM>(s1, g), p2 = pressure<DIM>(s2, g);
-    double v1 = 0.0, v2 = 0.0;
+  if constexpr (RS == RKDG_RS_RUSANOV) {
+    const SidePrim<DIM> q1 = side_prim<DIM>(s1, g), q2 = side_prim<DIM>(s2, g);
+    ok = side_ok<DIM>(s1, q1) && side_ok<DIM>(s2, q2);
+    const double m1 = sqrt(q1.m2) * q1.inv + side_sound<DIM>(s1, q1, g);
+    const double m2 = sqrt(q2.m2) * q2.inv + side_sound<DIM>(s2, q2, g);
+    const double mE = m1 > m2 ? m1 : m2;
+    double vn1 = 0.0, vn2 = 0.0, nm = 0.0;
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-      v1 += s1[1 + d] * s1[1 + d];
-      v2 += s2[1 + d] * s2[1 + d];
+      vn1 += s1[1 + d] * n[d];
+      vn2 += s2[1 + d] * n[d];
+      nm += n[d] * n[d];
     }
-    v1 /= s1[0];
-    v2 /= s2[0];
-    const double m1 = sqrt(v1 / s1[0]) + sound_speed_p<DIM>(s1, p1, g);
-    const double m2 = sqrt(v2 / s2[0]) + sound_speed_p<DIM>(s2, p2, g);
-    const double mE = m1 > m2 ? m1 : m2;
-    flux_dot_n<DIM>(s1, n, p1, g, f1);
-    flux_dot_n<DIM>(s2, n, p2, g, f2);
-    double nm = 0.0;
+    const double a1 = vn1 * q1.inv, a2 = vn2 * q2.inv;
+    f1[0] = vn1;
+    f2[0] = vn2;
 #pragma unroll
-    for (int d = 0; d < DIM; d++) nm += n[d] * n[d];
-    const double inm = 1.0 / sqrt(nm);
+    for (int d = 0; d < DIM; d++) {
+      f1[1 + d] = a1 * s1[1 + d] + q1.p * n[d];
+      f2[1 + d] = a2 * s2[1 + d] + q2.p * n[d];
+    }
+    f1[1 + DIM] = a1 * (s1[1 + DIM] + q1.p);
+    f2[1 + DIM] = a2 * (s2[1 + DIM] + q2.p);
+    // the unit normal's own norm (rs_rusanov.cpp:27-32); 1 up to rounding
+    const double inm = 0.5 * recip(sqrt(nm));
+    const double hm = 0.5 * mE;
 #pragma unroll
-    for (int i = 0; i < NE; i++) flux[i] = 0.5 * (f1[i] + f2[i]) * inm - 0.5 * mE * (s2[i] - s1[i]);
+    for (int i = 0; i < NE; i++) flux[i] = (f1[i] + f2[i]) * inm - hm * (s2[i] - s1[i]);
     return mE;
-  }
+  } else {
   rotate<DIM>(s1, n);
   rotate<DIM>(s2, n);
-  const double p1 = pressure<DIM>(s1, g), p2 = pressure<DIM>(s2, g);
-  flux_f<DIM>(s1, p1, f1);
-  flux_f<DIM>(s2, p2, f2);
+  const SidePrim<DIM> q1 = side_prim<DIM>(s1, g), q2 = side_prim<DIM>(s2, g);
+  ok = side_ok<DIM>(s1, q1) && side_ok<DIM>(s2, q2);
+  const double p1 = q1.p, p2 = q2.p;
+  {
+    const double u1 = s1[1] * q1.inv, u2 = s2[1] * q2.inv;
+    f1[0] = s1[1];
+    f2[0] = s2[1];
+    f1[1] = u1 * s1[1] + p1;
+    f2[1] = u2 * s2[1] + p2;
+#pragma unroll
+    for (int d = 1; d < DIM; d++) {
+      f1[d + 1] = u1 * s1[d + 1];
+      f2[d + 1] = u2 * s2[d + 1];
+    }
+    f1[1 + DIM] = u1 * (s1[1 + DIM] + p1);
+    f2[1 + DIM] = u2 * (s2[1 + DIM] + p2);
+  }
   double SL, SR;
   wave_speeds<DIM>(s1, s2, p1, p2, g, SL, SR);
   double mE;
-  if (g.riemann == RKDG_RS_LLF) {
+  if constexpr (RS == RKDG_RS_LLF) {
     mE = fmax(fabs(SL), fabs(SR));
 #pragma unroll
     for (int i = 0; i < NE; i++) flux[i] = 0.5 * (f1[i] + f2[i]) - 0.5 * mE * (s2[i] - s1[i]);
-  } else if (g.riemann == RKDG_RS_HLL) {
+  } else if constexpr (RS == RKDG_RS_HLL) {
     mE = fmax(fabs(SL), fabs(SR));
     if (SL >= 0) {
 #pragma unroll
@@ -288,6 +351,19 @@ RKDG_HD double riemann(const SchemeParams& g, const double* uL, const double* uR
   }
   inverse_rotate<DIM>(flux, n);
   return mE;
+  }
+}
+
+// runtime-dispatched form (host unit tests and cold paths)
+template <int DIM>
+RKDG_HD double riemann(const SchemeParams& g, const double* uL, const double* uR, const double* n,
+                       double* flux, bool& ok) {
+  switch (g.riemann) {
+    case RKDG_RS_RUSANOV: return riemann_t<DIM, RKDG_RS_RUSANOV>(g, uL, uR, n, flux, ok);
+    case RKDG_RS_LLF: return riemann_t<DIM, RKDG_RS_LLF>(g, uL, uR, n, flux, ok);
+    case RKDG_RS_HLL: return riemann_t<DIM, RKDG_RS_HLL>(g, uL, uR, n, flux, ok);
+    default: return riemann_t<DIM, RKDG_RS_HLLC>(g, uL, uR, n, flux, ok);
+  }
 }
 
 // Ghost state for boundary slot bi (src/bc/boundary_integrator_*.cpp

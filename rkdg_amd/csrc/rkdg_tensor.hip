@@ -7,8 +7,9 @@
 // src/limiter/limiter_multiplier.cpp:12-99, src/limiter/limiter_findiff.cpp:3-33).
 //
 // Work decomposition: one team of T lanes per element (T = next power of two of
-// max(nodes, 2 * face points)), 256-lane workgroups holding EPB = 256/T
-// elements. The element's state is staged in LDS; the volume term is the
+// max(nodes, 2 * face points)); a workgroup is one 64-lane wavefront holding
+// EPB = 64/T elements, so every barrier is a single-wave barrier. The 1D basis
+// tables are staged in LDS, as is the element's state. The volume term is the
 // collocated weak derivative (Gauss-Legendre nodes are the volume quadrature
 // points, so Aflux reduces to a 1D derivative matrix per axis); faces are
 // processed one axis at a time (the -r and +r faces together): normal-direction
@@ -16,7 +17,8 @@
 // points, Riemann solve, lift back. Every face flux is evaluated with the
 // arguments ordered as (Elem1 state, Elem2 state, Elem1 normal), so both
 // elements of a face see bitwise identical fluxes and the scheme is
-// conservative to round-off.
+// conservative to round-off. Kernels are specialised on the Riemann solver and
+// on the presence of boundary faces.
 #include "rkdg_kernels.hpp"
 
 namespace rkdg {
@@ -35,7 +37,7 @@ struct TC {
   static constexpr int NIP = DIM == 3 ? 4 : 2;  // indicator points per face (order 2 rule)
   static constexpr int m1 = ND > 2 * NFP ? ND : 2 * NFP;
   static constexpr int T = m1 <= 4 ? 4 : (m1 <= 8 ? 8 : (m1 <= 16 ? 16 : (m1 <= 32 ? 32 : 64)));
-  static constexpr int BLOCK = 256;
+  static constexpr int BLOCK = 64;
   static constexpr int EPB = BLOCK / T;
 };
 
@@ -107,21 +109,33 @@ __device__ __forceinline__ void flag_error(long long* err, int64_t e1, int64_t e
   }
 }
 
+__device__ __forceinline__ void stage_tables(TensorTables& dst, const TensorTables* src) {
+  constexpr int n = sizeof(TensorTables) / sizeof(double);
+  const double* s = reinterpret_cast<const double*>(src);
+  double* d = reinterpret_cast<double*>(&dst);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) d[i] = s[i];
+}
+
 // ---------------------------------------------------------------------------
-template <int DIM, int P>
-__global__ __launch_bounds__(256) void tensor_rhs_kernel(const RhsArgs A, const SchemeParams g,
-                                                         const TensorTables tb) {
+template <int DIM, int P, int RS, bool BC>
+__global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const SchemeParams g,
+                                                        const TensorTables* __restrict__ gtb) {
   using C = TC<DIM, P>;
   constexpr int N = C::N, ND = C::ND, NEQ = C::NEQ, NQ = C::NQ, NFP = C::NFP, NFN = C::NFN;
-  constexpr int T = C::T, EPB = C::EPB, BLOCK = C::BLOCK;
+  constexpr int NF = C::NF, T = C::T, EPB = C::EPB, BLOCK = C::BLOCK;
   constexpr int GD = DIM * DIM + 1;
-
-  __shared__ double s_u[EPB][NEQ][ND];
-  __shared__ double s_tro[EPB][2][NEQ][NFN];
-  __shared__ double s_trn[EPB][2][NEQ][NFN];
   constexpr int WORK = (NEQ + 1) * ND > 2 * NEQ * NFP ? (NEQ + 1) * ND : 2 * NEQ * NFP;
+  constexpr int NBITEMS = (NF * NFN + T - 1) / T;  // neighbour face nodes per lane
+
+  __shared__ TensorTables tb;
+  __shared__ double s_u[EPB][NEQ][ND];
+  __shared__ double s_trn[EPB][NF][NEQ][NFN];  // neighbour traces, every face
+  __shared__ double s_tro[EPB][2][NEQ][NFN];
   __shared__ double s_work[EPB][WORK];
-  __shared__ double s_red[BLOCK / 64];
+  __shared__ double s_nrm[EPB][NF][DIM + 2];   // unit normal, |n|, flux sign
+  __shared__ double s_geo[EPB][GD];
+  __shared__ int s_nb[EPB][NF];                // by MFEM local face
+  __shared__ int s_code[EPB][NF];
 
   const int tid = threadIdx.x;
   const int team = tid / T, lane = tid % T;
@@ -132,6 +146,9 @@ __global__ __launch_bounds__(256) void tensor_rhs_kernel(const RhsArgs A, const 
   const int64_t e = e0 + team;
   const bool active = e < NE;
 
+  // ---- phase 0: tables, element state, connectivity and geometry into LDS
+  stage_tables(tb, gtb);
+#pragma unroll
   for (int v = 0; v < NEQ; v++) {
     const double* xv = A.x + v * NDOFS + e0 * ND;
     for (int idx = tid; idx < EPB * ND; idx += BLOCK) {
@@ -139,10 +156,81 @@ __global__ __launch_bounds__(256) void tensor_rhs_kernel(const RhsArgs A, const 
       s_u[te][v][tn] = (e0 + te < NE) ? xv[idx] : 1.0;
     }
   }
-  double adj[DIM * DIM], detJ = 1.0;
+  for (int i = lane; active && i < NF; i += T) {
+    const int nb = A.nbr[i * NE + e];
+    s_nb[team][i] = nb;
+    s_code[team][i] = nb >= 0 ? A.fcode[i * NE + e] : 0;
+  }
+  for (int i = lane; active && i < GD; i += T) s_geo[team][i] = A.geo[i * NE + e];
+  __syncthreads();
+
+  // ---- phase 1: neighbour normal-direction traces for all faces (the only
+  // dependent global loads), and Elem1's unit normal per face
+  if (active) {
+    double vals[NBITEMS][NEQ][N];
+    int dst[NBITEMS];
 #pragma unroll
-  for (int i = 0; i < DIM * DIM; i++) adj[i] = active ? A.geo[i * NE + e] : 0.0;
-  if (active) detJ = A.geo[(GD - 1) * NE + e];
+    for (int q = 0; q < NBITEMS; q++) {
+      const int it = lane + q * T;
+      dst[q] = -1;
+      if (it < NF * NFN) {
+        const int f = it / NFN, fn = it - f * NFN;
+        const int lf = lf_of<DIM>(f / 2, f % 2);
+        const int nb = s_nb[team][lf];
+        if (nb >= 0) {
+          int rn, cn;
+          rc_of<DIM>(s_code[team][lf] & 7, rn, cn);
+          const double* xn = A.x + (int64_t)nb * ND;
+#pragma unroll
+          for (int v = 0; v < NEQ; v++)
+#pragma unroll
+            for (int k = 0; k < N; k++) vals[q][v][k] = xn[v * NDOFS + vnode<DIM, N>(rn, k, fn % N, fn / N)];
+          dst[q] = it | (cn << 16);
+        }
+      }
+    }
+    for (int f = lane; f < NF; f += T) {
+      const int r = f / 2, c = f % 2;
+      const int lf = lf_of<DIM>(r, c);
+      const int nb = s_nb[team][lf];
+      const int code = s_code[team][lf];
+      double an[DIM];
+      double sign = -1.0;
+      if (nb >= 0 && !((code >> 6) & 1)) {
+        int rn, cn;
+        rc_of<DIM>(code & 7, rn, cn);
+        const double sn = cn ? 1.0 : -1.0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) an[d] = sn * A.geo[(rn * DIM + d) * NE + nb];
+        sign = 1.0;
+      } else {
+        const double sg = c ? 1.0 : -1.0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) an[d] = sg * s_geo[team][r * DIM + d];
+      }
+      double nm2 = 0.0;
+#pragma unroll
+      for (int d = 0; d < DIM; d++) nm2 += an[d] * an[d];
+      const double inv = 1.0 / norml2_scaled<DIM>(an);
+#pragma unroll
+      for (int d = 0; d < DIM; d++) s_nrm[team][f][d] = an[d] * inv;
+      s_nrm[team][f][DIM] = sqrt(nm2);
+      s_nrm[team][f][DIM + 1] = sign;
+    }
+#pragma unroll
+    for (int q = 0; q < NBITEMS; q++) {
+      if (dst[q] < 0) continue;
+      const int it = dst[q] & 0xffff, cn = dst[q] >> 16;
+      const int f = it / NFN, fn = it - f * NFN;
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; k++) t += vals[q][v][k] * tb.E[cn][k];
+        s_trn[team][f][v][fn] = t;
+      }
+    }
+  }
   __syncthreads();
 
   const int j = lane;
@@ -164,27 +252,37 @@ __global__ __launch_bounds__(256) void tensor_rhs_kernel(const RhsArgs A, const 
 
   // ---- volume term: sum_r D^T (w * adj(J) row r . F(u)) along axis r
   if (!A.face_only) {
-    double F[NEQ * DIM];
-    double wj = 1.0;
+    double G[DIM][NEQ];
     if (node_lane) {
       double st[NEQ];
 #pragma unroll
       for (int v = 0; v < NEQ; v++) st[v] = s_u[team][v][j];
-      flux_tensor<DIM>(st, g, F);
-      lmax = max_char_speed<DIM>(st, g);
-      wj = tb.w[jj[0]] * tb.w[jj[1]];
+      const SidePrim<DIM> q = side_prim<DIM>(st, g);
+      lmax = sqrt(q.m2) * q.inv + side_sound<DIM>(st, q, g);
+      double wj = tb.w[jj[0]] * tb.w[jj[1]];
       if (DIM == 3) wj *= tb.w[jj[2]];
+      // contravariant fluxes G^r = w_j adj(J)(r,:) . F(u): with m.a_r = sum_d a_rd m_d,
+      // G^r = (m.a_r, (m.a_r) u + p a_r, (m.a_r) H)
+#pragma unroll
+      for (int r = 0; r < DIM; r++) {
+        double ar[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) ar[d] = wj * s_geo[team][r * DIM + d];
+        double ma = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) ma += st[1 + d] * ar[d];
+        const double mai = ma * q.inv;
+        G[r][0] = ma;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) G[r][1 + d] = mai * st[1 + d] + q.p * ar[d];
+        G[r][NEQ - 1] = mai * (st[NEQ - 1] + q.p);
+      }
     }
 #pragma unroll
     for (int r = 0; r < DIM; r++) {
       if (node_lane) {
 #pragma unroll
-        for (int v = 0; v < NEQ; v++) {
-          double gv = 0.0;
-#pragma unroll
-          for (int d = 0; d < DIM; d++) gv += adj[r * DIM + d] * F[v * DIM + d];
-          work[v * ND + j] = wj * gv;
-        }
+        for (int v = 0; v < NEQ; v++) work[v * ND + j] = G[r][v];
       }
       __syncthreads();
       if (node_lane) {
@@ -200,7 +298,7 @@ __global__ __launch_bounds__(256) void tensor_rhs_kernel(const RhsArgs A, const 
     }
   }
 
-  // ---- face terms, one axis at a time
+  // ---- face terms, one axis at a time (LDS only)
   bool bad = false;
 #pragma unroll
   for (int r = 0; r < DIM; r++) {
@@ -217,35 +315,15 @@ __global__ __launch_bounds__(256) void tensor_rhs_kernel(const RhsArgs A, const 
         s_tro[team][c][v][fn] = t;
       }
     }
-    // neighbour traces (neighbour's own face frame), read from global memory
-    for (int it = lane; it < 2 * NFN; it += T) {
-      if (!active) break;
-      const int c = it / NFN, fn = it - c * NFN;
-      const int lf = lf_of<DIM>(r, c);
-      const int nb = A.nbr[lf * NE + e];
-      if (nb < 0) continue;
-      const int code = A.fcode[lf * NE + e];
-      int rn, cn;
-      rc_of<DIM>(code & 7, rn, cn);
-      const int a = fn % N, b = fn / N;
-#pragma unroll
-      for (int v = 0; v < NEQ; v++) {
-        const double* xn = A.x + v * NDOFS + (int64_t)nb * ND;
-        double t = 0.0;
-#pragma unroll
-        for (int k = 0; k < N; k++) t += xn[vnode<DIM, N>(rn, k, a, b)] * tb.E[cn][k];
-        s_trn[team][c][v][fn] = t;
-      }
-    }
     __syncthreads();
     // face points: traces, Riemann solve, weighted flux
     for (int it = lane; it < 2 * NFP; it += T) {
       if (!active) break;
       const int c = it / NFP, q = it - c * NFP;
       const int qi = q % NQ, qj = DIM == 3 ? q / NQ : 0;
+      const int f = 2 * r + c;
       const int lf = lf_of<DIM>(r, c);
-      const int nb = A.nbr[lf * NE + e];
-      const int code = nb >= 0 ? A.fcode[lf * NE + e] : 0;
+      const int nb = s_nb[team][lf];
       double um[NEQ], up[NEQ];
 #pragma unroll
       for (int v = 0; v < NEQ; v++) {
@@ -264,11 +342,16 @@ __global__ __launch_bounds__(256) void tensor_rhs_kernel(const RhsArgs A, const 
         }
         um[v] = t;
       }
-      double an[DIM];
-      const double sgn = c ? 1.0 : -1.0;
-      bool elem1 = true;
-      if (nb >= 0) {
-        const int q2 = map_point<DIM, NQ>(code, qi, qj);
+      double nrm[DIM];
+#pragma unroll
+      for (int d = 0; d < DIM; d++) nrm[d] = s_nrm[team][f][d];
+      const double normag = s_nrm[team][f][DIM];
+      double fl[NEQ];
+      double factor = -1.0;
+      bool ok = true;
+      double lam = 0.0;
+      if (!BC || nb >= 0) {
+        const int q2 = map_point<DIM, NQ>(s_code[team][lf], qi, qj);
         const int pi = q2 % NQ, pj = DIM == 3 ? q2 / NQ : 0;
 #pragma unroll
         for (int v = 0; v < NEQ; v++) {
@@ -278,61 +361,36 @@ __global__ __launch_bounds__(256) void tensor_rhs_kernel(const RhsArgs A, const 
             for (int b = 0; b < N; b++) {
               double tt = 0.0;
 #pragma unroll
-              for (int a = 0; a < N; a++) tt += s_trn[team][c][v][a + N * b] * tb.B[pi][a];
+              for (int a = 0; a < N; a++) tt += s_trn[team][f][v][a + N * b] * tb.B[pi][a];
               t += tt * tb.B[pj][b];
             }
           } else {
 #pragma unroll
-            for (int a = 0; a < N; a++) t += s_trn[team][c][v][a] * tb.B[pi][a];
+            for (int a = 0; a < N; a++) t += s_trn[team][f][v][a] * tb.B[pi][a];
           }
           up[v] = t;
         }
-        elem1 = (code >> 6) & 1;
-        if (elem1) {
+        factor = s_nrm[team][f][DIM + 1];
+        const bool elem1 = factor < 0.0;
+        double sL[NEQ], sR[NEQ];
 #pragma unroll
-          for (int d = 0; d < DIM; d++) an[d] = sgn * adj[r * DIM + d];
-        } else {
-          int rn, cn;
-          rc_of<DIM>(code & 7, rn, cn);
-          const double sn = cn ? 1.0 : -1.0;
-#pragma unroll
-          for (int d = 0; d < DIM; d++) an[d] = sn * A.geo[(rn * DIM + d) * NE + nb];
+        for (int v = 0; v < NEQ; v++) {
+          sL[v] = elem1 ? um[v] : up[v];
+          sR[v] = elem1 ? up[v] : um[v];
         }
-      } else {
-#pragma unroll
-        for (int d = 0; d < DIM; d++) an[d] = sgn * adj[r * DIM + d];
-      }
-      double nm2 = 0.0;
-#pragma unroll
-      for (int d = 0; d < DIM; d++) nm2 += an[d] * an[d];
-      const double normag = sqrt(nm2);
-      const double inv = 1.0 / norml2_scaled<DIM>(an);
-      double nrm[DIM];
-#pragma unroll
-      for (int d = 0; d < DIM; d++) nrm[d] = an[d] * inv;
-      double fl[NEQ];
-      double factor = -1.0;
-      bool ok = true;
-      double lam = 0.0;
-      if (nb >= 0) {
-        if (elem1) {
-          lam = riemann<DIM>(g, um, up, nrm, fl, ok);
-        } else {
-          lam = riemann<DIM>(g, up, um, nrm, fl, ok);
-          factor = 1.0;
-        }
-      } else {
+        lam = riemann_t<DIM, RS>(g, sL, sR, nrm, fl, ok);
+      } else if constexpr (BC) {
         const int slot = -1 - nb;
         if (slot < g.num_bc) {
           bc_state<DIM>(g, slot, um, nrm, up);
-          lam = riemann<DIM>(g, um, up, nrm, fl, ok);
+          lam = riemann_t<DIM, RS>(g, um, up, nrm, fl, ok);
         } else {
 #pragma unroll
           for (int v = 0; v < NEQ; v++) fl[v] = 0.0;
         }
       }
       if (!ok || lam < 0.0) {
-        if (!bad) flag_error(A.err, elem1 ? e : nb, elem1 ? (nb >= 0 ? nb : -1) : e, lf);
+        if (!bad) flag_error(A.err, factor < 0.0 ? e : nb, factor < 0.0 ? (nb >= 0 ? nb : -1) : e, lf);
         bad = true;
       } else {
         lmax = lam > lmax ? lam : lmax;
@@ -350,11 +408,17 @@ __global__ __launch_bounds__(256) void tensor_rhs_kernel(const RhsArgs A, const 
 #pragma unroll
       for (int v = 0; v < NEQ; v++) {
         double L = 0.0;
+        if (DIM == 3) {
 #pragma unroll
-        for (int q = 0; q < NFP; q++) {
-          const int qi = q % NQ, qj = DIM == 3 ? q / NQ : 0;
-          const double bb = DIM == 3 ? tb.B[qi][a] * tb.B[qj][b] : tb.B[qi][a];
-          L += work[(c * NEQ + v) * NFP + q] * bb;
+          for (int qj = 0; qj < NQ; qj++) {
+            double tt = 0.0;
+#pragma unroll
+            for (int qi = 0; qi < NQ; qi++) tt += work[(c * NEQ + v) * NFP + qi + NQ * qj] * tb.B[qi][a];
+            L += tt * tb.B[qj][b];
+          }
+        } else {
+#pragma unroll
+          for (int qi = 0; qi < NQ; qi++) L += work[(c * NEQ + v) * NFP + qi] * tb.B[qi][a];
         }
         s_tro[team][c][v][fn] = L;
       }
@@ -365,36 +429,49 @@ __global__ __launch_bounds__(256) void tensor_rhs_kernel(const RhsArgs A, const 
       const int a = r == 0 ? jj[1] : jj[0];
       const int b = DIM == 3 ? (r == 2 ? jj[1] : jj[2]) : 0;
       const int fn = DIM == 3 ? a + N * b : a;
+      const double e0v = tb.E[0][k], e1v = tb.E[1][k];
 #pragma unroll
-      for (int v = 0; v < NEQ; v++)
-        z[v] += s_tro[team][0][v][fn] * tb.E[0][k] + s_tro[team][1][v][fn] * tb.E[1][k];
+      for (int v = 0; v < NEQ; v++) z[v] += s_tro[team][0][v][fn] * e0v + s_tro[team][1][v][fn] * e1v;
     }
-    __syncthreads();
+    if (r + 1 < DIM) __syncthreads();
   }
 
   // ---- inverse mass (affine: diagonal det(J) * w_j) and stage epilogue
   if (!A.face_only) {
     double outv[NEQ];
+    double detJ = 1.0;
     if (node_lane) {
+      detJ = s_geo[team][GD - 1];
       double iw = tb.invw[jj[0]] * tb.invw[jj[1]];
       if (DIM == 3) iw *= tb.invw[jj[2]];
-      const double im = iw / detJ;
+      const double im = iw * recip(detJ);
+      double base[4][NEQ];
+#pragma unroll
+      for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) {
+          const int64_t idx = v * NDOFS + e * ND + j;
+          base[m][v] = (m < A.nout && A.outs[m].base) ? A.outs[m].base[idx] : s_u[team][v][j];
+        }
 #pragma unroll
       for (int v = 0; v < NEQ; v++) {
         const double yv = z[v] * im;
         const int64_t idx = v * NDOFS + e * ND + j;
         if (A.k) A.k[idx] = yv;
         outv[v] = 0.0;
-        for (int m = 0; m < A.nout; m++) {
-          const double base = A.outs[m].base ? A.outs[m].base[idx] : s_u[team][v][j];
-          const double o = base + A.outs[m].coef * yv;
-          A.outs[m].out[idx] = o;
-          if (m == A.means_of) outv[v] = o;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+          if (m < A.nout) {
+            const double o = base[m][v] + A.outs[m].coef * yv;
+            A.outs[m].out[idx] = o;
+            if (m == A.means_of) outv[v] = o;
+          }
         }
       }
     }
     if (A.means) {
       // element means of the selected output: sum_k (w_k detJ) u_k / sum_k w_k detJ
+      __syncthreads();
       if (node_lane) {
         double wj = tb.w[jj[0]] * tb.w[jj[1]];
         if (DIM == 3) wj *= tb.w[jj[2]];
@@ -421,14 +498,18 @@ __global__ __launch_bounds__(256) void tensor_rhs_kernel(const RhsArgs A, const 
     const double o = __shfl_xor(lmax, off, 64);
     lmax = o > lmax ? o : lmax;
   }
-  if ((tid & 63) == 0) atomic_max_nonneg(A.lam, lmax);
+  if (tid == 0) atomic_max_nonneg(A.lam, lmax);
 }
 
 // ---------------------------------------------------------------------------
 template <int DIM, int P>
-__global__ __launch_bounds__(256) void tensor_means_kernel(const MeansArgs A, const TensorTables tb) {
+__global__ __launch_bounds__(256) void tensor_means_kernel(const MeansArgs A,
+                                                           const TensorTables* __restrict__ gtb) {
   using C = TC<DIM, P>;
   constexpr int N = C::N, ND = C::ND, NEQ = C::NEQ;
+  __shared__ double w[kMaxN];
+  if (threadIdx.x < N) w[threadIdx.x] = gtb->w[threadIdx.x];
+  __syncthreads();
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= A.NE) return;
   const int64_t NDOFS = A.NE * ND;
@@ -438,8 +519,8 @@ __global__ __launch_bounds__(256) void tensor_means_kernel(const MeansArgs A, co
 #pragma unroll
   for (int v = 0; v < NEQ; v++) s[v] = 0.0;
   for (int k = 0; k < ND; k++) {
-    double wk = tb.w[k % N] * tb.w[(k / N) % N];
-    if (DIM == 3) wk *= tb.w[k / (N * N)];
+    double wk = w[k % N] * w[(k / N) % N];
+    if (DIM == 3) wk *= w[k / (N * N)];
     const double mk = wk * detJ;
     ms += mk;
 #pragma unroll
@@ -455,14 +536,15 @@ __global__ __launch_bounds__(256) void tensor_means_kernel(const MeansArgs A, co
 // every interior face, the limiter function, then LimiterMultiplier (with the
 // last-hope reset at the reference vertices) or LimiterFinDiff.
 template <int DIM, int P>
-__global__ __launch_bounds__(256) void tensor_limit_kernel(const LimitArgs A, const SchemeParams g,
-                                                           const TensorTables tb) {
+__global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, const SchemeParams g,
+                                                          const TensorTables* __restrict__ gtb) {
   using C = TC<DIM, P>;
   constexpr int N = C::N, ND = C::ND, NEQ = C::NEQ, NFN = C::NFN, NF = C::NF, NIP = C::NIP;
   constexpr int T = C::T, EPB = C::EPB, BLOCK = C::BLOCK;
   constexpr int NPT = NF * NIP;  // indicator points per element
   constexpr int NIT = NPT > C::NVERT ? NPT : C::NVERT;
 
+  __shared__ TensorTables tb;
   __shared__ double s_u[EPB][NEQ][ND];
   __shared__ double s_tr[EPB][NF][NEQ][NFN];
   __shared__ double s_y[EPB][NIT][NEQ];
@@ -472,10 +554,12 @@ __global__ __launch_bounds__(256) void tensor_limit_kernel(const LimitArgs A, co
   const int team = tid / T, lane = tid % T;
   const int64_t NE = A.NE;
   const int64_t NDOFS = NE * ND;
-  const int64_t e0 = (int64_t)blockIdx.x * EPB;
+  const int64_t e0 = xcd_remap(blockIdx.x, gridDim.x) * EPB;
   const int64_t e = e0 + team;
   const bool active = e < NE;
 
+  stage_tables(tb, gtb);
+#pragma unroll
   for (int v = 0; v < NEQ; v++) {
     const double* xv = A.x + v * NDOFS + e0 * ND;
     for (int idx = tid; idx < EPB * ND; idx += BLOCK) {
@@ -493,28 +577,33 @@ __global__ __launch_bounds__(256) void tensor_limit_kernel(const LimitArgs A, co
   }
   __syncthreads();
 
-  // normal-direction traces on all faces of the element (own frame)
-  for (int it = lane; it < NF * NFN; it += T) {
-    if (!active) break;
-    const int f = it / NFN, fn = it - f * NFN;
-    const int r = f / 2, c = f % 2;
-    const int a = fn % N, b = fn / N;
+  const bool ind_bj = g.indicator == RKDG_IND_BJ;
+  const bool hope = g.limiter == RKDG_LIM_MULTI && g.last_hope;
+  // normal-direction traces on all faces of the element (own frame): the
+  // indicator points and the reference vertices lie on them
+  if (ind_bj || hope) {
+    for (int it = lane; it < NF * NFN; it += T) {
+      if (!active) break;
+      const int f = it / NFN, fn = it - f * NFN;
+      const int r = f / 2, c = f % 2;
+      const int a = fn % N, b = fn / N;
 #pragma unroll
-    for (int v = 0; v < NEQ; v++) {
-      double t = 0.0;
+      for (int v = 0; v < NEQ; v++) {
+        double t = 0.0;
 #pragma unroll
-      for (int k = 0; k < N; k++) t += s_u[team][v][vnode<DIM, N>(r, k, a, b)] * tb.E[c][k];
-      s_tr[team][f][v][fn] = t;
+        for (int k = 0; k < N; k++) t += s_u[team][v][vnode<DIM, N>(r, k, a, b)] * tb.E[c][k];
+        s_tr[team][f][v][fn] = t;
+      }
     }
   }
   // stencil bounds (cell + face neighbours), in every lane
-  const bool ind_bj = g.indicator == RKDG_IND_BJ;
   if (active && ind_bj) {
 #pragma unroll
     for (int v = 0; v < NEQ; v++) {
       mI[v] = mean[v] < mI[v] ? mean[v] : mI[v];
       MI[v] = mean[v] > MI[v] ? mean[v] : MI[v];
     }
+#pragma unroll
     for (int lf = 0; lf < NF; lf++) {
       const int nb = A.nbr[lf * NE + e];
       if (nb < 0) continue;
@@ -528,59 +617,65 @@ __global__ __launch_bounds__(256) void tensor_limit_kernel(const LimitArgs A, co
   }
   __syncthreads();
 
-  double hK = 0.0;
-  if (active && g.phi == RKDG_PHI_VENKAT && g.venkat_k > 0.0)
-    hK = pow(A.geo[DIM * DIM * NE + e], 1.0 / DIM);
+  // Venkatakrishnan eps^2 = (K h)^3 with h = |element|^(1/dim): h^3 = det J
+  // (3D) or det J^(3/2) (2D) for affine elements
+  double eps2 = 0.0;
+  if (active && g.phi == RKDG_PHI_VENKAT && g.venkat_k > 0.0) {
+    const double dj = A.geo[DIM * DIM * NE + e];
+    const double k3 = g.venkat_k * g.venkat_k * g.venkat_k;
+    eps2 = k3 * (DIM == 3 ? dj : dj * sqrt(dj));
+  }
 
   // indicator values at the points of interior faces
-  for (int it = lane; it < NPT; it += T) {
-    if (!active) break;
-    const int f = it / NIP, ip = it - f * NIP;
-    const int r = f / 2, c = f % 2;
-    const int lf = lf_of<DIM>(r, c);
-    double yv[NEQ];
+  if (ind_bj) {
+    for (int it = lane; it < NPT; it += T) {
+      if (!active) break;
+      const int f = it / NIP, ip = it - f * NIP;
+      const int r = f / 2, c = f % 2;
+      const int lf = lf_of<DIM>(r, c);
+      double yv[NEQ];
 #pragma unroll
-    for (int v = 0; v < NEQ; v++) yv[v] = 1e9;
-    if (ind_bj && A.nbr[lf * NE + e] >= 0) {
-      const int pi = ip % 2, pj = ip / 2;
+      for (int v = 0; v < NEQ; v++) yv[v] = 1e9;
+      if (A.nbr[lf * NE + e] >= 0) {
+        const int pi = ip % 2, pj = ip / 2;
 #pragma unroll
-      for (int v = 0; v < NEQ; v++) {
-        double u = 0.0;
-        if (DIM == 3) {
+        for (int v = 0; v < NEQ; v++) {
+          double u = 0.0;
+          if (DIM == 3) {
 #pragma unroll
-          for (int b = 0; b < N; b++) {
-            double tt = 0.0;
+            for (int b = 0; b < N; b++) {
+              double tt = 0.0;
 #pragma unroll
-            for (int a = 0; a < N; a++) tt += s_tr[team][f][v][a + N * b] * tb.BI[pi][a];
-            u += tt * tb.BI[pj][b];
-          }
-        } else {
-#pragma unroll
-          for (int a = 0; a < N; a++) u += s_tr[team][f][v][a] * tb.BI[pi][a];
-        }
-        const double diff = u - mean[v];
-        const double y = indicator_y(diff, mean[v], mI[v], MI[v]);
-        double yc;
-        if (g.phi == RKDG_PHI_BJ) {
-          yc = y < 1.0 ? y : 1.0;
-        } else if (g.phi == RKDG_PHI_VENKAT) {
-          if (g.venkat_k > 0.0) {
-            const double eps2 = pow(g.venkat_k * hK, 3.0);
-            const double d2 = diff * diff;
-            yc = (d2 * (y * y + 2.0 * y) + eps2) / (d2 * (y * y + y + 2.0) + eps2);
+              for (int a = 0; a < N; a++) tt += s_tr[team][f][v][a + N * b] * tb.BI[pi][a];
+              u += tt * tb.BI[pj][b];
+            }
           } else {
-            yc = (y * y + 2.0 * y) / (y * y + y + 2.0);
-          }
-        } else {
-          const double yStar = 1.5;
-          const double yRel = y / yStar;
-          yc = y < 1.0 ? y + (3.0 - 2.0 * yStar) * yRel * yRel + (yStar - 2.0) * yRel * yRel * yRel : 1.0;
-        }
-        yv[v] = yc;
-      }
-    }
 #pragma unroll
-    for (int v = 0; v < NEQ; v++) s_y[team][it][v] = yv[v];
+            for (int a = 0; a < N; a++) u += s_tr[team][f][v][a] * tb.BI[pi][a];
+          }
+          const double diff = u - mean[v];
+          const double y = indicator_y(diff, mean[v], mI[v], MI[v]);
+          double yc;
+          if (g.phi == RKDG_PHI_BJ) {
+            yc = y < 1.0 ? y : 1.0;
+          } else if (g.phi == RKDG_PHI_VENKAT) {
+            if (g.venkat_k > 0.0) {
+              const double d2 = diff * diff;
+              yc = (d2 * (y * y + 2.0 * y) + eps2) / (d2 * (y * y + y + 2.0) + eps2);
+            } else {
+              yc = (y * y + 2.0 * y) / (y * y + y + 2.0);
+            }
+          } else {
+            const double yStar = 1.5;
+            const double yRel = y / yStar;
+            yc = y < 1.0 ? y + (3.0 - 2.0 * yStar) * yRel * yRel + (yStar - 2.0) * yRel * yRel * yRel : 1.0;
+          }
+          yv[v] = yc;
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) s_y[team][it][v] = yv[v];
+    }
   }
   __syncthreads();
 
@@ -611,36 +706,40 @@ __global__ __launch_bounds__(256) void tensor_limit_kernel(const LimitArgs A, co
   if (g.limiter == RKDG_LIM_MULTI) {
     const double yr = yMin[0];
     if (yr < 1.0) trouble = 1;
-    const int reps = g.pow_nd ? ND : 1;
+    // u <- mean + y (u - mean) applied nd times (limiter_multiplier.cpp:24-38)
+    // is u <- mean + y^nd (u - mean); y^nd by repeated squaring
+    double f = yr;
+    if (g.pow_nd) {
+      double acc = 1.0, b = yr;
+      for (int e2 = ND; e2 > 0; e2 >>= 1) {
+        if (e2 & 1) acc *= b;
+        b *= b;
+      }
+      f = acc;
+    }
     if (node_lane) {
 #pragma unroll
-      for (int v = 0; v < NEQ; v++) {
-        double uv = u[v];
-        for (int rr = 0; rr < reps; rr++) uv = mean[v] + yr * (uv - mean[v]);
-        u[v] = uv;
-      }
+      for (int v = 0; v < NEQ; v++) u[v] = mean[v] + f * (u[v] - mean[v]);
     }
     if (g.last_hope) {
-      __syncthreads();
-      if (node_lane) {
-#pragma unroll
-        for (int v = 0; v < NEQ; v++) s_u[team][v][j] = u[v];
-      }
-      __syncthreads();
+      // the limited state at the reference vertices: mean + f (u(vertex) - mean),
+      // u(vertex) from the x-face traces (the nodal basis sums to one)
       for (int it = lane; it < C::NVERT; it += T) {
         if (!active) break;
-        // reference vertex (bits of it select 0/1 per axis)
         const int cx = it & 1, cy = (it >> 1) & 1, cz = (it >> 2) & 1;
+        const int fx = cx;  // own face (r = 0, c = cx)
         double vs[NEQ];
 #pragma unroll
         for (int v = 0; v < NEQ; v++) {
           double s = 0.0;
-          for (int k = 0; k < ND; k++) {
-            double sh = tb.E[cx][k % N] * tb.E[cy][(k / N) % N];
-            if (DIM == 3) sh *= tb.E[cz][k / (N * N)];
-            s += s_u[team][v][k] * sh;
+#pragma unroll
+          for (int b = 0; b < (DIM == 3 ? N : 1); b++) {
+            double tt = 0.0;
+#pragma unroll
+            for (int a = 0; a < N; a++) tt += s_tr[team][fx][v][a + N * b] * tb.E[cy][a];
+            s += DIM == 3 ? tt * tb.E[cz][b] : tt;
           }
-          vs[v] = s;
+          vs[v] = mean[v] + f * (s - mean[v]);
         }
         if (!physical<DIM>(vs, g)) atomicOr(&s_flag[team], 1);
       }
@@ -700,22 +799,38 @@ bool tensor_supported(int dim, int p) {
   return false;
 }
 
-hipError_t launch_tensor_rhs(int dim, int p, const RhsArgs& a, const SchemeParams& g,
-                             const TensorTables& t, hipStream_t s) {
-#define X(D, PP)                                                           \
-  if (dim == D && p == PP) {                                               \
-    using C = TC<D, PP>;                                                   \
-    const int64_t nb = (a.NE + C::EPB - 1) / C::EPB;                       \
-    tensor_rhs_kernel<D, PP><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t);   \
-    return hipGetLastError();                                              \
+template <int D, int PP, int RS>
+hipError_t launch_rhs_rs(const RhsArgs& a, const SchemeParams& g, const TensorTables* t, bool bc,
+                         hipStream_t s) {
+  using C = TC<D, PP>;
+  const int64_t nb = (a.NE + C::EPB - 1) / C::EPB;
+  if (bc) tensor_rhs_kernel<D, PP, RS, true><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t);
+  else tensor_rhs_kernel<D, PP, RS, false><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t);
+  return hipGetLastError();
+}
+
+template <int D, int PP>
+hipError_t launch_rhs_dp(const RhsArgs& a, const SchemeParams& g, const TensorTables* t, bool bc,
+                         hipStream_t s) {
+  switch (g.riemann) {
+    case RKDG_RS_RUSANOV: return launch_rhs_rs<D, PP, RKDG_RS_RUSANOV>(a, g, t, bc, s);
+    case RKDG_RS_LLF: return launch_rhs_rs<D, PP, RKDG_RS_LLF>(a, g, t, bc, s);
+    case RKDG_RS_HLL: return launch_rhs_rs<D, PP, RKDG_RS_HLL>(a, g, t, bc, s);
+    default: return launch_rhs_rs<D, PP, RKDG_RS_HLLC>(a, g, t, bc, s);
   }
+}
+
+hipError_t launch_tensor_rhs(int dim, int p, const RhsArgs& a, const SchemeParams& g,
+                             const TensorTables* t, bool has_bdr, hipStream_t s) {
+#define X(D, PP) \
+  if (dim == D && p == PP) return launch_rhs_dp<D, PP>(a, g, t, has_bdr, s);
   RKDG_TENSOR_CASES(X)
 #undef X
   return hipErrorInvalidValue;
 }
 
 hipError_t launch_tensor_limit(int dim, int p, const LimitArgs& a, const SchemeParams& g,
-                               const TensorTables& t, hipStream_t s) {
+                               const TensorTables* t, hipStream_t s) {
 #define X(D, PP)                                                           \
   if (dim == D && p == PP) {                                               \
     using C = TC<D, PP>;                                                   \
@@ -728,12 +843,12 @@ hipError_t launch_tensor_limit(int dim, int p, const LimitArgs& a, const SchemeP
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_tensor_means(int dim, int p, const MeansArgs& a, const TensorTables& t,
+hipError_t launch_tensor_means(int dim, int p, const MeansArgs& a, const TensorTables* t,
                                hipStream_t s) {
-#define X(D, PP)                                                              \
-  if (dim == D && p == PP) {                                                  \
+#define X(D, PP)                                                                      \
+  if (dim == D && p == PP) {                                                          \
     tensor_means_kernel<D, PP><<<(unsigned)((a.NE + 255) / 256), 256, 0, s>>>(a, t); \
-    return hipGetLastError();                                                 \
+    return hipGetLastError();                                                         \
   }
   RKDG_TENSOR_CASES(X)
 #undef X
