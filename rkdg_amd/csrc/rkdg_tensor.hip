@@ -125,11 +125,10 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
   constexpr int NF = C::NF, T = C::T, EPB = C::EPB, BLOCK = C::BLOCK;
   constexpr int GD = DIM * DIM + 1;
   constexpr int WORK = (NEQ + 1) * ND > 2 * NEQ * NFP ? (NEQ + 1) * ND : 2 * NEQ * NFP;
-  constexpr int NBITEMS = (NF * NFN + T - 1) / T;  // neighbour face nodes per lane
 
   __shared__ TensorTables tb;
   __shared__ double s_u[EPB][NEQ][ND];
-  __shared__ double s_trn[EPB][NF][NEQ][NFN];  // neighbour traces, every face
+  __shared__ double s_trn[EPB][2][NEQ][NFN];   // neighbour traces, faces (r,0), (r,1)
   __shared__ double s_tro[EPB][2][NEQ][NFN];
   __shared__ double s_work[EPB][WORK];
   __shared__ double s_nrm[EPB][NF][DIM + 2];   // unit normal, |n|, flux sign
@@ -164,31 +163,8 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
   for (int i = lane; active && i < GD; i += T) s_geo[team][i] = A.geo[i * NE + e];
   __syncthreads();
 
-  // ---- phase 1: neighbour normal-direction traces for all faces (the only
-  // dependent global loads), and Elem1's unit normal per face
+  // ---- phase 1: Elem1's unit normal per face
   if (active) {
-    double vals[NBITEMS][NEQ][N];
-    int dst[NBITEMS];
-#pragma unroll
-    for (int q = 0; q < NBITEMS; q++) {
-      const int it = lane + q * T;
-      dst[q] = -1;
-      if (it < NF * NFN) {
-        const int f = it / NFN, fn = it - f * NFN;
-        const int lf = lf_of<DIM>(f / 2, f % 2);
-        const int nb = s_nb[team][lf];
-        if (nb >= 0) {
-          int rn, cn;
-          rc_of<DIM>(s_code[team][lf] & 7, rn, cn);
-          const double* xn = A.x + (int64_t)nb * ND;
-#pragma unroll
-          for (int v = 0; v < NEQ; v++)
-#pragma unroll
-            for (int k = 0; k < N; k++) vals[q][v][k] = xn[v * NDOFS + vnode<DIM, N>(rn, k, fn % N, fn / N)];
-          dst[q] = it | (cn << 16);
-        }
-      }
-    }
     for (int f = lane; f < NF; f += T) {
       const int r = f / 2, c = f % 2;
       const int lf = lf_of<DIM>(r, c);
@@ -216,19 +192,6 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       for (int d = 0; d < DIM; d++) s_nrm[team][f][d] = an[d] * inv;
       s_nrm[team][f][DIM] = sqrt(nm2);
       s_nrm[team][f][DIM + 1] = sign;
-    }
-#pragma unroll
-    for (int q = 0; q < NBITEMS; q++) {
-      if (dst[q] < 0) continue;
-      const int it = dst[q] & 0xffff, cn = dst[q] >> 16;
-      const int f = it / NFN, fn = it - f * NFN;
-#pragma unroll
-      for (int v = 0; v < NEQ; v++) {
-        double t = 0.0;
-#pragma unroll
-        for (int k = 0; k < N; k++) t += vals[q][v][k] * tb.E[cn][k];
-        s_trn[team][f][v][fn] = t;
-      }
     }
   }
   __syncthreads();
@@ -302,17 +265,41 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
   bool bad = false;
 #pragma unroll
   for (int r = 0; r < DIM; r++) {
-    // own normal-direction traces on faces (r,0), (r,1)
-    for (int it = lane; it < 2 * NFN; it += T) {
+    // own normal-direction traces on faces (r,0), (r,1); neighbour traces in
+    // the neighbour's own face frame, read from global memory
+    for (int it = lane; it < 4 * NFN; it += T) {
       if (!active) break;
-      const int c = it / NFN, fn = it - c * NFN;
+      const int nbside = it >= 2 * NFN;
+      const int it2 = nbside ? it - 2 * NFN : it;
+      const int c = it2 / NFN, fn = it2 - c * NFN;
       const int a = fn % N, b = fn / N;
+      if (!nbside) {
 #pragma unroll
-      for (int v = 0; v < NEQ; v++) {
-        double t = 0.0;
+        for (int v = 0; v < NEQ; v++) {
+          double t = 0.0;
 #pragma unroll
-        for (int k = 0; k < N; k++) t += s_u[team][v][vnode<DIM, N>(r, k, a, b)] * tb.E[c][k];
-        s_tro[team][c][v][fn] = t;
+          for (int k = 0; k < N; k++) t += s_u[team][v][vnode<DIM, N>(r, k, a, b)] * tb.E[c][k];
+          s_tro[team][c][v][fn] = t;
+        }
+      } else {
+        const int lf = lf_of<DIM>(r, c);
+        const int nb = s_nb[team][lf];
+        if (nb < 0) continue;
+        int rn, cn;
+        rc_of<DIM>(s_code[team][lf] & 7, rn, cn);
+        const double* xn = A.x + (int64_t)nb * ND;
+        double vals[NEQ][N];
+#pragma unroll
+        for (int v = 0; v < NEQ; v++)
+#pragma unroll
+          for (int k = 0; k < N; k++) vals[v][k] = xn[v * NDOFS + vnode<DIM, N>(rn, k, a, b)];
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) {
+          double t = 0.0;
+#pragma unroll
+          for (int k = 0; k < N; k++) t += vals[v][k] * tb.E[cn][k];
+          s_trn[team][c][v][fn] = t;
+        }
       }
     }
     __syncthreads();
@@ -361,12 +348,12 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
             for (int b = 0; b < N; b++) {
               double tt = 0.0;
 #pragma unroll
-              for (int a = 0; a < N; a++) tt += s_trn[team][f][v][a + N * b] * tb.B[pi][a];
+              for (int a = 0; a < N; a++) tt += s_trn[team][c][v][a + N * b] * tb.B[pi][a];
               t += tt * tb.B[pj][b];
             }
           } else {
 #pragma unroll
-            for (int a = 0; a < N; a++) t += s_trn[team][f][v][a] * tb.B[pi][a];
+            for (int a = 0; a < N; a++) t += s_trn[team][c][v][a] * tb.B[pi][a];
           }
           up[v] = t;
         }
