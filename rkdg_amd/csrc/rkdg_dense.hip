@@ -1,0 +1,609 @@
+// rkdg_dense.hip — gfx950 kernels for small 2D elements whose reference
+// operators are kept as dense matrices: affine triangles P_p (the
+// L2_TriangleElement nodal basis) and non-affine (bilinear) quadrilaterals Q_p.
+// Same path as rkdg_tensor.hip: FE_Evolution::Mult
+// (src/fe_evolution/fe_evolution.cpp:102-216) fused with the RK-stage epilogue,
+// element means (src/averager/averager.cpp:84-109) and the limiter
+// (src/limiter/limiter.cpp:22-94, src/indicator/indicator_bj.cpp:15-200,
+// src/limiter/limiter_multiplier.cpp:12-99, src/limiter/limiter_findiff.cpp:3-33).
+//
+// Volume term: z_j += sum_r sum_k K_r(j,k) G^r_k with G^r_k = adj(J)(r,:) . F(u_k)
+// (nodal flux, GetFlux fe_evolution.cpp:219-245) and K_r(j,k) the exact integral
+// of phi_k d_r phi_j (triangles; DomainIntegrator with intorder 2p is exact) or
+// the collocated w_k d_r phi_j(x_k) with adj(J) taken at node k (Q_p on
+// Gauss-Legendre nodes). Faces: per-face trace tables Phi_f[q][k]; the
+// neighbour's trace is evaluated at the mirrored point of its own face. Mass:
+// M^-1 = Mhat^-1 / det J (affine triangles) or a per-element dense inverse
+// (bilinear quads).
+#include "rkdg_kernels.hpp"
+
+namespace rkdg {
+
+template <int GEOM, int P>
+struct DC {
+  static constexpr bool TRI = GEOM == RKDG_GEOM_TRI;
+  static constexpr int DIM = 2;
+  static constexpr int NEQ = 4;
+  static constexpr int ND = TRI ? (P + 1) * (P + 2) / 2 : (P + 1) * (P + 1);
+  static constexpr int NF = TRI ? 3 : 4;
+  static constexpr int NV = TRI ? 3 : 4;
+  // face rule order OrderW + 2p (+1 for P_k): triangle 2p+1, bilinear quad 2p+1
+  static constexpr int NQ = TRI ? P + 1 : P + 1;
+  static constexpr int NPT = NF * NQ;
+  static constexpr int m1 = ND > NPT ? ND : NPT;
+  static constexpr int T = m1 <= 4 ? 4 : (m1 <= 8 ? 8 : (m1 <= 16 ? 16 : 32));
+  static constexpr int BLOCK = 64;
+  static constexpr int EPB = BLOCK / T;
+  // geometry record per element: affine triangle adj(J)[4], det J, area normals[2*NF];
+  // bilinear quad: adj(J) at nodes [4*ND], Minv[ND*ND], node mass weights[ND], area normals[2*NF]
+  static constexpr int G_ADJ = 0;
+  static constexpr int G_DET = TRI ? 4 : -1;
+  static constexpr int G_NRM = TRI ? 5 : 4 * ND + ND * ND + ND;
+  static constexpr int G_MINV = TRI ? -1 : 4 * ND;
+  static constexpr int G_MW = TRI ? -1 : 4 * ND + ND * ND;
+  static constexpr int GSIZE = G_NRM + 2 * NF;
+};
+
+template <int GEOM, int P>
+struct DenseTab {
+  using C = DC<GEOM, P>;
+  double K[2][C::ND][C::ND];        // volume: K[r][j][k]
+  double Phi[C::NF][C::NQ][C::ND];  // face traces
+  double wq[C::NQ];
+  double Minv[C::ND][C::ND];        // reference inverse mass (triangles)
+  double mhat[C::ND];               // integral of the reference basis (triangles)
+  double PhiI[C::NF][2][C::ND];     // indicator points (IntRules order 2 on edges)
+  double PhiV[C::NV][C::ND];        // reference vertices (last hope)
+};
+
+template <int GEOM, int P>
+__device__ __forceinline__ void stage_dense(DenseTab<GEOM, P>& d, const double* s) {
+  constexpr int n = sizeof(DenseTab<GEOM, P>) / sizeof(double);
+  double* o = reinterpret_cast<double*>(&d);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) o[i] = s[i];
+}
+
+__device__ __forceinline__ int64_t xcd_remap_d(int64_t b, int64_t nb) {
+  const int64_t q = nb / 8, r = nb % 8;
+  const int64_t x = b % 8, l = b / 8;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + l;
+}
+
+template <int GEOM, int P, int RS, bool BC>
+__global__ __launch_bounds__(64) void dense_rhs_kernel(const RhsArgs A, const SchemeParams g,
+                                                       const double* __restrict__ gtab) {
+  using C = DC<GEOM, P>;
+  constexpr int ND = C::ND, NEQ = C::NEQ, NF = C::NF, NQ = C::NQ, NPT = C::NPT;
+  constexpr int T = C::T, EPB = C::EPB, BLOCK = C::BLOCK, GS = C::GSIZE;
+  constexpr bool TRI = C::TRI;
+
+  __shared__ DenseTab<GEOM, P> tb;
+  __shared__ double s_u[EPB][NEQ][ND];
+  __shared__ double s_g[EPB][2][NEQ][ND];     // contravariant nodal fluxes, then z
+  __shared__ double s_f[EPB][NPT][NEQ];       // weighted face fluxes
+  __shared__ double s_geo[EPB][GS];
+  __shared__ int s_nb[EPB][NF], s_code[EPB][NF];
+
+  const int tid = threadIdx.x;
+  const int team = tid / T, lane = tid % T;
+  const int64_t NE = A.NE;
+  const int64_t NDOFS = NE * ND;
+  const int64_t e0 = xcd_remap_d(blockIdx.x, gridDim.x) * EPB;
+  const int64_t e = e0 + team;
+  const bool active = e < NE;
+
+  stage_dense<GEOM, P>(tb, gtab);
+#pragma unroll
+  for (int v = 0; v < NEQ; v++) {
+    const double* xv = A.x + v * NDOFS + e0 * ND;
+    for (int idx = tid; idx < EPB * ND; idx += BLOCK) {
+      const int te = idx / ND, tn = idx - te * ND;
+      s_u[te][v][tn] = (e0 + te < NE) ? xv[idx] : 1.0;
+    }
+  }
+  for (int i = lane; active && i < NF; i += T) {
+    const int nb = A.nbr[i * NE + e];
+    s_nb[team][i] = nb;
+    s_code[team][i] = nb >= 0 ? A.fcode[i * NE + e] : 0;
+  }
+  for (int i = lane; active && i < GS; i += T) s_geo[team][i] = A.geo[(int64_t)i * NE + e];
+  __syncthreads();
+
+  const int j = lane;
+  const bool node_lane = active && lane < ND;
+  double lmax = 0.0;
+  bool bad = false;
+
+  // ---- volume: nodal flux, contravariant components
+  if (!A.face_only && node_lane) {
+    double st[NEQ];
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) st[v] = s_u[team][v][j];
+    const SidePrim<2> q = side_prim<2>(st, g);
+    lmax = sqrt(q.m2) * q.inv + side_sound<2>(st, q, g);
+    const double* adj = TRI ? &s_geo[team][C::G_ADJ] : &s_geo[team][C::G_ADJ + 4 * j];
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+      const double a0 = adj[r * 2 + 0], a1 = adj[r * 2 + 1];
+      const double ma = st[1] * a0 + st[2] * a1;
+      const double mai = ma * q.inv;
+      s_g[team][r][0][j] = ma;
+      s_g[team][r][1][j] = mai * st[1] + q.p * a0;
+      s_g[team][r][2][j] = mai * st[2] + q.p * a1;
+      s_g[team][r][3][j] = mai * (st[3] + q.p);
+    }
+  }
+  // ---- face points: traces, Riemann solve, weighted flux
+  for (int it = lane; it < NPT; it += T) {
+    if (!active) break;
+    const int f = it / NQ, q = it - f * NQ;
+    const int nb = s_nb[team][f];
+    double um[NEQ], up[NEQ], fl[NEQ];
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < ND; k++) t += s_u[team][v][k] * tb.Phi[f][q][k];
+      um[v] = t;
+    }
+    double an[2];
+    double factor = -1.0;
+    bool ok = true;
+    double lam = 0.0;
+    const int code = s_code[team][f];
+    const bool elem1 = nb < 0 || ((code >> 6) & 1);
+    if (elem1) {
+      an[0] = s_geo[team][C::G_NRM + 2 * f];
+      an[1] = s_geo[team][C::G_NRM + 2 * f + 1];
+    } else {
+      const int fn = code & 7;
+      an[0] = A.geo[(int64_t)(C::G_NRM + 2 * fn) * NE + nb];
+      an[1] = A.geo[(int64_t)(C::G_NRM + 2 * fn + 1) * NE + nb];
+    }
+    const double normag = sqrt(an[0] * an[0] + an[1] * an[1]);
+    const double inv = 1.0 / norml2_scaled<2>(an);
+    const double nrm[2] = {an[0] * inv, an[1] * inv};
+    if (!BC || nb >= 0) {
+      const int fn = code & 7;
+      const int q2 = ((code >> 3) & 2) ? NQ - 1 - q : q;
+      const double* xn = A.x + (int64_t)nb * ND;
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < ND; k++) t += xn[v * NDOFS + k] * tb.Phi[fn][q2][k];
+        up[v] = t;
+      }
+      double sL[NEQ], sR[NEQ];
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) {
+        sL[v] = elem1 ? um[v] : up[v];
+        sR[v] = elem1 ? up[v] : um[v];
+      }
+      if (!elem1) factor = 1.0;
+      lam = riemann_t<2, RS>(g, sL, sR, nrm, fl, ok);
+    } else if constexpr (BC) {
+      const int slot = -1 - nb;
+      if (slot < g.num_bc) {
+        bc_state<2>(g, slot, um, nrm, up);
+        lam = riemann_t<2, RS>(g, um, up, nrm, fl, ok);
+      } else {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) fl[v] = 0.0;
+      }
+    }
+    if (!ok || lam < 0.0) {
+      if (!bad) {
+        const int64_t other = nb >= 0 ? (int64_t)nb : -1;
+        const bool mine = factor < 0.0;
+        if (atomicCAS(reinterpret_cast<unsigned long long*>(A.err), 0ull, 1ull) == 0ull) {
+          A.err[1] = mine ? e : other;
+          A.err[2] = mine ? other : e;
+          A.err[3] = f;
+        }
+      }
+      bad = true;
+    } else {
+      lmax = lam > lmax ? lam : lmax;
+    }
+    const double wgt = factor * tb.wq[q] * normag;
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) s_f[team][it][v] = fl[v] * wgt;
+  }
+  __syncthreads();
+
+  // ---- node lanes: volume contraction + lift
+  double z[NEQ];
+#pragma unroll
+  for (int v = 0; v < NEQ; v++) z[v] = 0.0;
+  if (node_lane) {
+    if (!A.face_only) {
+#pragma unroll
+      for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int k = 0; k < ND; k++) {
+          const double kk = tb.K[r][j][k];
+#pragma unroll
+          for (int v = 0; v < NEQ; v++) z[v] += kk * s_g[team][r][v][k];
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        const double ph = tb.Phi[f][q][j];
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) z[v] += s_f[team][f * NQ + q][v] * ph;
+      }
+  }
+  if (A.face_only) {
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(lmax, off, 64);
+      lmax = o > lmax ? o : lmax;
+    }
+    if (tid == 0 && lmax > 0.0)
+      atomicMax(reinterpret_cast<unsigned long long*>(A.lam), (unsigned long long)__double_as_longlong(lmax));
+    return;
+  }
+  __syncthreads();
+  if (node_lane) {
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) s_g[team][0][v][j] = z[v];
+  }
+  __syncthreads();
+  // ---- inverse mass and stage epilogue
+  double outv[NEQ];
+  double mk = 0.0;
+  if (node_lane) {
+    double y[NEQ];
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) y[v] = 0.0;
+    if (TRI) {
+      const double idet = recip(s_geo[team][C::G_DET]);
+#pragma unroll
+      for (int k = 0; k < ND; k++) {
+        const double mi = tb.Minv[j][k];
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) y[v] += mi * s_g[team][0][v][k];
+      }
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) y[v] *= idet;
+      mk = tb.mhat[j] * s_geo[team][C::G_DET];
+    } else {
+#pragma unroll
+      for (int k = 0; k < ND; k++) {
+        const double mi = s_geo[team][C::G_MINV + j * ND + k];
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) y[v] += mi * s_g[team][0][v][k];
+      }
+      mk = s_geo[team][C::G_MW + j];
+    }
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) {
+      const double yv = y[v];
+      const int64_t idx = v * NDOFS + e * ND + j;
+      if (A.k) A.k[idx] = yv;
+      outv[v] = 0.0;
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        if (m < A.nout) {
+          const double base = A.outs[m].base ? A.outs[m].base[idx] : s_u[team][v][j];
+          const double o = base + A.outs[m].coef * yv;
+          A.outs[m].out[idx] = o;
+          if (m == A.means_of) outv[v] = o;
+        }
+      }
+    }
+  }
+  if (A.means) {
+    __syncthreads();
+    if (node_lane) {
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) s_g[team][1][v][j] = outv[v];
+      s_g[team][0][0][j] = mk;
+    }
+    __syncthreads();
+    if (active && lane < NEQ) {
+      double s = 0.0, ms = 0.0;
+      for (int k = 0; k < ND; k++) {
+        const double w = s_g[team][0][0][k];
+        s += w * s_g[team][1][lane][k];
+        ms += w;
+      }
+      A.means[lane * NE + e] = s / ms;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(lmax, off, 64);
+    lmax = o > lmax ? o : lmax;
+  }
+  if (tid == 0 && lmax > 0.0)
+    atomicMax(reinterpret_cast<unsigned long long*>(A.lam), (unsigned long long)__double_as_longlong(lmax));
+}
+
+template <int GEOM, int P>
+__global__ __launch_bounds__(256) void dense_means_kernel(const MeansArgs A, const double* __restrict__ gtab) {
+  using C = DC<GEOM, P>;
+  constexpr int ND = C::ND, NEQ = C::NEQ;
+  const DenseTab<GEOM, P>* tb = reinterpret_cast<const DenseTab<GEOM, P>*>(gtab);
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= A.NE) return;
+  const int64_t NDOFS = A.NE * ND;
+  double s[NEQ], ms = 0.0;
+#pragma unroll
+  for (int v = 0; v < NEQ; v++) s[v] = 0.0;
+  for (int k = 0; k < ND; k++) {
+    const double mk = C::TRI ? tb->mhat[k] * A.geo[(int64_t)C::G_DET * A.NE + e]
+                             : A.geo[(int64_t)(C::G_MW + k) * A.NE + e];
+    ms += mk;
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) s[v] += mk * A.x[v * NDOFS + e * ND + k];
+  }
+#pragma unroll
+  for (int v = 0; v < NEQ; v++) A.means[v * A.NE + e] = s[v] / ms;
+}
+
+template <int GEOM, int P>
+__global__ __launch_bounds__(64) void dense_limit_kernel(const LimitArgs A, const SchemeParams g,
+                                                         const double* __restrict__ gtab) {
+  using C = DC<GEOM, P>;
+  constexpr int ND = C::ND, NEQ = C::NEQ, NF = C::NF, NV = C::NV;
+  constexpr int T = C::T, EPB = C::EPB, BLOCK = C::BLOCK;
+  constexpr int NPT = NF * 2;
+  constexpr int NIT = NPT > NV ? NPT : NV;
+
+  __shared__ DenseTab<GEOM, P> tb;
+  __shared__ double s_u[EPB][NEQ][ND];
+  __shared__ double s_y[EPB][NIT][NEQ];
+  __shared__ int s_flag[EPB];
+
+  const int tid = threadIdx.x;
+  const int team = tid / T, lane = tid % T;
+  const int64_t NE = A.NE;
+  const int64_t NDOFS = NE * ND;
+  const int64_t e0 = xcd_remap_d(blockIdx.x, gridDim.x) * EPB;
+  const int64_t e = e0 + team;
+  const bool active = e < NE;
+
+  stage_dense<GEOM, P>(tb, gtab);
+#pragma unroll
+  for (int v = 0; v < NEQ; v++) {
+    const double* xv = A.x + v * NDOFS + e0 * ND;
+    for (int idx = tid; idx < EPB * ND; idx += BLOCK) {
+      const int te = idx / ND, tn = idx - te * ND;
+      s_u[te][v][tn] = (e0 + te < NE) ? xv[idx] : 1.0;
+    }
+  }
+  if (lane == 0) s_flag[team] = 0;
+  double mean[NEQ], mI[NEQ], MI[NEQ];
+#pragma unroll
+  for (int v = 0; v < NEQ; v++) {
+    mean[v] = active ? A.means[v * NE + e] : 0.0;
+    mI[v] = 1e9;
+    MI[v] = -1e9;
+  }
+  const bool ind_bj = g.indicator == RKDG_IND_BJ;
+  if (active && ind_bj) {
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) {
+      mI[v] = mean[v] < mI[v] ? mean[v] : mI[v];
+      MI[v] = mean[v] > MI[v] ? mean[v] : MI[v];
+    }
+    for (int f = 0; f < NF; f++) {
+      const int nb = A.nbr[f * NE + e];
+      if (nb < 0) continue;
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) {
+        const double a = A.means[v * NE + nb];
+        mI[v] = a < mI[v] ? a : mI[v];
+        MI[v] = a > MI[v] ? a : MI[v];
+      }
+    }
+  }
+  __syncthreads();
+  double eps2 = 0.0;
+  if (active && g.phi == RKDG_PHI_VENKAT && g.venkat_k > 0.0) {
+    // (K h)^3 with h = |element|^(1/2)
+    double area = 0.0;
+    if (C::TRI) area = 0.5 * A.geo[(int64_t)C::G_DET * NE + e];
+    else
+      for (int k = 0; k < ND; k++) area += A.geo[(int64_t)(C::G_MW + k) * NE + e];
+    const double h = sqrt(area);
+    const double kh = g.venkat_k * h;
+    eps2 = kh * kh * kh;
+  }
+  if (ind_bj) {
+    for (int it = lane; it < NPT; it += T) {
+      if (!active) break;
+      const int f = it / 2, ip = it % 2;
+      double yv[NEQ];
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) yv[v] = 1e9;
+      if (A.nbr[f * NE + e] >= 0) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) {
+          double u = 0.0;
+#pragma unroll
+          for (int k = 0; k < ND; k++) u += s_u[team][v][k] * tb.PhiI[f][ip][k];
+          const double diff = u - mean[v];
+          const double y = indicator_y(diff, mean[v], mI[v], MI[v]);
+          double yc;
+          if (g.phi == RKDG_PHI_BJ) {
+            yc = y < 1.0 ? y : 1.0;
+          } else if (g.phi == RKDG_PHI_VENKAT) {
+            if (g.venkat_k > 0.0) {
+              const double d2 = diff * diff;
+              yc = (d2 * (y * y + 2.0 * y) + eps2) / (d2 * (y * y + y + 2.0) + eps2);
+            } else {
+              yc = (y * y + 2.0 * y) / (y * y + y + 2.0);
+            }
+          } else {
+            const double yStar = 1.5;
+            const double yRel = y / yStar;
+            yc = y < 1.0 ? y + (3.0 - 2.0 * yStar) * yRel * yRel + (yStar - 2.0) * yRel * yRel * yRel : 1.0;
+          }
+          yv[v] = yc;
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) s_y[team][it][v] = yv[v];
+    }
+  }
+  __syncthreads();
+  double yMin[NEQ];
+#pragma unroll
+  for (int v = 0; v < NEQ; v++) {
+    double m = 1e9;
+    if (g.indicator == RKDG_IND_NOWHERE) m = 1.0;
+    else if (g.indicator == RKDG_IND_EVERYWHERE) m = 0.0;
+    else
+      for (int it = 0; it < NPT; it++) {
+        const double y = s_y[team][it][v];
+        m = y < m ? y : m;
+      }
+    yMin[v] = m;
+  }
+  const int j = lane;
+  const bool node_lane = active && lane < ND;
+  double u[NEQ];
+  if (node_lane) {
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) u[v] = s_u[team][v][j];
+  }
+  int trouble = 0;
+  if (g.limiter == RKDG_LIM_MULTI) {
+    const double yr = yMin[0];
+    if (yr < 1.0) trouble = 1;
+    double f = yr;
+    if (g.pow_nd) {
+      double acc = 1.0, b = yr;
+      for (int e2 = ND; e2 > 0; e2 >>= 1) {
+        if (e2 & 1) acc *= b;
+        b *= b;
+      }
+      f = acc;
+    }
+    if (node_lane) {
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) u[v] = mean[v] + f * (u[v] - mean[v]);
+    }
+    if (g.last_hope) {
+      for (int it = lane; it < NV; it += T) {
+        if (!active) break;
+        double vs[NEQ];
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) {
+          double s = 0.0;
+#pragma unroll
+          for (int k = 0; k < ND; k++) s += s_u[team][v][k] * tb.PhiV[it][k];
+          vs[v] = mean[v] + f * (s - mean[v]);
+        }
+        if (!physical<2>(vs, g)) atomicOr(&s_flag[team], 1);
+      }
+      __syncthreads();
+      if (active && s_flag[team]) {
+        trouble |= 2;
+        if (node_lane) {
+#pragma unroll
+          for (int v = 0; v < NEQ; v++) u[v] = mean[v];
+        }
+      }
+    }
+  } else if (g.limiter == RKDG_LIM_FINDIFF) {
+    if (yMin[NEQ - 1] < 0.999) {
+      trouble = 1;
+      if (node_lane) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) u[v] = mean[v];
+      }
+    }
+  }
+  if (node_lane && g.limiter != RKDG_LIM_NONE) {
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) A.x[v * NDOFS + e * ND + j] = u[v];
+  }
+  if (A.troubled && active && lane == 0) A.troubled[e] = (int8_t)trouble;
+}
+
+// ---------------------------------------------------------------------------
+#define RKDG_DENSE_CASES(X) X(RKDG_GEOM_TRI, 1) X(RKDG_GEOM_TRI, 2) X(RKDG_GEOM_TRI, 3) \
+  X(RKDG_GEOM_QUAD, 1) X(RKDG_GEOM_QUAD, 2)
+
+bool dense_supported(int geom, int p) {
+#define X(G, PP) \
+  if (geom == G && p == PP) return true;
+  RKDG_DENSE_CASES(X)
+#undef X
+  return false;
+}
+
+int dense_table_size(int geom, int p) {
+#define X(G, PP) \
+  if (geom == G && p == PP) return (int)(sizeof(DenseTab<G, PP>) / sizeof(double));
+  RKDG_DENSE_CASES(X)
+#undef X
+  return 0;
+}
+
+int dense_geo_size(int geom, int p) {
+#define X(G, PP) \
+  if (geom == G && p == PP) return DC<G, PP>::GSIZE;
+  RKDG_DENSE_CASES(X)
+#undef X
+  return 0;
+}
+
+template <int G, int PP, int RS>
+hipError_t dense_rhs_rs(const RhsArgs& a, const SchemeParams& g, const double* t, bool bc, hipStream_t s) {
+  using C = DC<G, PP>;
+  const int64_t nb = (a.NE + C::EPB - 1) / C::EPB;
+  if (bc) dense_rhs_kernel<G, PP, RS, true><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t);
+  else dense_rhs_kernel<G, PP, RS, false><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t);
+  return hipGetLastError();
+}
+
+template <int G, int PP>
+hipError_t dense_rhs_gp(const RhsArgs& a, const SchemeParams& g, const double* t, bool bc, hipStream_t s) {
+  switch (g.riemann) {
+    case RKDG_RS_RUSANOV: return dense_rhs_rs<G, PP, RKDG_RS_RUSANOV>(a, g, t, bc, s);
+    case RKDG_RS_LLF: return dense_rhs_rs<G, PP, RKDG_RS_LLF>(a, g, t, bc, s);
+    case RKDG_RS_HLL: return dense_rhs_rs<G, PP, RKDG_RS_HLL>(a, g, t, bc, s);
+    default: return dense_rhs_rs<G, PP, RKDG_RS_HLLC>(a, g, t, bc, s);
+  }
+}
+
+hipError_t launch_dense_rhs(int geom, int p, const RhsArgs& a, const SchemeParams& g, const double* t,
+                            bool has_bdr, hipStream_t s) {
+#define X(G, PP) \
+  if (geom == G && p == PP) return dense_rhs_gp<G, PP>(a, g, t, has_bdr, s);
+  RKDG_DENSE_CASES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_dense_limit(int geom, int p, const LimitArgs& a, const SchemeParams& g, const double* t,
+                              hipStream_t s) {
+#define X(G, PP)                                                                \
+  if (geom == G && p == PP) {                                                   \
+    using C = DC<G, PP>;                                                        \
+    const int64_t nb = (a.NE + C::EPB - 1) / C::EPB;                            \
+    dense_limit_kernel<G, PP><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t);       \
+    return hipGetLastError();                                                   \
+  }
+  RKDG_DENSE_CASES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_dense_means(int geom, int p, const MeansArgs& a, const double* t, hipStream_t s) {
+#define X(G, PP)                                                                         \
+  if (geom == G && p == PP) {                                                            \
+    dense_means_kernel<G, PP><<<(unsigned)((a.NE + 255) / 256), 256, 0, s>>>(a, t);    \
+    return hipGetLastError();                                                            \
+  }
+  RKDG_DENSE_CASES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+}  // namespace rkdg

@@ -150,6 +150,9 @@ struct rkdg_ctx {
   long long* d_err = nullptr;
   double* d_means = nullptr;
   TensorTables* d_tb = nullptr;
+  int dense = 0;                 // 1: dense small-element kernels (triangles, bilinear quads)
+  double* d_dtab = nullptr;      // dense reference tables
+  std::vector<double> ref_nodes; // reference node coordinates [nd][dim] (dense path)
   double* d_stage[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<double> coords;  // [NE][nv][dim]
   std::vector<double> geo_h;   // [g][NE]
@@ -172,6 +175,7 @@ void free_ctx(rkdg_ctx* c) {
   (void)hipFree(c->d_err);
   (void)hipFree(c->d_means);
   (void)hipFree(c->d_tb);
+  (void)hipFree(c->d_dtab);
   for (auto& p : c->d_stage) (void)hipFree(p);
   for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
   delete c;
@@ -208,13 +212,384 @@ double min_singular(int d, const double* J) {
   return std::sqrt(std::max(0.0, lmin));
 }
 
+// ---------------------------------------------------------------------------
+// Dense small-element path: affine triangles P_p and bilinear quadrilaterals Q_p.
+
+bool invert_small(int n, const double* A, double* Ai) {
+  std::vector<double> M(A, A + n * n), I(n * n, 0.0);
+  for (int i = 0; i < n; i++) I[i * n + i] = 1.0;
+  for (int c = 0; c < n; c++) {
+    int piv = c;
+    for (int r = c + 1; r < n; r++)
+      if (std::fabs(M[r * n + c]) > std::fabs(M[piv * n + c])) piv = r;
+    if (M[piv * n + c] == 0.0) return false;
+    for (int k = 0; k < n; k++) {
+      std::swap(M[c * n + k], M[piv * n + k]);
+      std::swap(I[c * n + k], I[piv * n + k]);
+    }
+    const double inv = 1.0 / M[c * n + c];
+    for (int k = 0; k < n; k++) {
+      M[c * n + k] *= inv;
+      I[c * n + k] *= inv;
+    }
+    for (int r = 0; r < n; r++) {
+      if (r == c || M[r * n + c] == 0.0) continue;
+      const double f = M[r * n + c];
+      for (int k = 0; k < n; k++) {
+        M[r * n + k] -= f * M[c * n + k];
+        I[r * n + k] -= f * I[c * n + k];
+      }
+    }
+  }
+  std::memcpy(Ai, I.data(), sizeof(double) * n * n);
+  return true;
+}
+
+// nodal basis of a reference element: values and gradients at a point
+struct RefBasis {
+  int tri = 0, p = 1, nd = 4;
+  std::vector<double> x1;                 // 1D GL nodes (quads)
+  std::vector<std::array<int, 2>> mono;   // triangles: monomial exponents
+  std::vector<double> coef;               // triangles: coef[m*nd + k]
+  std::vector<double> nodes;              // [nd][2]
+  void init(int is_tri, int order) {
+    tri = is_tri;
+    p = order;
+    const int N = p + 1;
+    x1.assign(N, 0.0);
+    std::vector<double> w(N);
+    gl_rule(N, x1.data(), w.data());
+    if (!tri) {
+      nd = N * N;
+      nodes.resize(2 * nd);
+      for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) {
+          nodes[2 * (i + N * j)] = x1[i];
+          nodes[2 * (i + N * j) + 1] = x1[j];
+        }
+      return;
+    }
+    // L2_TriangleElement nodes from Gauss-Legendre open points (j outer, i inner)
+    nd = N * (N + 1) / 2;
+    nodes.clear();
+    for (int j = 0; j <= p; j++)
+      for (int i = 0; i + j <= p; i++) {
+        const double wsum = x1[i] + x1[j] + x1[p - i - j];
+        nodes.push_back(x1[i] / wsum);
+        nodes.push_back(x1[j] / wsum);
+      }
+    for (int a = 0; a <= p; a++)
+      for (int b = 0; a + b <= p; b++) mono.push_back({a, b});
+    std::vector<double> V(nd * nd);
+    for (int k = 0; k < nd; k++)
+      for (int m2 = 0; m2 < nd; m2++)
+        V[k * nd + m2] = std::pow(nodes[2 * k], mono[m2][0]) * std::pow(nodes[2 * k + 1], mono[m2][1]);
+    coef.assign(nd * nd, 0.0);
+    invert_small(nd, V.data(), coef.data());
+  }
+  void eval(double x, double y, double* phi, double* dx, double* dy) const {
+    if (tri) {
+      for (int k = 0; k < nd; k++) {
+        double v = 0, gx = 0, gy = 0;
+        for (int m2 = 0; m2 < nd; m2++) {
+          const int a = mono[m2][0], b = mono[m2][1];
+          const double c2 = coef[m2 * nd + k];
+          v += c2 * std::pow(x, a) * std::pow(y, b);
+          if (a > 0) gx += c2 * a * std::pow(x, a - 1) * std::pow(y, b);
+          if (b > 0) gy += c2 * b * std::pow(x, a) * std::pow(y, b - 1);
+        }
+        phi[k] = v;
+        if (dx) dx[k] = gx;
+        if (dy) dy[k] = gy;
+      }
+      return;
+    }
+    const int N = p + 1;
+    for (int j = 0; j < N; j++)
+      for (int i = 0; i < N; i++) {
+        const int k = i + N * j;
+        const double lx = lag(x1.data(), N, i, x), ly = lag(x1.data(), N, j, y);
+        phi[k] = lx * ly;
+        if (dx) dx[k] = dlag(x1.data(), N, i, x) * ly;
+        if (dy) dy[k] = lx * dlag(x1.data(), N, j, y);
+      }
+  }
+};
+
+// exact rules on the reference triangle (collapsed Gauss-Legendre) and square
+void tri_rule(int n, std::vector<double>& pts, std::vector<double>& w) {
+  std::vector<double> x(n), wx(n);
+  gl_rule(n, x.data(), wx.data());
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      pts.push_back(x[i]);
+      pts.push_back((1.0 - x[i]) * x[j]);
+      w.push_back(wx[i] * wx[j] * (1.0 - x[i]));
+    }
+}
+void quad_rule(int n, std::vector<double>& pts, std::vector<double>& w) {
+  std::vector<double> x(n), wx(n);
+  gl_rule(n, x.data(), wx.data());
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) {
+      pts.push_back(x[i]);
+      pts.push_back(x[j]);
+      w.push_back(wx[i] * wx[j]);
+    }
+}
+
+const int kTriEdge[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+const int kQuadEdge[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+const double kTriV[3][2] = {{0, 0}, {1, 0}, {0, 1}};
+const double kQuadV[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+
+// bilinear quad map: Jacobian at (x, y) from vertex coordinates X[4][2]; J row-major
+void bilinear_jac(const double* X, double x, double y, double* J) {
+  const double dNx[4] = {-(1 - y), (1 - y), y, -y};
+  const double dNy[4] = {-(1 - x), -x, x, (1 - x)};
+  for (int i = 0; i < 2; i++) {
+    J[i * 2 + 0] = 0.0;
+    J[i * 2 + 1] = 0.0;
+    for (int v = 0; v < 4; v++) {
+      J[i * 2 + 0] += X[v * 2 + i] * dNx[v];
+      J[i * 2 + 1] += X[v * 2 + i] * dNy[v];
+    }
+  }
+}
+
+int build_dense(rkdg_ctx* c, const rkdg_mesh_desc* m) {
+  const bool tri = m->geom == RKDG_GEOM_TRI;
+  if (!dense_supported(m->geom, c->p)) return fail(RKDG_E_UNSUPPORTED, "polynomial order not instantiated for this element");
+  c->dense = 1;
+  c->dim = 2;
+  c->neq = 4;
+  c->nv = tri ? 3 : 4;
+  c->nvf = 2;
+  c->nf = tri ? 3 : 4;
+  RefBasis rb;
+  rb.init(tri, c->p);
+  const int nd = rb.nd, nf = c->nf, p = c->p;
+  c->nd = nd;
+  c->ndofs = c->NE * nd;
+  c->ref_nodes = rb.nodes;
+  const int64_t NE = c->NE;
+  const int NQ = p + 1;
+
+  // reference tables, laid out exactly as DenseTab<GEOM, P>
+  const int tsize = dense_table_size(m->geom, p);
+  std::vector<double> tab(tsize, 0.0);
+  size_t o = 0;
+  double* K = &tab[o];
+  o += 2 * nd * nd;
+  double* Phi = &tab[o];
+  o += nf * NQ * nd;
+  double* WQ = &tab[o];
+  o += NQ;
+  double* Minv = &tab[o];
+  o += nd * nd;
+  double* mhat = &tab[o];
+  o += nd;
+  double* PhiI = &tab[o];
+  o += nf * 2 * nd;
+  double* PhiV = &tab[o];
+  o += c->nv * nd;
+  if ((int)o != tsize) return fail(RKDG_E_ARG, "dense table layout mismatch");
+  std::vector<double> phi(nd), dx(nd), dy(nd);
+  std::vector<double> gw1(p + 1), gx1(p + 1);
+  gl_rule(p + 1, gx1.data(), gw1.data());
+  if (tri) {
+    std::vector<double> pts, w;
+    tri_rule(p + 2, pts, w);
+    std::vector<double> M(nd * nd, 0.0);
+    for (size_t q = 0; q < w.size(); q++) {
+      rb.eval(pts[2 * q], pts[2 * q + 1], phi.data(), dx.data(), dy.data());
+      for (int j2 = 0; j2 < nd; j2++)
+        for (int k = 0; k < nd; k++) {
+          K[(0 * nd + j2) * nd + k] += w[q] * phi[k] * dx[j2];
+          K[(1 * nd + j2) * nd + k] += w[q] * phi[k] * dy[j2];
+          M[j2 * nd + k] += w[q] * phi[j2] * phi[k];
+        }
+      for (int k = 0; k < nd; k++) mhat[k] += w[q] * phi[k];
+    }
+    invert_small(nd, M.data(), Minv);
+  } else {
+    // collocated weak derivative on Gauss-Legendre nodes: K_r(j,k) = w_k d_r phi_j(x_k)
+    for (int k = 0; k < nd; k++) {
+      const int i = k % (p + 1), jq = k / (p + 1);
+      const double wk = gw1[i] * gw1[jq];
+      rb.eval(rb.nodes[2 * k], rb.nodes[2 * k + 1], phi.data(), dx.data(), dy.data());
+      for (int j2 = 0; j2 < nd; j2++) {
+        K[(0 * nd + j2) * nd + k] = wk * dx[j2];
+        K[(1 * nd + j2) * nd + k] = wk * dy[j2];
+      }
+    }
+  }
+  std::vector<double> sq(NQ), wq(NQ);
+  gl_rule(NQ, sq.data(), wq.data());
+  for (int q = 0; q < NQ; q++) WQ[q] = wq[q];
+  double s2[2], w2[2];
+  gl_rule(2, s2, w2);
+  const double(*RV)[2] = tri ? kTriV : kQuadV;
+  const int(*ED)[2] = tri ? kTriEdge : kQuadEdge;
+  for (int f = 0; f < nf; f++) {
+    const double* a = RV[ED[f][0]];
+    const double* b = RV[ED[f][1]];
+    for (int q = 0; q < NQ; q++) {
+      rb.eval(a[0] + sq[q] * (b[0] - a[0]), a[1] + sq[q] * (b[1] - a[1]), phi.data(), nullptr, nullptr);
+      for (int k = 0; k < nd; k++) Phi[(f * NQ + q) * nd + k] = phi[k];
+    }
+    for (int q = 0; q < 2; q++) {
+      rb.eval(a[0] + s2[q] * (b[0] - a[0]), a[1] + s2[q] * (b[1] - a[1]), phi.data(), nullptr, nullptr);
+      for (int k = 0; k < nd; k++) PhiI[(f * 2 + q) * nd + k] = phi[k];
+    }
+  }
+  for (int v = 0; v < c->nv; v++) {
+    rb.eval(RV[v][0], RV[v][1], phi.data(), nullptr, nullptr);
+    for (int k = 0; k < nd; k++) PhiV[v * nd + k] = phi[k];
+  }
+
+  // per-element geometry
+  const int GS = dense_geo_size(m->geom, p);
+  const int G_NRM = tri ? 5 : 4 * nd + nd * nd + nd;
+  c->geo_h.assign((size_t)GS * NE, 0.0);
+  std::vector<double> Mq, wqq;
+  std::vector<double> mpts;
+  if (!tri) quad_rule(p + 2, mpts, Mq);
+  double hmin = 1e300;
+  for (int64_t e = 0; e < NE; e++) {
+    const double* X = &c->coords[e * c->nv * 2];
+    auto G = [&](int gi) -> double& { return c->geo_h[(size_t)gi * NE + e]; };
+    if (tri) {
+      const double J[4] = {X[2] - X[0], X[4] - X[0], X[3] - X[1], X[5] - X[1]};
+      const double det = J[0] * J[3] - J[1] * J[2];
+      if (!(det > 0.0)) return fail(RKDG_E_ARG, "element " + std::to_string(e) + " has non-positive Jacobian");
+      G(0) = J[3];
+      G(1) = -J[1];
+      G(2) = -J[2];
+      G(3) = J[0];
+      G(4) = det;
+      // GetElementSize(i,1) on the equilateral (perfect) triangle
+      const double s3 = std::sqrt(3.0);
+      const double Pm[4] = {1.0, -1.0 / s3, 0.0, 2.0 / s3};
+      double Q[4] = {J[0] * Pm[0] + J[1] * Pm[2], J[0] * Pm[1] + J[1] * Pm[3],
+                     J[2] * Pm[0] + J[3] * Pm[2], J[2] * Pm[1] + J[3] * Pm[3]};
+      hmin = std::min(hmin, min_singular(2, Q));
+    } else {
+      std::vector<double> Mm(nd * nd, 0.0), mw(nd, 0.0);
+      for (int k = 0; k < nd; k++) {
+        double J[4];
+        bilinear_jac(X, rb.nodes[2 * k], rb.nodes[2 * k + 1], J);
+        G(4 * k + 0) = J[3];
+        G(4 * k + 1) = -J[1];
+        G(4 * k + 2) = -J[2];
+        G(4 * k + 3) = J[0];
+        if (!(J[0] * J[3] - J[1] * J[2] > 0.0))
+          return fail(RKDG_E_ARG, "element " + std::to_string(e) + " has non-positive Jacobian");
+      }
+      for (size_t q = 0; q < Mq.size(); q++) {
+        double J[4];
+        bilinear_jac(X, mpts[2 * q], mpts[2 * q + 1], J);
+        const double wdet = Mq[q] * (J[0] * J[3] - J[1] * J[2]);
+        rb.eval(mpts[2 * q], mpts[2 * q + 1], phi.data(), nullptr, nullptr);
+        for (int i2 = 0; i2 < nd; i2++) {
+          mw[i2] += wdet * phi[i2];
+          for (int k = 0; k < nd; k++) Mm[i2 * nd + k] += wdet * phi[i2] * phi[k];
+        }
+      }
+      std::vector<double> Mi(nd * nd);
+      invert_small(nd, Mm.data(), Mi.data());
+      for (int i2 = 0; i2 < nd * nd; i2++) G(4 * nd + i2) = Mi[i2];
+      for (int i2 = 0; i2 < nd; i2++) G(4 * nd + nd * nd + i2) = mw[i2];
+      double Jc[4];
+      bilinear_jac(X, 0.5, 0.5, Jc);
+      hmin = std::min(hmin, min_singular(2, Jc));
+    }
+    for (int f = 0; f < nf; f++) {
+      const double* a = &X[ED[f][0] * 2];
+      const double* b = &X[ED[f][1] * 2];
+      const double t0 = b[0] - a[0], t1 = b[1] - a[1];
+      G(G_NRM + 2 * f) = t1;
+      G(G_NRM + 2 * f + 1) = -t0;
+    }
+  }
+  c->hmin = hmin;
+  c->affine = tri ? 1 : 0;
+  // 1D Gauss-Legendre nodes for node coordinates of quads
+  gl_rule(p + 1, c->tb.x, c->tb.w);
+
+  // edges: pair by sorted vertex ids; flip when the neighbour's edge runs the other way
+  std::vector<FaceKey> keys((size_t)NE * nf);
+  for (int64_t e = 0; e < NE; e++)
+    for (int lf = 0; lf < nf; lf++) {
+      FaceKey& k = keys[e * nf + lf];
+      k.k = {m->elem_verts[e * c->nv + ED[lf][0]], m->elem_verts[e * c->nv + ED[lf][1]], -1, -1};
+      if (k.k[0] > k.k[1]) std::swap(k.k[0], k.k[1]);
+      k.e = e;
+      k.lf = lf;
+    }
+  std::sort(keys.begin(), keys.end());
+  std::vector<int> nbr((size_t)nf * NE, -1 - RKDG_MAX_BC), fcode((size_t)nf * NE, 0);
+  std::vector<std::pair<std::array<int64_t, 4>, std::pair<int64_t, int>>> open;
+  for (size_t i = 0; i < keys.size();) {
+    size_t j2 = i + 1;
+    while (j2 < keys.size() && keys[j2].k == keys[i].k) j2++;
+    if (j2 - i > 2) return fail(RKDG_E_ARG, "edge shared by more than two elements");
+    if (j2 - i == 2) {
+      const FaceKey& a = keys[i];
+      const FaceKey& b = keys[i + 1];
+      if (a.e == b.e) return fail(RKDG_E_ARG, "element adjacent to itself (periodic direction too short)");
+      const bool same = m->elem_verts[a.e * c->nv + ED[a.lf][0]] == m->elem_verts[b.e * c->nv + ED[b.lf][0]];
+      const int o2 = same ? 0 : 2;
+      nbr[a.lf * NE + a.e] = (int)b.e;
+      nbr[b.lf * NE + b.e] = (int)a.e;
+      fcode[a.lf * NE + a.e] = b.lf | (o2 << 3) | (1 << 6);
+      fcode[b.lf * NE + b.e] = a.lf | (o2 << 3);
+      c->n_int++;
+    } else {
+      open.push_back({keys[i].k, {keys[i].e, keys[i].lf}});
+    }
+    i = j2;
+  }
+  std::vector<std::pair<std::array<int64_t, 4>, int>> bk(m->num_bdr);
+  for (int64_t b = 0; b < m->num_bdr; b++) {
+    std::array<int64_t, 4> k = {m->bdr_verts[b * 2], m->bdr_verts[b * 2 + 1], -1, -1};
+    if (k[0] > k[1]) std::swap(k[0], k[1]);
+    bk[b] = {k, m->bdr_attr[b]};
+  }
+  std::sort(bk.begin(), bk.end());
+  for (auto& of : open) {
+    auto it = std::lower_bound(bk.begin(), bk.end(), std::make_pair(of.first, -2147483647));
+    int slot = RKDG_MAX_BC;
+    if (it != bk.end() && it->first == of.first)
+      for (int q = 0; q < c->sp.num_bc; q++)
+        if (c->sp.bc_attr[q] == it->second) slot = q;
+    nbr[of.second.second * NE + of.second.first] = -1 - slot;
+    c->n_bdr++;
+  }
+  HIP_TRY(hipMalloc(&c->d_geo, sizeof(double) * GS * NE));
+  HIP_TRY(hipMemcpy(c->d_geo, c->geo_h.data(), sizeof(double) * GS * NE, hipMemcpyHostToDevice));
+  HIP_TRY(hipMalloc(&c->d_nbr, sizeof(int) * nf * NE));
+  HIP_TRY(hipMemcpy(c->d_nbr, nbr.data(), sizeof(int) * nf * NE, hipMemcpyHostToDevice));
+  HIP_TRY(hipMalloc(&c->d_fcode, sizeof(int) * nf * NE));
+  HIP_TRY(hipMemcpy(c->d_fcode, fcode.data(), sizeof(int) * nf * NE, hipMemcpyHostToDevice));
+  HIP_TRY(hipMalloc(&c->d_dtab, sizeof(double) * tsize));
+  HIP_TRY(hipMemcpy(c->d_dtab, tab.data(), sizeof(double) * tsize, hipMemcpyHostToDevice));
+  HIP_TRY(hipMalloc(&c->d_lam, sizeof(double) * 8));
+  HIP_TRY(hipMemset(c->d_lam, 0, sizeof(double) * 8));
+  HIP_TRY(hipMalloc(&c->d_err, sizeof(long long) * 4));
+  HIP_TRY(hipMemset(c->d_err, 0, sizeof(long long) * 4));
+  HIP_TRY(hipMalloc(&c->d_means, sizeof(double) * c->neq * NE));
+  return RKDG_OK;
+}
+
 int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
-  if (m->geom == RKDG_GEOM_TRI) return fail(RKDG_E_UNSUPPORTED, "triangle elements are not built into this library");
-  if (m->geom != RKDG_GEOM_QUAD && m->geom != RKDG_GEOM_HEX) return fail(RKDG_E_ARG, "unknown geometry");
+  if (m->geom != RKDG_GEOM_QUAD && m->geom != RKDG_GEOM_HEX && m->geom != RKDG_GEOM_TRI)
+    return fail(RKDG_E_ARG, "unknown geometry");
   c->geom = m->geom;
   c->p = m->order;
   c->dim = m->geom == RKDG_GEOM_HEX ? 3 : 2;
-  if (!tensor_supported(c->dim, c->p)) return fail(RKDG_E_UNSUPPORTED, "polynomial order not instantiated");
+  if (m->geom != RKDG_GEOM_TRI && !tensor_supported(c->dim, c->p))
+    return fail(RKDG_E_UNSUPPORTED, "polynomial order not instantiated");
   c->nv = c->dim == 3 ? 8 : 4;
   c->nvf = c->dim == 3 ? 4 : 2;
   c->nf = 2 * c->dim;
@@ -228,7 +603,8 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
   if (s->num_bc < 0 || s->num_bc > RKDG_MAX_BC) return fail(RKDG_E_ARG, "num_bc out of range");
   const int dim = c->dim, nv = c->nv, NE_ = (int)c->NE;
   const int64_t NE = c->NE;
-  c->coords.assign(m->elem_coords, m->elem_coords + NE * nv * dim);
+  if (m->geom == RKDG_GEOM_TRI) c->nv = 3;
+  c->coords.assign(m->elem_coords, m->elem_coords + NE * c->nv * dim);
 
   // scheme
   SchemeParams& g = c->sp;
@@ -250,6 +626,18 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
   g.limiter = s->limiter;
   g.last_hope = s->last_hope;
   g.pow_nd = s->pow_nd;
+
+  // triangles and non-parallelogram quadrilaterals take the dense path
+  bool dense = m->geom == RKDG_GEOM_TRI;
+  if (m->geom == RKDG_GEOM_QUAD) {
+    for (int64_t e = 0; e < NE && !dense; e++) {
+      const double* X = &c->coords[e * 8];
+      const double sc = std::fabs(X[2] - X[0]) + std::fabs(X[3] - X[1]) + std::fabs(X[6] - X[0]) + std::fabs(X[7] - X[1]);
+      for (int i = 0; i < 2; i++)
+        if (std::fabs(X[4 + i] - (X[2 + i] + X[6 + i] - X[i])) > 1e-10 * sc) dense = true;
+    }
+  }
+  if (dense) return build_dense(c, m);
 
   // 1D tables
   TensorTables& t = c->tb;
@@ -442,7 +830,8 @@ void prof_end(rkdg_ctx* c, std::pair<int, int> pr) {
 
 hipError_t rhs_launch(rkdg_ctx* c, const RhsArgs& a) {
   auto pr = prof_begin(c, c->ev_rhs);
-  hipError_t e = launch_tensor_rhs(c->dim, c->p, a, c->sp, c->d_tb, c->n_bdr > 0, c->stream);
+  hipError_t e = c->dense ? launch_dense_rhs(c->geom, c->p, a, c->sp, c->d_dtab, c->n_bdr > 0, c->stream)
+                          : launch_tensor_rhs(c->dim, c->p, a, c->sp, c->d_tb, c->n_bdr > 0, c->stream);
   prof_end(c, pr);
   return e;
 }
@@ -489,11 +878,13 @@ int read_lambda(rkdg_ctx* c, int slot, double* lam) {
 int limit_impl(rkdg_ctx* c, double* x, int8_t* troubled, bool have_means) {
   if (!have_means) {
     MeansArgs ma{x, c->d_means, c->d_geo, c->NE};
-    HIP_TRY(launch_tensor_means(c->dim, c->p, ma, c->d_tb, c->stream));
+    HIP_TRY(c->dense ? launch_dense_means(c->geom, c->p, ma, c->d_dtab, c->stream)
+                     : launch_tensor_means(c->dim, c->p, ma, c->d_tb, c->stream));
   }
   LimitArgs la{x, c->d_means, c->d_geo, c->d_nbr, c->d_fcode, c->NE, troubled};
   auto pr = prof_begin(c, c->ev_lim);
-  hipError_t le = launch_tensor_limit(c->dim, c->p, la, c->sp, c->d_tb, c->stream);
+  hipError_t le = c->dense ? launch_dense_limit(c->geom, c->p, la, c->sp, c->d_dtab, c->stream)
+                           : launch_tensor_limit(c->dim, c->p, la, c->sp, c->d_tb, c->stream);
   prof_end(c, pr);
   HIP_TRY(le);
   return RKDG_OK;
@@ -584,7 +975,8 @@ int rkdg_limit(rkdg_ctx* c, double* x, int8_t* troubled) {
 int rkdg_means(rkdg_ctx* c, const double* x, double* means) {
   if (!c || !x || !means) return fail(RKDG_E_ARG, "null argument");
   MeansArgs ma{x, means, c->d_geo, c->NE};
-  HIP_TRY(launch_tensor_means(c->dim, c->p, ma, c->d_tb, c->stream));
+  HIP_TRY(c->dense ? launch_dense_means(c->geom, c->p, ma, c->d_dtab, c->stream)
+                   : launch_tensor_means(c->dim, c->p, ma, c->d_tb, c->stream));
   return RKDG_OK;
 }
 
@@ -646,6 +1038,17 @@ int rkdg_node_coords(rkdg_ctx* c, double* xyz) {
   if (!c || !xyz) return fail(RKDG_E_ARG, "null argument");
   const int N = c->p + 1, dim = c->dim;
   const int64_t NE = c->NE;
+  if (c->geom == RKDG_GEOM_TRI) {
+    for (int64_t e = 0; e < NE; e++) {
+      const double* X = &c->coords[e * 6];
+      for (int n = 0; n < c->nd; n++) {
+        const double xi = c->ref_nodes[2 * n], eta = c->ref_nodes[2 * n + 1];
+        for (int d = 0; d < 2; d++)
+          xyz[(e * c->nd + n) * 2 + d] = X[d] + (X[2 + d] - X[d]) * xi + (X[4 + d] - X[d]) * eta;
+      }
+    }
+    return RKDG_OK;
+  }
   for (int64_t e = 0; e < NE; e++) {
     const double* X = &c->coords[e * c->nv * dim];
     for (int n = 0; n < c->nd; n++) {

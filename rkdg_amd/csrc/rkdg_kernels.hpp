@@ -75,4 +75,15 @@ hipError_t launch_fill(double* p, double v, int64_t n, hipStream_t s);
 hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
 bool tensor_supported(int dim, int p);
 
+// launchers (rkdg_dense.hip): affine triangles and bilinear quadrilaterals;
+// t points to the device copy of the dense reference tables
+bool dense_supported(int geom, int p);
+int dense_table_size(int geom, int p);  // doubles
+int dense_geo_size(int geom, int p);    // doubles per element
+hipError_t launch_dense_rhs(int geom, int p, const RhsArgs& a, const SchemeParams& g, const double* t,
+                            bool has_bdr, hipStream_t s);
+hipError_t launch_dense_limit(int geom, int p, const LimitArgs& a, const SchemeParams& g, const double* t,
+                              hipStream_t s);
+hipError_t launch_dense_means(int geom, int p, const MeansArgs& a, const double* t, hipStream_t s);
+
 }  // namespace rkdg
