@@ -162,6 +162,46 @@ int rkdg_profile_enable(rkdg_ctx* ctx, int enable);
 int rkdg_profile_read(rkdg_ctx* ctx, double* rhs_ms, int64_t* rhs_launches, double* limit_ms,
                       int64_t* limit_launches);
 
+/* rkdg_create with flags. RKDG_CREATE_DENSE runs quadrilateral meshes on the
+ * dense bilinear-element kernels even where every element is a parallelogram
+ * (the kernels a batch uses, so a single instance can be compared with its
+ * batch bit for bit). Same counterpart as rkdg_create. */
+#define RKDG_CREATE_DENSE 1
+int rkdg_create_ex(const rkdg_mesh_desc* mesh, const rkdg_scheme_desc* scheme, int flags, rkdg_ctx** out);
+
+/* Batched ensembles. No counterpart in the reference, which runs one case per
+ * process through main.cpp's time loop (main.cpp:467-553, 701, 733); a batch
+ * runs that loop for `count` independent instances at once. Instances share
+ * element type, order and scheme (only bc_param may differ per instance); the
+ * context holds their union mesh (instance i owns elements
+ * [offsets[i], offsets[i+1]) and the matching slices of every variable of the
+ * byNODES state), so rkdg_rhs, rkdg_limit, rkdg_means and rkdg_node_coords
+ * work on the whole ensemble. Quadrilaterals and triangles only. */
+int rkdg_batch_create(const rkdg_mesh_desc* meshes, const rkdg_scheme_desc* schemes, int count,
+                      rkdg_ctx** out);
+/* [count+1] element offsets (HOST). */
+int rkdg_batch_offsets(rkdg_ctx* ctx, int64_t* elem_offsets);
+/* Start every instance's time loop: limit x, first dt from the face wave
+ * speeds (main.cpp:265, 467-484). t0, t_final: HOST [count]. max_steps > 0
+ * also stops an instance after that many steps. */
+int rkdg_batch_init(rkdg_ctx* ctx, double* x, const double* t0, const double* t_final, double cfl,
+                    int64_t max_steps);
+/* Up to nsteps steps of every active instance, each with its own dt
+ * (dt_real = min(dt, t_final - t); next dt = cfl*hmin/lambda/(2p+1); t += dt;
+ * stop at t >= t_final - 1e-8 dt), entirely on the device: a captured HIP graph
+ * per step. n_active (HOST, nullable): instances still running. 2 <= s <= 4. */
+int rkdg_batch_advance(rkdg_ctx* ctx, double* x, int s, const double* a, const double* b, const double* c,
+                       int64_t nsteps, int* n_active);
+/* HOST [count] copies of t, next dt, steps taken and the active flags (each nullable). */
+int rkdg_batch_status(rkdg_ctx* ctx, double* t, double* dt, int64_t* steps, int* active);
+/* use_graph 0/1; grid_cap: persistent grid of the dense right-hand-side kernel
+ * in workgroups (0: one per work block; -1: keep). */
+int rkdg_batch_set_options(rkdg_ctx* ctx, int use_graph, int grid_cap);
+/* Autotuner: times the right-hand side of x (unchanged) for grid_cap 0 and each
+ * candidate, keeps the fastest. ms: [ncand+1] per-launch milliseconds (nullable). */
+int rkdg_tune_grid(rkdg_ctx* ctx, const double* x, const int* candidates, int ncand, int reps, int* best,
+                   double* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -58,7 +58,11 @@ EXPORTS = [
     "rkdg_create", "rkdg_destroy", "rkdg_last_error", "rkdg_get_info", "rkdg_set_stream",
     "rkdg_rhs", "rkdg_face_lambda", "rkdg_limit", "rkdg_step", "rkdg_means", "rkdg_hmin",
     "rkdg_node_coords", "rkdg_check_error", "rkdg_profile_enable", "rkdg_profile_read",
+    "rkdg_create_ex", "rkdg_batch_create", "rkdg_batch_offsets", "rkdg_batch_init",
+    "rkdg_batch_advance", "rkdg_batch_status", "rkdg_batch_set_options", "rkdg_tune_grid",
 ]
+
+RKDG_CREATE_DENSE = 1
 
 _lib = None
 
@@ -90,6 +94,15 @@ def lib():
     L.rkdg_profile_enable.argtypes = [P, C.c_int]
     L.rkdg_profile_read.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                     C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    pd = C.POINTER(C.c_double)
+    L.rkdg_create_ex.argtypes = [C.POINTER(MeshDesc), C.POINTER(SchemeDesc), C.c_int, C.POINTER(C.c_void_p)]
+    L.rkdg_batch_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(SchemeDesc), C.c_int, C.POINTER(C.c_void_p)]
+    L.rkdg_batch_offsets.argtypes = [P, C.POINTER(C.c_int64)]
+    L.rkdg_batch_init.argtypes = [P, dp, pd, pd, C.c_double, C.c_int64]
+    L.rkdg_batch_advance.argtypes = [P, dp, C.c_int, pd, pd, pd, C.c_int64, C.POINTER(C.c_int)]
+    L.rkdg_batch_status.argtypes = [P, pd, pd, C.POINTER(C.c_int64), C.POINTER(C.c_int)]
+    L.rkdg_batch_set_options.argtypes = [P, C.c_int, C.c_int]
+    L.rkdg_tune_grid.argtypes = [P, dp, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_int), pd]
     _lib = L
     return L
 
@@ -143,25 +156,44 @@ def _ptr(t):
     return C.c_void_p(int(t))
 
 
-class Context:
-    """One rkdg_ctx: a flattened mesh + scheme resident on the GPU."""
+def _mesh_desc(mesh, order, md=None):
+    """Fills a MeshDesc; returns it with the arrays it points into (keep them alive)."""
+    keep = (np.ascontiguousarray(mesh.elem_coords, np.float64), np.ascontiguousarray(mesh.elem_verts, np.int64),
+            np.ascontiguousarray(mesh.bdr_verts, np.int64), np.ascontiguousarray(mesh.bdr_attr, np.int32))
+    md = md if md is not None else MeshDesc()
+    md.geom = mesh.geom
+    md.order = order
+    md.num_elements = mesh.num_elements
+    md.elem_coords = keep[0].ctypes.data_as(C.POINTER(C.c_double))
+    md.elem_verts = keep[1].ctypes.data_as(C.POINTER(C.c_int64))
+    md.num_bdr = len(mesh.bdr_attr)
+    md.bdr_verts = keep[2].ctypes.data_as(C.POINTER(C.c_int64))
+    md.bdr_attr = keep[3].ctypes.data_as(C.POINTER(C.c_int32))
+    return md, keep
 
-    def __init__(self, mesh, order: int, scheme: S.Scheme):
+
+def _tableau_arrays(tableau):
+    s, a, b, c = tableau
+    a = (C.c_double * max(1, len(a)))(*a) if a else (C.c_double * 1)(0.0)
+    b = (C.c_double * len(b))(*b)
+    c = (C.c_double * max(1, len(c)))(*c) if c else (C.c_double * 1)(0.0)
+    return s, a, b, c
+
+
+class Context:
+    """One rkdg_ctx: a flattened mesh + scheme resident on the GPU. dense=True
+    runs quadrilaterals on the dense bilinear kernels (RKDG_CREATE_DENSE)."""
+
+    def __init__(self, mesh, order: int, scheme: S.Scheme, dense: bool = False, _handle=None):
         L = lib()
-        self._keep = (np.ascontiguousarray(mesh.elem_coords), np.ascontiguousarray(mesh.elem_verts),
-                      np.ascontiguousarray(mesh.bdr_verts), np.ascontiguousarray(mesh.bdr_attr, np.int32))
-        md = MeshDesc()
-        md.geom = mesh.geom
-        md.order = order
-        md.num_elements = mesh.num_elements
-        md.elem_coords = self._keep[0].ctypes.data_as(C.POINTER(C.c_double))
-        md.elem_verts = self._keep[1].ctypes.data_as(C.POINTER(C.c_int64))
-        md.num_bdr = len(mesh.bdr_attr)
-        md.bdr_verts = self._keep[2].ctypes.data_as(C.POINTER(C.c_int64))
-        md.bdr_attr = self._keep[3].ctypes.data_as(C.POINTER(C.c_int32))
-        self._sd = scheme_desc(scheme)
-        h = C.c_void_p()
-        _check(L.rkdg_create(C.byref(md), C.byref(self._sd), C.byref(h)))
+        if _handle is None:
+            md, self._keep = _mesh_desc(mesh, order)
+            self._sd = scheme_desc(scheme)
+            h = C.c_void_p()
+            _check(L.rkdg_create_ex(C.byref(md), C.byref(self._sd), RKDG_CREATE_DENSE if dense else 0,
+                                    C.byref(h)))
+        else:
+            h = _handle
         self.h = h
         info = Info()
         _check(L.rkdg_get_info(self.h, C.byref(info)))
@@ -205,10 +237,7 @@ class Context:
         _check(lib().rkdg_means(self.h, _ptr(x), _ptr(out)))
 
     def step(self, x, dt, tableau, t=0.0, want_lambda=True):
-        s, a, b, c = tableau
-        a = (C.c_double * max(1, len(a)))(*a) if a else (C.c_double * 1)(0.0)
-        b = (C.c_double * len(b))(*b)
-        c = (C.c_double * max(1, len(c)))(*c) if c else (C.c_double * 1)(0.0)
+        s, a, b, c = _tableau_arrays(tableau)
         lam = C.c_double(0.0)
         _check(lib().rkdg_step(self.h, _ptr(x), t, dt, s, a, b, c,
                                C.byref(lam) if want_lambda else None))
@@ -237,3 +266,68 @@ class Context:
     def check_error(self):
         info = (C.c_int64 * 2)()
         _check(lib().rkdg_check_error(self.h, info))
+
+
+class BatchContext(Context):
+    """A batched ensemble (rkdg_batch_*): instances that share element type,
+    order and scheme (boundary parameters may differ) held as one union mesh.
+    The state of the union is the byNODES layout of the union mesh; instance i
+    owns elements offsets[i]:offsets[i+1] of every variable."""
+
+    def __init__(self, meshes, order: int, schemes):
+        L = lib()
+        n = len(meshes)
+        if len(schemes) != n or n < 1:
+            raise ValueError("one scheme per mesh")
+        mds = (MeshDesc * n)()
+        self._keep = []
+        for i, m in enumerate(meshes):
+            _, k = _mesh_desc(m, order, mds[i])
+            self._keep.append(k)
+        sds = (SchemeDesc * n)()
+        for i, s in enumerate(schemes):
+            sds[i] = scheme_desc(s)
+        self._sds = sds
+        h = C.c_void_p()
+        _check(L.rkdg_batch_create(mds, sds, n, C.byref(h)))
+        super().__init__(None, order, schemes[0], _handle=h)
+        self.count = n
+        self.meshes = meshes
+        off = (C.c_int64 * (n + 1))()
+        _check(L.rkdg_batch_offsets(self.h, off))
+        self.offsets = np.array(off[:], dtype=np.int64)
+
+    def init(self, x, t_final, cfl, t0=0.0, max_steps=0):
+        tf = np.broadcast_to(np.asarray(t_final, np.float64), (self.count,)).copy()
+        t0 = np.broadcast_to(np.asarray(t0, np.float64), (self.count,)).copy()
+        pd = C.POINTER(C.c_double)
+        _check(lib().rkdg_batch_init(self.h, _ptr(x), t0.ctypes.data_as(pd), tf.ctypes.data_as(pd),
+                                     float(cfl), int(max_steps)))
+
+    def advance(self, x, tableau, nsteps=1):
+        """Returns the number of instances still running."""
+        s, a, b, c = _tableau_arrays(tableau)
+        na = C.c_int(0)
+        _check(lib().rkdg_batch_advance(self.h, _ptr(x), s, a, b, c, int(nsteps), C.byref(na)))
+        return na.value
+
+    def status(self):
+        n = self.count
+        t, dt = np.zeros(n), np.zeros(n)
+        steps, act = np.zeros(n, np.int64), np.zeros(n, np.int32)
+        pd = C.POINTER(C.c_double)
+        _check(lib().rkdg_batch_status(self.h, t.ctypes.data_as(pd), dt.ctypes.data_as(pd),
+                                       steps.ctypes.data_as(C.POINTER(C.c_int64)),
+                                       act.ctypes.data_as(C.POINTER(C.c_int))))
+        return t, dt, steps, act
+
+    def set_options(self, use_graph=True, grid_cap=-1):
+        _check(lib().rkdg_batch_set_options(self.h, 1 if use_graph else 0, int(grid_cap)))
+
+    def tune_grid(self, x, candidates=(256, 512, 1024, 2048, 4096, 8192), reps=5):
+        """Autotuner: returns (chosen grid cap, {cap: ms per launch})."""
+        cand = (C.c_int * len(candidates))(*candidates)
+        ms = (C.c_double * (len(candidates) + 1))()
+        best = C.c_int(0)
+        _check(lib().rkdg_tune_grid(self.h, _ptr(x), cand, len(candidates), int(reps), C.byref(best), ms))
+        return best.value, dict(zip((0,) + tuple(candidates), ms[:]))

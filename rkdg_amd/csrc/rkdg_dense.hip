@@ -69,15 +69,15 @@ __device__ __forceinline__ int64_t xcd_remap_d(int64_t b, int64_t nb) {
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + l;
 }
 
+// one work block (EPB elements, one wavefront) of the right-hand side
 template <int GEOM, int P, int RS, bool BC>
-__global__ __launch_bounds__(64) void dense_rhs_kernel(const RhsArgs A, const SchemeParams g,
-                                                       const double* __restrict__ gtab) {
+__device__ __forceinline__ void dense_rhs_block(const RhsArgs& A, const SchemeParams& g, int64_t vb,
+                                                DenseTab<GEOM, P>& tb) {
   using C = DC<GEOM, P>;
   constexpr int ND = C::ND, NEQ = C::NEQ, NF = C::NF, NQ = C::NQ, NPT = C::NPT;
   constexpr int T = C::T, EPB = C::EPB, BLOCK = C::BLOCK, GS = C::GSIZE;
   constexpr bool TRI = C::TRI;
 
-  __shared__ DenseTab<GEOM, P> tb;
   __shared__ double s_u[EPB][NEQ][ND];
   __shared__ double s_g[EPB][2][NEQ][ND];     // contravariant nodal fluxes, then z
   __shared__ double s_f[EPB][NPT][NEQ];       // weighted face fluxes
@@ -88,11 +88,11 @@ __global__ __launch_bounds__(64) void dense_rhs_kernel(const RhsArgs A, const Sc
   const int team = tid / T, lane = tid % T;
   const int64_t NE = A.NE;
   const int64_t NDOFS = NE * ND;
-  const int64_t e0 = xcd_remap_d(blockIdx.x, gridDim.x) * EPB;
+  const int64_t e0 = vb * EPB;
   const int64_t e = e0 + team;
-  const bool active = e < NE;
+  const int ins = (A.inst && e < NE) ? A.inst[e] : 0;
+  const bool active = e < NE && (!A.active || A.active[ins]);
 
-  stage_dense<GEOM, P>(tb, gtab);
 #pragma unroll
   for (int v = 0; v < NEQ; v++) {
     const double* xv = A.x + v * NDOFS + e0 * ND;
@@ -185,7 +185,8 @@ __global__ __launch_bounds__(64) void dense_rhs_kernel(const RhsArgs A, const Sc
     } else if constexpr (BC) {
       const int slot = -1 - nb;
       if (slot < g.num_bc) {
-        bc_state<2>(g, slot, um, nrm, up);
+        const double* Pover = A.bcp ? A.bcp + ((int64_t)ins * RKDG_MAX_BC + slot) * 8 : nullptr;
+        bc_state<2>(g, slot, um, nrm, up, Pover);
         lam = riemann_t<2, RS>(g, um, up, nrm, fl, ok);
       } else {
 #pragma unroll
@@ -236,89 +237,107 @@ __global__ __launch_bounds__(64) void dense_rhs_kernel(const RhsArgs A, const Sc
         for (int v = 0; v < NEQ; v++) z[v] += s_f[team][f * NQ + q][v] * ph;
       }
   }
-  if (A.face_only) {
-    for (int off = 32; off > 0; off >>= 1) {
-      const double o = __shfl_xor(lmax, off, 64);
-      lmax = o > lmax ? o : lmax;
-    }
-    if (tid == 0 && lmax > 0.0)
-      atomicMax(reinterpret_cast<unsigned long long*>(A.lam), (unsigned long long)__double_as_longlong(lmax));
-    return;
-  }
-  __syncthreads();
-  if (node_lane) {
-#pragma unroll
-    for (int v = 0; v < NEQ; v++) s_g[team][0][v][j] = z[v];
-  }
-  __syncthreads();
-  // ---- inverse mass and stage epilogue
-  double outv[NEQ];
-  double mk = 0.0;
-  if (node_lane) {
-    double y[NEQ];
-#pragma unroll
-    for (int v = 0; v < NEQ; v++) y[v] = 0.0;
-    if (TRI) {
-      const double idet = recip(s_geo[team][C::G_DET]);
-#pragma unroll
-      for (int k = 0; k < ND; k++) {
-        const double mi = tb.Minv[j][k];
-#pragma unroll
-        for (int v = 0; v < NEQ; v++) y[v] += mi * s_g[team][0][v][k];
-      }
-#pragma unroll
-      for (int v = 0; v < NEQ; v++) y[v] *= idet;
-      mk = tb.mhat[j] * s_geo[team][C::G_DET];
-    } else {
-#pragma unroll
-      for (int k = 0; k < ND; k++) {
-        const double mi = s_geo[team][C::G_MINV + j * ND + k];
-#pragma unroll
-        for (int v = 0; v < NEQ; v++) y[v] += mi * s_g[team][0][v][k];
-      }
-      mk = s_geo[team][C::G_MW + j];
-    }
-#pragma unroll
-    for (int v = 0; v < NEQ; v++) {
-      const double yv = y[v];
-      const int64_t idx = v * NDOFS + e * ND + j;
-      if (A.k) A.k[idx] = yv;
-      outv[v] = 0.0;
-#pragma unroll
-      for (int m = 0; m < 4; m++) {
-        if (m < A.nout) {
-          const double base = A.outs[m].base ? A.outs[m].base[idx] : s_u[team][v][j];
-          const double o = base + A.outs[m].coef * yv;
-          A.outs[m].out[idx] = o;
-          if (m == A.means_of) outv[v] = o;
-        }
-      }
-    }
-  }
-  if (A.means) {
+  if (!A.face_only) {
     __syncthreads();
     if (node_lane) {
 #pragma unroll
-      for (int v = 0; v < NEQ; v++) s_g[team][1][v][j] = outv[v];
-      s_g[team][0][0][j] = mk;
+      for (int v = 0; v < NEQ; v++) s_g[team][0][v][j] = z[v];
     }
     __syncthreads();
-    if (active && lane < NEQ) {
-      double s = 0.0, ms = 0.0;
-      for (int k = 0; k < ND; k++) {
-        const double w = s_g[team][0][0][k];
-        s += w * s_g[team][1][lane][k];
-        ms += w;
+    // ---- inverse mass and stage epilogue
+    double outv[NEQ];
+    double mk = 0.0;
+    if (node_lane) {
+      double y[NEQ];
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) y[v] = 0.0;
+      if (TRI) {
+        const double idet = recip(s_geo[team][C::G_DET]);
+#pragma unroll
+        for (int k = 0; k < ND; k++) {
+          const double mi = tb.Minv[j][k];
+#pragma unroll
+          for (int v = 0; v < NEQ; v++) y[v] += mi * s_g[team][0][v][k];
+        }
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) y[v] *= idet;
+        mk = tb.mhat[j] * s_geo[team][C::G_DET];
+      } else {
+#pragma unroll
+        for (int k = 0; k < ND; k++) {
+          const double mi = s_geo[team][C::G_MINV + j * ND + k];
+#pragma unroll
+          for (int v = 0; v < NEQ; v++) y[v] += mi * s_g[team][0][v][k];
+        }
+        mk = s_geo[team][C::G_MW + j];
       }
-      A.means[lane * NE + e] = s / ms;
+      const double dtm = A.dt_inst ? A.dt_inst[ins] : 1.0;
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) {
+        const double yv = y[v];
+        const int64_t idx = v * NDOFS + e * ND + j;
+        if (A.k) A.k[idx] = yv;
+        outv[v] = 0.0;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+          if (m < A.nout) {
+            const double base = A.outs[m].base ? A.outs[m].base[idx] : s_u[team][v][j];
+            const double o = base + (A.outs[m].coef * dtm) * yv;
+            A.outs[m].out[idx] = o;
+            if (m == A.means_of) outv[v] = o;
+          }
+        }
+      }
+    }
+    if (A.means) {
+      __syncthreads();
+      if (node_lane) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) s_g[team][1][v][j] = outv[v];
+        s_g[team][0][0][j] = mk;
+      }
+      __syncthreads();
+      if (active && lane < NEQ) {
+        double s = 0.0, ms = 0.0;
+        for (int k = 0; k < ND; k++) {
+          const double w = s_g[team][0][0][k];
+          s += w * s_g[team][1][lane][k];
+          ms += w;
+        }
+        A.means[lane * NE + e] = s / ms;
+      }
     }
   }
-  for (int off = 32; off > 0; off >>= 1) {
+  // ---- max wave speed: per instance slot
+  for (int off = T / 2; off > 0; off >>= 1) {
     const double o = __shfl_xor(lmax, off, 64);
     lmax = o > lmax ? o : lmax;
   }
-  if (tid == 0 && lmax > 0.0)
-    atomicMax(reinterpret_cast<unsigned long long*>(A.lam), (unsigned long long)__double_as_longlong(lmax));
+  const int64_t elast = e0 + EPB - 1 < NE ? e0 + EPB - 1 : NE - 1;
+  const bool uniform = !A.inst || (e0 < NE && A.inst[e0] == A.inst[elast]);
+  if (uniform) {
+    for (int off = 32; off >= T; off >>= 1) {
+      const double o = __shfl_xor(lmax, off, 64);
+      lmax = o > lmax ? o : lmax;
+    }
+    const int i0 = A.inst && e0 < NE ? A.inst[e0] : 0;
+    if (tid == 0 && lmax > 0.0)
+      atomicMax(reinterpret_cast<unsigned long long*>(A.lam + (A.inst ? i0 : 0)),
+                (unsigned long long)__double_as_longlong(lmax));
+  } else if (lane == 0 && active && lmax > 0.0) {
+    atomicMax(reinterpret_cast<unsigned long long*>(A.lam + ins), (unsigned long long)__double_as_longlong(lmax));
+  }
+  __syncthreads();  // LDS is reused by the next work block
+}
+
+// persistent: the grid strides over the work blocks
+template <int GEOM, int P, int RS, bool BC>
+__global__ __launch_bounds__(64) void dense_rhs_kernel(const RhsArgs A, const SchemeParams g,
+                                                       const double* __restrict__ gtab, int64_t nvb) {
+  __shared__ DenseTab<GEOM, P> tb;
+  stage_dense<GEOM, P>(tb, gtab);
+  __syncthreads();
+  for (int64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) dense_rhs_block<GEOM, P, RS, BC>(A, g, vb, tb);
 }
 
 template <int GEOM, int P>
@@ -363,7 +382,7 @@ __global__ __launch_bounds__(64) void dense_limit_kernel(const LimitArgs A, cons
   const int64_t NDOFS = NE * ND;
   const int64_t e0 = xcd_remap_d(blockIdx.x, gridDim.x) * EPB;
   const int64_t e = e0 + team;
-  const bool active = e < NE;
+  const bool active = e < NE && (!A.active || A.active[A.inst[e]]);
 
   stage_dense<GEOM, P>(tb, gtab);
 #pragma unroll
@@ -556,11 +575,13 @@ int dense_geo_size(int geom, int p) {
 template <int G, int PP, int RS>
 hipError_t dense_rhs_rs(const RhsArgs& a, const SchemeParams& g, const double* t, bool bc, hipStream_t s) {
   using C = DC<G, PP>;
-  const int64_t nb = (a.NE + C::EPB - 1) / C::EPB;
-  if (bc) dense_rhs_kernel<G, PP, RS, true><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t);
-  else dense_rhs_kernel<G, PP, RS, false><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t);
+  const int64_t nvb = (a.NE + C::EPB - 1) / C::EPB;
+  const int64_t grid = a.grid_cap > 0 && a.grid_cap < nvb ? a.grid_cap : nvb;
+  if (bc) dense_rhs_kernel<G, PP, RS, true><<<(unsigned)grid, C::BLOCK, 0, s>>>(a, g, t, nvb);
+  else dense_rhs_kernel<G, PP, RS, false><<<(unsigned)grid, C::BLOCK, 0, s>>>(a, g, t, nvb);
   return hipGetLastError();
 }
+
 
 template <int G, int PP>
 hipError_t dense_rhs_gp(const RhsArgs& a, const SchemeParams& g, const double* t, bool bc, hipStream_t s) {

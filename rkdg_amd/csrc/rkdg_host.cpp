@@ -158,10 +158,28 @@ struct rkdg_ctx {
   std::vector<double> geo_h;   // [g][NE]
   int64_t n_int = 0, n_bdr = 0;
   double hmin = 0.0;
+  std::vector<double> h_elem;  // GetElementSize(e, 1) per element
   // instrumentation: event pairs around kernel launches
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, int>> ev_rhs, ev_lim;  // (start, stop) pool indices
+  // batched ensembles: count disjoint instances in one union mesh
+  int count = 1;
+  int force_dense = 0;
+  int grid_cap = 0;              // persistent grid of the dense right-hand-side kernel
+  std::vector<int64_t> inst_off; // [count+1] element offsets
+  int* d_inst = nullptr;         // [NE] element -> instance
+  double* d_bcp = nullptr;       // [count][RKDG_MAX_BC][8]
+  int* d_active = nullptr;       // [count]
+  double *d_t = nullptr, *d_dt = nullptr, *d_dtr = nullptr, *d_tf = nullptr, *d_hmin = nullptr;
+  long long* d_steps = nullptr;
+  int* d_nact = nullptr;
+  double cfl = 0.0;
+  long long max_steps = 0;
+  hipStream_t bstream = nullptr;    // stream of batch steps (graph capture needs a created stream)
+  hipGraphExec_t gexec = nullptr;   // captured batch step
+  std::vector<double> gkey;         // what the graph was captured for: x, tableau
+  int use_graph = 1;
 };
 
 namespace {
@@ -178,6 +196,18 @@ void free_ctx(rkdg_ctx* c) {
   (void)hipFree(c->d_dtab);
   for (auto& p : c->d_stage) (void)hipFree(p);
   for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
+  (void)hipFree(c->d_inst);
+  (void)hipFree(c->d_bcp);
+  (void)hipFree(c->d_active);
+  (void)hipFree(c->d_t);
+  (void)hipFree(c->d_dt);
+  (void)hipFree(c->d_dtr);
+  (void)hipFree(c->d_tf);
+  (void)hipFree(c->d_hmin);
+  (void)hipFree(c->d_steps);
+  (void)hipFree(c->d_nact);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->bstream) (void)hipStreamDestroy(c->bstream);
   delete c;
 }
 
@@ -455,6 +485,7 @@ int build_dense(rkdg_ctx* c, const rkdg_mesh_desc* m) {
   std::vector<double> Mq, wqq;
   std::vector<double> mpts;
   if (!tri) quad_rule(p + 2, mpts, Mq);
+  c->h_elem.assign(NE, 0.0);
   double hmin = 1e300;
   for (int64_t e = 0; e < NE; e++) {
     const double* X = &c->coords[e * c->nv * 2];
@@ -473,7 +504,8 @@ int build_dense(rkdg_ctx* c, const rkdg_mesh_desc* m) {
       const double Pm[4] = {1.0, -1.0 / s3, 0.0, 2.0 / s3};
       double Q[4] = {J[0] * Pm[0] + J[1] * Pm[2], J[0] * Pm[1] + J[1] * Pm[3],
                      J[2] * Pm[0] + J[3] * Pm[2], J[2] * Pm[1] + J[3] * Pm[3]};
-      hmin = std::min(hmin, min_singular(2, Q));
+      c->h_elem[e] = min_singular(2, Q);
+      hmin = std::min(hmin, c->h_elem[e]);
     } else {
       std::vector<double> Mm(nd * nd, 0.0), mw(nd, 0.0);
       for (int k = 0; k < nd; k++) {
@@ -502,7 +534,8 @@ int build_dense(rkdg_ctx* c, const rkdg_mesh_desc* m) {
       for (int i2 = 0; i2 < nd; i2++) G(4 * nd + nd * nd + i2) = mw[i2];
       double Jc[4];
       bilinear_jac(X, 0.5, 0.5, Jc);
-      hmin = std::min(hmin, min_singular(2, Jc));
+      c->h_elem[e] = min_singular(2, Jc);
+      hmin = std::min(hmin, c->h_elem[e]);
     }
     for (int f = 0; f < nf; f++) {
       const double* a = &X[ED[f][0] * 2];
@@ -574,8 +607,8 @@ int build_dense(rkdg_ctx* c, const rkdg_mesh_desc* m) {
   HIP_TRY(hipMemcpy(c->d_fcode, fcode.data(), sizeof(int) * nf * NE, hipMemcpyHostToDevice));
   HIP_TRY(hipMalloc(&c->d_dtab, sizeof(double) * tsize));
   HIP_TRY(hipMemcpy(c->d_dtab, tab.data(), sizeof(double) * tsize, hipMemcpyHostToDevice));
-  HIP_TRY(hipMalloc(&c->d_lam, sizeof(double) * 8));
-  HIP_TRY(hipMemset(c->d_lam, 0, sizeof(double) * 8));
+  HIP_TRY(hipMalloc(&c->d_lam, sizeof(double) * 8 * c->count));
+  HIP_TRY(hipMemset(c->d_lam, 0, sizeof(double) * 8 * c->count));
   HIP_TRY(hipMalloc(&c->d_err, sizeof(long long) * 4));
   HIP_TRY(hipMemset(c->d_err, 0, sizeof(long long) * 4));
   HIP_TRY(hipMalloc(&c->d_means, sizeof(double) * c->neq * NE));
@@ -627,8 +660,11 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
   g.last_hope = s->last_hope;
   g.pow_nd = s->pow_nd;
 
-  // triangles and non-parallelogram quadrilaterals take the dense path
-  bool dense = m->geom == RKDG_GEOM_TRI;
+  // triangles and non-parallelogram quadrilaterals take the dense path (any
+  // quadrilateral mesh when asked for)
+  bool dense = m->geom == RKDG_GEOM_TRI || (m->geom == RKDG_GEOM_QUAD && c->force_dense);
+  if (c->force_dense && m->geom == RKDG_GEOM_HEX)
+    return fail(RKDG_E_UNSUPPORTED, "hexahedra have no dense kernels");
   if (m->geom == RKDG_GEOM_QUAD) {
     for (int64_t e = 0; e < NE && !dense; e++) {
       const double* X = &c->coords[e * 8];
@@ -665,6 +701,7 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
   // geometry (affine tensor elements): adj(J) row-major, det(J)
   const int G = dim * dim + 1;
   c->geo_h.assign((size_t)G * NE, 0.0);
+  c->h_elem.assign(NE, 0.0);
   double hmin = 1e300;
   for (int64_t e = 0; e < NE; e++) {
     const double* X = &c->coords[e * nv * dim];
@@ -707,7 +744,8 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
     if (!(det > 0.0)) return fail(RKDG_E_ARG, "element " + std::to_string(e) + " has non-positive Jacobian");
     for (int i = 0; i < dim * dim; i++) c->geo_h[i * NE + e] = A[i];
     c->geo_h[(G - 1) * NE + e] = det;
-    hmin = std::min(hmin, min_singular(dim, J));
+    c->h_elem[e] = min_singular(dim, J);
+    hmin = std::min(hmin, c->h_elem[e]);
   }
   if (!c->affine) return fail(RKDG_E_UNSUPPORTED, "non-affine tensor elements are not built into this library");
   c->hmin = hmin;
@@ -798,8 +836,8 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
   HIP_TRY(hipMemcpy(c->d_nbr, nbr.data(), sizeof(int) * nf * NE, hipMemcpyHostToDevice));
   HIP_TRY(hipMalloc(&c->d_fcode, sizeof(int) * nf * NE));
   HIP_TRY(hipMemcpy(c->d_fcode, fcode.data(), sizeof(int) * nf * NE, hipMemcpyHostToDevice));
-  HIP_TRY(hipMalloc(&c->d_lam, sizeof(double) * 8));
-  HIP_TRY(hipMemset(c->d_lam, 0, sizeof(double) * 8));
+  HIP_TRY(hipMalloc(&c->d_lam, sizeof(double) * 8 * c->count));
+  HIP_TRY(hipMemset(c->d_lam, 0, sizeof(double) * 8 * c->count));
   HIP_TRY(hipMalloc(&c->d_err, sizeof(long long) * 4));
   HIP_TRY(hipMemset(c->d_err, 0, sizeof(long long) * 4));
   HIP_TRY(hipMalloc(&c->d_means, sizeof(double) * c->neq * NE));
@@ -850,6 +888,9 @@ RhsArgs base_args(rkdg_ctx* c, const double* x) {
   a.lam = c->d_lam;
   a.err = c->d_err;
   a.face_only = 0;
+  a.inst = c->d_inst;
+  a.bcp = c->d_bcp;
+  a.grid_cap = c->grid_cap;
   return a;
 }
 
@@ -870,18 +911,23 @@ int check_device_error(rkdg_ctx* c, int64_t* info) {
   return RKDG_OK;
 }
 
+// wave speed of stage slot; on a batch, the maximum over the instances
 int read_lambda(rkdg_ctx* c, int slot, double* lam) {
-  HIP_TRY(hipMemcpyAsync(lam, c->d_lam + slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  return check_device_error(c, nullptr);
+  std::vector<double> h(c->count);
+  HIP_TRY(hipMemcpyAsync(h.data(), c->d_lam + (int64_t)slot * c->count, sizeof(double) * c->count,
+                         hipMemcpyDeviceToHost, c->stream));
+  const int rc = check_device_error(c, nullptr);
+  *lam = *std::max_element(h.begin(), h.end());
+  return rc;
 }
 
-int limit_impl(rkdg_ctx* c, double* x, int8_t* troubled, bool have_means) {
+int limit_impl(rkdg_ctx* c, double* x, int8_t* troubled, bool have_means, const int* active = nullptr) {
   if (!have_means) {
     MeansArgs ma{x, c->d_means, c->d_geo, c->NE};
     HIP_TRY(c->dense ? launch_dense_means(c->geom, c->p, ma, c->d_dtab, c->stream)
                      : launch_tensor_means(c->dim, c->p, ma, c->d_tb, c->stream));
   }
-  LimitArgs la{x, c->d_means, c->d_geo, c->d_nbr, c->d_fcode, c->NE, troubled};
+  LimitArgs la{x, c->d_means, c->d_geo, c->d_nbr, c->d_fcode, c->NE, troubled, c->d_inst, active};
   auto pr = prof_begin(c, c->ev_lim);
   hipError_t le = c->dense ? launch_dense_limit(c->geom, c->p, la, c->sp, c->d_dtab, c->stream)
                            : launch_tensor_limit(c->dim, c->p, la, c->sp, c->d_tb, c->stream);
@@ -897,6 +943,62 @@ int ensure_stage_buffers(rkdg_ctx* c, int n) {
   return RKDG_OK;
 }
 
+BatchTime batch_time(rkdg_ctx* c, const double* lam) {
+  BatchTime b{};
+  b.count = c->count;
+  b.t = c->d_t;
+  b.dt = c->d_dt;
+  b.dt_real = c->d_dtr;
+  b.t_final = c->d_tf;
+  b.hmin = c->d_hmin;
+  b.lam = lam;
+  b.active = c->d_active;
+  b.steps = c->d_steps;
+  b.n_active = c->d_nact;
+  b.cfl = c->cfl;
+  b.p = c->p;
+  b.max_steps = c->max_steps;
+  return b;
+}
+
+// one batch step on c->stream: per-instance dt, stages with the per-instance
+// dt scaling every coefficient, limiter after each stage, dt/t/active update
+int batch_step_launch(rkdg_ctx* c, double* x, int s, const double* a, const double* b) {
+  double* W[4] = {nullptr, c->d_stage[0], c->d_stage[1], c->d_stage[2]};
+  double* acc = c->d_stage[3];
+  HIP_TRY(launch_fill(c->d_lam, 0.0, (int64_t)s * c->count, c->stream));
+  BatchTime bt = batch_time(c, c->d_lam + (int64_t)(s - 1) * c->count);
+  HIP_TRY(launch_batch_pre(bt, c->stream));
+  auto A_ = [&](int i, int j) { return a[i * (i - 1) / 2 + j]; };
+  for (int i = 0; i < s; i++) {
+    const double* in = i == 0 ? x : W[i];
+    RhsArgs ra = base_args(c, in);
+    ra.lam = c->d_lam + (int64_t)i * c->count;
+    ra.dt_inst = c->d_dtr;
+    ra.active = c->d_active;
+    int n = 0;
+    for (int j = i + 1; j < s; j++) {
+      ra.outs[n].out = W[j];
+      ra.outs[n].base = i == 0 ? nullptr : W[j];
+      ra.outs[n].coef = A_(j, i);
+      n++;
+    }
+    const bool last = i == s - 1;
+    ra.outs[n].out = last ? x : acc;
+    ra.outs[n].base = i == 0 ? nullptr : acc;
+    ra.outs[n].coef = b[i];
+    n++;
+    ra.nout = n;
+    ra.means = c->d_means;
+    ra.means_of = last ? n - 1 : 0;
+    HIP_TRY(rhs_launch(c, ra));
+    int rc = limit_impl(c, last ? x : W[i + 1], nullptr, true, c->d_active);
+    if (rc) return rc;
+  }
+  HIP_TRY(launch_batch_post(bt, c->stream));
+  return RKDG_OK;
+}
+
 }  // namespace
 
 // ============================================================================
@@ -905,9 +1007,14 @@ extern "C" {
 const char* rkdg_last_error(void) { return g_last_error.c_str(); }
 
 int rkdg_create(const rkdg_mesh_desc* mesh, const rkdg_scheme_desc* scheme, rkdg_ctx** out) {
+  return rkdg_create_ex(mesh, scheme, 0, out);
+}
+
+int rkdg_create_ex(const rkdg_mesh_desc* mesh, const rkdg_scheme_desc* scheme, int flags, rkdg_ctx** out) {
   if (!mesh || !scheme || !out) return fail(RKDG_E_ARG, "null argument");
   *out = nullptr;
   rkdg_ctx* c = new rkdg_ctx();
+  c->force_dense = (flags & RKDG_CREATE_DENSE) ? 1 : 0;
   int rc = build_ctx(c, mesh, scheme);
   if (rc != RKDG_OK) {
     std::string keep = g_last_error;
@@ -950,7 +1057,7 @@ int rkdg_set_stream(rkdg_ctx* c, void* stream) {
 
 int rkdg_rhs(rkdg_ctx* c, const double* x, double* y, double* lambda_max) {
   if (!c || !x || !y) return fail(RKDG_E_ARG, "null argument");
-  HIP_TRY(launch_fill(c->d_lam, 0.0, 1, c->stream));
+  HIP_TRY(launch_fill(c->d_lam, 0.0, c->count, c->stream));
   RhsArgs a = base_args(c, x);
   a.k = y;
   HIP_TRY(rhs_launch(c, a));
@@ -960,7 +1067,7 @@ int rkdg_rhs(rkdg_ctx* c, const double* x, double* y, double* lambda_max) {
 
 int rkdg_face_lambda(rkdg_ctx* c, const double* x, double* lambda_max) {
   if (!c || !x || !lambda_max) return fail(RKDG_E_ARG, "null argument");
-  HIP_TRY(launch_fill(c->d_lam, 0.0, 1, c->stream));
+  HIP_TRY(launch_fill(c->d_lam, 0.0, c->count, c->stream));
   RhsArgs a = base_args(c, x);
   a.face_only = 1;
   HIP_TRY(rhs_launch(c, a));
@@ -997,12 +1104,12 @@ int rkdg_step(rkdg_ctx* c, double* x, double t, double dt, int s, const double* 
   if (rc) return rc;
   double* W[4] = {nullptr, c->d_stage[0], c->d_stage[1], c->d_stage[2]};
   double* acc = c->d_stage[3];
-  HIP_TRY(launch_fill(c->d_lam, 0.0, 8, c->stream));
+  HIP_TRY(launch_fill(c->d_lam, 0.0, 8 * c->count, c->stream));
   auto A_ = [&](int i, int j) { return a[i * (i - 1) / 2 + j]; };
   for (int i = 0; i < s; i++) {
     const double* in = i == 0 ? x : W[i];
     RhsArgs ra = base_args(c, in);
-    ra.lam = c->d_lam + i;
+    ra.lam = c->d_lam + (int64_t)i * c->count;
     int n = 0;
     for (int j = i + 1; j < s; j++) {
       ra.outs[n].out = W[j];
@@ -1104,6 +1211,287 @@ int rkdg_profile_read(rkdg_ctx* c, double* rhs_ms, int64_t* rhs_n, double* lim_m
 int rkdg_check_error(rkdg_ctx* c, int64_t* info) {
   if (!c) return fail(RKDG_E_ARG, "null context");
   return check_device_error(c, info);
+}
+
+// ---------------------------------------------------------------------------
+// Batched ensembles
+
+int rkdg_batch_create(const rkdg_mesh_desc* meshes, const rkdg_scheme_desc* schemes, int count,
+                      rkdg_ctx** out) {
+  if (!meshes || !schemes || !out || count < 1) return fail(RKDG_E_ARG, "null argument or count < 1");
+  *out = nullptr;
+  const rkdg_mesh_desc& m0 = meshes[0];
+  if (m0.geom == RKDG_GEOM_HEX) return fail(RKDG_E_UNSUPPORTED, "batched ensembles of hexahedra are not built");
+  const int nv = m0.geom == RKDG_GEOM_TRI ? 3 : 4;
+  const rkdg_scheme_desc& s0 = schemes[0];
+  // every instance shares the discretisation; only boundary parameters differ
+  for (int i = 1; i < count; i++) {
+    const rkdg_scheme_desc& si = schemes[i];
+    bool same = si.riemann == s0.riemann && si.wave_speeds == s0.wave_speeds && si.hllc_abs == s0.hllc_abs &&
+                si.gamma == s0.gamma && si.gas_constant == s0.gas_constant && si.covolume == s0.covolume &&
+                si.num_bc == s0.num_bc && si.indicator == s0.indicator && si.phi == s0.phi &&
+                si.venkat_k == s0.venkat_k && si.limiter == s0.limiter && si.last_hope == s0.last_hope &&
+                si.pow_nd == s0.pow_nd;
+    for (int q = 0; q < RKDG_MAX_BC && same; q++)
+      same = si.bc_attr[q] == s0.bc_attr[q] && si.bc_type[q] == s0.bc_type[q];
+    if (!same) return fail(RKDG_E_ARG, "instance " + std::to_string(i) + ": scheme differs beyond boundary parameters");
+    if (meshes[i].geom != m0.geom || meshes[i].order != m0.order)
+      return fail(RKDG_E_ARG, "instance " + std::to_string(i) + ": element type or order differs");
+  }
+  // union mesh: instances concatenated, vertex ids shifted apart
+  std::vector<double> coords;
+  std::vector<int64_t> verts, bverts;
+  std::vector<int32_t> battr;
+  std::vector<int64_t> off(count + 1, 0);
+  int64_t vbase = 0;
+  for (int i = 0; i < count; i++) {
+    const rkdg_mesh_desc& m = meshes[i];
+    if (m.num_elements <= 0) return fail(RKDG_E_ARG, "instance " + std::to_string(i) + ": empty mesh");
+    off[i + 1] = off[i] + m.num_elements;
+    coords.insert(coords.end(), m.elem_coords, m.elem_coords + m.num_elements * nv * 2);
+    int64_t vmax = -1;
+    for (int64_t k = 0; k < m.num_elements * nv; k++) {
+      verts.push_back(m.elem_verts[k] + vbase);
+      vmax = std::max(vmax, m.elem_verts[k]);
+    }
+    for (int64_t k = 0; k < m.num_bdr * 2; k++) {
+      bverts.push_back(m.bdr_verts[k] + vbase);
+      vmax = std::max(vmax, m.bdr_verts[k]);
+    }
+    battr.insert(battr.end(), m.bdr_attr, m.bdr_attr + m.num_bdr);
+    vbase += vmax + 1;
+  }
+  rkdg_mesh_desc u = m0;
+  u.num_elements = off[count];
+  u.elem_coords = coords.data();
+  u.elem_verts = verts.data();
+  u.num_bdr = (int64_t)battr.size();
+  u.bdr_verts = bverts.data();
+  u.bdr_attr = battr.data();
+
+  rkdg_ctx* c = new rkdg_ctx();
+  c->count = count;
+  c->force_dense = 1;
+  auto bail = [&](int rc) {
+    std::string keep = g_last_error;
+    free_ctx(c);
+    g_last_error = keep;
+    return rc;
+  };
+  int rc = build_ctx(c, &u, &s0);
+  if (rc != RKDG_OK) return bail(rc);
+  c->inst_off = off;
+  std::vector<int> inst(c->NE);
+  std::vector<double> bcp((size_t)count * RKDG_MAX_BC * 8), hm(count, 1e300);
+  for (int i = 0; i < count; i++) {
+    for (int64_t e = off[i]; e < off[i + 1]; e++) {
+      inst[e] = i;
+      hm[i] = std::min(hm[i], c->h_elem[e]);
+    }
+    for (int q = 0; q < RKDG_MAX_BC; q++)
+      for (int k = 0; k < 8; k++) bcp[((size_t)i * RKDG_MAX_BC + q) * 8 + k] = schemes[i].bc_param[q][k];
+  }
+  auto up = [&](auto** d, const auto* h, size_t n) -> hipError_t {
+    hipError_t e = hipMalloc((void**)d, sizeof(**d) * n);
+    if (e != hipSuccess) return e;
+    return h ? hipMemcpy(*d, h, sizeof(**d) * n, hipMemcpyHostToDevice) : hipMemset(*d, 0, sizeof(**d) * n);
+  };
+  const double* nod = nullptr;
+  const int* noi = nullptr;
+  const long long* noll = nullptr;
+  hipError_t he = hipSuccess;
+  if (he == hipSuccess) he = up(&c->d_inst, inst.data(), (size_t)c->NE);
+  if (he == hipSuccess) he = up(&c->d_bcp, bcp.data(), bcp.size());
+  if (he == hipSuccess) he = up(&c->d_hmin, hm.data(), (size_t)count);
+  if (he == hipSuccess) he = up(&c->d_active, noi, (size_t)count);
+  if (he == hipSuccess) he = up(&c->d_t, nod, (size_t)count);
+  if (he == hipSuccess) he = up(&c->d_dt, nod, (size_t)count);
+  if (he == hipSuccess) he = up(&c->d_dtr, nod, (size_t)count);
+  if (he == hipSuccess) he = up(&c->d_tf, nod, (size_t)count);
+  if (he == hipSuccess) he = up(&c->d_steps, noll, (size_t)count);
+  if (he == hipSuccess) he = up(&c->d_nact, noi, (size_t)1);
+  if (he == hipSuccess) he = hipStreamCreate(&c->bstream);
+  if (he != hipSuccess) return bail(fail(RKDG_E_HIP, std::string("batch buffers: ") + hipGetErrorString(he)));
+  *out = c;
+  return RKDG_OK;
+}
+
+int rkdg_batch_offsets(rkdg_ctx* c, int64_t* elem_offsets) {
+  if (!c || !elem_offsets) return fail(RKDG_E_ARG, "null argument");
+  if (c->inst_off.empty()) {
+    elem_offsets[0] = 0;
+    elem_offsets[1] = c->NE;
+    return RKDG_OK;
+  }
+  std::copy(c->inst_off.begin(), c->inst_off.end(), elem_offsets);
+  return RKDG_OK;
+}
+
+// Start of the time loop for every instance (main.cpp:265, 467-484): limit the
+// projected state, then the first dt from the face wave speeds.
+int rkdg_batch_init(rkdg_ctx* c, double* x, const double* t0, const double* t_final, double cfl,
+                    int64_t max_steps) {
+  if (!c || !x || !t0 || !t_final) return fail(RKDG_E_ARG, "null argument");
+  if (!c->d_inst) return fail(RKDG_E_ARG, "not a batch context");
+  if (!(cfl > 0.0)) return fail(RKDG_E_ARG, "cfl must be positive");
+  c->cfl = cfl;
+  c->max_steps = max_steps > 0 ? max_steps : 0;
+  if (c->gexec) {
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->gkey.clear();
+  }
+  hipStream_t keep = c->stream;
+  c->stream = c->bstream;
+  HIP_TRY(hipStreamSynchronize(keep));
+  HIP_TRY(hipMemcpyAsync(c->d_tf, t_final, sizeof(double) * c->count, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->d_t, t0, sizeof(double) * c->count, hipMemcpyHostToDevice, c->stream));
+  int rc = limit_impl(c, x, nullptr, false);
+  if (rc == RKDG_OK) {
+    HIP_TRY(launch_fill(c->d_lam, 0.0, c->count, c->stream));
+    RhsArgs a = base_args(c, x);
+    a.face_only = 1;
+    HIP_TRY(rhs_launch(c, a));
+    BatchTime b = batch_time(c, c->d_lam);
+    HIP_TRY(launch_batch_init(b, c->stream));
+    rc = check_device_error(c, nullptr);
+  }
+  c->stream = keep;
+  return rc;
+}
+
+// Advances every active instance by up to nsteps steps (one HIP graph launch
+// per step when graphs are on); n_active (host, nullable) receives the number
+// of instances still running.
+int rkdg_batch_advance(rkdg_ctx* c, double* x, int s, const double* a, const double* b, const double* cc,
+                       int64_t nsteps, int* n_active) {
+  (void)cc;
+  if (!c || !x || !a || !b) return fail(RKDG_E_ARG, "null argument");
+  if (!c->d_inst) return fail(RKDG_E_ARG, "not a batch context");
+  if (s < 2 || s > 4) return fail(RKDG_E_ARG, "2 <= s <= 4 stages supported for batches");
+  int rc = ensure_stage_buffers(c, 4);
+  if (rc) return rc;
+  hipStream_t keep = c->stream;
+  HIP_TRY(hipStreamSynchronize(keep));
+  c->stream = c->bstream;
+  const bool prof_keep = c->prof;
+  std::vector<double> key;
+  key.push_back((double)(uintptr_t)x);
+  key.push_back(s);
+  for (int i = 0; i < s * (s - 1) / 2; i++) key.push_back(a[i]);
+  for (int i = 0; i < s; i++) key.push_back(b[i]);
+  if (c->use_graph && (!c->gexec || key != c->gkey)) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->prof = false;  // events are not recorded into the graph
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    rc = batch_step_launch(c, x, s, a, b);
+    hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+    c->prof = prof_keep;
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      c->stream = keep;
+      return rc;
+    }
+    HIP_TRY(ce);
+    hipError_t ie = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_TRY(ie);
+    c->gkey = key;
+  }
+  for (int64_t n = 0; n < nsteps; n++) {
+    if (c->use_graph) {
+      HIP_TRY(hipGraphLaunch(c->gexec, c->stream));
+    } else {
+      rc = batch_step_launch(c, x, s, a, b);
+      if (rc) {
+        c->stream = keep;
+        return rc;
+      }
+    }
+  }
+  int h = 0;
+  HIP_TRY(hipMemcpyAsync(&h, c->d_nact, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  rc = check_device_error(c, nullptr);
+  c->stream = keep;
+  if (n_active) *n_active = h;
+  return rc;
+}
+
+// Host copies of the per-instance time state; any pointer may be NULL.
+int rkdg_batch_status(rkdg_ctx* c, double* t, double* dt, int64_t* steps, int* active) {
+  if (!c) return fail(RKDG_E_ARG, "null context");
+  if (!c->d_inst) return fail(RKDG_E_ARG, "not a batch context");
+  const size_t n = c->count;
+  HIP_TRY(hipStreamSynchronize(c->bstream));
+  if (t) HIP_TRY(hipMemcpy(t, c->d_t, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (dt) HIP_TRY(hipMemcpy(dt, c->d_dt, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (steps) {
+    std::vector<long long> h(n);
+    HIP_TRY(hipMemcpy(h.data(), c->d_steps, sizeof(long long) * n, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) steps[i] = h[i];
+  }
+  if (active) HIP_TRY(hipMemcpy(active, c->d_active, sizeof(int) * n, hipMemcpyDeviceToHost));
+  return RKDG_OK;
+}
+
+// use_graph: 1 = capture a step once and replay it; grid_cap: persistent grid of
+// the right-hand-side kernel (0 = one workgroup per work block, -1 = keep)
+int rkdg_batch_set_options(rkdg_ctx* c, int use_graph, int grid_cap) {
+  if (!c) return fail(RKDG_E_ARG, "null context");
+  c->use_graph = use_graph ? 1 : 0;
+  if (grid_cap >= 0 && grid_cap != c->grid_cap) {
+    c->grid_cap = grid_cap;
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->gkey.clear();
+  }
+  return RKDG_OK;
+}
+
+// Autotuner: times the right-hand side of state x (read only) for each
+// candidate persistent grid size (0 and the given list) with HIP events and
+// keeps the fastest. Results: chosen cap and the per-candidate milliseconds.
+int rkdg_tune_grid(rkdg_ctx* c, const double* x, const int* candidates, int ncand, int reps, int* best,
+                   double* ms_out) {
+  if (!c || !x || (ncand > 0 && !candidates)) return fail(RKDG_E_ARG, "null argument");
+  if (!c->dense) return fail(RKDG_E_UNSUPPORTED, "grid tuning applies to the dense kernels");
+  int rc = ensure_stage_buffers(c, 4);
+  if (rc) return rc;
+  reps = std::max(reps, 1);
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  const int keep_cap = c->grid_cap;
+  double best_ms = 1e300;
+  int best_cap = 0;
+  for (int k = -1; k < ncand; k++) {
+    const int cap = k < 0 ? 0 : candidates[k];
+    c->grid_cap = cap;
+    RhsArgs ra = base_args(c, x);
+    ra.k = c->d_stage[3];
+    HIP_TRY(rhs_launch(c, ra));  // warm-up
+    HIP_TRY(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; r++) HIP_TRY(rhs_launch(c, ra));
+    HIP_TRY(hipEventRecord(e1, c->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    const double per = (double)ms / reps;
+    if (ms_out) ms_out[k + 1] = per;
+    if (per < best_ms) {
+      best_ms = per;
+      best_cap = cap;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->grid_cap = keep_cap;
+  rc = check_device_error(c, nullptr);
+  if (rc) return rc;
+  if (best) *best = best_cap;
+  return rkdg_batch_set_options(c, c->use_graph, best_cap);
 }
 
 }  // extern "C"

@@ -44,6 +44,15 @@ struct RhsArgs {
   double* lam;         // device max wave speed (non-negative doubles compared as int64)
   long long* err;      // device error word [4]: flag, e1, e2, local face
   int face_only;       // first-dt path: face terms and wave speeds only, no outputs
+  // batched ensembles (all null for a single instance): element -> instance,
+  // per-instance step size multiplying every stage coefficient, per-instance
+  // boundary parameters [inst][RKDG_MAX_BC][8], per-instance active flag; lam
+  // then holds one slot per instance
+  const int* inst;
+  const double* dt_inst;
+  const double* bcp;
+  const int* active;
+  int grid_cap;  // dense kernel: persistent grid of at most this many workgroups (0: one per work block)
 };
 
 struct LimitArgs {
@@ -54,6 +63,8 @@ struct LimitArgs {
   const int* fcode;
   int64_t NE;
   int8_t* troubled;
+  const int* inst;    // batched ensembles: element -> instance (nullable)
+  const int* active;  // per-instance active flag (nullable)
 };
 
 struct MeansArgs {
@@ -85,5 +96,25 @@ hipError_t launch_dense_rhs(int geom, int p, const RhsArgs& a, const SchemeParam
 hipError_t launch_dense_limit(int geom, int p, const LimitArgs& a, const SchemeParams& g, const double* t,
                               hipStream_t s);
 hipError_t launch_dense_means(int geom, int p, const MeansArgs& a, const double* t, hipStream_t s);
+
+// per-instance time control of a batched ensemble (rkdg_batch.hip)
+struct BatchTime {
+  int count;
+  double* t;
+  double* dt;           // next step size (from the last stage's wave speed)
+  double* dt_real;      // size of the step being taken: min(dt, t_final - t)
+  const double* t_final;
+  const double* hmin;
+  const double* lam;    // wave-speed slot of each instance (first dt: face speeds; after a step: last stage)
+  int* active;
+  long long* steps;
+  int* n_active;        // [1]
+  double cfl;
+  int p;
+  long long max_steps;  // 0: no step limit
+};
+hipError_t launch_batch_init(const BatchTime& b, hipStream_t s);  // t holds the start times
+hipError_t launch_batch_pre(const BatchTime& b, hipStream_t s);
+hipError_t launch_batch_post(const BatchTime& b, hipStream_t s);
 
 }  // namespace rkdg

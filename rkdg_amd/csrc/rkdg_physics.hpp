@@ -369,9 +369,10 @@ RKDG_HD double riemann(const SchemeParams& g, const double* uL, const double* uR
 // Ghost state for boundary slot bi (src/bc/boundary_integrator_*.cpp
 // computeRightState); n is the unit outward normal.
 template <int DIM>
-RKDG_HD void bc_state(const SchemeParams& g, int bi, const double* s1, const double* n, double* s2) {
+RKDG_HD void bc_state(const SchemeParams& g, int bi, const double* s1, const double* n, double* s2,
+                      const double* Pover = nullptr) {
   constexpr int NE = DIM + 2;
-  const double* P = g.bc_param[bi];
+  const double* P = Pover ? Pover : g.bc_param[bi];
   const double gm = g.gamma, R = g.gas_constant;
 #pragma unroll
   for (int i = 0; i < NE; i++) s2[i] = s1[i];
