@@ -14,7 +14,11 @@ geometry and connectivity). The CPU baseline is the oracle (a restatement of
 the reference path with its data structures) timed on this host on a smaller
 mesh of the same workload.
 
+The same line carries the batched-ensemble figure (256 nozzle instances,
+batched vs sequential solves, bit-identical check) under "ensemble".
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 96] [--no-cpu]
+                       [--no-ensemble | --ensemble-only]
 """
 from __future__ import annotations
 
@@ -66,6 +70,19 @@ def stage_bytes(neq, ndofs, ne, dim, s=3):
     return [a * vec + per_el * ne for a in arrays[:s]]
 
 
+def measured_traffic(n):
+    """HBM bytes per stage-kernel launch from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
+    when it was taken on this mesh size; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "traffic_rhs_stage_q2hex96.md")
+    if n != 96 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch ="):
+            return float(line.split("=", 1)[1])
+    return None
+
+
 def cpu_baseline(n, steps, threads):
     from oracle.refcpu import RefSolver, lib
     from rkdg_amd import mesh as M
@@ -86,6 +103,89 @@ def cpu_baseline(n, steps, threads):
     return dofs * 3 * steps / el, el, m.num_elements
 
 
+def ensemble_bench(dev, count=256, steps=20, warmup=3, seed=12345):
+    """The batched ensemble: ``count`` converging-diverging nozzles (P1 quads,
+    HLL, Michalak multiplier limiter, SSP-RK3, per-instance dynamic dt), element
+    counts log-uniform in [2000, 40000]. Every instance takes warmup + steps
+    steps; the batch (one captured HIP graph per step for all instances) and the
+    sequential solves (the scalar time loop, one instance after another, on the
+    same kernels) are timed over the last ``steps`` steps and their final states
+    compared bit for bit."""
+    import torch
+
+    from rkdg_amd import cases
+    from rkdg_amd import scheme as S
+    from rkdg_amd.solver import Ensemble, ExplicitRKLimitedSolver, FE_Evolution, Limiter
+
+    t0 = time.perf_counter()
+    css = [cases.nozzle(ne, ar, pr) for ne, ar, pr in cases.nozzle_ensemble(count, seed)]
+    tab = S.TABLEAUS["ssprk3"]
+    ens = Ensemble([c.mesh for c in css], 1, [c.scheme for c in css])
+    xyz = ens.node_coords()
+    x = torch.from_numpy(ens.pack([c.initial_state(p) for c, p in zip(css, xyz)])).to(dev)
+    ne_total = int(ens.offsets[-1])
+    dof = ens.neq * ne_total * ens.nd
+    print(f"[ensemble] {count} instances, {ne_total} elements, {dof} DOF, setup "
+          f"{time.perf_counter() - t0:.1f}s", flush=True)
+    best, ms = ens.ctx.tune_grid(x, reps=3)
+    ens.ctx.init(x, np.full(count, 1e30), css[0].cfl, max_steps=warmup + steps)
+    ens.ctx.advance(x, tab, warmup)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    left = ens.ctx.advance(x, tab, steps)
+    torch.cuda.synchronize()
+    batch_s = time.perf_counter() - t1
+    _, _, nsteps, _ = ens.ctx.status()
+    assert left == 0 and (nsteps == warmup + steps).all()
+    xb = x.cpu().numpy()
+    states = ens.unpack(xb)
+    del x
+    ens.ctx.close()
+
+    seq_s = 0.0
+    identical = True
+    for i, cs in enumerate(css):
+        f = FE_Evolution(cs.mesh, 1, cs.scheme, dense=True)
+        lim = Limiter(f)
+        sol = ExplicitRKLimitedSolver.from_order(3, lim)
+        sol.Init(f)
+        xi = torch.from_numpy(cs.initial_state(f.ctx.node_coords())).to(dev)
+        lim.update(xi)
+        hmin = f.hmin()
+        dt = cs.cfl * hmin / f.FaceMaxCharSpeed(xi) / 3
+        t = 0.0
+        for k in range(warmup + steps):
+            if k == warmup:
+                torch.cuda.synchronize()
+                ts = time.perf_counter()
+            sol.Step(xi, t, dt)
+            dt = cs.cfl * hmin / f.max_char_speed / 3
+            t += dt
+        torch.cuda.synchronize()
+        seq_s += time.perf_counter() - ts
+        identical = identical and np.array_equal(xi.cpu().numpy(), states[i])
+        f.ctx.close()
+    work = dof * 3 * steps
+    out = {
+        "instances": count,
+        "elements": ne_total,
+        "dof": dof,
+        "steps_timed": steps,
+        "batched_dof_stage_per_s": work / batch_s,
+        "sequential_dof_stage_per_s": work / seq_s,
+        "ratio": seq_s / batch_s,
+        "batched_ms_per_step": batch_s * 1e3 / steps,
+        "sequential_ms_per_step": seq_s * 1e3 / steps,
+        "bit_identical": bool(identical),
+        "grid_cap": best,
+        "tuner_ms": {str(k): v for k, v in ms.items()},
+    }
+    print(f"[ensemble] batched {out['batched_ms_per_step']:.2f} ms/step, sequential "
+          f"{out['sequential_ms_per_step']:.2f} ms/step, ratio {out['ratio']:.2f}, "
+          f"identical {identical}, grid {best}", flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +196,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=48)
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 1 GiB read + 1 GiB write elementwise kernel (3 launches), "
+                         "the reference for scaling FETCH_SIZE/WRITE_SIZE counters")
+    ap.add_argument("--ensemble-only", action="store_true")
+    ap.add_argument("--ensemble-count", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -109,6 +215,16 @@ def main():
     from rkdg_amd.capi import Context
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_bench(dev, args.ensemble_count)}), flush=True)
+        return
     n = args.n
     t0 = time.perf_counter()
     m = M.structured_hex(n, n, n)
@@ -148,13 +264,7 @@ def main():
     lim_avg_ms = lim_ms / max(1, lim_n)
     achieved = bytes_per_launch / (rhs_avg_ms * 1e-3) / 1e9
     lim_bytes = 2 * 8 * dofs + 8 * 5 * info.num_elements * 7
-    traffic = None
-    tr_file = os.path.join(ROOT, "profiles", "traffic_rhs_stage_q2hex96.json")
-    if os.path.exists(tr_file):
-        try:
-            traffic = json.load(open(tr_file)).get("bytes_per_launch")
-        except Exception:
-            traffic = None
+    traffic = measured_traffic(n)
     print(f"[bench] stage kernel avg {rhs_avg_ms:.3f} ms over {rhs_n} launches "
           f"({achieved:.0f} GB/s algorithmic), limiter avg {lim_avg_ms:.3f} ms "
           f"({lim_bytes / (lim_avg_ms * 1e-3) / 1e9:.0f} GB/s), step {ms_step:.3f} ms, "
@@ -170,6 +280,13 @@ def main():
                          f"{args.cpu_steps} SSP-RK3 steps after 1 warm-up step, {cel:.1f} s, "
                          f"OpenMP {threads} threads"}
         print(f"[bench] cpu baseline {cv:.3e} DOF-stage/s ({cel:.1f}s)", flush=True)
+
+    ens = None
+    if not args.no_ensemble:
+        del x
+        ctx.close()
+        torch.cuda.empty_cache()
+        ens = ensemble_bench(dev, args.ensemble_count)
 
     out = {
         "metric": METRIC,
@@ -204,6 +321,7 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
         "cpu_baseline": cpu,
+        "ensemble": ens,
     }
     print(json.dumps(out), flush=True)
 

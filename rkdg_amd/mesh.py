@@ -124,23 +124,27 @@ def _local_faces(geom):
 
 
 def _boundary_from_topology(geom, verts, coords, attr_fn):
-    """Faces seen once are boundary faces; attribute from attr_fn(midpoint, outward normal)."""
+    """Faces seen once are boundary faces (in element, then local-face order);
+    attribute from attr_fn(midpoint, outward normal)."""
     lf = _local_faces(geom)
-    seen = {}
-    for e in range(verts.shape[0]):
-        for f, loc in enumerate(lf):
-            key = tuple(sorted(int(verts[e, v]) for v in loc))
-            if key in seen:
-                seen[key] = None
-            else:
-                seen[key] = (e, f)
+    nvf = len(lf[0])
+    loc = np.array(lf, np.int64)                       # (nf, nvf)
+    fv = verts[:, loc].reshape(-1, nvf)                # (NE*nf, nvf), element-major
+    keys = np.sort(fv, axis=1)
+    order = np.lexsort(keys.T[::-1])
+    ks = keys[order]
+    new_group = np.ones(len(ks), bool)
+    new_group[1:] = np.any(ks[1:] != ks[:-1], axis=1)
+    gid = np.cumsum(new_group) - 1
+    cnt = np.bincount(gid)
+    single = np.empty(len(ks), bool)
+    single[order] = cnt[gid] == 1
+    once = np.flatnonzero(single)
     bv, ba = [], []
-    for key, val in seen.items():
-        if val is None:
-            continue
-        e, f = val
-        loc = lf[f]
-        pts = coords[e, list(loc)]
+    nf = len(lf)
+    for idx in once:
+        e, f = divmod(int(idx), nf)
+        pts = coords[e, list(lf[f])]
         mid = pts.mean(axis=0)
         if pts.shape[1] == 2:
             t = pts[1] - pts[0]
@@ -148,9 +152,8 @@ def _boundary_from_topology(geom, verts, coords, attr_fn):
         else:
             n = np.cross(pts[1] - pts[0], pts[3] - pts[0])
         n = n / np.linalg.norm(n)
-        bv.append([int(verts[e, v]) for v in loc])
+        bv.append(fv[idx])
         ba.append(attr_fn(mid, n))
-    nvf = len(lf[0])
     return np.array(bv, np.int64).reshape(-1, nvf), np.array(ba, np.int32)
 
 
