@@ -15,7 +15,9 @@ the reference path with its data structures) timed on this host on a smaller
 mesh of the same workload.
 
 The same line carries the batched-ensemble figure (256 nozzle instances,
-batched vs sequential solves, bit-identical check) under "ensemble".
+batched vs sequential solves, bit-identical check) under "ensemble", and the
+adjoint/forward time ratio of the discrete adjoint (Mach 3 forward-facing
+step, 2000 steps) with its finite-difference checks under "adjoint".
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 96] [--no-cpu]
                        [--no-ensemble | --ensemble-only]
@@ -186,6 +188,116 @@ def ensemble_bench(dev, count=256, steps=20, warmup=3, seed=12345):
     return out
 
 
+def adjoint_bench(dev, nsteps=2000, nx=240, ny=80, oracle_steps=200, fd_steps=1000, threads=None):
+    """The discrete adjoint on the Mach 3 forward-facing step (240x80 minus the
+    step, P1 quads, HLLC, Venkatakrishnan K=5, SSP-RK3): objective = pressure
+    integrated over the step face and over time; parameters = inflow Mach and
+    a seeded 1e-3 density perturbation field. The dt schedule of one dynamic run
+    is frozen. Times one forward solve and one gradient (sqrt(N) checkpointed
+    adjoint, its own forward pass included) over nsteps steps, and checks the
+    gradient by central differences (h = 1e-6) of the GPU forward solve over
+    fd_steps steps and of the oracle's forward solve over oracle_steps steps.
+    Beyond about 1000 steps J(M) stops being differentiable at the scale of h
+    (limiter thresholds and resets switch inside +-h: the differences change by
+    orders of magnitude with h), so differences cannot check a gradient there;
+    the full horizon is reported for the time ratio only."""
+    import torch
+
+    from rkdg_amd.adjoint import FFSProblem
+
+    P = FFSProblem(nx, ny)
+    prob = P.prob
+    dts = P.schedule(nsteps, device=dev)
+    mach = torch.tensor(3.0, dtype=torch.float64, device=dev)
+    delta = torch.from_numpy(P.perturbation()).to(dev)
+
+    def x0_of(m, d):
+        prob.set_bc(P.inlet_slot, P.inlet_params(m).cpu().numpy())
+        return prob.limit(P.initial_state(m, d))
+
+    x0 = x0_of(mach, delta)
+    prob.gradient(x0, dts[:50])  # allocations and warm-up
+    prob.objective(x0, dts[:50])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J = prob.objective(x0, dts)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Ja, x0bar, pbar = prob.gradient(x0, dts)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    # full parameter gradient through the harness (initial limiter + IC map)
+    dfd = dts[:fd_steps]
+    mr = mach.clone().requires_grad_(True)
+    dr = delta.clone().requires_grad_(True)
+    Jt = P.J(mr, dr, dfd)
+    Jt.backward()
+    gm = float(mr.grad)
+    rng = np.random.default_rng(77)
+    dvec = torch.from_numpy(rng.uniform(-1.0, 1.0, delta.numel())).to(dev)
+    gd = float((dr.grad * dvec).sum())
+    h = 1e-6
+    with torch.no_grad():
+        fdm = (float(P.J(mach + h, delta, dfd)) - float(P.J(mach - h, delta, dfd))) / (2 * h)
+        fdd = (float(P.J(mach, delta + h * dvec, dfd)) - float(P.J(mach, delta - h * dvec, dfd))) / (2 * h)
+    out = {
+        "workload": f"Mach 3 forward-facing step, {P.mesh.num_elements} P1 quads, HLLC, Venkatakrishnan K=5, "
+                    f"SSP-RK3, {nsteps} steps (frozen dt schedule)",
+        "steps": nsteps,
+        "J": J,
+        "forward_s": t_fwd,
+        "gradient_s": t_adj,
+        "adjoint_forward_ratio": t_adj / t_fwd,
+        "fd_steps": fd_steps,
+        "dJ_dMach": gm,
+        "fd_dJ_dMach": fdm,
+        "rel_err_mach": abs(gm - fdm) / abs(fdm),
+        "dJ_ddelta_dot_d": gd,
+        "fd_dJ_ddelta_dot_d": fdd,
+        "rel_err_delta": abs(gd - fdd) / abs(fdd),
+        "J_adjoint_pass_matches": abs(Ja - J) <= 1e-13 * abs(J),
+    }
+    print(f"[adjoint] forward {t_fwd:.3f}s gradient {t_adj:.3f}s ratio {t_adj / t_fwd:.2f}; "
+          f"over {fd_steps} steps dJ/dM {gm:.9e} "
+          f"(fd {fdm:.9e}, rel {out['rel_err_mach']:.2e}); dJ/ddelta.d {gd:.9e} (fd {fdd:.9e}, "
+          f"rel {out['rel_err_delta']:.2e})", flush=True)
+    if oracle_steps:
+        from oracle.refcpu import RefSolver
+        from rkdg_amd.adjoint import ffs_scheme
+
+        th = threads or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        ds = dts[:oracle_steps]
+        tab = None
+        from rkdg_amd import scheme as S
+
+        tab = S.TABLEAUS["ssprk3"]
+
+        def oracle_J(m):
+            ref = RefSolver(P.mesh, 1, ffs_scheme(m), threads=th)
+            xi = P.initial_state(torch.tensor(m, dtype=torch.float64), delta.cpu()).numpy()
+            x = ref.limit(xi)[0]
+            total = 0.0
+            for n in range(len(ds)):
+                x, _ = ref.step(x, ds[n], tab)
+                total += ds[n] * ref.patch_pressure(x, 5)
+            return total
+
+        t0 = time.perf_counter()
+        fdo = (oracle_J(3.0 + h) - oracle_J(3.0 - h)) / (2 * h)
+        t_or = time.perf_counter() - t0
+        mr = mach.clone().requires_grad_(True)
+        Jt = P.J(mr, delta, ds)
+        Jt.backward()
+        gmo = float(mr.grad)
+        out.update({"oracle_steps": oracle_steps, "oracle_fd_dJ_dMach": fdo, "dJ_dMach_at_oracle_steps": gmo,
+                    "rel_err_mach_vs_oracle_fd": abs(gmo - fdo) / abs(fdo), "oracle_fd_s": t_or,
+                    "oracle_threads": th})
+        print(f"[adjoint] oracle FD over {oracle_steps} steps: {fdo:.9e} vs adjoint {gmo:.9e} "
+              f"(rel {out['rel_err_mach_vs_oracle_fd']:.2e}, {t_or:.1f}s)", flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,6 +314,10 @@ def main():
                          "the reference for scaling FETCH_SIZE/WRITE_SIZE counters")
     ap.add_argument("--ensemble-only", action="store_true")
     ap.add_argument("--ensemble-count", type=int, default=256)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-only", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
+    ap.add_argument("--oracle-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -224,6 +340,10 @@ def main():
         del src, dst
     if args.ensemble_only:
         print(json.dumps({"ensemble": ensemble_bench(dev, args.ensemble_count)}), flush=True)
+        return
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_bench(dev, args.adjoint_steps, oracle_steps=args.oracle_steps)}),
+              flush=True)
         return
     n = args.n
     t0 = time.perf_counter()
@@ -282,11 +402,15 @@ def main():
         print(f"[bench] cpu baseline {cv:.3e} DOF-stage/s ({cel:.1f}s)", flush=True)
 
     ens = None
-    if not args.no_ensemble:
+    if not (args.no_ensemble and args.no_adjoint):
         del x
         ctx.close()
         torch.cuda.empty_cache()
+    if not args.no_ensemble:
         ens = ensemble_bench(dev, args.ensemble_count)
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_bench(dev, args.adjoint_steps, oracle_steps=0 if args.no_cpu else args.oracle_steps)
 
     out = {
         "metric": METRIC,
@@ -322,6 +446,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "ensemble": ens,
+        "adjoint": adj,
     }
     print(json.dumps(out), flush=True)
 

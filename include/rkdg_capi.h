@@ -202,6 +202,39 @@ int rkdg_batch_set_options(rkdg_ctx* ctx, int use_graph, int grid_cap);
 int rkdg_tune_grid(rkdg_ctx* ctx, const double* x, const int* candidates, int ncand, int reps, int* best,
                    double* ms);
 
+/* Discrete adjoint (dense 2D elements: triangles, and quadrilaterals on the
+ * dense kernels; boundary types slip, outlet and supersonicInlet). No
+ * counterpart in the reference, which has no adjoint: these are the backward
+ * operators of rkdg_rhs, rkdg_limit and rkdg_step, with the forward branch
+ * taken at every min/max and threshold switch. DEVICE pointers unless noted. */
+/* Replace the 8 parameters of boundary slot `slot` (HOST array). */
+int rkdg_set_bc_params(rkdg_ctx* ctx, int slot, const double* params);
+/* xbar = addend + (d rkdg_rhs / dx)^T kbar; bc_param_bar [RKDG_MAX_BC*8]
+ * accumulates the adjoint of the boundary parameters (both nullable). */
+int rkdg_rhs_adjoint(rkdg_ctx* ctx, const double* x, const double* kbar, double* xbar, const double* addend,
+                     double* bc_param_bar);
+/* xbar = (d rkdg_limit / dx)^T ubar at the pre-limit state x. */
+int rkdg_limit_adjoint(rkdg_ctx* ctx, const double* x, const double* ubar, double* xbar);
+/* *J += coef * sum over the faces of boundary slot bc_slot of the integral of
+ * the pressure of the interior trace; xbar += coef * its gradient (J, xbar
+ * nullable, both DEVICE). */
+int rkdg_objective(rkdg_ctx* ctx, const double* x, int bc_slot, double coef, double* J, double* xbar);
+/* xbar = (d rkdg_step(x, dt) / dx)^T ybar; bc_param_bar: HOST [RKDG_MAX_BC*8] (nullable). */
+int rkdg_step_adjoint(rkdg_ctx* ctx, const double* x, double dt, int s, const double* a, const double* b,
+                      const double* c, const double* ybar, double* xbar, double* bc_param_bar);
+/* nsteps steps with the HOST dt schedule dts; J (HOST, nullable) =
+ * sum_n dts[n] * P(x_{n+1}) over the faces of boundary slot obj_slot (< 0: none). */
+int rkdg_forward_solve(rkdg_ctx* ctx, double* x, int64_t nsteps, const double* dts, int s, const double* a,
+                       const double* b, const double* c, int obj_slot, double* J);
+/* Gradient of that J with respect to the initial state x0 (x0bar) and the
+ * boundary parameters (bc_param_bar, HOST [RKDG_MAX_BC*8]) for the fixed dt
+ * schedule, by sqrt(N) checkpointing: checkpoints every `every` steps (0:
+ * ceil(sqrt(nsteps))), each segment recomputed once with its stage states
+ * recorded. J: HOST; xN (nullable): final state. */
+int rkdg_adjoint_solve(rkdg_ctx* ctx, const double* x0, int64_t nsteps, const double* dts, int s,
+                       const double* a, const double* b, const double* c, int obj_slot, int64_t every,
+                       double* J, double* x0bar, double* bc_param_bar, double* xN);
+
 #ifdef __cplusplus
 }
 #endif

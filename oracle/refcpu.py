@@ -185,6 +185,42 @@ class RefSolver:
     def hmin(self):
         return lib().ref_hmin(self.h)
 
+    def patch_pressure(self, x, attr, gamma=1.4):
+        """Integral of the interior-trace pressure over the boundary faces with
+        attribute attr (2D): sum over faces and the p+1 Gauss-Legendre points of
+        the edge of w_q |edge| p(u(x_q)). The objective of the adjoint checks."""
+        m = self.mesh
+        if m.dim != 2:
+            raise ValueError("2D meshes only")
+        if not hasattr(self, "_patch"):
+            self._patch = {}
+        if attr not in self._patch:
+            tri = m.geom == 1
+            edges = [(0, 1), (1, 2), (2, 0)] if tri else [(0, 1), (1, 2), (2, 3), (3, 0)]
+            rv = np.array([[0, 0], [1, 0], [0, 1]] if tri else [[0, 0], [1, 0], [1, 1], [0, 1]], float)
+            s, w = gl_rule(self.order + 1)
+            where = {}
+            for e in range(m.num_elements):
+                for f, (a, b) in enumerate(edges):
+                    where[tuple(sorted((int(m.elem_verts[e, a]), int(m.elem_verts[e, b]))))] = (e, f)
+            rows = []
+            for k in np.flatnonzero(m.bdr_attr == attr):
+                e, f = where[tuple(sorted(int(v) for v in m.bdr_verts[k]))]
+                a, b = edges[f]
+                length = np.linalg.norm(m.elem_coords[e, b] - m.elem_coords[e, a])
+                for q in range(len(s)):
+                    xi = rv[a] + s[q] * (rv[b] - rv[a])
+                    phi, _ = shape(m.geom, self.order, xi)
+                    rows.append((e, w[q] * length, phi))
+            self._patch[attr] = rows
+        x = np.asarray(x, np.float64).reshape(self.neq, m.num_elements, self.nd)
+        total = 0.0
+        for e, wl, phi in self._patch[attr]:
+            u = x[:, e, :] @ phi
+            p = (gamma - 1.0) * (u[3] - 0.5 * (u[1] * u[1] + u[2] * u[2]) / u[0])
+            total += wl * p
+        return total
+
     def mass_inverse(self, e):
         out = np.zeros((self.nd, self.nd))
         lib().ref_element_mass_inverse(self.h, e, _dp(out))

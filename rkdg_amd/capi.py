@@ -60,6 +60,15 @@ EXPORTS = [
     "rkdg_node_coords", "rkdg_check_error", "rkdg_profile_enable", "rkdg_profile_read",
     "rkdg_create_ex", "rkdg_batch_create", "rkdg_batch_offsets", "rkdg_batch_init",
     "rkdg_batch_advance", "rkdg_batch_status", "rkdg_batch_set_options", "rkdg_tune_grid",
+    "rkdg_set_bc_params", "rkdg_rhs_adjoint", "rkdg_limit_adjoint", "rkdg_objective",
+    "rkdg_step_adjoint", "rkdg_forward_solve", "rkdg_adjoint_solve",
+]
+
+# include/rkdg_selftest.h (host-only)
+SELFTEST_EXPORTS = [
+    "rkdg_selftest_riemann", "rkdg_selftest_riemann_vjp", "rkdg_selftest_volume_flux",
+    "rkdg_selftest_volume_flux_vjp", "rkdg_selftest_bc_state", "rkdg_selftest_bc_state_vjp",
+    "rkdg_selftest_limiter_phi", "rkdg_selftest_pressure",
 ]
 
 RKDG_CREATE_DENSE = 1
@@ -103,6 +112,26 @@ def lib():
     L.rkdg_batch_status.argtypes = [P, pd, pd, C.POINTER(C.c_int64), C.POINTER(C.c_int)]
     L.rkdg_batch_set_options.argtypes = [P, C.c_int, C.c_int]
     L.rkdg_tune_grid.argtypes = [P, dp, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_int), pd]
+    L.rkdg_set_bc_params.argtypes = [P, C.c_int, pd]
+    L.rkdg_rhs_adjoint.argtypes = [P, dp, dp, dp, dp, dp]
+    L.rkdg_limit_adjoint.argtypes = [P, dp, dp, dp]
+    L.rkdg_objective.argtypes = [P, dp, C.c_int, C.c_double, dp, dp]
+    L.rkdg_step_adjoint.argtypes = [P, dp, C.c_double, C.c_int, pd, pd, pd, dp, dp, pd]
+    L.rkdg_forward_solve.argtypes = [P, dp, C.c_int64, pd, C.c_int, pd, pd, pd, C.c_int, pd]
+    L.rkdg_adjoint_solve.argtypes = [P, dp, C.c_int64, pd, C.c_int, pd, pd, pd, C.c_int, C.c_int64,
+                                     pd, dp, pd, dp]
+    SD = C.POINTER(SchemeDesc)
+    L.rkdg_selftest_riemann.restype = C.c_double
+    L.rkdg_selftest_riemann.argtypes = [SD, pd, pd, pd, pd]
+    L.rkdg_selftest_riemann_vjp.argtypes = [SD, pd, pd, pd, pd, pd, pd]
+    L.rkdg_selftest_volume_flux.argtypes = [SD, pd, pd, pd]
+    L.rkdg_selftest_volume_flux_vjp.argtypes = [SD, pd, pd, pd, pd]
+    L.rkdg_selftest_bc_state.argtypes = [SD, C.c_int, pd, pd, pd]
+    L.rkdg_selftest_bc_state_vjp.argtypes = [SD, C.c_int, pd, pd, pd, pd, pd]
+    L.rkdg_selftest_limiter_phi.restype = C.c_double
+    L.rkdg_selftest_limiter_phi.argtypes = [SD, C.c_double, C.c_double, C.c_double, pd]
+    L.rkdg_selftest_pressure.restype = C.c_double
+    L.rkdg_selftest_pressure.argtypes = [SD, pd, pd]
     _lib = L
     return L
 
@@ -263,6 +292,50 @@ class Context:
         _check(lib().rkdg_profile_read(self.h, C.byref(a), C.byref(b), C.byref(c_), C.byref(d)))
         return a.value, b.value, c_.value, d.value
 
+    # --- discrete adjoint (dense 2D contexts)
+    def set_bc_params(self, slot, params):
+        p = np.zeros(8)
+        p[:len(params)] = params
+        _check(lib().rkdg_set_bc_params(self.h, int(slot), p.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def rhs_adjoint(self, x, kbar, xbar, addend=None, pbar=None):
+        _check(lib().rkdg_rhs_adjoint(self.h, _ptr(x), _ptr(kbar), _ptr(xbar), _ptr(addend), _ptr(pbar)))
+
+    def limit_adjoint(self, x, ubar, xbar):
+        _check(lib().rkdg_limit_adjoint(self.h, _ptr(x), _ptr(ubar), _ptr(xbar)))
+
+    def objective(self, x, slot, coef, J=None, xbar=None):
+        _check(lib().rkdg_objective(self.h, _ptr(x), int(slot), float(coef), _ptr(J), _ptr(xbar)))
+
+    def step_adjoint(self, x, dt, tableau, ybar, xbar):
+        s, a, b, c = _tableau_arrays(tableau)
+        pbar = np.zeros(S.MAX_BC * 8)
+        pd = C.POINTER(C.c_double)
+        _check(lib().rkdg_step_adjoint(self.h, _ptr(x), float(dt), s, a, b, c, _ptr(ybar), _ptr(xbar),
+                                       pbar.ctypes.data_as(pd)))
+        return pbar.reshape(S.MAX_BC, 8)
+
+    def forward_solve(self, x, dts, tableau, obj_slot=-1):
+        s, a, b, c = _tableau_arrays(tableau)
+        d = np.ascontiguousarray(dts, np.float64)
+        J = C.c_double(0.0)
+        pd = C.POINTER(C.c_double)
+        _check(lib().rkdg_forward_solve(self.h, _ptr(x), len(d), d.ctypes.data_as(pd), s, a, b, c,
+                                        int(obj_slot), C.byref(J)))
+        return J.value
+
+    def adjoint_solve(self, x0, dts, tableau, obj_slot, x0bar, every=0, xN=None):
+        """Returns (J, bc_param_bar[MAX_BC, 8]); x0bar receives dJ/dx0."""
+        s, a, b, c = _tableau_arrays(tableau)
+        d = np.ascontiguousarray(dts, np.float64)
+        J = C.c_double(0.0)
+        pbar = np.zeros(S.MAX_BC * 8)
+        pd = C.POINTER(C.c_double)
+        _check(lib().rkdg_adjoint_solve(self.h, _ptr(x0), len(d), d.ctypes.data_as(pd), s, a, b, c,
+                                        int(obj_slot), int(every), C.byref(J), _ptr(x0bar),
+                                        pbar.ctypes.data_as(pd), _ptr(xN)))
+        return J.value, pbar.reshape(S.MAX_BC, 8)
+
     def check_error(self):
         info = (C.c_int64 * 2)()
         _check(lib().rkdg_check_error(self.h, info))
@@ -331,3 +404,68 @@ class BatchContext(Context):
         best = C.c_int(0)
         _check(lib().rkdg_tune_grid(self.h, _ptr(x), cand, len(candidates), int(reps), C.byref(best), ms))
         return best.value, dict(zip((0,) + tuple(candidates), ms[:]))
+
+
+class SelfTest:
+    """Host evaluation of the 2D physics and its hand-written derivatives
+    (include/rkdg_selftest.h); needs no GPU."""
+
+    def __init__(self, scheme: S.Scheme):
+        self.sd = scheme_desc(scheme)
+        self.L = lib()
+
+    @staticmethod
+    def _a(v, n=4):
+        return np.ascontiguousarray(np.asarray(v, np.float64).reshape(-1)[:n] if v is not None else np.zeros(n))
+
+    def _p(self, a):
+        return a.ctypes.data_as(C.POINTER(C.c_double))
+
+    def riemann(self, uL, uR, n):
+        uL, uR, n = self._a(uL), self._a(uR), self._a(n, 2)
+        f = np.zeros(4)
+        lam = self.L.rkdg_selftest_riemann(C.byref(self.sd), self._p(uL), self._p(uR), self._p(n), self._p(f))
+        return f, lam
+
+    def riemann_vjp(self, uL, uR, n, Fbar):
+        uL, uR, n, Fbar = self._a(uL), self._a(uR), self._a(n, 2), self._a(Fbar)
+        a, b = np.zeros(4), np.zeros(4)
+        self.L.rkdg_selftest_riemann_vjp(C.byref(self.sd), self._p(uL), self._p(uR), self._p(n), self._p(Fbar),
+                                         self._p(a), self._p(b))
+        return a, b
+
+    def volume_flux(self, u, a):
+        u, a = self._a(u), self._a(a, 2)
+        G = np.zeros(4)
+        self.L.rkdg_selftest_volume_flux(C.byref(self.sd), self._p(u), self._p(a), self._p(G))
+        return G
+
+    def volume_flux_vjp(self, u, a, Gbar):
+        u, a, Gbar = self._a(u), self._a(a, 2), self._a(Gbar)
+        ub = np.zeros(4)
+        self.L.rkdg_selftest_volume_flux_vjp(C.byref(self.sd), self._p(u), self._p(a), self._p(Gbar), self._p(ub))
+        return ub
+
+    def bc_state(self, bi, u, n):
+        u, n = self._a(u), self._a(n, 2)
+        g = np.zeros(4)
+        self.L.rkdg_selftest_bc_state(C.byref(self.sd), int(bi), self._p(u), self._p(n), self._p(g))
+        return g
+
+    def bc_state_vjp(self, bi, u, n, gbar):
+        u, n, gbar = self._a(u), self._a(n, 2), self._a(gbar)
+        ub, pb = np.zeros(4), np.zeros(8)
+        self.L.rkdg_selftest_bc_state_vjp(C.byref(self.sd), int(bi), self._p(u), self._p(n), self._p(gbar),
+                                          self._p(ub), self._p(pb))
+        return ub, pb
+
+    def limiter_phi(self, y, diff, eps2):
+        d = np.zeros(3)
+        v = self.L.rkdg_selftest_limiter_phi(C.byref(self.sd), float(y), float(diff), float(eps2), self._p(d))
+        return v, d[1], d[2]
+
+    def pressure(self, u):
+        u = self._a(u)
+        dp = np.zeros(4)
+        p = self.L.rkdg_selftest_pressure(C.byref(self.sd), self._p(u), self._p(dp))
+        return p, dp

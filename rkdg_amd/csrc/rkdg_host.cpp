@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/rkdg_capi.h"
+#include "rkdg_adjoint.hpp"
 #include "rkdg_kernels.hpp"
 
 using namespace rkdg;
@@ -180,6 +181,16 @@ struct rkdg_ctx {
   hipGraphExec_t gexec = nullptr;   // captured batch step
   std::vector<double> gkey;         // what the graph was captured for: x, tableau
   int use_graph = 1;
+  // adjoint work space (dense 2D path)
+  double* d_ameans = nullptr;     // means of a pre-limit state [neq*NE]
+  double* d_meanbar = nullptr;    // [neq*NE]
+  int64_t* d_send_to = nullptr;   // [NE]
+  double* d_send_val = nullptr;   // [NE]
+  double* d_pbar = nullptr;       // [RKDG_MAX_BC*8]
+  double* d_J = nullptr;          // [1]
+  std::vector<double*> d_adj;     // adjoint vectors: u0bar, kbar[4], tmp[2]
+  std::vector<double*> d_rec;     // recorded steps of a segment
+  std::vector<double*> d_cp;      // checkpoints
 };
 
 namespace {
@@ -208,6 +219,15 @@ void free_ctx(rkdg_ctx* c) {
   (void)hipFree(c->d_nact);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->bstream) (void)hipStreamDestroy(c->bstream);
+  (void)hipFree(c->d_ameans);
+  (void)hipFree(c->d_meanbar);
+  (void)hipFree(c->d_send_to);
+  (void)hipFree(c->d_send_val);
+  (void)hipFree(c->d_pbar);
+  (void)hipFree(c->d_J);
+  for (auto p : c->d_adj) (void)hipFree(p);
+  for (auto p : c->d_rec) (void)hipFree(p);
+  for (auto p : c->d_cp) (void)hipFree(p);
   delete c;
 }
 
@@ -927,7 +947,7 @@ int limit_impl(rkdg_ctx* c, double* x, int8_t* troubled, bool have_means, const 
     HIP_TRY(c->dense ? launch_dense_means(c->geom, c->p, ma, c->d_dtab, c->stream)
                      : launch_tensor_means(c->dim, c->p, ma, c->d_tb, c->stream));
   }
-  LimitArgs la{x, c->d_means, c->d_geo, c->d_nbr, c->d_fcode, c->NE, troubled, c->d_inst, active};
+  LimitArgs la{x, c->d_means, c->d_geo, c->d_nbr, c->d_fcode, c->NE, troubled, c->d_inst, active, nullptr};
   auto pr = prof_begin(c, c->ev_lim);
   hipError_t le = c->dense ? launch_dense_limit(c->geom, c->p, la, c->sp, c->d_dtab, c->stream)
                            : launch_tensor_limit(c->dim, c->p, la, c->sp, c->d_tb, c->stream);
@@ -996,6 +1016,177 @@ int batch_step_launch(rkdg_ctx* c, double* x, int s, const double* a, const doub
     if (rc) return rc;
   }
   HIP_TRY(launch_batch_post(bt, c->stream));
+  return RKDG_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// adjoint helpers (dense 2D path)
+
+int adjoint_supported(rkdg_ctx* c) {
+  if (!c->dense || c->dim != 2) return fail(RKDG_E_UNSUPPORTED, "the adjoint is built for the dense 2D kernels "
+                                                                "(triangles, or quadrilaterals created with RKDG_CREATE_DENSE)");
+  for (int q = 0; q < c->sp.num_bc; q++)
+    if (!adj::bc_has_vjp(c->sp.bc_type[q]))
+      return fail(RKDG_E_UNSUPPORTED, "boundary type " + std::to_string(c->sp.bc_type[q]) + " has no adjoint");
+  return RKDG_OK;
+}
+
+int alloc_vec(double** p, int64_t n) {
+  if (*p) return RKDG_OK;
+  HIP_TRY(hipMalloc(p, sizeof(double) * n));
+  return RKDG_OK;
+}
+
+int ensure_adjoint_space(rkdg_ctx* c) {
+  const int64_t NE = c->NE, sz = (int64_t)c->neq * c->ndofs;
+  if (!c->d_ameans) {
+    HIP_TRY(hipMalloc(&c->d_ameans, sizeof(double) * c->neq * NE));
+    HIP_TRY(hipMalloc(&c->d_meanbar, sizeof(double) * c->neq * NE));
+    HIP_TRY(hipMalloc(&c->d_send_to, sizeof(int64_t) * NE));
+    HIP_TRY(hipMalloc(&c->d_send_val, sizeof(double) * NE));
+    HIP_TRY(hipMalloc(&c->d_pbar, sizeof(double) * RKDG_MAX_BC * 8));
+    HIP_TRY(hipMalloc(&c->d_J, sizeof(double)));
+  }
+  while (c->d_adj.size() < 7) {
+    double* p = nullptr;
+    HIP_TRY(hipMalloc(&p, sizeof(double) * sz));
+    c->d_adj.push_back(p);
+  }
+  return RKDG_OK;
+}
+
+int rhs_adj_launch(rkdg_ctx* c, const double* x, const double* kbar, double* xbar, const double* addend,
+                   double* pbar) {
+  RhsAdjArgs a{x, kbar, xbar, addend, pbar, c->d_geo, c->d_nbr, c->d_fcode, c->NE};
+  HIP_TRY(launch_dense_rhs_adj(c->geom, c->p, a, c->sp, c->d_dtab, c->n_bdr > 0, c->stream));
+  return RKDG_OK;
+}
+
+// vbar = (dL/dv)^T ubar (+ objective of the limited state when obj_slot >= 0);
+// means: element means of v (computed here when null)
+int limit_adj_launch(rkdg_ctx* c, const double* v, const double* means, const double* ubar, double* vbar,
+                     int obj_slot, double obj_coef) {
+  if (!means) {
+    MeansArgs ma{v, c->d_ameans, c->d_geo, c->NE};
+    HIP_TRY(launch_dense_means(c->geom, c->p, ma, c->d_dtab, c->stream));
+    means = c->d_ameans;
+  }
+  LimAdjArgs a{v, means, ubar, vbar, c->d_meanbar, c->d_send_to, c->d_send_val, c->d_geo, c->d_nbr, c->d_fcode,
+               c->NE, obj_slot, obj_coef};
+  HIP_TRY(launch_dense_limit_adj(c->geom, c->p, a, c->sp, c->d_dtab, c->stream));
+  return RKDG_OK;
+}
+
+// one recorded step: the vectors of a step a reverse pass needs
+struct StepRec {
+  double* x;          // step input (limited)
+  double* v[4];       // pre-limit stage states v_1..v_s (v_s: before the final limiter)
+  double* w[4];       // limited stage inputs W_1..W_{s-1}
+  double* mean[4];    // element means of v_1..v_s
+};
+
+int rec_vectors(int s) { return 1 + s + (s - 1); }
+
+StepRec rec_at(rkdg_ctx* c, int s, size_t k) {
+  // layout per recorded step: x, v_1..v_s, w_1..w_{s-1}, then means (s blocks of neq*NE)
+  StepRec r{};
+  const size_t per = rec_vectors(s) + 1;  // + one vector-sized block holding the means
+  double** b = &c->d_rec[k * per];
+  r.x = b[0];
+  for (int i = 0; i < s; i++) r.v[i] = b[1 + i];
+  for (int i = 0; i < s - 1; i++) r.w[i] = b[1 + s + i];
+  for (int i = 0; i < s; i++) r.mean[i] = b[per - 1] + (size_t)i * c->neq * c->NE;
+  return r;
+}
+
+int ensure_records(rkdg_ctx* c, int s, int64_t steps) {
+  const size_t per = rec_vectors(s) + 1;
+  const int64_t sz = (int64_t)c->neq * c->ndofs;
+  if ((int64_t)s * c->neq * c->NE > sz) return fail(RKDG_E_UNSUPPORTED, "element means do not fit the record block");
+  while (c->d_rec.size() < per * (size_t)steps) {
+    double* p = nullptr;
+    HIP_TRY(hipMalloc(&p, sizeof(double) * sz));
+    c->d_rec.push_back(p);
+  }
+  return RKDG_OK;
+}
+
+// forward step from r.x recording the stage states; the limited output goes to xout
+int step_record(rkdg_ctx* c, const StepRec& r, double* xout, double dt, int s, const double* a, const double* b) {
+  auto A_ = [&](int i, int j) { return a[i * (i - 1) / 2 + j]; };
+  HIP_TRY(launch_fill(c->d_lam, 0.0, 8 * c->count, c->stream));
+  for (int i = 0; i < s; i++) {
+    const double* in = i == 0 ? r.x : r.w[i - 1];
+    RhsArgs ra = base_args(c, in);
+    ra.lam = c->d_lam + (int64_t)i * c->count;
+    int n = 0;
+    for (int j = i + 1; j < s; j++) {
+      ra.outs[n].out = r.v[j - 1];
+      ra.outs[n].base = i == 0 ? nullptr : r.v[j - 1];
+      ra.outs[n].coef = A_(j, i) * dt;
+      n++;
+    }
+    const bool last = i == s - 1;
+    ra.outs[n].out = r.v[s - 1];
+    ra.outs[n].base = i == 0 ? nullptr : r.v[s - 1];
+    ra.outs[n].coef = b[i] * dt;
+    n++;
+    ra.nout = n;
+    ra.means = last ? r.mean[s - 1] : r.mean[i];
+    ra.means_of = last ? n - 1 : 0;
+    HIP_TRY(rhs_launch(c, ra));
+    const double* vin = last ? r.v[s - 1] : r.v[i];
+    double* dst = last ? xout : r.w[i];
+    LimitArgs la{const_cast<double*>(vin), ra.means, c->d_geo, c->d_nbr, c->d_fcode, c->NE, nullptr, nullptr,
+                 nullptr, dst};
+    HIP_TRY(launch_dense_limit(c->geom, c->p, la, c->sp, c->d_dtab, c->stream));
+  }
+  return RKDG_OK;
+}
+
+// reverse of one recorded step: xbar = (dStep/dx)^T ybar, where ybar is the
+// adjoint of the step output plus, when obj_slot >= 0, obj_coef times the
+// objective gradient at the step output; pbar accumulates.
+int step_adjoint(rkdg_ctx* c, const StepRec& r, double dt, int s, const double* a, const double* b,
+                 const double* ybar, double* xbar, int obj_slot, double obj_coef) {
+  auto A_ = [&](int i, int j) { return a[i * (i - 1) / 2 + j]; };
+  const int64_t sz = (int64_t)c->neq * c->ndofs;
+  double* u0bar = c->d_adj[0];
+  double* kb[4] = {c->d_adj[1], c->d_adj[2], c->d_adj[3], c->d_adj[4]};
+  double* vb = c->d_adj[5];
+  double* Wb = c->d_adj[6];
+  int rc = limit_adj_launch(c, r.v[s - 1], r.mean[s - 1], ybar, vb, obj_slot, obj_coef);
+  if (rc) return rc;
+  {
+    // u0bar = vb; kbar_i = dt b_i vb
+    AxpyTargets t{};
+    t.n = 0;
+    t.y[t.n] = u0bar; t.c[t.n] = 1.0; t.set[t.n] = 1; t.n++;
+    for (int i = 0; i < s && t.n < 4; i++) { t.y[t.n] = kb[i]; t.c[t.n] = dt * b[i]; t.set[t.n] = 1; t.n++; }
+    HIP_TRY(launch_axpy_multi(vb, sz, t, c->stream));
+    if (s == 4) {
+      AxpyTargets t2{};
+      t2.n = 1; t2.y[0] = kb[3]; t2.c[0] = dt * b[3]; t2.set[0] = 1;
+      HIP_TRY(launch_axpy_multi(vb, sz, t2, c->stream));
+    }
+  }
+  for (int j = s - 1; j >= 1; j--) {
+    rc = rhs_adj_launch(c, r.w[j - 1], kb[j], Wb, nullptr, c->d_pbar);
+    if (rc) return rc;
+    rc = limit_adj_launch(c, r.v[j - 1], r.mean[j - 1], Wb, vb, -1, 0.0);
+    if (rc) return rc;
+    AxpyTargets t{};
+    t.y[t.n] = u0bar; t.c[t.n] = 1.0; t.set[t.n] = 0; t.n++;
+    for (int i = 0; i < j; i++) { t.y[t.n] = kb[i]; t.c[t.n] = dt * A_(j, i); t.set[t.n] = 0; t.n++; }
+    HIP_TRY(launch_axpy_multi(vb, sz, t, c->stream));
+  }
+  return rhs_adj_launch(c, r.x, kb[0], xbar, u0bar, c->d_pbar);
+}
+
+int objective_launch(rkdg_ctx* c, const double* x, int slot, double coef, double* J, double* xbar) {
+  ObjArgs o{x, J, xbar, coef, slot, c->d_geo, c->d_nbr, c->NE};
+  HIP_TRY(launch_dense_objective(c->geom, c->p, o, c->sp, c->d_dtab, c->stream));
   return RKDG_OK;
 }
 
@@ -1211,6 +1402,163 @@ int rkdg_profile_read(rkdg_ctx* c, double* rhs_ms, int64_t* rhs_n, double* lim_m
 int rkdg_check_error(rkdg_ctx* c, int64_t* info) {
   if (!c) return fail(RKDG_E_ARG, "null context");
   return check_device_error(c, info);
+}
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint
+
+int rkdg_set_bc_params(rkdg_ctx* c, int slot, const double* params) {
+  if (!c || !params) return fail(RKDG_E_ARG, "null argument");
+  if (slot < 0 || slot >= c->sp.num_bc) return fail(RKDG_E_ARG, "boundary slot out of range");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < 8; k++) c->sp.bc_param[slot][k] = params[k];
+  return RKDG_OK;
+}
+
+int rkdg_rhs_adjoint(rkdg_ctx* c, const double* x, const double* kbar, double* xbar, const double* addend,
+                     double* bc_param_bar) {
+  if (!c || !x || !kbar || !xbar) return fail(RKDG_E_ARG, "null argument");
+  int rc = adjoint_supported(c);
+  if (rc) return rc;
+  return rhs_adj_launch(c, x, kbar, xbar, addend, bc_param_bar);
+}
+
+int rkdg_limit_adjoint(rkdg_ctx* c, const double* x, const double* ubar, double* xbar) {
+  if (!c || !x || !ubar || !xbar) return fail(RKDG_E_ARG, "null argument");
+  int rc = adjoint_supported(c);
+  if (rc) return rc;
+  rc = ensure_adjoint_space(c);
+  if (rc) return rc;
+  return limit_adj_launch(c, x, nullptr, ubar, xbar, -1, 0.0);
+}
+
+int rkdg_objective(rkdg_ctx* c, const double* x, int bc_slot, double coef, double* J, double* xbar) {
+  if (!c || !x) return fail(RKDG_E_ARG, "null argument");
+  if (!c->dense || c->dim != 2) return fail(RKDG_E_UNSUPPORTED, "objective is built for the dense 2D kernels");
+  if (bc_slot < 0 || bc_slot >= c->sp.num_bc) return fail(RKDG_E_ARG, "boundary slot out of range");
+  return objective_launch(c, x, bc_slot, coef, J, xbar);
+}
+
+int rkdg_step_adjoint(rkdg_ctx* c, const double* x, double dt, int s, const double* a, const double* b,
+                      const double* cc, const double* ybar, double* xbar, double* bc_param_bar) {
+  (void)cc;
+  if (!c || !x || !a || !b || !ybar || !xbar) return fail(RKDG_E_ARG, "null argument");
+  if (s < 2 || s > 4) return fail(RKDG_E_ARG, "2 <= s <= 4 stages supported");
+  int rc = adjoint_supported(c);
+  if (!rc) rc = ensure_adjoint_space(c);
+  if (!rc) rc = ensure_records(c, s, 1);
+  if (!rc) rc = ensure_stage_buffers(c, 4);
+  if (rc) return rc;
+  StepRec r = rec_at(c, s, 0);
+  HIP_TRY(launch_copy(r.x, x, (int64_t)c->neq * c->ndofs, c->stream));
+  rc = step_record(c, r, c->d_stage[3], dt, s, a, b);
+  if (rc) return rc;
+  HIP_TRY(launch_fill(c->d_pbar, 0.0, RKDG_MAX_BC * 8, c->stream));
+  rc = step_adjoint(c, r, dt, s, a, b, ybar, xbar, -1, 0.0);
+  if (rc) return rc;
+  if (bc_param_bar) {
+    double h[RKDG_MAX_BC * 8];
+    HIP_TRY(hipMemcpyAsync(h, c->d_pbar, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < RKDG_MAX_BC * 8; k++) bc_param_bar[k] = h[k];
+  }
+  return check_device_error(c, nullptr);
+}
+
+int rkdg_forward_solve(rkdg_ctx* c, double* x, int64_t nsteps, const double* dts, int s, const double* a,
+                       const double* b, const double* cc, int obj_slot, double* J) {
+  if (!c || !x || !dts || !a || !b) return fail(RKDG_E_ARG, "null argument");
+  if (obj_slot >= c->sp.num_bc) return fail(RKDG_E_ARG, "boundary slot out of range");
+  if (obj_slot >= 0 && (!c->dense || c->dim != 2)) return fail(RKDG_E_UNSUPPORTED, "objective needs the dense 2D kernels");
+  int rc = ensure_adjoint_space(c);
+  if (rc) return rc;
+  HIP_TRY(launch_fill(c->d_J, 0.0, 1, c->stream));
+  for (int64_t n = 0; n < nsteps; n++) {
+    rc = rkdg_step(c, x, 0.0, dts[n], s, a, b, cc, nullptr);
+    if (rc) return rc;
+    if (obj_slot >= 0) {
+      rc = objective_launch(c, x, obj_slot, dts[n], c->d_J, nullptr);
+      if (rc) return rc;
+    }
+  }
+  double h = 0.0;
+  HIP_TRY(hipMemcpyAsync(&h, c->d_J, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  rc = check_device_error(c, nullptr);
+  if (J) *J = h;
+  return rc;
+}
+
+// J = sum_{n=1..N} dt_{n-1} P(x_n) with x_{n+1} = Step(x_n, dt_n) (dt schedule
+// given, held fixed), P = pressure integrated over the faces of boundary slot
+// obj_slot. Forward pass with a checkpoint every K steps, then per segment
+// (last first) a recomputation recording the stage states and the reverse
+// sweep through it: 2N forward steps and N adjoint steps, K recorded steps and
+// N/K checkpoints resident (K = ceil(sqrt(N)) by default).
+int rkdg_adjoint_solve(rkdg_ctx* c, const double* x0, int64_t N, const double* dts, int s, const double* a,
+                       const double* b, const double* cc, int obj_slot, int64_t every, double* J, double* x0bar,
+                       double* bc_param_bar, double* xN) {
+  if (!c || !x0 || !dts || !a || !b || !x0bar) return fail(RKDG_E_ARG, "null argument");
+  if (s < 2 || s > 4) return fail(RKDG_E_ARG, "2 <= s <= 4 stages supported");
+  if (N < 1) return fail(RKDG_E_ARG, "nsteps must be positive");
+  if (obj_slot < 0 || obj_slot >= c->sp.num_bc) return fail(RKDG_E_ARG, "boundary slot out of range");
+  int rc = adjoint_supported(c);
+  if (!rc) rc = ensure_adjoint_space(c);
+  if (!rc) rc = ensure_stage_buffers(c, 4);
+  if (rc) return rc;
+  const int64_t K = every > 0 ? every : (int64_t)std::ceil(std::sqrt((double)N));
+  const int64_t ncp = (N + K - 1) / K;
+  rc = ensure_records(c, s, K);
+  if (rc) return rc;
+  const int64_t sz = (int64_t)c->neq * c->ndofs;
+  while ((int64_t)c->d_cp.size() < ncp + 1) {
+    double* p = nullptr;
+    HIP_TRY(hipMalloc(&p, sizeof(double) * sz));
+    c->d_cp.push_back(p);
+  }
+  double* x = c->d_cp[ncp];  // running state of the forward pass
+  HIP_TRY(launch_copy(x, x0, sz, c->stream));
+  HIP_TRY(launch_fill(c->d_J, 0.0, 1, c->stream));
+  HIP_TRY(launch_fill(c->d_pbar, 0.0, RKDG_MAX_BC * 8, c->stream));
+  for (int64_t n = 0; n < N; n++) {
+    if (n % K == 0) HIP_TRY(launch_copy(c->d_cp[n / K], x, sz, c->stream));
+    rc = rkdg_step(c, x, 0.0, dts[n], s, a, b, cc, nullptr);
+    if (rc) return rc;
+    rc = objective_launch(c, x, obj_slot, dts[n], c->d_J, nullptr);
+    if (rc) return rc;
+  }
+  if (xN) HIP_TRY(launch_copy(xN, x, sz, c->stream));
+  // reverse: lam holds the adjoint of the current step output (zero at the end:
+  // the final objective term enters through the fused objective of the last step)
+  double* lam = x0bar;
+  double* lam2 = c->d_cp[ncp];
+  HIP_TRY(launch_fill(lam, 0.0, sz, c->stream));
+  for (int64_t seg = ncp - 1; seg >= 0; seg--) {
+    const int64_t start = seg * K, end = std::min(N, start + K);
+    for (int64_t n = start; n < end; n++) {
+      StepRec r = rec_at(c, s, (size_t)(n - start));
+      if (n == start) HIP_TRY(launch_copy(r.x, c->d_cp[seg], sz, c->stream));
+      double* xout = n + 1 < end ? rec_at(c, s, (size_t)(n + 1 - start)).x : c->d_stage[3];
+      rc = step_record(c, r, xout, dts[n], s, a, b);
+      if (rc) return rc;
+    }
+    for (int64_t n = end - 1; n >= start; n--) {
+      StepRec r = rec_at(c, s, (size_t)(n - start));
+      // the objective term of x_{n+1} (weight dt_n) joins the adjoint of the step output
+      rc = step_adjoint(c, r, dts[n], s, a, b, lam, lam2, obj_slot, dts[n]);
+      if (rc) return rc;
+      std::swap(lam, lam2);
+    }
+  }
+  if (lam != x0bar) HIP_TRY(launch_copy(x0bar, lam, sz, c->stream));
+  double hJ = 0.0;
+  double hp[RKDG_MAX_BC * 8];
+  HIP_TRY(hipMemcpyAsync(&hJ, c->d_J, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(hp, c->d_pbar, sizeof(hp), hipMemcpyDeviceToHost, c->stream));
+  rc = check_device_error(c, nullptr);
+  if (J) *J = hJ;
+  if (bc_param_bar)
+    for (int k = 0; k < RKDG_MAX_BC * 8; k++) bc_param_bar[k] = hp[k];
+  return rc;
 }
 
 // ---------------------------------------------------------------------------

@@ -65,6 +65,7 @@ struct LimitArgs {
   int8_t* troubled;
   const int* inst;    // batched ensembles: element -> instance (nullable)
   const int* active;  // per-instance active flag (nullable)
+  double* out;        // dense kernels: write the limited state here instead of in place (nullable)
 };
 
 struct MeansArgs {
@@ -96,6 +97,57 @@ hipError_t launch_dense_rhs(int geom, int p, const RhsArgs& a, const SchemeParam
 hipError_t launch_dense_limit(int geom, int p, const LimitArgs& a, const SchemeParams& g, const double* t,
                               hipStream_t s);
 hipError_t launch_dense_means(int geom, int p, const MeansArgs& a, const double* t, hipStream_t s);
+
+// adjoint kernels of the dense 2D path (rkdg_adjoint.hip)
+struct RhsAdjArgs {
+  const double* x;       // state the right-hand side was evaluated at
+  const double* kbar;    // adjoint of the right-hand side
+  double* xbar;          // out: addend + (dR/dx)^T kbar
+  const double* addend;  // nullable
+  double* pbar;          // [RKDG_MAX_BC][8] boundary-parameter adjoint, accumulated (nullable)
+  const double* geo;
+  const int* nbr;
+  const int* fcode;
+  int64_t NE;
+};
+struct LimAdjArgs {
+  const double* x;      // pre-limit state
+  const double* means;  // its element means [v*NE + e]
+  const double* ubar;   // adjoint of the limited state
+  double* vbar;         // out: adjoint of the pre-limit state
+  double* meanbar;      // scratch [NEQ*NE]
+  int64_t* send_to;     // scratch [NE]
+  double* send_val;     // scratch [NE]
+  const double* geo;
+  const int* nbr;
+  const int* fcode;
+  int64_t NE;
+  int obj_slot;         // >= 0: add obj_coef * d(objective)/du of the limited state to ubar first
+  double obj_coef;
+};
+struct ObjArgs {
+  const double* x;
+  double* J;     // device accumulator (nullable)
+  double* xbar;  // nullable: += coef * dJ/dx
+  double coef;
+  int slot;      // boundary slot whose faces carry the objective
+  const double* geo;
+  const int* nbr;
+  int64_t NE;
+};
+struct AxpyTargets {
+  int n;
+  double* y[4];
+  double c[4];
+  int set[4];
+};
+hipError_t launch_dense_rhs_adj(int geom, int p, const RhsAdjArgs& a, const SchemeParams& g, const double* t,
+                                bool has_bdr, hipStream_t s);
+hipError_t launch_dense_limit_adj(int geom, int p, const LimAdjArgs& a, const SchemeParams& g, const double* t,
+                                  hipStream_t s);
+hipError_t launch_dense_objective(int geom, int p, const ObjArgs& a, const SchemeParams& g, const double* t,
+                                  hipStream_t s);
+hipError_t launch_axpy_multi(const double* v, int64_t n, const AxpyTargets& t, hipStream_t s);
 
 // per-instance time control of a batched ensemble (rkdg_batch.hip)
 struct BatchTime {

@@ -257,13 +257,14 @@ def nozzle_quad(num_elements, area_ratio, length=1.0, inlet_half_height=0.5) -> 
 
 def forward_step_quad(nx=240, ny=80, length=3.0, height=1.0, step_x=0.6, step_h=0.2) -> Mesh:
     """Mach-3 forward-facing step: nx*ny quads on [0,L]x[0,H] minus [step_x,L]x[0,step_h].
-    Attributes follow the outward normal: 1 faces pointing -y (floor and step top),
-    2 +x (outlet), 3 +y (top wall), 4 -x (inlet and step front)."""
+    Attributes: 1 floor (outward normal -y), 2 outlet (+x), 3 top wall (+y),
+    4 inlet (-x), 5 step front (x = step_x, normal +x), 6 step top (y = step_h)."""
     def remove(xc, yc):
         return (xc > step_x) & (yc < step_h)
 
     m = structured_quad(nx, ny, 0.0, length, 0.0, height, remove=remove)
-    # re-label the step's front face (normal -x at x == step_x) as attribute 5
+    # re-label the step's front face (x == step_x; the domain's outward normal
+    # there points +x, into the step) as attribute 5 and its top as 6
     bv, ba = m.bdr_verts, m.bdr_attr.copy()
     vx = {}
     for e in range(m.num_elements):
@@ -272,7 +273,7 @@ def forward_step_quad(nx=240, ny=80, length=3.0, height=1.0, step_x=0.6, step_h=
     for b in range(len(ba)):
         p0 = vx[int(bv[b, 0])]
         p1 = vx[int(bv[b, 1])]
-        if ba[b] == 4 and abs(p0[0] - step_x) < 1e-12 and abs(p1[0] - step_x) < 1e-12:
+        if ba[b] == 2 and abs(p0[0] - step_x) < 1e-12 and abs(p1[0] - step_x) < 1e-12:
             ba[b] = 5
         if ba[b] == 1 and abs(p0[1] - step_h) < 1e-12 and abs(p1[1] - step_h) < 1e-12:
             ba[b] = 6
