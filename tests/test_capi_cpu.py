@@ -40,3 +40,13 @@ def test_struct_layouts_match_header(tmp_path):
     want = [ctypes.sizeof(capi.MeshDesc), ctypes.sizeof(capi.SchemeDesc), ctypes.sizeof(capi.Info),
             capi.SchemeDesc.bc_param.offset, capi.SchemeDesc.pow_nd.offset, capi.Info.affine.offset]
     assert got == want
+
+
+def test_library_exports_selftest_symbols():
+    txt = open(os.path.join(ROOT, "include", "rkdg_selftest.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = sorted(set(re.findall(r"\b(rkdg_selftest_[a-z_0-9]+)\s*\(", txt)))
+    L = ctypes.CDLL(capi.LIB_PATH)
+    assert names and set(names) == set(capi.SELFTEST_EXPORTS)
+    for n in names:
+        assert hasattr(L, n), n

@@ -440,3 +440,73 @@ def _sod_l1(nx, order=1):
 def test_sod_shock_tube_converges_to_exact_solution():
     e1, e2 = _sod_l1(50), _sod_l1(100)
     assert e1 < 0.03 and e2 < 0.7 * e1, (e1, e2)
+
+
+# ---------------------------------------------------------------- ghost states by invariants
+def _prim2(u, g=1.4):
+    rho = u[0]
+    vel = np.array(u[1:3]) / rho
+    p = (g - 1.0) * (u[3] - 0.5 * rho * vel @ vel)
+    return rho, vel, p
+
+
+@pytest.mark.parametrize("mach", [0.05, 0.3, 0.7])
+def test_subsonic_inlet_total_pressure_ghost_invariants(mach):
+    """The ghost of subsonicInletTotalPressure keeps the interior velocity and has
+    the prescribed total pressure and total temperature (isentropic relations
+    with the ghost's own Mach number), with p = rho R T."""
+    g, Rg, pTot, TTot = 1.4, 1.0, 2.3, 1.7
+    sch = S.Scheme(bcs=[S.BC(4, S.BC_SUBSONIC_INLET_TOTAL_PRESSURE, (pTot, TTot))], gas_constant=Rg)
+    rho_i, p_i = 1.1, 1.6
+    c_i = math.sqrt(g * p_i / rho_i)
+    u = np.array(conservative(rho_i, [mach * c_i * 0.8, mach * c_i * 0.6], p_i))
+    n = np.array([-1.0, 0.0])
+    gh = R.bc_state(sch, 2, 0, u, n)
+    rho, vel, p = _prim2(gh)
+    np.testing.assert_allclose(vel, np.array(u[1:3]) / u[0], rtol=1e-14)
+    T = p / (rho * Rg)
+    M2 = vel @ vel / (g * Rg * T)
+    np.testing.assert_allclose(T * (1 + 0.5 * (g - 1) * M2), TTot, rtol=1e-13)
+    np.testing.assert_allclose(p * (1 + 0.5 * (g - 1) * M2) ** (g / (g - 1)), pTot, rtol=1e-13)
+
+
+def test_fixed_pressure_outlet_ghost_invariants():
+    """The ghost of fixedPressureOutlet has the prescribed pressure and the
+    interior velocity; its specific internal energy is the reference's
+    (rhoE - |rho u|^2 / 2) / rho (boundary_integrator_open_fixed_pressure.cpp:18,
+    which subtracts |rho u|^2/2 where the kinetic energy is |rho u|^2/(2 rho))."""
+    g, pFix = 1.4, 0.55
+    sch = S.Scheme(bcs=[S.BC(2, S.BC_FIXED_PRESSURE_OUTLET, (pFix,))])
+    u = np.array(conservative(0.9, [0.7, -0.2], 0.8))
+    gh = R.bc_state(sch, 2, 0, u, np.array([1.0, 0.0]))
+    rho, vel, p = _prim2(gh)
+    rho_i, vel_i, p_i = _prim2(u)
+    np.testing.assert_allclose(p, pFix, rtol=1e-13)
+    np.testing.assert_allclose(vel, vel_i, rtol=1e-14)
+    eps_ref = (u[3] - 0.5 * (u[1] ** 2 + u[2] ** 2)) / u[0]
+    np.testing.assert_allclose(p / ((g - 1) * rho), eps_ref, rtol=1e-13)
+    # it equals the physical internal energy only at unit density
+    u1 = np.array(conservative(1.0, [0.7, -0.2], 0.8))
+    _, _, p1 = _prim2(R.bc_state(sch, 2, 0, u1, np.array([1.0, 0.0])))
+    np.testing.assert_allclose(p1, pFix, rtol=1e-13)
+
+
+def test_supersonic_inlet_ghost_is_prescribed_state():
+    sch = S.Scheme(bcs=[S.BC(4, S.BC_SUPERSONIC_INLET, (1.4, 3.0, 0.5, 0.0, 1.0))])
+    gh = R.bc_state(sch, 2, 0, np.array(conservative(1.0, [0.1, 0.0], 1.0)), np.array([-1.0, 0.0]))
+    rho, vel, p = _prim2(gh)
+    np.testing.assert_allclose([rho, vel[0], vel[1], p], [1.4, 3.0, 0.5, 1.0], rtol=1e-14)
+
+
+def test_subsonic_inlet_fixed_pressure_ghost_invariants():
+    """subsonicInletFixedPressure: ghost pressure is the prescribed one, the
+    velocity is the interior one, and the total temperature is TTot."""
+    g, Rg, pFix, TTot = 1.4, 1.0, 1.3, 1.7
+    sch = S.Scheme(bcs=[S.BC(4, S.BC_SUBSONIC_INLET_FIXED_PRESSURE, (pFix, TTot))], gas_constant=Rg)
+    u = np.array(conservative(1.2, [0.4, 0.1], 1.5))
+    gh = R.bc_state(sch, 2, 0, u, np.array([-1.0, 0.0]))
+    rho, vel, p = _prim2(gh)
+    np.testing.assert_allclose(p, pFix, rtol=1e-13)
+    np.testing.assert_allclose(vel, np.array(u[1:3]) / u[0], rtol=1e-14)
+    T = p / (rho * Rg)
+    np.testing.assert_allclose(T * (1 + 0.5 * (g - 1) * (vel @ vel) / (g * Rg * T)), TTot, rtol=1e-13)
