@@ -312,6 +312,9 @@ __global__ __launch_bounds__(256) void dense_means_kernel(const MeansArgs A, con
   for (int v = 0; v < NEQ; v++) A.means[v * A.NE + e] = s[v] / ms;
 }
 
+// Only the deciding variable's indicator is formed (density for the
+// multiplier, indicator_bj.cpp:193; the last variable for FinDiff,
+// limiter_findiff.cpp:7-8).
 template <int GEOM, int P>
 __global__ __launch_bounds__(64) void dense_limit_kernel(const LimitArgs A, const SchemeParams g,
                                                          const double* __restrict__ gtab) {
@@ -319,11 +322,10 @@ __global__ __launch_bounds__(64) void dense_limit_kernel(const LimitArgs A, cons
   constexpr int ND = C::ND, NEQ = C::NEQ, NF = C::NF, NV = C::NV;
   constexpr int T = C::T, EPB = C::EPB, BLOCK = C::BLOCK;
   constexpr int NPT = NF * 2;
-  constexpr int NIT = NPT > NV ? NPT : NV;
 
   __shared__ DenseTab<GEOM, P> tb;
   __shared__ double s_u[EPB][NEQ][ND];
-  __shared__ double s_y[EPB][NIT][NEQ];
+  __shared__ double s_y[EPB][NPT];
   __shared__ int s_flag[EPB];
 
   const int tid = threadIdx.x;
@@ -333,6 +335,7 @@ __global__ __launch_bounds__(64) void dense_limit_kernel(const LimitArgs A, cons
   const int64_t e0 = xcd_remap_d(blockIdx.x, gridDim.x) * EPB;
   const int64_t e = e0 + team;
   const bool active = e < NE && (!A.active || A.active[A.inst[e]]);
+  const int vsel = g.limiter == RKDG_LIM_FINDIFF ? NEQ - 1 : 0;
 
   stage_dense<GEOM, P>(tb, gtab);
 #pragma unroll
@@ -344,29 +347,21 @@ __global__ __launch_bounds__(64) void dense_limit_kernel(const LimitArgs A, cons
     }
   }
   if (lane == 0) s_flag[team] = 0;
-  double mean[NEQ], mI[NEQ], MI[NEQ];
+  double mean[NEQ];
 #pragma unroll
-  for (int v = 0; v < NEQ; v++) {
-    mean[v] = active ? A.means[v * NE + e] : 0.0;
-    mI[v] = 1e9;
-    MI[v] = -1e9;
-  }
+  for (int v = 0; v < NEQ; v++) mean[v] = active ? A.means[v * NE + e] : 0.0;
+  double mI = 1e9, MI = -1e9;
   const bool ind_bj = g.indicator == RKDG_IND_BJ;
   if (active && ind_bj) {
-#pragma unroll
-    for (int v = 0; v < NEQ; v++) {
-      mI[v] = mean[v] < mI[v] ? mean[v] : mI[v];
-      MI[v] = mean[v] > MI[v] ? mean[v] : MI[v];
-    }
+    const double mv = mean[vsel];
+    mI = mv < mI ? mv : mI;
+    MI = mv > MI ? mv : MI;
     for (int f = 0; f < NF; f++) {
       const int nb = A.nbr[f * NE + e];
       if (nb < 0) continue;
-#pragma unroll
-      for (int v = 0; v < NEQ; v++) {
-        const double a = A.means[v * NE + nb];
-        mI[v] = a < mI[v] ? a : mI[v];
-        MI[v] = a > MI[v] ? a : MI[v];
-      }
+      const double a = A.means[vsel * NE + nb];
+      mI = a < mI ? a : mI;
+      MI = a > MI ? a : MI;
     }
   }
   __syncthreads();
@@ -385,53 +380,40 @@ __global__ __launch_bounds__(64) void dense_limit_kernel(const LimitArgs A, cons
     for (int it = lane; it < NPT; it += T) {
       if (!active) break;
       const int f = it / 2, ip = it % 2;
-      double yv[NEQ];
-#pragma unroll
-      for (int v = 0; v < NEQ; v++) yv[v] = 1e9;
+      double yc = 1e9;
       if (A.nbr[f * NE + e] >= 0) {
+        double u = 0.0;
 #pragma unroll
-        for (int v = 0; v < NEQ; v++) {
-          double u = 0.0;
-#pragma unroll
-          for (int k = 0; k < ND; k++) u += s_u[team][v][k] * tb.PhiI[f][ip][k];
-          const double diff = u - mean[v];
-          const double y = indicator_y(diff, mean[v], mI[v], MI[v]);
-          double yc;
-          if (g.phi == RKDG_PHI_BJ) {
-            yc = y < 1.0 ? y : 1.0;
-          } else if (g.phi == RKDG_PHI_VENKAT) {
-            if (g.venkat_k > 0.0) {
-              const double d2 = diff * diff;
-              yc = (d2 * (y * y + 2.0 * y) + eps2) / (d2 * (y * y + y + 2.0) + eps2);
-            } else {
-              yc = (y * y + 2.0 * y) / (y * y + y + 2.0);
-            }
+        for (int k = 0; k < ND; k++) u += s_u[team][vsel][k] * tb.PhiI[f][ip][k];
+        const double diff = u - mean[vsel];
+        const double y = indicator_y(diff, mean[vsel], mI, MI);
+        if (g.phi == RKDG_PHI_BJ) {
+          yc = y < 1.0 ? y : 1.0;
+        } else if (g.phi == RKDG_PHI_VENKAT) {
+          if (g.venkat_k > 0.0) {
+            const double d2 = diff * diff;
+            yc = (d2 * (y * y + 2.0 * y) + eps2) / (d2 * (y * y + y + 2.0) + eps2);
           } else {
-            const double yStar = 1.5;
-            const double yRel = y / yStar;
-            yc = y < 1.0 ? y + (3.0 - 2.0 * yStar) * yRel * yRel + (yStar - 2.0) * yRel * yRel * yRel : 1.0;
+            yc = (y * y + 2.0 * y) / (y * y + y + 2.0);
           }
-          yv[v] = yc;
+        } else {
+          const double yStar = 1.5;
+          const double yRel = y / yStar;
+          yc = y < 1.0 ? y + (3.0 - 2.0 * yStar) * yRel * yRel + (yStar - 2.0) * yRel * yRel * yRel : 1.0;
         }
       }
-#pragma unroll
-      for (int v = 0; v < NEQ; v++) s_y[team][it][v] = yv[v];
+      s_y[team][it] = yc;
     }
   }
   __syncthreads();
-  double yMin[NEQ];
-#pragma unroll
-  for (int v = 0; v < NEQ; v++) {
-    double m = 1e9;
-    if (g.indicator == RKDG_IND_NOWHERE) m = 1.0;
-    else if (g.indicator == RKDG_IND_EVERYWHERE) m = 0.0;
-    else
-      for (int it = 0; it < NPT; it++) {
-        const double y = s_y[team][it][v];
-        m = y < m ? y : m;
-      }
-    yMin[v] = m;
-  }
+  double yr = 1e9;
+  if (g.indicator == RKDG_IND_NOWHERE) yr = 1.0;
+  else if (g.indicator == RKDG_IND_EVERYWHERE) yr = 0.0;
+  else
+    for (int it = 0; it < NPT; it++) {
+      const double y = s_y[team][it];
+      yr = y < yr ? y : yr;
+    }
   const int j = lane;
   const bool node_lane = active && lane < ND;
   double u[NEQ];
@@ -441,7 +423,6 @@ __global__ __launch_bounds__(64) void dense_limit_kernel(const LimitArgs A, cons
   }
   int trouble = 0;
   if (g.limiter == RKDG_LIM_MULTI) {
-    const double yr = yMin[0];
     if (yr < 1.0) trouble = 1;
     double f = yr;
     if (g.pow_nd) {
@@ -479,7 +460,7 @@ __global__ __launch_bounds__(64) void dense_limit_kernel(const LimitArgs A, cons
       }
     }
   } else if (g.limiter == RKDG_LIM_FINDIFF) {
-    if (yMin[NEQ - 1] < 0.999) {
+    if (yr < 0.999) {
       trouble = 1;
       if (node_lane) {
 #pragma unroll
@@ -496,7 +477,6 @@ __global__ __launch_bounds__(64) void dense_limit_kernel(const LimitArgs A, cons
 }
 
 // ---------------------------------------------------------------------------
-
 bool dense_supported(int geom, int p) {
 #define X(G, PP) \
   if (geom == G && p == PP) return true;

@@ -1309,6 +1309,7 @@ int rkdg_step(rkdg_ctx* c, double* x, double t, double dt, int s, const double* 
       n++;
     }
     const bool last = i == s - 1;
+    ra.no_lambda = (!last || !lambda_max) ? 1 : 0;  // only the last stage's wave speed is read
     ra.outs[n].out = last ? (s == 1 ? acc : x) : acc;
     ra.outs[n].base = i == 0 ? nullptr : acc;
     ra.outs[n].coef = b[i] * dt;

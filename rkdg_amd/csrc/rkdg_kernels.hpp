@@ -53,6 +53,7 @@ struct RhsArgs {
   const double* bcp;
   const int* active;
   int grid_cap;  // dense kernel: persistent grid of at most this many workgroups (0: one per work block)
+  int no_lambda; // 1: the wave speed of this launch is not needed (tensor kernel skips the nodal one)
 };
 
 struct LimitArgs {

@@ -221,7 +221,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
 #pragma unroll
       for (int v = 0; v < NEQ; v++) st[v] = s_u[team][v][j];
       const SidePrim<DIM> q = side_prim<DIM>(st, g);
-      lmax = sqrt(q.m2) * q.inv + side_sound<DIM>(st, q, g);
+      if (!A.no_lambda) lmax = sqrt(q.m2) * q.inv + side_sound<DIM>(st, q, g);
       double wj = tb.w[jj[0]] * tb.w[jj[1]];
       if (DIM == 3) wj *= tb.w[jj[2]];
       // contravariant fluxes G^r = w_j adj(J)(r,:) . F(u): with m.a_r = sum_d a_rd m_d,
@@ -265,34 +265,34 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
   bool bad = false;
 #pragma unroll
   for (int r = 0; r < DIM; r++) {
-    // own normal-direction traces on faces (r,0), (r,1); neighbour traces in
-    // the neighbour's own face frame, read from global memory
-    for (int it = lane; it < 4 * NFN; it += T) {
+    // own normal-direction traces on faces (r,0), (r,1) and the neighbour's
+    // traces in its own face frame (read from global memory), one face node of
+    // both per lane
+    for (int it = lane; it < 2 * NFN; it += T) {
       if (!active) break;
-      const int nbside = it >= 2 * NFN;
-      const int it2 = nbside ? it - 2 * NFN : it;
-      const int c = it2 / NFN, fn = it2 - c * NFN;
+      const int c = it / NFN, fn = it - c * NFN;
       const int a = fn % N, b = fn / N;
-      if (!nbside) {
-#pragma unroll
-        for (int v = 0; v < NEQ; v++) {
-          double t = 0.0;
-#pragma unroll
-          for (int k = 0; k < N; k++) t += s_u[team][v][vnode<DIM, N>(r, k, a, b)] * tb.E[c][k];
-          s_tro[team][c][v][fn] = t;
-        }
-      } else {
-        const int lf = lf_of<DIM>(r, c);
-        const int nb = s_nb[team][lf];
-        if (nb < 0) continue;
-        int rn, cn;
+      const int lf = lf_of<DIM>(r, c);
+      const int nb = s_nb[team][lf];
+      double vals[NEQ][N];
+      int cn = 0;
+      if (nb >= 0) {
+        int rn;
         rc_of<DIM>(s_code[team][lf] & 7, rn, cn);
         const double* xn = A.x + (int64_t)nb * ND;
-        double vals[NEQ][N];
 #pragma unroll
         for (int v = 0; v < NEQ; v++)
 #pragma unroll
           for (int k = 0; k < N; k++) vals[v][k] = xn[v * NDOFS + vnode<DIM, N>(rn, k, a, b)];
+      }
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; k++) t += s_u[team][v][vnode<DIM, N>(r, k, a, b)] * tb.E[c][k];
+        s_tro[team][c][v][fn] = t;
+      }
+      if (nb >= 0) {
 #pragma unroll
         for (int v = 0; v < NEQ; v++) {
           double t = 0.0;
@@ -485,7 +485,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
     const double o = __shfl_xor(lmax, off, 64);
     lmax = o > lmax ? o : lmax;
   }
-  if (tid == 0) atomic_max_nonneg(A.lam, lmax);
+  if (tid == 0 && !A.no_lambda) atomic_max_nonneg(A.lam, lmax);
 }
 
 // ---------------------------------------------------------------------------
@@ -521,7 +521,10 @@ __global__ __launch_bounds__(256) void tensor_means_kernel(const MeansArgs A,
 // Limiter::update for one element per team: stencil bounds of the means over
 // the face neighbours, the indicator at two Gauss points per face direction of
 // every interior face, the limiter function, then LimiterMultiplier (with the
-// last-hope reset at the reference vertices) or LimiterFinDiff.
+// last-hope reset at the reference vertices) or LimiterFinDiff. Only the
+// indicator of the variable that decides is formed: the multiplier uses the
+// density's (minValues = yMin[rho], indicator_bj.cpp:193), FinDiff the last
+// variable's (limiter_findiff.cpp:7-8).
 template <int DIM, int P>
 __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, const SchemeParams g,
                                                           const TensorTables* __restrict__ gtb) {
@@ -529,12 +532,12 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
   constexpr int N = C::N, ND = C::ND, NEQ = C::NEQ, NFN = C::NFN, NF = C::NF, NIP = C::NIP;
   constexpr int T = C::T, EPB = C::EPB, BLOCK = C::BLOCK;
   constexpr int NPT = NF * NIP;  // indicator points per element
-  constexpr int NIT = NPT > C::NVERT ? NPT : C::NVERT;
+  constexpr int NVERT = C::NVERT;
 
   __shared__ TensorTables tb;
   __shared__ double s_u[EPB][NEQ][ND];
-  __shared__ double s_tr[EPB][NF][NEQ][NFN];
-  __shared__ double s_y[EPB][NIT][NEQ];
+  __shared__ double s_tr[EPB][NF][NFN];   // traces of the deciding variable
+  __shared__ double s_y[EPB][NPT];
   __shared__ int s_flag[EPB];
 
   const int tid = threadIdx.x;
@@ -544,6 +547,11 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
   const int64_t e0 = xcd_remap(blockIdx.x, gridDim.x) * EPB;
   const int64_t e = e0 + team;
   const bool active = e < NE;
+  if (g.limiter == RKDG_LIM_NONE) {
+    if (A.troubled && active && lane == 0) A.troubled[e] = 0;
+    return;
+  }
+  const int vsel = g.limiter == RKDG_LIM_FINDIFF ? NEQ - 1 : 0;
 
   stage_tables(tb, gtb);
 #pragma unroll
@@ -555,50 +563,37 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
     }
   }
   if (lane == 0) s_flag[team] = 0;
-  double mean[NEQ], mI[NEQ], MI[NEQ];
+  double mean[NEQ];
 #pragma unroll
-  for (int v = 0; v < NEQ; v++) {
-    mean[v] = active ? A.means[v * NE + e] : 0.0;
-    mI[v] = 1e9;
-    MI[v] = -1e9;
-  }
+  for (int v = 0; v < NEQ; v++) mean[v] = active ? A.means[v * NE + e] : 0.0;
+  double mI = 1e9, MI = -1e9;
+  const bool ind_bj = g.indicator == RKDG_IND_BJ;
   __syncthreads();
 
-  const bool ind_bj = g.indicator == RKDG_IND_BJ;
-  const bool hope = g.limiter == RKDG_LIM_MULTI && g.last_hope;
-  // normal-direction traces on all faces of the element (own frame): the
-  // indicator points and the reference vertices lie on them
-  if (ind_bj || hope) {
+  // normal-direction traces of the deciding variable on every face
+  if (ind_bj) {
     for (int it = lane; it < NF * NFN; it += T) {
       if (!active) break;
       const int f = it / NFN, fn = it - f * NFN;
       const int r = f / 2, c = f % 2;
       const int a = fn % N, b = fn / N;
+      double t = 0.0;
 #pragma unroll
-      for (int v = 0; v < NEQ; v++) {
-        double t = 0.0;
-#pragma unroll
-        for (int k = 0; k < N; k++) t += s_u[team][v][vnode<DIM, N>(r, k, a, b)] * tb.E[c][k];
-        s_tr[team][f][v][fn] = t;
-      }
+      for (int k = 0; k < N; k++) t += s_u[team][vsel][vnode<DIM, N>(r, k, a, b)] * tb.E[c][k];
+      s_tr[team][f][fn] = t;
     }
-  }
-  // stencil bounds (cell + face neighbours), in every lane
-  if (active && ind_bj) {
+    // stencil bounds (cell + face neighbours) of the deciding variable's means
+    if (active) {
+      const double mv = mean[vsel];
+      mI = mv < mI ? mv : mI;
+      MI = mv > MI ? mv : MI;
 #pragma unroll
-    for (int v = 0; v < NEQ; v++) {
-      mI[v] = mean[v] < mI[v] ? mean[v] : mI[v];
-      MI[v] = mean[v] > MI[v] ? mean[v] : MI[v];
-    }
-#pragma unroll
-    for (int lf = 0; lf < NF; lf++) {
-      const int nb = A.nbr[lf * NE + e];
-      if (nb < 0) continue;
-#pragma unroll
-      for (int v = 0; v < NEQ; v++) {
-        const double a = A.means[v * NE + nb];
-        mI[v] = a < mI[v] ? a : mI[v];
-        MI[v] = a > MI[v] ? a : MI[v];
+      for (int lf = 0; lf < NF; lf++) {
+        const int nb = A.nbr[lf * NE + e];
+        if (nb < 0) continue;
+        const double a = A.means[vsel * NE + nb];
+        mI = a < mI ? a : mI;
+        MI = a > MI ? a : MI;
       }
     }
   }
@@ -620,66 +615,52 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
       const int f = it / NIP, ip = it - f * NIP;
       const int r = f / 2, c = f % 2;
       const int lf = lf_of<DIM>(r, c);
-      double yv[NEQ];
-#pragma unroll
-      for (int v = 0; v < NEQ; v++) yv[v] = 1e9;
+      double yc = 1e9;
       if (A.nbr[lf * NE + e] >= 0) {
         const int pi = ip % 2, pj = ip / 2;
+        double u = 0.0;
+        if (DIM == 3) {
 #pragma unroll
-        for (int v = 0; v < NEQ; v++) {
-          double u = 0.0;
-          if (DIM == 3) {
+          for (int b = 0; b < N; b++) {
+            double tt = 0.0;
 #pragma unroll
-            for (int b = 0; b < N; b++) {
-              double tt = 0.0;
-#pragma unroll
-              for (int a = 0; a < N; a++) tt += s_tr[team][f][v][a + N * b] * tb.BI[pi][a];
-              u += tt * tb.BI[pj][b];
-            }
-          } else {
-#pragma unroll
-            for (int a = 0; a < N; a++) u += s_tr[team][f][v][a] * tb.BI[pi][a];
+            for (int a = 0; a < N; a++) tt += s_tr[team][f][a + N * b] * tb.BI[pi][a];
+            u += tt * tb.BI[pj][b];
           }
-          const double diff = u - mean[v];
-          const double y = indicator_y(diff, mean[v], mI[v], MI[v]);
-          double yc;
-          if (g.phi == RKDG_PHI_BJ) {
-            yc = y < 1.0 ? y : 1.0;
-          } else if (g.phi == RKDG_PHI_VENKAT) {
-            if (g.venkat_k > 0.0) {
-              const double d2 = diff * diff;
-              yc = (d2 * (y * y + 2.0 * y) + eps2) / (d2 * (y * y + y + 2.0) + eps2);
-            } else {
-              yc = (y * y + 2.0 * y) / (y * y + y + 2.0);
-            }
+        } else {
+#pragma unroll
+          for (int a = 0; a < N; a++) u += s_tr[team][f][a] * tb.BI[pi][a];
+        }
+        const double diff = u - mean[vsel];
+        const double y = indicator_y(diff, mean[vsel], mI, MI);
+        if (g.phi == RKDG_PHI_BJ) {
+          yc = y < 1.0 ? y : 1.0;
+        } else if (g.phi == RKDG_PHI_VENKAT) {
+          if (g.venkat_k > 0.0) {
+            const double d2 = diff * diff;
+            yc = (d2 * (y * y + 2.0 * y) + eps2) / (d2 * (y * y + y + 2.0) + eps2);
           } else {
-            const double yStar = 1.5;
-            const double yRel = y / yStar;
-            yc = y < 1.0 ? y + (3.0 - 2.0 * yStar) * yRel * yRel + (yStar - 2.0) * yRel * yRel * yRel : 1.0;
+            yc = (y * y + 2.0 * y) / (y * y + y + 2.0);
           }
-          yv[v] = yc;
+        } else {
+          const double yStar = 1.5;
+          const double yRel = y / yStar;
+          yc = y < 1.0 ? y + (3.0 - 2.0 * yStar) * yRel * yRel + (yStar - 2.0) * yRel * yRel * yRel : 1.0;
         }
       }
-#pragma unroll
-      for (int v = 0; v < NEQ; v++) s_y[team][it][v] = yv[v];
+      s_y[team][it] = yc;
     }
   }
   __syncthreads();
 
-  double yMin[NEQ];
-#pragma unroll
-  for (int v = 0; v < NEQ; v++) {
-    double m = 1e9;
-    if (g.indicator == RKDG_IND_NOWHERE) m = 1.0;
-    else if (g.indicator == RKDG_IND_EVERYWHERE) m = 0.0;
-    else {
-      for (int it = 0; it < NPT; it++) {
-        const double y = s_y[team][it][v];
-        m = y < m ? y : m;
-      }
+  double yr = 1e9;
+  if (g.indicator == RKDG_IND_NOWHERE) yr = 1.0;
+  else if (g.indicator == RKDG_IND_EVERYWHERE) yr = 0.0;
+  else
+    for (int it = 0; it < NPT; it++) {
+      const double y = s_y[team][it];
+      yr = y < yr ? y : yr;
     }
-    yMin[v] = m;
-  }
 
   // limit: each node lane owns one node
   const int j = lane;
@@ -691,7 +672,6 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
   }
   int trouble = 0;
   if (g.limiter == RKDG_LIM_MULTI) {
-    const double yr = yMin[0];
     if (yr < 1.0) trouble = 1;
     // u <- mean + y (u - mean) applied nd times (limiter_multiplier.cpp:24-38)
     // is u <- mean + y^nd (u - mean); y^nd by repeated squaring
@@ -709,22 +689,26 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
       for (int v = 0; v < NEQ; v++) u[v] = mean[v] + f * (u[v] - mean[v]);
     }
     if (g.last_hope) {
-      // the limited state at the reference vertices: mean + f (u(vertex) - mean),
-      // u(vertex) from the x-face traces (the nodal basis sums to one)
-      for (int it = lane; it < C::NVERT; it += T) {
+      // the limited state at the reference vertices: mean + f (u(vertex) - mean);
+      // u(vertex) = sum over nodes of E-products (the nodal basis sums to one)
+      for (int it = lane; it < NVERT; it += T) {
         if (!active) break;
         const int cx = it & 1, cy = (it >> 1) & 1, cz = (it >> 2) & 1;
-        const int fx = cx;  // own face (r = 0, c = cx)
         double vs[NEQ];
 #pragma unroll
         for (int v = 0; v < NEQ; v++) {
           double s = 0.0;
 #pragma unroll
-          for (int b = 0; b < (DIM == 3 ? N : 1); b++) {
-            double tt = 0.0;
+          for (int kz = 0; kz < (DIM == 3 ? N : 1); kz++) {
+            double sy = 0.0;
 #pragma unroll
-            for (int a = 0; a < N; a++) tt += s_tr[team][fx][v][a + N * b] * tb.E[cy][a];
-            s += DIM == 3 ? tt * tb.E[cz][b] : tt;
+            for (int ky = 0; ky < N; ky++) {
+              double sx = 0.0;
+#pragma unroll
+              for (int kx = 0; kx < N; kx++) sx += s_u[team][v][kx + N * (ky + N * kz)] * tb.E[cx][kx];
+              sy += sx * tb.E[cy][ky];
+            }
+            s += DIM == 3 ? sy * tb.E[cz][kz] : sy;
           }
           vs[v] = mean[v] + f * (s - mean[v]);
         }
@@ -740,7 +724,7 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
       }
     }
   } else if (g.limiter == RKDG_LIM_FINDIFF) {
-    if (yMin[NEQ - 1] < 0.999) {
+    if (yr < 0.999) {
       trouble = 1;
       if (node_lane) {
 #pragma unroll
@@ -748,7 +732,7 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
       }
     }
   }
-  if (node_lane && g.limiter != RKDG_LIM_NONE) {
+  if (node_lane) {
 #pragma unroll
     for (int v = 0; v < NEQ; v++) A.x[v * NDOFS + e * ND + j] = u[v];
   }
