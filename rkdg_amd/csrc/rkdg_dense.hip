@@ -271,11 +271,9 @@ __device__ __forceinline__ void dense_rhs_block(const RhsArgs& A, const SchemePa
       lmax = o > lmax ? o : lmax;
     }
     const int i0 = A.inst && e0 < NE ? A.inst[e0] : 0;
-    if (tid == 0 && lmax > 0.0)
-      atomicMax(reinterpret_cast<unsigned long long*>(A.lam + (A.inst ? i0 : 0)),
-                (unsigned long long)__double_as_longlong(lmax));
-  } else if (lane == 0 && active && lmax > 0.0) {
-    atomicMax(reinterpret_cast<unsigned long long*>(A.lam + ins), (unsigned long long)__double_as_longlong(lmax));
+    if (tid == 0) atomic_max_pos(A.lam + (A.inst ? i0 : 0), lmax);
+  } else if (lane == 0 && active) {
+    atomic_max_pos(A.lam + ins, lmax);
   }
   __syncthreads();  // LDS is reused by the next work block
 }

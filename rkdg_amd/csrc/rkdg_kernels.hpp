@@ -76,6 +76,18 @@ struct MeansArgs {
   int64_t NE;
 };
 
+// Max-accumulate a non-negative double into *addr (non-negative doubles order
+// like their bit patterns). The current value is read first and only a larger
+// value issues the atomic: one same-address atomic per workgroup over a whole
+// grid would serialise in L2 and cost milliseconds.
+__device__ __forceinline__ void atomic_max_pos(double* addr, double v) {
+  if (!(v > 0.0)) return;
+  unsigned long long* p = reinterpret_cast<unsigned long long*>(addr);
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+  const unsigned long long cur = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (bits > cur) atomicMax(p, bits);
+}
+
 // launchers (rkdg_tensor.hip)
 // t: device copy of the tables; has_bdr selects the kernel with boundary faces
 hipError_t launch_tensor_rhs(int dim, int p, const RhsArgs& a, const SchemeParams& g,

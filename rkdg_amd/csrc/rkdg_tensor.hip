@@ -96,10 +96,6 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nb) {
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + l;
 }
 
-__device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
-  if (!(v > 0.0)) return;
-  atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
-}
 
 __device__ __forceinline__ void flag_error(long long* err, int64_t e1, int64_t e2, int lf) {
   if (atomicCAS(reinterpret_cast<unsigned long long*>(err), 0ull, 1ull) == 0ull) {
@@ -485,7 +481,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
     const double o = __shfl_xor(lmax, off, 64);
     lmax = o > lmax ? o : lmax;
   }
-  if (tid == 0 && !A.no_lambda) atomic_max_nonneg(A.lam, lmax);
+  if (tid == 0 && !A.no_lambda) atomic_max_pos(A.lam, lmax);
 }
 
 // ---------------------------------------------------------------------------
