@@ -292,6 +292,19 @@ class Context:
         _check(lib().rkdg_profile_read(self.h, C.byref(a), C.byref(b), C.byref(c_), C.byref(d)))
         return a.value, b.value, c_.value, d.value
 
+    def tune_grid(self, x, candidates=(256, 512, 1024, 2048, 4096, 8192), reps=5):
+        """Autotuner (rkdg_tune_grid): times the right-hand side of x for the
+        persistent grid sizes 0 (one workgroup per work block) and candidates,
+        keeps the fastest; returns (chosen cap, {cap: ms per launch})."""
+        cand = (C.c_int * len(candidates))(*candidates)
+        ms = (C.c_double * (len(candidates) + 1))()
+        best = C.c_int(0)
+        _check(lib().rkdg_tune_grid(self.h, _ptr(x), cand, len(candidates), int(reps), C.byref(best), ms))
+        return best.value, dict(zip((0,) + tuple(candidates), ms[:]))
+
+    def set_grid_cap(self, cap):
+        _check(lib().rkdg_batch_set_options(self.h, 1, int(cap)))
+
     # --- discrete adjoint (dense 2D contexts)
     def set_bc_params(self, slot, params):
         p = np.zeros(8)
@@ -397,13 +410,6 @@ class BatchContext(Context):
     def set_options(self, use_graph=True, grid_cap=-1):
         _check(lib().rkdg_batch_set_options(self.h, 1 if use_graph else 0, int(grid_cap)))
 
-    def tune_grid(self, x, candidates=(256, 512, 1024, 2048, 4096, 8192), reps=5):
-        """Autotuner: returns (chosen grid cap, {cap: ms per launch})."""
-        cand = (C.c_int * len(candidates))(*candidates)
-        ms = (C.c_double * (len(candidates) + 1))()
-        best = C.c_int(0)
-        _check(lib().rkdg_tune_grid(self.h, _ptr(x), cand, len(candidates), int(reps), C.byref(best), ms))
-        return best.value, dict(zip((0,) + tuple(candidates), ms[:]))
 
 
 class SelfTest:

@@ -280,12 +280,16 @@ __device__ __forceinline__ void dense_rhs_block(const RhsArgs& A, const SchemePa
 
 // persistent: the grid strides over the work blocks
 template <int GEOM, int P, int RS, bool BC>
-__global__ __launch_bounds__(64) void dense_rhs_kernel(const RhsArgs A, const SchemeParams g,
+__global__ __launch_bounds__(64, 4) void dense_rhs_kernel(const RhsArgs A, const SchemeParams g,
                                                        const double* __restrict__ gtab, int64_t nvb) {
   __shared__ DenseTab<GEOM, P> tb;
   stage_dense<GEOM, P>(tb, gtab);
   __syncthreads();
-  for (int64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) dense_rhs_block<GEOM, P, RS, BC>(A, g, vb, tb);
+  for (int64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+    // keep table reads inside the iteration (hoisted, they would hold registers across the loop)
+    asm volatile("" ::: "memory");
+    dense_rhs_block<GEOM, P, RS, BC>(A, g, vb, tb);
+  }
 }
 
 template <int GEOM, int P>
@@ -314,7 +318,7 @@ __global__ __launch_bounds__(256) void dense_means_kernel(const MeansArgs A, con
 // multiplier, indicator_bj.cpp:193; the last variable for FinDiff,
 // limiter_findiff.cpp:7-8).
 template <int GEOM, int P>
-__global__ __launch_bounds__(64) void dense_limit_kernel(const LimitArgs A, const SchemeParams g,
+__global__ __launch_bounds__(64, 4) void dense_limit_kernel(const LimitArgs A, const SchemeParams g,
                                                          const double* __restrict__ gtab) {
   using C = DC<GEOM, P>;
   constexpr int ND = C::ND, NEQ = C::NEQ, NF = C::NF, NV = C::NV;

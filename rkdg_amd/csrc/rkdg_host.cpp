@@ -1805,7 +1805,6 @@ int rkdg_batch_set_options(rkdg_ctx* c, int use_graph, int grid_cap) {
 int rkdg_tune_grid(rkdg_ctx* c, const double* x, const int* candidates, int ncand, int reps, int* best,
                    double* ms_out) {
   if (!c || !x || (ncand > 0 && !candidates)) return fail(RKDG_E_ARG, "null argument");
-  if (!c->dense) return fail(RKDG_E_UNSUPPORTED, "grid tuning applies to the dense kernels");
   int rc = ensure_stage_buffers(c, 4);
   if (rc) return rc;
   reps = std::max(reps, 1);
