@@ -243,9 +243,10 @@ RKDG_HD bool side_ok(const double* s, const SidePrim<DIM>& q) {
   return (s[0] >= 0.0) && (s[1 + DIM] > 0.0) && (q.pid > 0.0);
 }
 
+// inm: 0.5 / |n| when the caller has it per face (Rusanov only; 0 = form it here)
 template <int DIM, int RS>
 RKDG_HD double riemann_t(const SchemeParams& g, const double* uL, const double* uR, const double* n,
-                         double* flux, bool& ok) {
+                         double* flux, bool& ok, double inm_face = 0.0) {
   constexpr int NE = DIM + 2;
   double s1[NE], s2[NE], f1[NE], f2[NE];
 #pragma unroll
@@ -277,7 +278,7 @@ RKDG_HD double riemann_t(const SchemeParams& g, const double* uL, const double* 
     f1[1 + DIM] = a1 * (s1[1 + DIM] + q1.p);
     f2[1 + DIM] = a2 * (s2[1 + DIM] + q2.p);
     // the unit normal's own norm (rs_rusanov.cpp:27-32); 1 up to rounding
-    const double inm = 0.5 * recip(sqrt(nm));
+    const double inm = inm_face != 0.0 ? inm_face : 0.5 * recip(sqrt(nm));
     const double hm = 0.5 * mE;
 #pragma unroll
     for (int i = 0; i < NE; i++) flux[i] = (f1[i] + f2[i]) * inm - hm * (s2[i] - s1[i]);

@@ -105,11 +105,36 @@ __device__ __forceinline__ void flag_error(long long* err, int64_t e1, int64_t e
   }
 }
 
-__device__ __forceinline__ void stage_tables(TensorTables& dst, const TensorTables* src) {
-  constexpr int n = sizeof(TensorTables) / sizeof(double);
-  const double* s = reinterpret_cast<const double*>(src);
-  double* d = reinterpret_cast<double*>(&dst);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) d[i] = s[i];
+// the part of TensorTables an element of order N-1 uses, packed for LDS
+template <int N, int NQ>
+struct TT {
+  double w[N], invw[N];
+  double D[N][N];
+  double E[2][N];
+  double B[NQ][N];
+  double wq[NQ];
+  double BI[2][N];
+};
+
+template <int N, int NQ>
+__device__ __forceinline__ void stage_tables(TT<N, NQ>& d, const TensorTables* s) {
+  constexpr int n = sizeof(TT<N, NQ>) / sizeof(double);
+  double* o = reinterpret_cast<double*>(&d);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double v;
+    int k = i;
+    if (k < N) v = s->w[k];
+    else if ((k -= N) < N) v = s->invw[k];
+    else if ((k -= N) < N * N) v = s->D[k / N][k % N];
+    else if ((k -= N * N) < 2 * N) v = s->E[k / N][k % N];
+    else if ((k -= 2 * N) < NQ * N) v = s->B[k / N][k % N];
+    else if ((k -= NQ * N) < NQ) v = s->wq[k];
+    else {
+      k -= NQ;
+      v = s->BI[k / N][k % N];
+    }
+    o[i] = v;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -122,7 +147,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
   constexpr int GD = DIM * DIM + 1;
   constexpr int WORK = (NEQ + 1) * ND > 2 * NEQ * NFP ? (NEQ + 1) * ND : 2 * NEQ * NFP;
 
-  __shared__ TensorTables tb;
+  __shared__ TT<C::N, C::NQ> tb;
   __shared__ double s_u[EPB][NEQ][ND];
   __shared__ double s_trn[EPB][2][NEQ][NFN];   // neighbour traces, faces (r,0), (r,1)
   __shared__ double s_tro[EPB][2][NEQ][NFN];
@@ -530,9 +555,10 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
   constexpr int NPT = NF * NIP;  // indicator points per element
   constexpr int NVERT = C::NVERT;
 
-  __shared__ TensorTables tb;
+  __shared__ TT<C::N, C::NQ> tb;
   __shared__ double s_u[EPB][NEQ][ND];
   __shared__ double s_tr[EPB][NF][NFN];   // traces of the deciding variable
+  __shared__ double s_tx[EPB][2][NEQ][NFN]; // traces of every variable on the two x faces (last hope)
   __shared__ double s_y[EPB][NPT];
   __shared__ int s_flag[EPB];
 
@@ -564,6 +590,22 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
   for (int v = 0; v < NEQ; v++) mean[v] = active ? A.means[v * NE + e] : 0.0;
   double mI = 1e9, MI = -1e9;
   const bool ind_bj = g.indicator == RKDG_IND_BJ;
+  if (ind_bj) {
+    // stencil bounds (cell + face neighbours) of the deciding variable's means
+    if (active) {
+      const double mv = mean[vsel];
+      mI = mv < mI ? mv : mI;
+      MI = mv > MI ? mv : MI;
+#pragma unroll
+      for (int lf = 0; lf < NF; lf++) {
+        const int nb = A.nbr[lf * NE + e];
+        if (nb < 0) continue;
+        const double a = A.means[vsel * NE + nb];
+        mI = a < mI ? a : mI;
+        MI = a > MI ? a : MI;
+      }
+    }
+  }
   __syncthreads();
 
   // normal-direction traces of the deciding variable on every face
@@ -578,18 +620,18 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
       for (int k = 0; k < N; k++) t += s_u[team][vsel][vnode<DIM, N>(r, k, a, b)] * tb.E[c][k];
       s_tr[team][f][fn] = t;
     }
-    // stencil bounds (cell + face neighbours) of the deciding variable's means
-    if (active) {
-      const double mv = mean[vsel];
-      mI = mv < mI ? mv : mI;
-      MI = mv > MI ? mv : MI;
+  }
+  if (g.limiter == RKDG_LIM_MULTI && g.last_hope) {
+    for (int it = lane; it < 2 * NFN; it += T) {
+      if (!active) break;
+      const int cx = it / NFN, fn = it - cx * NFN;
+      const int a = fn % N, b = fn / N;
 #pragma unroll
-      for (int lf = 0; lf < NF; lf++) {
-        const int nb = A.nbr[lf * NE + e];
-        if (nb < 0) continue;
-        const double a = A.means[vsel * NE + nb];
-        mI = a < mI ? a : mI;
-        MI = a > MI ? a : MI;
+      for (int v = 0; v < NEQ; v++) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; k++) t += s_u[team][v][vnode<DIM, N>(0, k, a, b)] * tb.E[cx][k];
+        s_tx[team][cx][v][fn] = t;
       }
     }
   }
@@ -685,8 +727,8 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
       for (int v = 0; v < NEQ; v++) u[v] = mean[v] + f * (u[v] - mean[v]);
     }
     if (g.last_hope) {
-      // the limited state at the reference vertices: mean + f (u(vertex) - mean);
-      // u(vertex) = sum over nodes of E-products (the nodal basis sums to one)
+      // the limited state at the reference vertices: mean + f (u(vertex) - mean),
+      // u(vertex) from the x-face traces (the nodal basis sums to one)
       for (int it = lane; it < NVERT; it += T) {
         if (!active) break;
         const int cx = it & 1, cy = (it >> 1) & 1, cz = (it >> 2) & 1;
@@ -695,16 +737,11 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
         for (int v = 0; v < NEQ; v++) {
           double s = 0.0;
 #pragma unroll
-          for (int kz = 0; kz < (DIM == 3 ? N : 1); kz++) {
-            double sy = 0.0;
+          for (int b = 0; b < (DIM == 3 ? N : 1); b++) {
+            double tt = 0.0;
 #pragma unroll
-            for (int ky = 0; ky < N; ky++) {
-              double sx = 0.0;
-#pragma unroll
-              for (int kx = 0; kx < N; kx++) sx += s_u[team][v][kx + N * (ky + N * kz)] * tb.E[cx][kx];
-              sy += sx * tb.E[cy][ky];
-            }
-            s += DIM == 3 ? sy * tb.E[cz][kz] : sy;
+            for (int a = 0; a < N; a++) tt += s_tx[team][cx][v][a + N * b] * tb.E[cy][a];
+            s += DIM == 3 ? tt * tb.E[cz][b] : tt;
           }
           vs[v] = mean[v] + f * (s - mean[v]);
         }
