@@ -194,11 +194,13 @@ int rkdg_batch_advance(rkdg_ctx* ctx, double* x, int s, const double* a, const d
                        int64_t nsteps, int* n_active);
 /* HOST [count] copies of t, next dt, steps taken and the active flags (each nullable). */
 int rkdg_batch_status(rkdg_ctx* ctx, double* t, double* dt, int64_t* steps, int* active);
-/* use_graph 0/1; grid_cap: persistent grid of the dense right-hand-side kernel
- * in workgroups (0: one per work block; -1: keep). */
+/* use_graph 0/1 (batches); grid_cap: persistent grid of the dense
+ * right-hand-side kernel in workgroups (0: one per work block; -1: keep). Any
+ * context accepts it; the tensor kernels ignore grid_cap. */
 int rkdg_batch_set_options(rkdg_ctx* ctx, int use_graph, int grid_cap);
-/* Autotuner: times the right-hand side of x (unchanged) for grid_cap 0 and each
- * candidate, keeps the fastest. ms: [ncand+1] per-launch milliseconds (nullable). */
+/* Autotuner (any context): times the right-hand side of x (unchanged) for
+ * grid_cap 0 and each candidate, keeps the fastest. ms: [ncand+1] per-launch
+ * milliseconds (nullable). */
 int rkdg_tune_grid(rkdg_ctx* ctx, const double* x, const int* candidates, int ncand, int reps, int* best,
                    double* ms);
 
