@@ -528,8 +528,8 @@ __global__ __launch_bounds__(256) void dense_limit_adj_b_kernel(const LimAdjArgs
 // ---------------------------------------------------------------------------
 // objective: J += coef * sum over faces of boundary slot obj_slot of
 // sum_q w_q |n| p(trace); optionally xbar += coef * dJ/dx. One workgroup
-// strides over the elements and reduces in a fixed order, so J is the same
-// double on every run (finite differences of J need that).
+// strides over the (few) elements that touch the slot and reduces in a fixed
+// order, so J is the same double on every run (finite differences of J need that).
 template <int GEOM, int P>
 __global__ __launch_bounds__(256) void dense_objective_kernel(const ObjArgs A, const SchemeParams g,
                                                               const double* __restrict__ gtab) {
@@ -539,7 +539,8 @@ __global__ __launch_bounds__(256) void dense_objective_kernel(const ObjArgs A, c
   const DenseTab<GEOM, P>* tb = reinterpret_cast<const DenseTab<GEOM, P>*>(gtab);
   double jl = 0.0;
   const int64_t NE = A.NE, NDOFS = NE * ND;
-  for (int64_t e = threadIdx.x; e < NE; e += blockDim.x) {
+  for (int t = threadIdx.x; t < A.nelems; t += blockDim.x) {
+    const int64_t e = A.elems[t];
     for (int fc = 0; fc < NF; fc++) {
       if (A.nbr[fc * NE + e] != -1 - A.slot) continue;
       const double an0 = A.geo[(int64_t)(C::G_NRM + 2 * fc) * NE + e];

@@ -327,7 +327,6 @@ __global__ __launch_bounds__(64, 4) void dense_limit_kernel(const LimitArgs A, c
 
   __shared__ DenseTab<GEOM, P> tb;
   __shared__ double s_u[EPB][NEQ][ND];
-  __shared__ double s_y[EPB][NPT];
   __shared__ int s_flag[EPB];
 
   const int tid = threadIdx.x;
@@ -378,6 +377,7 @@ __global__ __launch_bounds__(64, 4) void dense_limit_kernel(const LimitArgs A, c
     const double kh = g.venkat_k * h;
     eps2 = kh * kh * kh;
   }
+  double ylane = 1e9;  // this lane's minimum over its indicator points
   if (ind_bj) {
     for (int it = lane; it < NPT; it += T) {
       if (!active) break;
@@ -404,18 +404,18 @@ __global__ __launch_bounds__(64, 4) void dense_limit_kernel(const LimitArgs A, c
           yc = y < 1.0 ? y + (3.0 - 2.0 * yStar) * yRel * yRel + (yStar - 2.0) * yRel * yRel * yRel : 1.0;
         }
       }
-      s_y[team][it] = yc;
+      ylane = yc < ylane ? yc : ylane;
     }
   }
-  __syncthreads();
-  double yr = 1e9;
+  // minimum over the team's lanes (min is exact, so the order does not matter)
+#pragma unroll
+  for (int off = T / 2; off > 0; off >>= 1) {
+    const double o = __shfl_xor(ylane, off, 64);
+    ylane = o < ylane ? o : ylane;
+  }
+  double yr = ylane;
   if (g.indicator == RKDG_IND_NOWHERE) yr = 1.0;
   else if (g.indicator == RKDG_IND_EVERYWHERE) yr = 0.0;
-  else
-    for (int it = 0; it < NPT; it++) {
-      const double y = s_y[team][it];
-      yr = y < yr ? y : yr;
-    }
   const int j = lane;
   const bool node_lane = active && lane < ND;
   double u[NEQ];

@@ -191,6 +191,10 @@ struct rkdg_ctx {
   std::vector<double*> d_adj;     // adjoint vectors: u0bar, kbar[4], tmp[2]
   std::vector<double*> d_rec;     // recorded steps of a segment
   std::vector<double*> d_cp;      // checkpoints
+  std::vector<int> nbr_h;         // host copy of the neighbour table [lf][NE]
+  int obj_slot_cached = -1;       // boundary slot of d_obj_elems
+  int* d_obj_elems = nullptr;     // elements with a face on that slot
+  int n_obj_elems = 0;
 };
 
 namespace {
@@ -228,6 +232,7 @@ void free_ctx(rkdg_ctx* c) {
   for (auto p : c->d_adj) (void)hipFree(p);
   for (auto p : c->d_rec) (void)hipFree(p);
   for (auto p : c->d_cp) (void)hipFree(p);
+  (void)hipFree(c->d_obj_elems);
   delete c;
 }
 
@@ -623,6 +628,7 @@ int build_dense(rkdg_ctx* c, const rkdg_mesh_desc* m) {
   HIP_TRY(hipMemcpy(c->d_geo, c->geo_h.data(), sizeof(double) * GS * NE, hipMemcpyHostToDevice));
   HIP_TRY(hipMalloc(&c->d_nbr, sizeof(int) * nf * NE));
   HIP_TRY(hipMemcpy(c->d_nbr, nbr.data(), sizeof(int) * nf * NE, hipMemcpyHostToDevice));
+  c->nbr_h = nbr;
   HIP_TRY(hipMalloc(&c->d_fcode, sizeof(int) * nf * NE));
   HIP_TRY(hipMemcpy(c->d_fcode, fcode.data(), sizeof(int) * nf * NE, hipMemcpyHostToDevice));
   HIP_TRY(hipMalloc(&c->d_dtab, sizeof(double) * tsize));
@@ -854,6 +860,7 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
   HIP_TRY(hipMemcpy(c->d_geo, c->geo_h.data(), sizeof(double) * G * NE, hipMemcpyHostToDevice));
   HIP_TRY(hipMalloc(&c->d_nbr, sizeof(int) * nf * NE));
   HIP_TRY(hipMemcpy(c->d_nbr, nbr.data(), sizeof(int) * nf * NE, hipMemcpyHostToDevice));
+  c->nbr_h = nbr;
   HIP_TRY(hipMalloc(&c->d_fcode, sizeof(int) * nf * NE));
   HIP_TRY(hipMemcpy(c->d_fcode, fcode.data(), sizeof(int) * nf * NE, hipMemcpyHostToDevice));
   HIP_TRY(hipMalloc(&c->d_lam, sizeof(double) * 8 * c->count));
@@ -1185,7 +1192,25 @@ int step_adjoint(rkdg_ctx* c, const StepRec& r, double dt, int s, const double* 
 }
 
 int objective_launch(rkdg_ctx* c, const double* x, int slot, double coef, double* J, double* xbar) {
-  ObjArgs o{x, J, xbar, coef, slot, c->d_geo, c->d_nbr, c->NE};
+  if (c->obj_slot_cached != slot) {
+    std::vector<int> el;
+    for (int64_t e = 0; e < c->NE; e++)
+      for (int f = 0; f < c->nf; f++)
+        if (c->nbr_h[(size_t)f * c->NE + e] == -1 - slot) {
+          el.push_back((int)e);
+          break;
+        }
+    (void)hipFree(c->d_obj_elems);
+    c->d_obj_elems = nullptr;
+    if (!el.empty()) {
+      HIP_TRY(hipMalloc(&c->d_obj_elems, sizeof(int) * el.size()));
+      HIP_TRY(hipMemcpy(c->d_obj_elems, el.data(), sizeof(int) * el.size(), hipMemcpyHostToDevice));
+    }
+    c->n_obj_elems = (int)el.size();
+    c->obj_slot_cached = slot;
+  }
+  if (c->n_obj_elems == 0) return RKDG_OK;
+  ObjArgs o{x, J, xbar, coef, slot, c->d_geo, c->d_nbr, c->NE, c->d_obj_elems, c->n_obj_elems};
   HIP_TRY(launch_dense_objective(c->geom, c->p, o, c->sp, c->d_dtab, c->stream));
   return RKDG_OK;
 }

@@ -147,6 +147,8 @@ struct ObjArgs {
   const double* geo;
   const int* nbr;
   int64_t NE;
+  const int* elems;  // the elements with a face on the slot, ascending
+  int nelems;
 };
 struct AxpyTargets {
   int n;
