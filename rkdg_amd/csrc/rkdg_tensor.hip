@@ -151,6 +151,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
   __shared__ double s_u[EPB][NEQ][ND];
   __shared__ double s_trn[EPB][2][NEQ][NFN];   // neighbour traces, faces (r,0), (r,1)
   __shared__ double s_tro[EPB][2][NEQ][NFN];
+  __shared__ double s_lift[EPB][2][NEQ][NFN];  // face fluxes lifted to face nodes
   __shared__ double s_work[EPB][WORK];
   __shared__ double s_nrm[EPB][NF][DIM + 2];   // unit normal, |n|, flux sign
   __shared__ double s_geo[EPB][GD];
@@ -282,13 +283,14 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
     }
   }
 
-  // ---- face terms, one axis at a time (LDS only)
+  // ---- face terms, one axis at a time. Phases per axis r: points(r) |
+  // lift(r) + traces(r+1) | node accumulation(r) + points(r+1) ..., so each
+  // axis costs two barriers.
   bool bad = false;
-#pragma unroll
-  for (int r = 0; r < DIM; r++) {
-    // own normal-direction traces on faces (r,0), (r,1) and the neighbour's
-    // traces in its own face frame (read from global memory), one face node of
-    // both per lane
+  // own normal-direction traces on faces (r,0), (r,1) and the neighbour's
+  // traces in its own face frame (read from global memory), one face node of
+  // both per lane
+  auto traces = [&](const int r) {
     for (int it = lane; it < 2 * NFN; it += T) {
       if (!active) break;
       const int c = it / NFN, fn = it - c * NFN;
@@ -323,7 +325,11 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
         }
       }
     }
-    __syncthreads();
+  };
+  traces(0);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < DIM; r++) {
     // face points: traces, Riemann solve, weighted flux
     for (int it = lane; it < 2 * NFP; it += T) {
       if (!active) break;
@@ -408,7 +414,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       for (int v = 0; v < NEQ; v++) work[(c * NEQ + v) * NFP + q] = fl[v] * wgt;
     }
     __syncthreads();
-    // lift to face nodes (reuses s_tro)
+    // lift to face nodes
     for (int it = lane; it < 2 * NFN; it += T) {
       if (!active) break;
       const int c = it / NFN, fn = it - c * NFN;
@@ -428,9 +434,10 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
 #pragma unroll
           for (int qi = 0; qi < NQ; qi++) L += work[(c * NEQ + v) * NFP + qi] * tb.B[qi][a];
         }
-        s_tro[team][c][v][fn] = L;
+        s_lift[team][c][v][fn] = L;
       }
     }
+    if (r + 1 < DIM) traces(r + 1);
     __syncthreads();
     if (node_lane) {
       const int k = jj[r];
@@ -439,9 +446,8 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       const int fn = DIM == 3 ? a + N * b : a;
       const double e0v = tb.E[0][k], e1v = tb.E[1][k];
 #pragma unroll
-      for (int v = 0; v < NEQ; v++) z[v] += s_tro[team][0][v][fn] * e0v + s_tro[team][1][v][fn] * e1v;
+      for (int v = 0; v < NEQ; v++) z[v] += s_lift[team][0][v][fn] * e0v + s_lift[team][1][v][fn] * e1v;
     }
-    if (r + 1 < DIM) __syncthreads();
   }
 
   // ---- inverse mass (affine: diagonal det(J) * w_j) and stage epilogue
