@@ -53,8 +53,9 @@ __device__ __forceinline__ void dense_rhs_block(const RhsArgs& A, const SchemePa
   }
   for (int i = lane; active && i < NF; i += T) {
     const int nb = A.nbr[i * NE + e];
+    const int cd = A.fcode[i * NE + e];  // loaded together with nb, used for interior faces
     s_nb[team][i] = nb;
-    s_code[team][i] = nb >= 0 ? A.fcode[i * NE + e] : 0;
+    s_code[team][i] = nb >= 0 ? cd : 0;
   }
   for (int i = lane; active && i < GS; i += T) s_geo[team][i] = A.geo[(int64_t)i * NE + e];
   __syncthreads();
@@ -327,7 +328,6 @@ __global__ __launch_bounds__(64, 4) void dense_limit_kernel(const LimitArgs A, c
 
   __shared__ DenseTab<GEOM, P> tb;
   __shared__ double s_u[EPB][NEQ][ND];
-  __shared__ int s_flag[EPB];
 
   const int tid = threadIdx.x;
   const int team = tid / T, lane = tid % T;
@@ -347,43 +347,50 @@ __global__ __launch_bounds__(64, 4) void dense_limit_kernel(const LimitArgs A, c
       s_u[te][v][tn] = (e0 + te < NE) ? xv[idx] : 1.0;
     }
   }
-  if (lane == 0) s_flag[team] = 0;
   double mean[NEQ];
 #pragma unroll
   for (int v = 0; v < NEQ; v++) mean[v] = active ? A.means[v * NE + e] : 0.0;
   double mI = 1e9, MI = -1e9;
   const bool ind_bj = g.indicator == RKDG_IND_BJ;
+  int nbmask = 0;  // bit f set: face f has a neighbour element
   if (active && ind_bj) {
     const double mv = mean[vsel];
     mI = mv < mI ? mv : mI;
     MI = mv > MI ? mv : MI;
+    // branch-free, so the index loads and then the means loads are in flight together
+    int nbs[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) nbs[f] = A.nbr[f * NE + e];
+    double nm[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) nm[f] = A.means[vsel * NE + (nbs[f] >= 0 ? (int64_t)nbs[f] : e)];
+#pragma unroll
     for (int f = 0; f < NF; f++) {
-      const int nb = A.nbr[f * NE + e];
-      if (nb < 0) continue;
-      const double a = A.means[vsel * NE + nb];
-      mI = a < mI ? a : mI;
-      MI = a > MI ? a : MI;
+      nbmask |= (nbs[f] >= 0 ? 1 : 0) << f;
+      mI = nm[f] < mI ? nm[f] : mI;
+      MI = nm[f] > MI ? nm[f] : MI;
     }
   }
-  __syncthreads();
   double eps2 = 0.0;
   if (active && g.phi == RKDG_PHI_VENKAT && g.venkat_k > 0.0) {
     // (K h)^3 with h = |element|^(1/2)
     double area = 0.0;
     if (C::TRI) area = 0.5 * A.geo[(int64_t)C::G_DET * NE + e];
     else
+#pragma unroll
       for (int k = 0; k < ND; k++) area += A.geo[(int64_t)(C::G_MW + k) * NE + e];
     const double h = sqrt(area);
     const double kh = g.venkat_k * h;
     eps2 = kh * kh * kh;
   }
+  __syncthreads();
   double ylane = 1e9;  // this lane's minimum over its indicator points
   if (ind_bj) {
     for (int it = lane; it < NPT; it += T) {
       if (!active) break;
       const int f = it / 2, ip = it % 2;
       double yc = 1e9;
-      if (A.nbr[f * NE + e] >= 0) {
+      if ((nbmask >> f) & 1) {
         double u = 0.0;
 #pragma unroll
         for (int k = 0; k < ND; k++) u += s_u[team][vsel][k] * tb.PhiI[f][ip][k];
@@ -440,6 +447,7 @@ __global__ __launch_bounds__(64, 4) void dense_limit_kernel(const LimitArgs A, c
       for (int v = 0; v < NEQ; v++) u[v] = mean[v] + f * (u[v] - mean[v]);
     }
     if (g.last_hope) {
+      bool bad_v = false;
       for (int it = lane; it < NV; it += T) {
         if (!active) break;
         double vs[NEQ];
@@ -450,10 +458,12 @@ __global__ __launch_bounds__(64, 4) void dense_limit_kernel(const LimitArgs A, c
           for (int k = 0; k < ND; k++) s += s_u[team][v][k] * tb.PhiV[it][k];
           vs[v] = mean[v] + f * (s - mean[v]);
         }
-        if (!physical<2>(vs, g)) atomicOr(&s_flag[team], 1);
+        if (!physical<2>(vs, g)) bad_v = true;
       }
-      __syncthreads();
-      if (active && s_flag[team]) {
+      // any unphysical vertex of the team's element (one ballot, no barrier)
+      const unsigned long long bm = __ballot(bad_v);
+      const unsigned long long tm = T == 64 ? ~0ull : (((1ull << T) - 1ull) << (team * T));
+      if (active && (bm & tm)) {
         trouble |= 2;
         if (node_lane) {
 #pragma unroll

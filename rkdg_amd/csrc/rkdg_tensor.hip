@@ -167,7 +167,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
   const int64_t e = e0 + team;
   const bool active = e < NE;
 
-  // ---- phase 0: tables, element state, connectivity and geometry into LDS
+  // ---- phase 0: tables, element state, connectivity, normals and geometry into LDS
   stage_tables(tb, gtb);
 #pragma unroll
   for (int v = 0; v < NEQ; v++) {
@@ -177,45 +177,42 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       s_u[te][v][tn] = (e0 + te < NE) ? xv[idx] : 1.0;
     }
   }
+  // connectivity of local face i and Elem1's unit normal on it (the neighbour's
+  // geometry when the neighbour is Elem1); the face code is loaded together with
+  // the neighbour index so that only the normal's gather waits for it
   for (int i = lane; active && i < NF; i += T) {
     const int nb = A.nbr[i * NE + e];
+    const int cd = A.fcode[i * NE + e];
+    const int code = nb >= 0 ? cd : 0;
     s_nb[team][i] = nb;
-    s_code[team][i] = nb >= 0 ? A.fcode[i * NE + e] : 0;
+    s_code[team][i] = code;
+    int r, c;
+    rc_of<DIM>(i, r, c);
+    double an[DIM];
+    double sign = -1.0;
+    if (nb >= 0 && !((code >> 6) & 1)) {
+      int rn, cn;
+      rc_of<DIM>(code & 7, rn, cn);
+      const double sn = cn ? 1.0 : -1.0;
+#pragma unroll
+      for (int d = 0; d < DIM; d++) an[d] = sn * A.geo[(rn * DIM + d) * NE + nb];
+      sign = 1.0;
+    } else {
+      const double sg = c ? 1.0 : -1.0;
+#pragma unroll
+      for (int d = 0; d < DIM; d++) an[d] = sg * A.geo[(r * DIM + d) * NE + e];
+    }
+    double nm2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) nm2 += an[d] * an[d];
+    const double inv = 1.0 / norml2_scaled<DIM>(an);
+    const int f = 2 * r + c;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) s_nrm[team][f][d] = an[d] * inv;
+    s_nrm[team][f][DIM] = sqrt(nm2);
+    s_nrm[team][f][DIM + 1] = sign;
   }
   for (int i = lane; active && i < GD; i += T) s_geo[team][i] = A.geo[i * NE + e];
-  __syncthreads();
-
-  // ---- phase 1: Elem1's unit normal per face
-  if (active) {
-    for (int f = lane; f < NF; f += T) {
-      const int r = f / 2, c = f % 2;
-      const int lf = lf_of<DIM>(r, c);
-      const int nb = s_nb[team][lf];
-      const int code = s_code[team][lf];
-      double an[DIM];
-      double sign = -1.0;
-      if (nb >= 0 && !((code >> 6) & 1)) {
-        int rn, cn;
-        rc_of<DIM>(code & 7, rn, cn);
-        const double sn = cn ? 1.0 : -1.0;
-#pragma unroll
-        for (int d = 0; d < DIM; d++) an[d] = sn * A.geo[(rn * DIM + d) * NE + nb];
-        sign = 1.0;
-      } else {
-        const double sg = c ? 1.0 : -1.0;
-#pragma unroll
-        for (int d = 0; d < DIM; d++) an[d] = sg * s_geo[team][r * DIM + d];
-      }
-      double nm2 = 0.0;
-#pragma unroll
-      for (int d = 0; d < DIM; d++) nm2 += an[d] * an[d];
-      const double inv = 1.0 / norml2_scaled<DIM>(an);
-#pragma unroll
-      for (int d = 0; d < DIM; d++) s_nrm[team][f][d] = an[d] * inv;
-      s_nrm[team][f][DIM] = sqrt(nm2);
-      s_nrm[team][f][DIM + 1] = sign;
-    }
-  }
   __syncthreads();
 
   const int j = lane;
@@ -565,7 +562,6 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
   __shared__ double s_u[EPB][NEQ][ND];
   __shared__ double s_tr[EPB][NF][NFN];   // traces of the deciding variable
   __shared__ double s_tx[EPB][2][NEQ][NFN]; // traces of every variable on the two x faces (last hope)
-  __shared__ int s_flag[EPB];
 
   const int tid = threadIdx.x;
   const int team = tid / T, lane = tid % T;
@@ -589,27 +585,41 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
       s_u[te][v][tn] = (e0 + te < NE) ? xv[idx] : 1.0;
     }
   }
-  if (lane == 0) s_flag[team] = 0;
   double mean[NEQ];
 #pragma unroll
   for (int v = 0; v < NEQ; v++) mean[v] = active ? A.means[v * NE + e] : 0.0;
   double mI = 1e9, MI = -1e9;
   const bool ind_bj = g.indicator == RKDG_IND_BJ;
+  int nbmask = 0;  // bit lf set: local face lf has a neighbour element
   if (ind_bj) {
     // stencil bounds (cell + face neighbours) of the deciding variable's means
     if (active) {
       const double mv = mean[vsel];
       mI = mv < mI ? mv : mI;
       MI = mv > MI ? mv : MI;
+      // branch-free, so the six index loads and then the six means loads are
+      // in flight together
+      int nbs[NF];
+#pragma unroll
+      for (int lf = 0; lf < NF; lf++) nbs[lf] = A.nbr[lf * NE + e];
+      double nm[NF];
+#pragma unroll
+      for (int lf = 0; lf < NF; lf++) nm[lf] = A.means[vsel * NE + (nbs[lf] >= 0 ? (int64_t)nbs[lf] : e)];
 #pragma unroll
       for (int lf = 0; lf < NF; lf++) {
-        const int nb = A.nbr[lf * NE + e];
-        if (nb < 0) continue;
-        const double a = A.means[vsel * NE + nb];
-        mI = a < mI ? a : mI;
-        MI = a > MI ? a : MI;
+        nbmask |= (nbs[lf] >= 0 ? 1 : 0) << lf;
+        mI = nm[lf] < mI ? nm[lf] : mI;
+        MI = nm[lf] > MI ? nm[lf] : MI;
       }
     }
+  }
+  // Venkatakrishnan eps^2 = (K h)^3 with h = |element|^(1/dim): h^3 = det J
+  // (3D) or det J^(3/2) (2D) for affine elements
+  double eps2 = 0.0;
+  if (active && g.phi == RKDG_PHI_VENKAT && g.venkat_k > 0.0) {
+    const double dj = A.geo[DIM * DIM * NE + e];
+    const double k3 = g.venkat_k * g.venkat_k * g.venkat_k;
+    eps2 = k3 * (DIM == 3 ? dj : dj * sqrt(dj));
   }
   __syncthreads();
 
@@ -642,15 +652,6 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
   }
   __syncthreads();
 
-  // Venkatakrishnan eps^2 = (K h)^3 with h = |element|^(1/dim): h^3 = det J
-  // (3D) or det J^(3/2) (2D) for affine elements
-  double eps2 = 0.0;
-  if (active && g.phi == RKDG_PHI_VENKAT && g.venkat_k > 0.0) {
-    const double dj = A.geo[DIM * DIM * NE + e];
-    const double k3 = g.venkat_k * g.venkat_k * g.venkat_k;
-    eps2 = k3 * (DIM == 3 ? dj : dj * sqrt(dj));
-  }
-
   // indicator values at the points of interior faces
   double ylane = 1e9;  // this lane's minimum over its indicator points
   if (ind_bj) {
@@ -660,7 +661,7 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
       const int r = f / 2, c = f % 2;
       const int lf = lf_of<DIM>(r, c);
       double yc = 1e9;
-      if (A.nbr[lf * NE + e] >= 0) {
+      if ((nbmask >> lf) & 1) {
         const int pi = ip % 2, pj = ip / 2;
         double u = 0.0;
         if (DIM == 3) {
@@ -733,6 +734,7 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
       for (int v = 0; v < NEQ; v++) u[v] = mean[v] + f * (u[v] - mean[v]);
     }
     if (g.last_hope) {
+      bool bad_v = false;
       // the limited state at the reference vertices: mean + f (u(vertex) - mean),
       // u(vertex) from the x-face traces (the nodal basis sums to one)
       for (int it = lane; it < NVERT; it += T) {
@@ -751,10 +753,11 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
           }
           vs[v] = mean[v] + f * (s - mean[v]);
         }
-        if (!physical<DIM>(vs, g)) atomicOr(&s_flag[team], 1);
+        if (!physical<DIM>(vs, g)) bad_v = true;
       }
-      __syncthreads();
-      if (active && s_flag[team]) {
+      const unsigned long long bm = __ballot(bad_v);
+      const unsigned long long tm = T == 64 ? ~0ull : (((1ull << T) - 1ull) << (team * T));
+      if (active && (bm & tm)) {
         trouble |= 2;
         if (node_lane) {
 #pragma unroll
