@@ -153,7 +153,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
   __shared__ double s_tro[EPB][2][NEQ][NFN];
   __shared__ double s_lift[EPB][2][NEQ][NFN];  // face fluxes lifted to face nodes
   __shared__ double s_work[EPB][WORK];
-  __shared__ double s_nrm[EPB][NF][DIM + 2];   // unit normal, |n|, flux sign
+  __shared__ double s_nrm[EPB][NF][DIM + 2];   // unit normal, flux sign * |n|, 0.5 / |unit normal|
   __shared__ double s_geo[EPB][GD];
   __shared__ int s_nb[EPB][NF];                // by MFEM local face
   __shared__ int s_code[EPB][NF];
@@ -207,10 +207,15 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
     for (int d = 0; d < DIM; d++) nm2 += an[d] * an[d];
     const double inv = 1.0 / norml2_scaled<DIM>(an);
     const int f = 2 * r + c;
+    double nu2 = 0.0;
 #pragma unroll
-    for (int d = 0; d < DIM; d++) s_nrm[team][f][d] = an[d] * inv;
-    s_nrm[team][f][DIM] = sqrt(nm2);
-    s_nrm[team][f][DIM + 1] = sign;
+    for (int d = 0; d < DIM; d++) {
+      const double nu = an[d] * inv;
+      s_nrm[team][f][d] = nu;
+      nu2 += nu * nu;
+    }
+    s_nrm[team][f][DIM] = sign * sqrt(nm2);
+    s_nrm[team][f][DIM + 1] = 0.5 * recip(sqrt(nu2));
   }
   for (int i = lane; active && i < GD; i += T) s_geo[team][i] = A.geo[i * NE + e];
   __syncthreads();
@@ -240,7 +245,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
 #pragma unroll
       for (int v = 0; v < NEQ; v++) st[v] = s_u[team][v][j];
       const SidePrim<DIM> q = side_prim<DIM>(st, g);
-      if (!A.no_lambda) lmax = sqrt(q.m2) * q.inv + side_sound<DIM>(st, q, g);
+      if (!A.no_lambda) lmax = sqrt_ws(q.m2) * q.inv + side_sound_ws<DIM>(st, q, g);
       double wj = tb.w[jj[0]] * tb.w[jj[1]];
       if (DIM == 3) wj *= tb.w[jj[2]];
       // contravariant fluxes G^r = w_j adj(J)(r,:) . F(u): with m.a_r = sum_d a_rd m_d,
@@ -356,7 +361,8 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       double nrm[DIM];
 #pragma unroll
       for (int d = 0; d < DIM; d++) nrm[d] = s_nrm[team][f][d];
-      const double normag = s_nrm[team][f][DIM];
+      const double snm = s_nrm[team][f][DIM];  // -|n| on Elem1's side, +|n| on Elem2's
+      const double inm = s_nrm[team][f][DIM + 1];
       double fl[NEQ];
       double factor = -1.0;
       bool ok = true;
@@ -381,7 +387,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
           }
           up[v] = t;
         }
-        factor = s_nrm[team][f][DIM + 1];
+        factor = snm < 0.0 ? -1.0 : 1.0;
         const bool elem1 = factor < 0.0;
         double sL[NEQ], sR[NEQ];
 #pragma unroll
@@ -389,12 +395,12 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
           sL[v] = elem1 ? um[v] : up[v];
           sR[v] = elem1 ? up[v] : um[v];
         }
-        lam = riemann_t<DIM, RS>(g, sL, sR, nrm, fl, ok);
+        lam = riemann_t<DIM, RS>(g, sL, sR, nrm, fl, ok, inm);
       } else if constexpr (BC) {
         const int slot = -1 - nb;
         if (slot < g.num_bc) {
           bc_state<DIM>(g, slot, um, nrm, up);
-          lam = riemann_t<DIM, RS>(g, um, up, nrm, fl, ok);
+          lam = riemann_t<DIM, RS>(g, um, up, nrm, fl, ok, inm);
         } else {
 #pragma unroll
           for (int v = 0; v < NEQ; v++) fl[v] = 0.0;
@@ -406,7 +412,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       } else {
         lmax = lam > lmax ? lam : lmax;
       }
-      const double wgt = factor * (DIM == 3 ? tb.wq[qi] * tb.wq[qj] : tb.wq[qi]) * normag;
+      const double wgt = (DIM == 3 ? tb.wq[qi] * tb.wq[qj] : tb.wq[qi]) * snm;  // flux sign * w_q * |n|
 #pragma unroll
       for (int v = 0; v < NEQ; v++) work[(c * NEQ + v) * NFP + q] = fl[v] * wgt;
     }
