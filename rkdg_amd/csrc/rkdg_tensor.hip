@@ -73,6 +73,21 @@ __device__ __forceinline__ int vnode(int r, int k, int a, int b) {
   return a + N * (b + N * k);
 }
 
+// vnode for an axis known only at run time, without branches: the node index
+// is a * sa + b * sb + k * sk
+template <int DIM, int N>
+__device__ __forceinline__ void vstrides(int r, int& sk, int& sa, int& sb) {
+  if (DIM == 2) {
+    sk = r == 0 ? 1 : N;
+    sa = r == 0 ? N : 1;
+    sb = 0;
+  } else {
+    sk = r == 0 ? 1 : (r == 1 ? N : N * N);
+    sa = r == 0 ? N : 1;
+    sb = r == 2 ? N : N * N;
+  }
+}
+
 // orientation: my face-point indices (i, j) -> neighbour frame (i', j')
 template <int DIM, int NQ>
 __device__ __forceinline__ int map_point(int code, int qi, int qj) {
@@ -302,13 +317,14 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       double vals[NEQ][N];
       int cn = 0;
       if (nb >= 0) {
-        int rn;
+        int rn, sk, sa, sb;
         rc_of<DIM>(s_code[team][lf] & 7, rn, cn);
-        const double* xn = A.x + (int64_t)nb * ND;
+        vstrides<DIM, N>(rn, sk, sa, sb);
+        const double* xn = A.x + (int64_t)nb * ND + (a * sa + b * sb);
 #pragma unroll
         for (int v = 0; v < NEQ; v++)
 #pragma unroll
-          for (int k = 0; k < N; k++) vals[v][k] = xn[v * NDOFS + vnode<DIM, N>(rn, k, a, b)];
+          for (int k = 0; k < N; k++) vals[v][k] = xn[v * NDOFS + k * sk];
       }
 #pragma unroll
       for (int v = 0; v < NEQ; v++) {
@@ -636,9 +652,12 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
       const int f = it / NFN, fn = it - f * NFN;
       const int r = f / 2, c = f % 2;
       const int a = fn % N, b = fn / N;
+      int sk, sa, sb;
+      vstrides<DIM, N>(r, sk, sa, sb);
+      const double* uv = &s_u[team][vsel][a * sa + b * sb];
       double t = 0.0;
 #pragma unroll
-      for (int k = 0; k < N; k++) t += s_u[team][vsel][vnode<DIM, N>(r, k, a, b)] * tb.E[c][k];
+      for (int k = 0; k < N; k++) t += uv[k * sk] * tb.E[c][k];
       s_tr[team][f][fn] = t;
     }
   }
