@@ -161,6 +161,10 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
   constexpr int NF = C::NF, T = C::T, EPB = C::EPB, BLOCK = C::BLOCK;
   constexpr int GD = DIM * DIM + 1;
   constexpr int WORK = (NEQ + 1) * ND > 2 * NEQ * NFP ? (NEQ + 1) * ND : 2 * NEQ * NFP;
+  // face-point fluxes are stored in rows of NFP per (face, variable); a row's
+  // entries are XOR-permuted by the row index (NFP a power of two) so that the
+  // lift's lanes, which read ten rows at once, hit different LDS banks
+  constexpr int SWZ = (NFP & (NFP - 1)) == 0 ? NFP - 1 : 0;
 
   __shared__ TT<C::N, C::NQ> tb;
   __shared__ double s_u[EPB][NEQ][ND];
@@ -305,41 +309,43 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
   // axis costs two barriers.
   bool bad = false;
   // own normal-direction traces on faces (r,0), (r,1) and the neighbour's
-  // traces in its own face frame (read from global memory), one face node of
-  // both per lane
+  // traces in its own face frame (read from global memory). One lane per
+  // (face, variable, first tangential index a): the N face nodes (a, b) of
+  // that row, so 2 NEQ N lanes share the work.
+  constexpr int NB = DIM == 3 ? N : 1;  // face nodes per row
   auto traces = [&](const int r) {
-    for (int it = lane; it < 2 * NFN; it += T) {
+    for (int it = lane; it < 2 * NEQ * N; it += T) {
       if (!active) break;
-      const int c = it / NFN, fn = it - c * NFN;
-      const int a = fn % N, b = fn / N;
+      const int c = it / (NEQ * N), rem = it - c * (NEQ * N);
+      const int v = rem / N, a = rem - v * N;
       const int lf = lf_of<DIM>(r, c);
       const int nb = s_nb[team][lf];
-      double vals[NEQ][N];
+      double vals[NB][N];
       int cn = 0;
       if (nb >= 0) {
         int rn, sk, sa, sb;
         rc_of<DIM>(s_code[team][lf] & 7, rn, cn);
         vstrides<DIM, N>(rn, sk, sa, sb);
-        const double* xn = A.x + (int64_t)nb * ND + (a * sa + b * sb);
+        const double* xn = A.x + v * NDOFS + (int64_t)nb * ND + a * sa;
 #pragma unroll
-        for (int v = 0; v < NEQ; v++)
+        for (int b = 0; b < NB; b++)
 #pragma unroll
-          for (int k = 0; k < N; k++) vals[v][k] = xn[v * NDOFS + k * sk];
+          for (int k = 0; k < N; k++) vals[b][k] = xn[b * sb + k * sk];
       }
 #pragma unroll
-      for (int v = 0; v < NEQ; v++) {
+      for (int b = 0; b < NB; b++) {
         double t = 0.0;
 #pragma unroll
         for (int k = 0; k < N; k++) t += s_u[team][v][vnode<DIM, N>(r, k, a, b)] * tb.E[c][k];
-        s_tro[team][c][v][fn] = t;
+        s_tro[team][c][v][a + N * b] = t;
       }
       if (nb >= 0) {
 #pragma unroll
-        for (int v = 0; v < NEQ; v++) {
+        for (int b = 0; b < NB; b++) {
           double t = 0.0;
 #pragma unroll
-          for (int k = 0; k < N; k++) t += vals[v][k] * tb.E[cn][k];
-          s_trn[team][c][v][fn] = t;
+          for (int k = 0; k < N; k++) t += vals[b][k] * tb.E[cn][k];
+          s_trn[team][c][v][a + N * b] = t;
         }
       }
     }
@@ -430,30 +436,38 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       }
       const double wgt = (DIM == 3 ? tb.wq[qi] * tb.wq[qj] : tb.wq[qi]) * snm;  // flux sign * w_q * |n|
 #pragma unroll
-      for (int v = 0; v < NEQ; v++) work[(c * NEQ + v) * NFP + q] = fl[v] * wgt;
+      for (int v = 0; v < NEQ; v++) work[(c * NEQ + v) * NFP + (q ^ ((c * NEQ + v) & SWZ))] = fl[v] * wgt;
     }
     __syncthreads();
-    // lift to face nodes
-    for (int it = lane; it < 2 * NFN; it += T) {
+    // lift to face nodes, sum-factorised: one lane per (face, variable, a)
+    // forms W(qj) = sum_qi F(qi, qj) B(qi, a) once and the N nodes (a, b) from it
+    for (int it = lane; it < 2 * NEQ * N; it += T) {
       if (!active) break;
-      const int c = it / NFN, fn = it - c * NFN;
-      const int a = fn % N, b = fn / N;
+      const int c = it / (NEQ * N), rem = it - c * (NEQ * N);
+      const int v = rem / N, a = rem - v * N;
+      const int rv = c * NEQ + v, sw = rv & SWZ;
+      const double* w = work + rv * NFP;
+      if (DIM == 3) {
+        double W[NQ];
 #pragma unroll
-      for (int v = 0; v < NEQ; v++) {
-        double L = 0.0;
-        if (DIM == 3) {
+        for (int qj = 0; qj < NQ; qj++) {
+          double tt = 0.0;
 #pragma unroll
-          for (int qj = 0; qj < NQ; qj++) {
-            double tt = 0.0;
-#pragma unroll
-            for (int qi = 0; qi < NQ; qi++) tt += work[(c * NEQ + v) * NFP + qi + NQ * qj] * tb.B[qi][a];
-            L += tt * tb.B[qj][b];
-          }
-        } else {
-#pragma unroll
-          for (int qi = 0; qi < NQ; qi++) L += work[(c * NEQ + v) * NFP + qi] * tb.B[qi][a];
+          for (int qi = 0; qi < NQ; qi++) tt += w[(qi + NQ * qj) ^ sw] * tb.B[qi][a];
+          W[qj] = tt;
         }
-        s_lift[team][c][v][fn] = L;
+#pragma unroll
+        for (int b = 0; b < N; b++) {
+          double L = 0.0;
+#pragma unroll
+          for (int qj = 0; qj < NQ; qj++) L += W[qj] * tb.B[qj][b];
+          s_lift[team][c][v][a + N * b] = L;
+        }
+      } else {
+        double L = 0.0;
+#pragma unroll
+        for (int qi = 0; qi < NQ; qi++) L += w[qi ^ sw] * tb.B[qi][a];
+        s_lift[team][c][v][a] = L;
       }
     }
     if (r + 1 < DIM) traces(r + 1);
