@@ -165,13 +165,22 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
   // entries are XOR-permuted by the row index (NFP a power of two) so that the
   // lift's lanes, which read ten rows at once, hit different LDS banks
   constexpr int SWZ = (NFP & (NFP - 1)) == 0 ? NFP - 1 : 0;
+  constexpr int NB = DIM == 3 ? N : 1;  // face nodes per row (second tangential index b)
+  // The lifted fluxes may share the face-point flux rows when every phase is a
+  // single pass of the team (one wavefront per workgroup executes in order, so a
+  // phase's reads of the rows come before its writes of the lifted values, and
+  // the node accumulation reads them before the next points phase writes rows).
+  constexpr bool LIFT_IN_WORK = 2 * NEQ * N <= T && 2 * NFP <= T;
+  constexpr int LIFT = 2 * NEQ * NFN;
 
   __shared__ TT<C::N, C::NQ> tb;
   __shared__ double s_u[EPB][NEQ][ND];
-  __shared__ double s_trn[EPB][2][NEQ][NFN];   // neighbour traces, faces (r,0), (r,1)
-  __shared__ double s_tro[EPB][2][NEQ][NFN];
-  __shared__ double s_lift[EPB][2][NEQ][NFN];  // face fluxes lifted to face nodes
-  __shared__ double s_work[EPB][WORK];
+  // traces interpolated along the first tangential direction to the face Gauss
+  // abscissae: [face c][variable][qi][b], own and neighbour (its face frame)
+  __shared__ double s_to[EPB][2][NEQ][NQ][NB];
+  __shared__ double s_tn[EPB][2][NEQ][NQ][NB];
+  __shared__ double s_work[EPB][WORK > LIFT ? WORK : LIFT];
+  __shared__ double s_lift[LIFT_IN_WORK ? 1 : EPB][LIFT_IN_WORK ? 1 : LIFT];  // face fluxes lifted to face nodes
   __shared__ double s_nrm[EPB][NF][DIM + 2];   // unit normal, flux sign * |n|, 0.5 / |unit normal|
   __shared__ double s_geo[EPB][GD];
   __shared__ int s_nb[EPB][NF];                // by MFEM local face
@@ -255,6 +264,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
   for (int v = 0; v < NEQ; v++) z[v] = 0.0;
   double lmax = 0.0;
   double* work = s_work[team];
+  double* lift = LIFT_IN_WORK ? work : s_lift[LIFT_IN_WORK ? 0 : team];  // [c][v][face node]
 
   // ---- volume term: sum_r D^T (w * adj(J) row r . F(u)) along axis r
   if (!A.face_only) {
@@ -309,43 +319,57 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
   // axis costs two barriers.
   bool bad = false;
   // own normal-direction traces on faces (r,0), (r,1) and the neighbour's
-  // traces in its own face frame (read from global memory). One lane per
-  // (face, variable, first tangential index a): the N face nodes (a, b) of
-  // that row, so 2 NEQ N lanes share the work.
-  constexpr int NB = DIM == 3 ? N : 1;  // face nodes per row
+  // traces in its own face frame (read from global memory), then interpolated
+  // along the first tangential direction a to the Gauss abscissae qi. One lane
+  // per (face, variable, second tangential index b).
   auto traces = [&](const int r) {
-    for (int it = lane; it < 2 * NEQ * N; it += T) {
+    for (int it = lane; it < 2 * NEQ * NB; it += T) {
       if (!active) break;
-      const int c = it / (NEQ * N), rem = it - c * (NEQ * N);
-      const int v = rem / N, a = rem - v * N;
+      const int c = it / (NEQ * NB), rem = it - c * (NEQ * NB);
+      const int v = rem / NB, b = rem - v * NB;
       const int lf = lf_of<DIM>(r, c);
       const int nb = s_nb[team][lf];
-      double vals[NB][N];
+      double vals[N][N];
       int cn = 0;
       if (nb >= 0) {
         int rn, sk, sa, sb;
         rc_of<DIM>(s_code[team][lf] & 7, rn, cn);
         vstrides<DIM, N>(rn, sk, sa, sb);
-        const double* xn = A.x + v * NDOFS + (int64_t)nb * ND + a * sa;
+        const double* xn = A.x + v * NDOFS + (int64_t)nb * ND + b * sb;
 #pragma unroll
-        for (int b = 0; b < NB; b++)
+        for (int a = 0; a < N; a++)
 #pragma unroll
-          for (int k = 0; k < N; k++) vals[b][k] = xn[b * sb + k * sk];
+          for (int k = 0; k < N; k++) vals[a][k] = xn[a * sa + k * sk];
       }
+      double tr[N];
 #pragma unroll
-      for (int b = 0; b < NB; b++) {
+      for (int a = 0; a < N; a++) {
         double t = 0.0;
 #pragma unroll
         for (int k = 0; k < N; k++) t += s_u[team][v][vnode<DIM, N>(r, k, a, b)] * tb.E[c][k];
-        s_tro[team][c][v][a + N * b] = t;
+        tr[a] = t;
+      }
+#pragma unroll
+      for (int qi = 0; qi < NQ; qi++) {
+        double tt = 0.0;
+#pragma unroll
+        for (int a = 0; a < N; a++) tt += tr[a] * tb.B[qi][a];
+        s_to[team][c][v][qi][b] = tt;
       }
       if (nb >= 0) {
 #pragma unroll
-        for (int b = 0; b < NB; b++) {
+        for (int a = 0; a < N; a++) {
           double t = 0.0;
 #pragma unroll
-          for (int k = 0; k < N; k++) t += vals[b][k] * tb.E[cn][k];
-          s_trn[team][c][v][a + N * b] = t;
+          for (int k = 0; k < N; k++) t += vals[a][k] * tb.E[cn][k];
+          tr[a] = t;
+        }
+#pragma unroll
+        for (int qi = 0; qi < NQ; qi++) {
+          double tt = 0.0;
+#pragma unroll
+          for (int a = 0; a < N; a++) tt += tr[a] * tb.B[qi][a];
+          s_tn[team][c][v][qi][b] = tt;
         }
       }
     }
@@ -368,15 +392,9 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
         double t = 0.0;
         if (DIM == 3) {
 #pragma unroll
-          for (int b = 0; b < N; b++) {
-            double tt = 0.0;
-#pragma unroll
-            for (int a = 0; a < N; a++) tt += s_tro[team][c][v][a + N * b] * tb.B[qi][a];
-            t += tt * tb.B[qj][b];
-          }
+          for (int b = 0; b < N; b++) t += s_to[team][c][v][qi][b] * tb.B[qj][b];
         } else {
-#pragma unroll
-          for (int a = 0; a < N; a++) t += s_tro[team][c][v][a] * tb.B[qi][a];
+          t = s_to[team][c][v][qi][0];
         }
         um[v] = t;
       }
@@ -397,15 +415,9 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
           double t = 0.0;
           if (DIM == 3) {
 #pragma unroll
-            for (int b = 0; b < N; b++) {
-              double tt = 0.0;
-#pragma unroll
-              for (int a = 0; a < N; a++) tt += s_trn[team][c][v][a + N * b] * tb.B[pi][a];
-              t += tt * tb.B[pj][b];
-            }
+            for (int b = 0; b < N; b++) t += s_tn[team][c][v][pi][b] * tb.B[pj][b];
           } else {
-#pragma unroll
-            for (int a = 0; a < N; a++) t += s_trn[team][c][v][a] * tb.B[pi][a];
+            t = s_tn[team][c][v][pi][0];
           }
           up[v] = t;
         }
@@ -461,13 +473,13 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
           double L = 0.0;
 #pragma unroll
           for (int qj = 0; qj < NQ; qj++) L += W[qj] * tb.B[qj][b];
-          s_lift[team][c][v][a + N * b] = L;
+          lift[(c * NEQ + v) * NFN + a + N * b] = L;
         }
       } else {
         double L = 0.0;
 #pragma unroll
         for (int qi = 0; qi < NQ; qi++) L += w[qi ^ sw] * tb.B[qi][a];
-        s_lift[team][c][v][a] = L;
+        lift[(c * NEQ + v) * NFN + a] = L;
       }
     }
     if (r + 1 < DIM) traces(r + 1);
@@ -479,7 +491,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       const int fn = DIM == 3 ? a + N * b : a;
       const double e0v = tb.E[0][k], e1v = tb.E[1][k];
 #pragma unroll
-      for (int v = 0; v < NEQ; v++) z[v] += s_lift[team][0][v][fn] * e0v + s_lift[team][1][v][fn] * e1v;
+      for (int v = 0; v < NEQ; v++) z[v] += lift[v * NFN + fn] * e0v + lift[(NEQ + v) * NFN + fn] * e1v;
     }
   }
 
