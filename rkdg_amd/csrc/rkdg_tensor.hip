@@ -41,26 +41,24 @@ struct TC {
   static constexpr int EPB = BLOCK / T;
 };
 
-// MFEM local face numbering <-> (normal axis r, side c)
+// MFEM local face numbering <-> (normal axis r, side c). 3D: (r, c) = (0,0)
+// -> 4, (0,1) -> 2, (1,0) -> 1, (1,1) -> 3, (2,0) -> 0, (2,1) -> 5; 2D: (0,0) ->
+// 3, (0,1) -> 1, (1,0) -> 0, (1,1) -> 2. The tables are packed into integer
+// constants (3 or 2 bits per entry) so that a run-time index is a shift, not a
+// memory lookup.
 template <int DIM>
 __host__ __device__ __forceinline__ int lf_of(int r, int c) {
-  if (DIM == 3) {
-    const int t[3][2] = {{4, 2}, {1, 3}, {0, 5}};
-    return t[r][c];
-  }
-  const int t[2][2] = {{3, 1}, {0, 2}};
-  return t[r][c];
+  if (DIM == 3) return (0x28654 >> (3 * (2 * r + c))) & 7;  // 4,2,1,3,0,5
+  return (0x87 >> (2 * (2 * r + c))) & 3;                  // 3,1,0,2
 }
 template <int DIM>
 __host__ __device__ __forceinline__ void rc_of(int lf, int& r, int& c) {
   if (DIM == 3) {
-    const int rr[6] = {2, 1, 0, 1, 0, 2}, cc[6] = {0, 0, 1, 1, 0, 1};
-    r = rr[lf];
-    c = cc[lf];
+    r = (0x846 >> (2 * lf)) & 3;  // 2,1,0,1,0,2
+    c = (0x2c >> lf) & 1;         // 0,0,1,1,0,1
   } else {
-    const int rr[4] = {1, 0, 1, 0}, cc[4] = {0, 1, 1, 0};
-    r = rr[lf];
-    c = cc[lf];
+    r = (0x5 >> lf) & 1;  // 1,0,1,0
+    c = (0x6 >> lf) & 1;  // 0,1,1,0
   }
 }
 
@@ -160,11 +158,11 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
   constexpr int N = C::N, ND = C::ND, NEQ = C::NEQ, NQ = C::NQ, NFP = C::NFP, NFN = C::NFN;
   constexpr int NF = C::NF, T = C::T, EPB = C::EPB, BLOCK = C::BLOCK;
   constexpr int GD = DIM * DIM + 1;
-  constexpr int WORK = (NEQ + 1) * ND > 2 * NEQ * NFP ? (NEQ + 1) * ND : 2 * NEQ * NFP;
-  // face-point fluxes are stored in rows of NFP per (face, variable); a row's
-  // entries are XOR-permuted by the row index (NFP a power of two) so that the
-  // lift's lanes, which read ten rows at once, hit different LDS banks
-  constexpr int SWZ = (NFP & (NFP - 1)) == 0 ? NFP - 1 : 0;
+  // face-point fluxes are stored in rows per (face, variable), padded by one
+  // entry when NFP is a power of two so that the lift's lanes, which read ten
+  // rows at once, hit different LDS banks
+  constexpr int RW = (NFP & (NFP - 1)) == 0 ? NFP + 1 : NFP;
+  constexpr int WORK = (NEQ + 1) * ND > 2 * NEQ * RW ? (NEQ + 1) * ND : 2 * NEQ * RW;
   constexpr int NB = DIM == 3 ? N : 1;  // face nodes per row (second tangential index b)
   // The lifted fluxes may share the face-point flux rows when every phase is a
   // single pass of the team (one wavefront per workgroup executes in order, so a
@@ -448,7 +446,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       }
       const double wgt = (DIM == 3 ? tb.wq[qi] * tb.wq[qj] : tb.wq[qi]) * snm;  // flux sign * w_q * |n|
 #pragma unroll
-      for (int v = 0; v < NEQ; v++) work[(c * NEQ + v) * NFP + (q ^ ((c * NEQ + v) & SWZ))] = fl[v] * wgt;
+      for (int v = 0; v < NEQ; v++) work[(c * NEQ + v) * RW + q] = fl[v] * wgt;
     }
     __syncthreads();
     // lift to face nodes, sum-factorised: one lane per (face, variable, a)
@@ -457,15 +455,14 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       if (!active) break;
       const int c = it / (NEQ * N), rem = it - c * (NEQ * N);
       const int v = rem / N, a = rem - v * N;
-      const int rv = c * NEQ + v, sw = rv & SWZ;
-      const double* w = work + rv * NFP;
+      const double* w = work + (c * NEQ + v) * RW;
       if (DIM == 3) {
         double W[NQ];
 #pragma unroll
         for (int qj = 0; qj < NQ; qj++) {
           double tt = 0.0;
 #pragma unroll
-          for (int qi = 0; qi < NQ; qi++) tt += w[(qi + NQ * qj) ^ sw] * tb.B[qi][a];
+          for (int qi = 0; qi < NQ; qi++) tt += w[qi + NQ * qj] * tb.B[qi][a];
           W[qj] = tt;
         }
 #pragma unroll
@@ -478,7 +475,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       } else {
         double L = 0.0;
 #pragma unroll
-        for (int qi = 0; qi < NQ; qi++) L += w[qi ^ sw] * tb.B[qi][a];
+        for (int qi = 0; qi < NQ; qi++) L += w[qi] * tb.B[qi][a];
         lift[(c * NEQ + v) * NFN + a] = L;
       }
     }
@@ -504,28 +501,40 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       double iw = tb.invw[jj[0]] * tb.invw[jj[1]];
       if (DIM == 3) iw *= tb.invw[jj[2]];
       const double im = iw * recip(detJ);
+      // every base is loaded before the first store (an output may be a base)
+      const int64_t i0 = e * ND + j;
       double base[4][NEQ];
 #pragma unroll
-      for (int m = 0; m < 4; m++)
+      for (int m = 0; m < 4; m++) {
+        const double* bp = m < A.nout ? A.outs[m].base : nullptr;
+        if (bp) {
+          bp += i0;
 #pragma unroll
-        for (int v = 0; v < NEQ; v++) {
-          const int64_t idx = v * NDOFS + e * ND + j;
-          base[m][v] = (m < A.nout && A.outs[m].base) ? A.outs[m].base[idx] : s_u[team][v][j];
+          for (int v = 0; v < NEQ; v++) base[m][v] = bp[v * NDOFS];
+        } else {
+#pragma unroll
+          for (int v = 0; v < NEQ; v++) base[m][v] = s_u[team][v][j];
         }
+      }
+      double* kp = A.k ? A.k + i0 : nullptr;
 #pragma unroll
-      for (int v = 0; v < NEQ; v++) {
-        const double yv = z[v] * im;
-        const int64_t idx = v * NDOFS + e * ND + j;
-        if (A.k) A.k[idx] = yv;
-        outv[v] = 0.0;
+      for (int v = 0; v < NEQ; v++) outv[v] = 0.0;
 #pragma unroll
-        for (int m = 0; m < 4; m++) {
-          if (m < A.nout) {
-            const double o = base[m][v] + A.outs[m].coef * yv;
-            A.outs[m].out[idx] = o;
+      for (int m = 0; m < 4; m++) {
+        if (m < A.nout) {
+          double* op = A.outs[m].out + i0;
+          const double cf = A.outs[m].coef;
+#pragma unroll
+          for (int v = 0; v < NEQ; v++) {
+            const double o = base[m][v] + cf * (z[v] * im);
+            op[v * NDOFS] = o;
             if (m == A.means_of) outv[v] = o;
           }
         }
+      }
+      if (kp) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) kp[v * NDOFS] = z[v] * im;
       }
     }
     if (A.means) {
