@@ -893,7 +893,14 @@ void prof_end(rkdg_ctx* c, std::pair<int, int> pr) {
   if (pr.second >= 0) (void)hipEventRecord(c->ev_pool[pr.second], c->stream);
 }
 
-hipError_t rhs_launch(rkdg_ctx* c, const RhsArgs& a) {
+hipError_t rhs_launch(rkdg_ctx* c, const RhsArgs& a0) {
+  // the tensor kernel forms the means of outs[0]: move the selected output
+  // there (the outputs are independent, every base is read before any store)
+  RhsArgs a = a0;
+  if (a.means && a.means_of > 0 && a.means_of < a.nout) {
+    std::swap(a.outs[0], a.outs[a.means_of]);
+    a.means_of = 0;
+  }
   auto pr = prof_begin(c, c->ev_rhs);
   hipError_t e = c->dense ? launch_dense_rhs(c->geom, c->p, a, c->sp, c->d_dtab, c->n_bdr > 0, c->stream)
                           : launch_tensor_rhs(c->dim, c->p, a, c->sp, c->d_tb, c->n_bdr > 0, c->stream);

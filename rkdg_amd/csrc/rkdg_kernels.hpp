@@ -36,7 +36,7 @@ struct RhsArgs {
   int nout;
   StageOut outs[4];
   double* means;     // optional: element means of outs[means_of] ([v*NE + e])
-  int means_of;
+  int means_of;      // the tensor kernel takes 0 (rhs_launch reorders the outputs)
   const double* geo;   // affine tensor geometry [g][NE]: adj(J) row-major, then det(J)
   const int* nbr;      // [lf][NE] neighbour element, or -1-bc_slot for boundary faces
   const int* fcode;    // [lf][NE] bits 0-2 neighbour local face, 3-5 orientation, 6 = Elem1

@@ -228,10 +228,13 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
 #pragma unroll
       for (int d = 0; d < DIM; d++) an[d] = sg * A.geo[(r * DIM + d) * NE + e];
     }
+    // |n| as the plain Euclidean norm (face normals are O(h^2): the scaling of
+    // MFEM's Norml2 against overflow changes nothing but the last bit)
     double nm2 = 0.0;
 #pragma unroll
     for (int d = 0; d < DIM; d++) nm2 += an[d] * an[d];
-    const double inv = 1.0 / norml2_scaled<DIM>(an);
+    const double nmag = sqrt_ws(nm2);
+    const double inv = recip(nmag);
     const int f = 2 * r + c;
     double nu2 = 0.0;
 #pragma unroll
@@ -240,8 +243,8 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       s_nrm[team][f][d] = nu;
       nu2 += nu * nu;
     }
-    s_nrm[team][f][DIM] = sign * sqrt(nm2);
-    s_nrm[team][f][DIM + 1] = 0.5 * recip(sqrt(nu2));
+    s_nrm[team][f][DIM] = sign * nmag;
+    s_nrm[team][f][DIM + 1] = 0.5 * recip(sqrt_ws(nu2));
   }
   for (int i = lane; active && i < GD; i += T) s_geo[team][i] = A.geo[i * NE + e];
   __syncthreads();
@@ -528,7 +531,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
           for (int v = 0; v < NEQ; v++) {
             const double o = base[m][v] + cf * (z[v] * im);
             op[v * NDOFS] = o;
-            if (m == A.means_of) outv[v] = o;
+            if (m == 0) outv[v] = o;  // means of outs[0] (the host puts the selected output first)
           }
         }
       }
