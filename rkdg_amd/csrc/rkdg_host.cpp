@@ -723,6 +723,20 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
   gl_rule(2, si, wi);
   for (int q = 0; q < 2; q++)
     for (int b = 0; b < N; b++) t.BI[q][b] = lag(t.x, N, b, si[q]);
+  {
+    int k = 0;
+    for (int a = 0; a < N; a++) t.packed[k++] = t.w[a];
+    for (int a = 0; a < N; a++) t.packed[k++] = t.invw[a];
+    for (int a = 0; a < N; a++)
+      for (int b = 0; b < N; b++) t.packed[k++] = t.D[a][b];
+    for (int c2 = 0; c2 < 2; c2++)
+      for (int b = 0; b < N; b++) t.packed[k++] = t.E[c2][b];
+    for (int q = 0; q < NQ; q++)
+      for (int b = 0; b < N; b++) t.packed[k++] = t.B[q][b];
+    for (int q = 0; q < NQ; q++) t.packed[k++] = t.wq[q];
+    for (int q = 0; q < 2; q++)
+      for (int b = 0; b < N; b++) t.packed[k++] = t.BI[q][b];
+  }
 
   // geometry (affine tensor elements): adj(J) row-major, det(J)
   const int G = dim * dim + 1;

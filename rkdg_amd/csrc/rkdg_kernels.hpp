@@ -11,6 +11,7 @@ namespace rkdg {
 
 constexpr int kMaxN = 6;  // up to p = 5 nodes per direction
 constexpr int kMaxQ = 8;  // face quadrature points per direction
+constexpr int kMaxPacked = 3 * kMaxN + kMaxN * kMaxN + 2 * kMaxN + kMaxQ * kMaxN + kMaxQ + 2 * kMaxN;
 
 // 1D tables of a tensor-product element of order p (N = p+1 Gauss-Legendre
 // nodes) and its face rule (NQ Gauss-Legendre points).
@@ -21,6 +22,9 @@ struct TensorTables {
   double B[kMaxQ][kMaxN];   // B[q][b] = psi_b(s_q), face points s_q
   double wq[kMaxQ];         // face weights
   double BI[2][kMaxN];      // psi_b at the two indicator points (IntRules order 2)
+  // the tables of this order packed as the kernels keep them in LDS: w[N],
+  // invw[N], D[N][N], E[2][N], B[NQ][N], wq[NQ], BI[2][N]
+  double packed[kMaxPacked];
 };
 
 // out = (base ? base : stage input) + coef * k, element-wise

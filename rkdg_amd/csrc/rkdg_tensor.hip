@@ -129,26 +129,45 @@ struct TT {
   double BI[2][N];
 };
 
-template <int N, int NQ>
-__device__ __forceinline__ void stage_tables(TT<N, NQ>& d, const TensorTables* s) {
-  constexpr int n = sizeof(TT<N, NQ>) / sizeof(double);
-  double* o = reinterpret_cast<double*>(&d);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    double v;
-    int k = i;
-    if (k < N) v = s->w[k];
-    else if ((k -= N) < N) v = s->invw[k];
-    else if ((k -= N) < N * N) v = s->D[k / N][k % N];
-    else if ((k -= N * N) < 2 * N) v = s->E[k / N][k % N];
-    else if ((k -= 2 * N) < NQ * N) v = s->B[k / N][k % N];
-    else if ((k -= NQ * N) < NQ) v = s->wq[k];
-    else {
-      k -= NQ;
-      v = s->BI[k / N][k % N];
-    }
-    o[i] = v;
+// LDS staging of the setup data in two steps, so that a workgroup issues every
+// global load of its setup before the first wait: load_* fill registers,
+// store_* write them to LDS.
+template <int N, int NQ, int BLOCK>
+struct TableRegs {
+  static constexpr int n = sizeof(TT<N, NQ>) / sizeof(double);
+  static constexpr int passes = (n + BLOCK - 1) / BLOCK;
+  double v[passes];
+  __device__ __forceinline__ void load(const TensorTables* s, int tid) {
+#pragma unroll
+    for (int p = 0; p < passes; p++) v[p] = tid + p * BLOCK < n ? s->packed[tid + p * BLOCK] : 0.0;
   }
-}
+  __device__ __forceinline__ void store(TT<N, NQ>& d, int tid) const {
+    double* o = reinterpret_cast<double*>(&d);
+#pragma unroll
+    for (int p = 0; p < passes; p++)
+      if (tid + p * BLOCK < n) o[tid + p * BLOCK] = v[p];
+  }
+};
+
+// the states of the EPB elements from e0 in one pass: lane tid holds node
+// tid % ND of element e0 + tid / ND (past the last element: 1.0)
+template <int EPB, int NEQ, int ND, int BLOCK>
+struct StateRegs {
+  static_assert(EPB * ND <= BLOCK, "one pass of the workgroup covers its elements' nodes");
+  double v[NEQ];
+  __device__ __forceinline__ void load(const double* x, int64_t NDOFS, int64_t e0, int64_t NE, int tid) {
+    const bool ok = tid < EPB * ND && e0 + tid / ND < NE;
+#pragma unroll
+    for (int q = 0; q < NEQ; q++) v[q] = ok ? x[q * NDOFS + e0 * ND + tid] : 1.0;
+  }
+  __device__ __forceinline__ void store(double (&su)[EPB][NEQ][ND], int tid) const {
+    if (tid < EPB * ND) {
+      const int te = tid / ND, tn = tid - te * ND;
+#pragma unroll
+      for (int q = 0; q < NEQ; q++) su[te][q][tn] = v[q];
+    }
+  }
+};
 
 // ---------------------------------------------------------------------------
 template <int DIM, int P, int RS, bool BC>
@@ -193,41 +212,41 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
   const int64_t e = e0 + team;
   const bool active = e < NE;
 
-  // ---- phase 0: tables, element state, connectivity, normals and geometry into LDS
-  stage_tables(tb, gtb);
+  // ---- phase 0: tables, element state, connectivity, normals and geometry
+  // into LDS. Every load is issued before the first LDS store: one memory round
+  // trip, plus one for the normal's geometry, which needs the connectivity.
+  static_assert(NF <= T, "one lane per local face");
+  TableRegs<N, NQ, BLOCK> tbr;
+  tbr.load(gtb, tid);
+  StateRegs<EPB, NEQ, ND, BLOCK> str;
+  str.load(A.x, NDOFS, e0, NE, tid);
+  constexpr int GDP = (GD + T - 1) / T;
+  double gv[GDP];
 #pragma unroll
-  for (int v = 0; v < NEQ; v++) {
-    const double* xv = A.x + v * NDOFS + e0 * ND;
-    for (int idx = tid; idx < EPB * ND; idx += BLOCK) {
-      const int te = idx / ND, tn = idx - te * ND;
-      s_u[te][v][tn] = (e0 + te < NE) ? xv[idx] : 1.0;
-    }
-  }
-  // connectivity of local face i and Elem1's unit normal on it (the neighbour's
-  // geometry when the neighbour is Elem1); the face code is loaded together with
-  // the neighbour index so that only the normal's gather waits for it
-  for (int i = lane; active && i < NF; i += T) {
-    const int nb = A.nbr[i * NE + e];
-    const int cd = A.fcode[i * NE + e];
-    const int code = nb >= 0 ? cd : 0;
+  for (int p = 0; p < GDP; p++) gv[p] = (active && lane + p * T < GD) ? A.geo[(lane + p * T) * NE + e] : 0.0;
+  // connectivity of local face i = lane and Elem1's unit normal on it (the
+  // neighbour's geometry when the neighbour is Elem1)
+  const bool face_lane = active && lane < NF;
+  const int nb0 = face_lane ? A.nbr[lane * NE + e] : -1;
+  const int cd0 = face_lane ? A.fcode[lane * NE + e] : 0;
+  if (face_lane) {
+    const int i = lane;
+    const int nb = nb0;
+    const int code = nb >= 0 ? cd0 : 0;
     s_nb[team][i] = nb;
     s_code[team][i] = code;
     int r, c;
     rc_of<DIM>(i, r, c);
+    int rn, cn;
+    rc_of<DIM>(code & 7, rn, cn);
+    const bool from_nb = nb >= 0 && !((code >> 6) & 1);
+    const double sign = from_nb ? 1.0 : -1.0;
+    const double sg = from_nb ? (cn ? 1.0 : -1.0) : (c ? 1.0 : -1.0);
+    const int64_t ge = from_nb ? (int64_t)nb : e;
+    const int gr = from_nb ? rn : r;
     double an[DIM];
-    double sign = -1.0;
-    if (nb >= 0 && !((code >> 6) & 1)) {
-      int rn, cn;
-      rc_of<DIM>(code & 7, rn, cn);
-      const double sn = cn ? 1.0 : -1.0;
 #pragma unroll
-      for (int d = 0; d < DIM; d++) an[d] = sn * A.geo[(rn * DIM + d) * NE + nb];
-      sign = 1.0;
-    } else {
-      const double sg = c ? 1.0 : -1.0;
-#pragma unroll
-      for (int d = 0; d < DIM; d++) an[d] = sg * A.geo[(r * DIM + d) * NE + e];
-    }
+    for (int d = 0; d < DIM; d++) an[d] = sg * A.geo[(gr * DIM + d) * NE + ge];
     // |n| as the plain Euclidean norm (face normals are O(h^2): the scaling of
     // MFEM's Norml2 against overflow changes nothing but the last bit)
     double nm2 = 0.0;
@@ -246,7 +265,11 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
     s_nrm[team][f][DIM] = sign * nmag;
     s_nrm[team][f][DIM + 1] = 0.5 * recip(sqrt_ws(nu2));
   }
-  for (int i = lane; active && i < GD; i += T) s_geo[team][i] = A.geo[i * NE + e];
+  tbr.store(tb, tid);
+  str.store(s_u, tid);
+#pragma unroll
+  for (int p = 0; p < GDP; p++)
+    if (active && lane + p * T < GD) s_geo[team][lane + p * T] = gv[p];
   __syncthreads();
 
   const int j = lane;
@@ -636,15 +659,11 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
   }
   const int vsel = g.limiter == RKDG_LIM_FINDIFF ? NEQ - 1 : 0;
 
-  stage_tables(tb, gtb);
-#pragma unroll
-  for (int v = 0; v < NEQ; v++) {
-    const double* xv = A.x + v * NDOFS + e0 * ND;
-    for (int idx = tid; idx < EPB * ND; idx += BLOCK) {
-      const int te = idx / ND, tn = idx - te * ND;
-      s_u[te][v][tn] = (e0 + te < NE) ? xv[idx] : 1.0;
-    }
-  }
+  // every load of the setup is issued before the first LDS store
+  TableRegs<N, C::NQ, BLOCK> tbr;
+  tbr.load(gtb, tid);
+  StateRegs<EPB, NEQ, ND, BLOCK> str;
+  str.load(A.x, NDOFS, e0, NE, tid);
   double mean[NEQ];
 #pragma unroll
   for (int v = 0; v < NEQ; v++) mean[v] = active ? A.means[v * NE + e] : 0.0;
@@ -681,6 +700,8 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
     const double k3 = g.venkat_k * g.venkat_k * g.venkat_k;
     eps2 = k3 * (DIM == 3 ? dj : dj * sqrt(dj));
   }
+  tbr.store(tb, tid);
+  str.store(s_u, tid);
   __syncthreads();
 
   // normal-direction traces of the deciding variable on every face
