@@ -47,17 +47,15 @@ __global__ __launch_bounds__(64) void dense_rhs_adj_kernel(const RhsAdjArgs A, c
   const int64_t e = e0 + team;
   const bool active = e < NE;
 
-  stage_dense<GEOM, P>(tb, gtab);
-#pragma unroll
-  for (int v = 0; v < NEQ; v++) {
-    const int64_t base = v * NDOFS + e0 * ND;
-    for (int idx = tid; idx < EPB * ND; idx += BLOCK) {
-      const int te = idx / ND, tn = idx - te * ND;
-      const bool in = e0 + te < NE;
-      s_u[te][v][tn] = in ? A.x[base + idx] : 1.0;
-      s_kb[te][v][tn] = in ? A.kbar[base + idx] : 0.0;
-    }
-  }
+  // every load of the states is issued before the first LDS store
+  CopyRegs<sizeof(DenseTab<GEOM, P>) / sizeof(double), BLOCK> tbr;
+  tbr.load(gtab, tid);
+  StateRegs<EPB, NEQ, ND, BLOCK> sx, skb;
+  sx.load(A.x, NDOFS, e0, NE, tid, 1.0);
+  skb.load(A.kbar, NDOFS, e0, NE, tid, 0.0);
+  tbr.store(reinterpret_cast<double*>(&tb), tid);
+  sx.store(s_u, tid);
+  skb.store(s_kb, tid);
   for (int i = lane; active && i < NF; i += T) {
     const int nb = A.nbr[i * NE + e];
     s_nb[team][i] = nb;
@@ -271,17 +269,15 @@ __global__ __launch_bounds__(64) void dense_limit_adj_a_kernel(const LimAdjArgs 
   const int64_t e = e0 + team;
   const bool active = e < NE;
 
-  stage_dense<GEOM, P>(tb, gtab);
-#pragma unroll
-  for (int v = 0; v < NEQ; v++) {
-    const int64_t base = v * NDOFS + e0 * ND;
-    for (int idx = tid; idx < EPB * ND; idx += BLOCK) {
-      const int te = idx / ND, tn = idx - te * ND;
-      const bool in = e0 + te < NE;
-      s_u[te][v][tn] = in ? A.x[base + idx] : 1.0;
-      s_ub[te][v][tn] = in ? A.ubar[base + idx] : 0.0;
-    }
-  }
+  // every load of the states is issued before the first LDS store
+  CopyRegs<sizeof(DenseTab<GEOM, P>) / sizeof(double), BLOCK> tbr;
+  tbr.load(gtab, tid);
+  StateRegs<EPB, NEQ, ND, BLOCK> sx, sub;
+  sx.load(A.x, NDOFS, e0, NE, tid, 1.0);
+  sub.load(A.ubar, NDOFS, e0, NE, tid, 0.0);
+  tbr.store(reinterpret_cast<double*>(&tb), tid);
+  sx.store(s_u, tid);
+  sub.store(s_ub, tid);
   __syncthreads();
   // one lane per element does the (small) serial recomputation
   if (active && lane == 0) {

@@ -43,21 +43,25 @@ __device__ __forceinline__ void dense_rhs_block(const RhsArgs& A, const SchemePa
   const int ins = (A.inst && e < NE) ? A.inst[e] : 0;
   const bool active = e < NE && (!A.active || A.active[ins]);
 
+  // every load of the block's setup is issued before the first LDS store
+  static_assert(NF <= T, "one lane per face");
+  StateRegs<EPB, NEQ, ND, BLOCK> str;
+  str.load(A.x, NDOFS, e0, NE, tid);
+  constexpr int GSP = (GS + T - 1) / T;
+  double gv[GSP];
 #pragma unroll
-  for (int v = 0; v < NEQ; v++) {
-    const double* xv = A.x + v * NDOFS + e0 * ND;
-    for (int idx = tid; idx < EPB * ND; idx += BLOCK) {
-      const int te = idx / ND, tn = idx - te * ND;
-      s_u[te][v][tn] = (e0 + te < NE) ? xv[idx] : 1.0;
-    }
+  for (int p = 0; p < GSP; p++) gv[p] = (active && lane + p * T < GS) ? A.geo[(int64_t)(lane + p * T) * NE + e] : 0.0;
+  const bool face_lane = active && lane < NF;
+  const int nb0 = face_lane ? A.nbr[lane * NE + e] : -1;
+  const int cd0 = face_lane ? A.fcode[lane * NE + e] : 0;
+  str.store(s_u, tid);
+  if (face_lane) {
+    s_nb[team][lane] = nb0;
+    s_code[team][lane] = nb0 >= 0 ? cd0 : 0;
   }
-  for (int i = lane; active && i < NF; i += T) {
-    const int nb = A.nbr[i * NE + e];
-    const int cd = A.fcode[i * NE + e];  // loaded together with nb, used for interior faces
-    s_nb[team][i] = nb;
-    s_code[team][i] = nb >= 0 ? cd : 0;
-  }
-  for (int i = lane; active && i < GS; i += T) s_geo[team][i] = A.geo[(int64_t)i * NE + e];
+#pragma unroll
+  for (int p = 0; p < GSP; p++)
+    if (active && lane + p * T < GS) s_geo[team][lane + p * T] = gv[p];
   __syncthreads();
 
   const int j = lane;
@@ -338,15 +342,11 @@ __global__ __launch_bounds__(64, 4) void dense_limit_kernel(const LimitArgs A, c
   const bool active = e < NE && (!A.active || A.active[A.inst[e]]);
   const int vsel = g.limiter == RKDG_LIM_FINDIFF ? NEQ - 1 : 0;
 
-  stage_dense<GEOM, P>(tb, gtab);
-#pragma unroll
-  for (int v = 0; v < NEQ; v++) {
-    const double* xv = A.x + v * NDOFS + e0 * ND;
-    for (int idx = tid; idx < EPB * ND; idx += BLOCK) {
-      const int te = idx / ND, tn = idx - te * ND;
-      s_u[te][v][tn] = (e0 + te < NE) ? xv[idx] : 1.0;
-    }
-  }
+  // every load of the setup is issued before the first LDS store
+  CopyRegs<sizeof(DenseTab<GEOM, P>) / sizeof(double), BLOCK> tbr;
+  tbr.load(gtab, tid);
+  StateRegs<EPB, NEQ, ND, BLOCK> str;
+  str.load(A.x, NDOFS, e0, NE, tid);
   double mean[NEQ];
 #pragma unroll
   for (int v = 0; v < NEQ; v++) mean[v] = active ? A.means[v * NE + e] : 0.0;
@@ -383,6 +383,8 @@ __global__ __launch_bounds__(64, 4) void dense_limit_kernel(const LimitArgs A, c
     const double kh = g.venkat_k * h;
     eps2 = kh * kh * kh;
   }
+  tbr.store(reinterpret_cast<double*>(&tb), tid);
+  str.store(s_u, tid);
   __syncthreads();
   double ylane = 1e9;  // this lane's minimum over its indicator points
   if (ind_bj) {

@@ -80,6 +80,49 @@ struct MeansArgs {
   int64_t NE;
 };
 
+// LDS staging of a kernel's setup data in two steps, so that a workgroup issues
+// every global load of its setup before the first wait: load() fills
+// registers, store() writes them to LDS. (A strided copy loop that the
+// compiler keeps as a loop waits for each load before its store.)
+//
+// n doubles from s, copied by the BLOCK lanes of the workgroup
+template <int n, int BLOCK>
+struct CopyRegs {
+  static constexpr int passes = (n + BLOCK - 1) / BLOCK;
+  double v[passes];
+  __device__ __forceinline__ void load(const double* s, int tid) {
+#pragma unroll
+    for (int p = 0; p < passes; p++) v[p] = tid + p * BLOCK < n ? s[tid + p * BLOCK] : 0.0;
+  }
+  __device__ __forceinline__ void store(double* o, int tid) const {
+#pragma unroll
+    for (int p = 0; p < passes; p++)
+      if (tid + p * BLOCK < n) o[tid + p * BLOCK] = v[p];
+  }
+};
+
+// the NEQ-variable states of the EPB elements from e0 (byNODES layout) in one
+// pass: lane tid holds node tid % ND of element e0 + tid / ND; past the last
+// element the value is `fill`
+template <int EPB, int NEQ, int ND, int BLOCK>
+struct StateRegs {
+  static_assert(EPB * ND <= BLOCK, "one pass of the workgroup covers its elements' nodes");
+  double v[NEQ];
+  __device__ __forceinline__ void load(const double* x, int64_t NDOFS, int64_t e0, int64_t NE, int tid,
+                                       double fill = 1.0) {
+    const bool ok = tid < EPB * ND && e0 + tid / ND < NE;
+#pragma unroll
+    for (int q = 0; q < NEQ; q++) v[q] = ok ? x[q * NDOFS + e0 * ND + tid] : fill;
+  }
+  __device__ __forceinline__ void store(double (&su)[EPB][NEQ][ND], int tid) const {
+    if (tid < EPB * ND) {
+      const int te = tid / ND, tn = tid - te * ND;
+#pragma unroll
+      for (int q = 0; q < NEQ; q++) su[te][q][tn] = v[q];
+    }
+  }
+};
+
 // Max-accumulate a non-negative double into *addr (non-negative doubles order
 // like their bit patterns). The current value is read first and only a larger
 // value issues the atomic: one same-address atomic per workgroup over a whole

@@ -129,46 +129,6 @@ struct TT {
   double BI[2][N];
 };
 
-// LDS staging of the setup data in two steps, so that a workgroup issues every
-// global load of its setup before the first wait: load_* fill registers,
-// store_* write them to LDS.
-template <int N, int NQ, int BLOCK>
-struct TableRegs {
-  static constexpr int n = sizeof(TT<N, NQ>) / sizeof(double);
-  static constexpr int passes = (n + BLOCK - 1) / BLOCK;
-  double v[passes];
-  __device__ __forceinline__ void load(const TensorTables* s, int tid) {
-#pragma unroll
-    for (int p = 0; p < passes; p++) v[p] = tid + p * BLOCK < n ? s->packed[tid + p * BLOCK] : 0.0;
-  }
-  __device__ __forceinline__ void store(TT<N, NQ>& d, int tid) const {
-    double* o = reinterpret_cast<double*>(&d);
-#pragma unroll
-    for (int p = 0; p < passes; p++)
-      if (tid + p * BLOCK < n) o[tid + p * BLOCK] = v[p];
-  }
-};
-
-// the states of the EPB elements from e0 in one pass: lane tid holds node
-// tid % ND of element e0 + tid / ND (past the last element: 1.0)
-template <int EPB, int NEQ, int ND, int BLOCK>
-struct StateRegs {
-  static_assert(EPB * ND <= BLOCK, "one pass of the workgroup covers its elements' nodes");
-  double v[NEQ];
-  __device__ __forceinline__ void load(const double* x, int64_t NDOFS, int64_t e0, int64_t NE, int tid) {
-    const bool ok = tid < EPB * ND && e0 + tid / ND < NE;
-#pragma unroll
-    for (int q = 0; q < NEQ; q++) v[q] = ok ? x[q * NDOFS + e0 * ND + tid] : 1.0;
-  }
-  __device__ __forceinline__ void store(double (&su)[EPB][NEQ][ND], int tid) const {
-    if (tid < EPB * ND) {
-      const int te = tid / ND, tn = tid - te * ND;
-#pragma unroll
-      for (int q = 0; q < NEQ; q++) su[te][q][tn] = v[q];
-    }
-  }
-};
-
 // ---------------------------------------------------------------------------
 template <int DIM, int P, int RS, bool BC>
 __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const SchemeParams g,
@@ -216,8 +176,8 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
   // into LDS. Every load is issued before the first LDS store: one memory round
   // trip, plus one for the normal's geometry, which needs the connectivity.
   static_assert(NF <= T, "one lane per local face");
-  TableRegs<N, NQ, BLOCK> tbr;
-  tbr.load(gtb, tid);
+  CopyRegs<sizeof(TT<N, NQ>) / sizeof(double), BLOCK> tbr;
+  tbr.load(gtb->packed, tid);
   StateRegs<EPB, NEQ, ND, BLOCK> str;
   str.load(A.x, NDOFS, e0, NE, tid);
   constexpr int GDP = (GD + T - 1) / T;
@@ -265,7 +225,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
     s_nrm[team][f][DIM] = sign * nmag;
     s_nrm[team][f][DIM + 1] = 0.5 * recip(sqrt_ws(nu2));
   }
-  tbr.store(tb, tid);
+  tbr.store(reinterpret_cast<double*>(&tb), tid);
   str.store(s_u, tid);
 #pragma unroll
   for (int p = 0; p < GDP; p++)
@@ -660,8 +620,8 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
   const int vsel = g.limiter == RKDG_LIM_FINDIFF ? NEQ - 1 : 0;
 
   // every load of the setup is issued before the first LDS store
-  TableRegs<N, C::NQ, BLOCK> tbr;
-  tbr.load(gtb, tid);
+  CopyRegs<sizeof(TT<N, C::NQ>) / sizeof(double), BLOCK> tbr;
+  tbr.load(gtb->packed, tid);
   StateRegs<EPB, NEQ, ND, BLOCK> str;
   str.load(A.x, NDOFS, e0, NE, tid);
   double mean[NEQ];
@@ -700,7 +660,7 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
     const double k3 = g.venkat_k * g.venkat_k * g.venkat_k;
     eps2 = k3 * (DIM == 3 ? dj : dj * sqrt(dj));
   }
-  tbr.store(tb, tid);
+  tbr.store(reinterpret_cast<double*>(&tb), tid);
   str.store(s_u, tid);
   __syncthreads();
 
