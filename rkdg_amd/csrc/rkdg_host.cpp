@@ -917,7 +917,7 @@ hipError_t rhs_launch(rkdg_ctx* c, const RhsArgs& a0) {
   }
   auto pr = prof_begin(c, c->ev_rhs);
   hipError_t e = c->dense ? launch_dense_rhs(c->geom, c->p, a, c->sp, c->d_dtab, c->n_bdr > 0, c->stream)
-                          : launch_tensor_rhs(c->dim, c->p, a, c->sp, c->d_tb, c->n_bdr > 0, c->stream);
+                          : launch_tensor_rhs(c->dim, c->p, a, c->sp, c->d_tb, c->tb, c->n_bdr > 0, c->stream);
   prof_end(c, pr);
   return e;
 }

@@ -136,9 +136,11 @@ __device__ __forceinline__ void atomic_max_pos(double* addr, double v) {
 }
 
 // launchers (rkdg_tensor.hip)
-// t: device copy of the tables; has_bdr selects the kernel with boundary faces
+// t: device copy of the tables, th: the host copy (its packed tables also go to
+// the kernel as an argument, for reads at wave-uniform indices from scalar
+// registers); has_bdr selects the kernel with boundary faces
 hipError_t launch_tensor_rhs(int dim, int p, const RhsArgs& a, const SchemeParams& g,
-                             const TensorTables* t, bool has_bdr, hipStream_t s);
+                             const TensorTables* t, const TensorTables& th, bool has_bdr, hipStream_t s);
 hipError_t launch_tensor_limit(int dim, int p, const LimitArgs& a, const SchemeParams& g,
                                const TensorTables* t, hipStream_t s);
 hipError_t launch_tensor_means(int dim, int p, const MeansArgs& a, const TensorTables* t,

@@ -19,6 +19,8 @@
 // elements of a face see bitwise identical fluxes and the scheme is
 // conservative to round-off. Kernels are specialised on the Riemann solver and
 // on the presence of boundary faces.
+#include <cstring>
+
 #include "rkdg_kernels.hpp"
 
 namespace rkdg {
@@ -132,7 +134,8 @@ struct TT {
 // ---------------------------------------------------------------------------
 template <int DIM, int P, int RS, bool BC>
 __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const SchemeParams g,
-                                                        const TensorTables* __restrict__ gtb) {
+                                                        const TensorTables* __restrict__ gtb,
+                                                        const TT<P + 1, TC<DIM, P>::NQ> kt) {
   using C = TC<DIM, P>;
   constexpr int N = C::N, ND = C::ND, NEQ = C::NEQ, NQ = C::NQ, NFP = C::NFP, NFN = C::NFN;
   constexpr int NF = C::NF, T = C::T, EPB = C::EPB, BLOCK = C::BLOCK;
@@ -337,7 +340,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       for (int qi = 0; qi < NQ; qi++) {
         double tt = 0.0;
 #pragma unroll
-        for (int a = 0; a < N; a++) tt += tr[a] * tb.B[qi][a];
+        for (int a = 0; a < N; a++) tt += tr[a] * kt.B[qi][a];
         s_to[team][c][v][qi][b] = tt;
       }
       if (nb >= 0) {
@@ -352,7 +355,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
         for (int qi = 0; qi < NQ; qi++) {
           double tt = 0.0;
 #pragma unroll
-          for (int a = 0; a < N; a++) tt += tr[a] * tb.B[qi][a];
+          for (int a = 0; a < N; a++) tt += tr[a] * kt.B[qi][a];
           s_tn[team][c][v][qi][b] = tt;
         }
       }
@@ -455,7 +458,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
         for (int b = 0; b < N; b++) {
           double L = 0.0;
 #pragma unroll
-          for (int qj = 0; qj < NQ; qj++) L += W[qj] * tb.B[qj][b];
+          for (int qj = 0; qj < NQ; qj++) L += W[qj] * kt.B[qj][b];
           lift[(c * NEQ + v) * NFN + a + N * b] = L;
         }
       } else {
@@ -857,30 +860,32 @@ bool tensor_supported(int dim, int p) {
 }
 
 template <int D, int PP, int RS>
-hipError_t launch_rhs_rs(const RhsArgs& a, const SchemeParams& g, const TensorTables* t, bool bc,
-                         hipStream_t s) {
+hipError_t launch_rhs_rs(const RhsArgs& a, const SchemeParams& g, const TensorTables* t,
+                         const TensorTables& th, bool bc, hipStream_t s) {
   using C = TC<D, PP>;
   const int64_t nb = (a.NE + C::EPB - 1) / C::EPB;
-  if (bc) tensor_rhs_kernel<D, PP, RS, true><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t);
-  else tensor_rhs_kernel<D, PP, RS, false><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t);
+  TT<C::N, C::NQ> kt;
+  std::memcpy(&kt, th.packed, sizeof(kt));
+  if (bc) tensor_rhs_kernel<D, PP, RS, true><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t, kt);
+  else tensor_rhs_kernel<D, PP, RS, false><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t, kt);
   return hipGetLastError();
 }
 
 template <int D, int PP>
-hipError_t launch_rhs_dp(const RhsArgs& a, const SchemeParams& g, const TensorTables* t, bool bc,
-                         hipStream_t s) {
+hipError_t launch_rhs_dp(const RhsArgs& a, const SchemeParams& g, const TensorTables* t,
+                         const TensorTables& th, bool bc, hipStream_t s) {
   switch (g.riemann) {
-    case RKDG_RS_RUSANOV: return launch_rhs_rs<D, PP, RKDG_RS_RUSANOV>(a, g, t, bc, s);
-    case RKDG_RS_LLF: return launch_rhs_rs<D, PP, RKDG_RS_LLF>(a, g, t, bc, s);
-    case RKDG_RS_HLL: return launch_rhs_rs<D, PP, RKDG_RS_HLL>(a, g, t, bc, s);
-    default: return launch_rhs_rs<D, PP, RKDG_RS_HLLC>(a, g, t, bc, s);
+    case RKDG_RS_RUSANOV: return launch_rhs_rs<D, PP, RKDG_RS_RUSANOV>(a, g, t, th, bc, s);
+    case RKDG_RS_LLF: return launch_rhs_rs<D, PP, RKDG_RS_LLF>(a, g, t, th, bc, s);
+    case RKDG_RS_HLL: return launch_rhs_rs<D, PP, RKDG_RS_HLL>(a, g, t, th, bc, s);
+    default: return launch_rhs_rs<D, PP, RKDG_RS_HLLC>(a, g, t, th, bc, s);
   }
 }
 
 hipError_t launch_tensor_rhs(int dim, int p, const RhsArgs& a, const SchemeParams& g,
-                             const TensorTables* t, bool has_bdr, hipStream_t s) {
+                             const TensorTables* t, const TensorTables& th, bool has_bdr, hipStream_t s) {
 #define X(D, PP) \
-  if (dim == D && p == PP) return launch_rhs_dp<D, PP>(a, g, t, has_bdr, s);
+  if (dim == D && p == PP) return launch_rhs_dp<D, PP>(a, g, t, th, has_bdr, s);
   RKDG_TENSOR_CASES(X)
 #undef X
   return hipErrorInvalidValue;
