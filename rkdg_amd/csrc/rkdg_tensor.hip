@@ -131,11 +131,23 @@ struct TT {
   double BI[2][N];
 };
 
+// tables read at wave-uniform indices, passed by value as a kernel argument
+// (scalar registers): the 1D tables and the tensor-product quadrature weights
+// w_k = w_a w_b (w_c) of the nodes with their sum
+template <int DIM, int N, int NQ>
+struct TArg {
+  static constexpr int ND = DIM == 3 ? N * N * N : N * N;
+  TT<N, NQ> t;
+  double wk[ND];
+  double wsum;
+};
+
 // ---------------------------------------------------------------------------
 template <int DIM, int P, int RS, bool BC>
 __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const SchemeParams g,
                                                         const TensorTables* __restrict__ gtb,
-                                                        const TT<P + 1, TC<DIM, P>::NQ> kt) {
+                                                        const TArg<DIM, P + 1, TC<DIM, P>::NQ> ka) {
+  const TT<P + 1, TC<DIM, P>::NQ>& kt = ka.t;
   using C = TC<DIM, P>;
   constexpr int N = C::N, ND = C::ND, NEQ = C::NEQ, NQ = C::NQ, NFP = C::NFP, NFN = C::NFN;
   constexpr int NF = C::NF, T = C::T, EPB = C::EPB, BLOCK = C::BLOCK;
@@ -305,6 +317,26 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
   // lift(r) + traces(r+1) | node accumulation(r) + points(r+1) ..., so each
   // axis costs two barriers.
   bool bad = false;
+  // Table values that depend on the lane but not on the axis, read from LDS
+  // once (the phases below are one pass of the team when ONE_PASS; otherwise
+  // they read the tables per item).
+  constexpr bool ONE_PASS = 2 * NEQ * N <= T && 2 * NFP <= T;
+  double ec[N];         // traces: E[c][k] of the lane's face side
+  double bqj[N];        // points: B[qj][b] of the lane's point
+  double bqa[NQ];       // lift: B[qi][a] of the lane's row
+  double wq2 = 0.0;     // points: the lane's face weight
+  if (ONE_PASS) {
+    const int ct = lane / (NEQ * NB);
+    const int at = (lane - (lane / (NEQ * N)) * (NEQ * N)) % N;
+    const int qt = lane % NFP, qit = qt % NQ, qjt = DIM == 3 ? qt / NQ : 0;
+#pragma unroll
+    for (int k = 0; k < N; k++) ec[k] = tb.E[ct < 2 ? ct : 0][k];
+#pragma unroll
+    for (int b = 0; b < N; b++) bqj[b] = tb.B[qjt][b];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) bqa[q] = tb.B[q][at];
+    wq2 = DIM == 3 ? tb.wq[qit] * tb.wq[qjt] : tb.wq[qit];
+  }
   // own normal-direction traces on faces (r,0), (r,1) and the neighbour's
   // traces in its own face frame (read from global memory), then interpolated
   // along the first tangential direction a to the Gauss abscissae qi. One lane
@@ -333,7 +365,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       for (int a = 0; a < N; a++) {
         double t = 0.0;
 #pragma unroll
-        for (int k = 0; k < N; k++) t += s_u[team][v][vnode<DIM, N>(r, k, a, b)] * tb.E[c][k];
+        for (int k = 0; k < N; k++) t += s_u[team][v][vnode<DIM, N>(r, k, a, b)] * (ONE_PASS ? ec[k] : tb.E[c][k]);
         tr[a] = t;
       }
 #pragma unroll
@@ -379,7 +411,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
         double t = 0.0;
         if (DIM == 3) {
 #pragma unroll
-          for (int b = 0; b < N; b++) t += s_to[team][c][v][qi][b] * tb.B[qj][b];
+          for (int b = 0; b < N; b++) t += s_to[team][c][v][qi][b] * (ONE_PASS ? bqj[b] : tb.B[qj][b]);
         } else {
           t = s_to[team][c][v][qi][0];
         }
@@ -433,7 +465,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       } else {
         lmax = lam > lmax ? lam : lmax;
       }
-      const double wgt = (DIM == 3 ? tb.wq[qi] * tb.wq[qj] : tb.wq[qi]) * snm;  // flux sign * w_q * |n|
+      const double wgt = (ONE_PASS ? wq2 : (DIM == 3 ? tb.wq[qi] * tb.wq[qj] : tb.wq[qi])) * snm;  // flux sign * w_q * |n|
 #pragma unroll
       for (int v = 0; v < NEQ; v++) work[(c * NEQ + v) * RW + q] = fl[v] * wgt;
     }
@@ -451,7 +483,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
         for (int qj = 0; qj < NQ; qj++) {
           double tt = 0.0;
 #pragma unroll
-          for (int qi = 0; qi < NQ; qi++) tt += w[qi + NQ * qj] * tb.B[qi][a];
+          for (int qi = 0; qi < NQ; qi++) tt += w[qi + NQ * qj] * (ONE_PASS ? bqa[qi] : tb.B[qi][a]);
           W[qj] = tt;
         }
 #pragma unroll
@@ -492,6 +524,9 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       const double im = iw * recip(detJ);
       // every base is loaded before the first store (an output may be a base)
       const int64_t i0 = e * ND + j;
+      double u0[NEQ];  // the stage input at this node (the base of outputs without one)
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) u0[v] = s_u[team][v][j];
       double base[4][NEQ];
 #pragma unroll
       for (int m = 0; m < 4; m++) {
@@ -502,7 +537,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
           for (int v = 0; v < NEQ; v++) base[m][v] = bp[v * NDOFS];
         } else {
 #pragma unroll
-          for (int v = 0; v < NEQ; v++) base[m][v] = s_u[team][v][j];
+          for (int v = 0; v < NEQ; v++) base[m][v] = u0[v];
         }
       }
       double* kp = A.k ? A.k + i0 : nullptr;
@@ -527,35 +562,31 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       }
     }
     if (A.means) {
-      // element means of the selected output: sum_k (w_k detJ) u_k / sum_k w_k detJ
+      // element means of the selected output: sum_k (w_k detJ) u_k / sum_k w_k detJ;
+      // det J is constant on an affine element and cancels
       __syncthreads();
       if (node_lane) {
-        double wj = tb.w[jj[0]] * tb.w[jj[1]];
-        if (DIM == 3) wj *= tb.w[jj[2]];
-        const double mk = wj * detJ;
 #pragma unroll
         for (int v = 0; v < NEQ; v++) work[v * ND + j] = outv[v];
-        work[NEQ * ND + j] = mk;
       }
       __syncthreads();
       if (active && lane < NEQ) {
-        double s = 0.0, ms = 0.0;
-        for (int k = 0; k < ND; k++) {
-          const double mk = work[NEQ * ND + k];
-          s += mk * work[lane * ND + k];
-          ms += mk;
-        }
-        A.means[lane * NE + e] = s / ms;
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < ND; k++) s += ka.wk[k] * work[lane * ND + k];
+        A.means[lane * NE + e] = s / ka.wsum;
       }
     }
   }
 
   // ---- max wave speed: wave reduction, one atomic per wave
-  for (int off = 32; off > 0; off >>= 1) {
-    const double o = __shfl_xor(lmax, off, 64);
-    lmax = o > lmax ? o : lmax;
+  if (!A.no_lambda) {
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(lmax, off, 64);
+      lmax = o > lmax ? o : lmax;
+    }
+    if (tid == 0) atomic_max_pos(A.lam, lmax);
   }
-  if (tid == 0 && !A.no_lambda) atomic_max_pos(A.lam, lmax);
 }
 
 // ---------------------------------------------------------------------------
@@ -864,10 +895,18 @@ hipError_t launch_rhs_rs(const RhsArgs& a, const SchemeParams& g, const TensorTa
                          const TensorTables& th, bool bc, hipStream_t s) {
   using C = TC<D, PP>;
   const int64_t nb = (a.NE + C::EPB - 1) / C::EPB;
-  TT<C::N, C::NQ> kt;
-  std::memcpy(&kt, th.packed, sizeof(kt));
-  if (bc) tensor_rhs_kernel<D, PP, RS, true><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t, kt);
-  else tensor_rhs_kernel<D, PP, RS, false><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t, kt);
+  TArg<D, C::N, C::NQ> ka;
+  std::memcpy(&ka.t, th.packed, sizeof(ka.t));
+  ka.wsum = 0.0;
+  for (int k = 0; k < C::ND; k++) {
+    const int i0 = k % C::N, i1 = (k / C::N) % C::N, i2 = k / (C::N * C::N);
+    double w = th.w[i0] * th.w[i1];
+    if (D == 3) w *= th.w[i2];
+    ka.wk[k] = w;
+    ka.wsum += w;
+  }
+  if (bc) tensor_rhs_kernel<D, PP, RS, true><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t, ka);
+  else tensor_rhs_kernel<D, PP, RS, false><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t, ka);
   return hipGetLastError();
 }
 
