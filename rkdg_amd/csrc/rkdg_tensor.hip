@@ -590,32 +590,39 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
 }
 
 // ---------------------------------------------------------------------------
+// element means (Averager::computeMeanValues): one team per element, lanes on
+// nodes (coalesced loads), sum_k w_k u_k / sum_k w_k by a shuffle reduction
+// over the team (det J is constant on an affine element and cancels)
 template <int DIM, int P>
-__global__ __launch_bounds__(256) void tensor_means_kernel(const MeansArgs A,
-                                                           const TensorTables* __restrict__ gtb) {
+__global__ __launch_bounds__(64) void tensor_means_kernel(const MeansArgs A,
+                                                          const TensorTables* __restrict__ gtb) {
   using C = TC<DIM, P>;
-  constexpr int N = C::N, ND = C::ND, NEQ = C::NEQ;
-  __shared__ double w[kMaxN];
-  if (threadIdx.x < N) w[threadIdx.x] = gtb->w[threadIdx.x];
-  __syncthreads();
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= A.NE) return;
-  const int64_t NDOFS = A.NE * ND;
-  const double detJ = A.geo[DIM * DIM * A.NE + e];
-  double ms = 0.0;
-  double s[NEQ];
+  constexpr int N = C::N, ND = C::ND, NEQ = C::NEQ, T = C::T, EPB = C::EPB;
+  const int tid = threadIdx.x;
+  const int team = tid / T, lane = tid % T;
+  const int64_t NE = A.NE;
+  const int64_t NDOFS = NE * ND;
+  const int64_t e = (int64_t)blockIdx.x * EPB + team;
+  const bool on = e < NE && lane < ND;
+  double wk = 0.0, s[NEQ];
 #pragma unroll
-  for (int v = 0; v < NEQ; v++) s[v] = 0.0;
-  for (int k = 0; k < ND; k++) {
-    double wk = w[k % N] * w[(k / N) % N];
-    if (DIM == 3) wk *= w[k / (N * N)];
-    const double mk = wk * detJ;
-    ms += mk;
-#pragma unroll
-    for (int v = 0; v < NEQ; v++) s[v] += mk * A.x[v * NDOFS + e * ND + k];
+  for (int v = 0; v < NEQ; v++) s[v] = on ? A.x[v * NDOFS + e * ND + lane] : 0.0;
+  if (on) {
+    wk = gtb->w[lane % N] * gtb->w[(lane / N) % N];
+    if (DIM == 3) wk *= gtb->w[lane / (N * N)];
   }
 #pragma unroll
-  for (int v = 0; v < NEQ; v++) A.means[v * A.NE + e] = s[v] / ms;
+  for (int v = 0; v < NEQ; v++) s[v] *= wk;
+#pragma unroll
+  for (int off = T / 2; off > 0; off >>= 1) {
+    wk += __shfl_xor(wk, off, 64);
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) s[v] += __shfl_xor(s[v], off, 64);
+  }
+  if (e < NE && lane == 0) {
+#pragma unroll
+    for (int v = 0; v < NEQ; v++) A.means[v * NE + e] = s[v] / wk;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -948,7 +955,8 @@ hipError_t launch_tensor_means(int dim, int p, const MeansArgs& a, const TensorT
                                hipStream_t s) {
 #define X(D, PP)                                                                      \
   if (dim == D && p == PP) {                                                          \
-    tensor_means_kernel<D, PP><<<(unsigned)((a.NE + 255) / 256), 256, 0, s>>>(a, t); \
+    using C = TC<D, PP>;                                                              \
+    tensor_means_kernel<D, PP><<<(unsigned)((a.NE + C::EPB - 1) / C::EPB), C::BLOCK, 0, s>>>(a, t); \
     return hipGetLastError();                                                         \
   }
   RKDG_TENSOR_CASES(X)
