@@ -645,7 +645,11 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
   __shared__ TT<C::N, C::NQ> tb;
   __shared__ double s_u[EPB][NEQ][ND];
   __shared__ double s_tr[EPB][NF][NFN];   // traces of the deciding variable
-  __shared__ double s_tx[EPB][2][NEQ][NFN]; // traces of every variable on the two x faces (last hope)
+  constexpr int NB = DIM == 3 ? N : 1;
+  // last hope: every variable on the x faces interpolated to y = 0, 1: [cx][cy][v][z node]
+  __shared__ double s_ty[EPB][2][2][NEQ][NB];
+  __shared__ double s_mean[EPB][NEQ];  // the element's means
+  __shared__ double s_nbm[EPB][NF];    // deciding variable's means of the face neighbours
 
   const int tid = threadIdx.x;
   const int team = tid / T, lane = tid % T;
@@ -665,34 +669,17 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
   tbr.load(gtb->packed, tid);
   StateRegs<EPB, NEQ, ND, BLOCK> str;
   str.load(A.x, NDOFS, e0, NE, tid);
-  double mean[NEQ];
-#pragma unroll
-  for (int v = 0; v < NEQ; v++) mean[v] = active ? A.means[v * NE + e] : 0.0;
-  double mI = 1e9, MI = -1e9;
+  // the element's means (lane v < NEQ) and the deciding variable's means of
+  // the face neighbours (lane lf < NF): one load instruction each, combined
+  // through LDS after the barrier (a wave-wide load per value would cost the
+  // memory pipeline 64 lanes for two distinct addresses)
+  static_assert(NEQ <= T && NF <= T, "one lane per variable and per face");
   const bool ind_bj = g.indicator == RKDG_IND_BJ;
-  int nbmask = 0;  // bit lf set: local face lf has a neighbour element
-  if (ind_bj) {
-    // stencil bounds (cell + face neighbours) of the deciding variable's means
-    if (active) {
-      const double mv = mean[vsel];
-      mI = mv < mI ? mv : mI;
-      MI = mv > MI ? mv : MI;
-      // branch-free, so the six index loads and then the six means loads are
-      // in flight together
-      int nbs[NF];
-#pragma unroll
-      for (int lf = 0; lf < NF; lf++) nbs[lf] = A.nbr[lf * NE + e];
-      double nm[NF];
-#pragma unroll
-      for (int lf = 0; lf < NF; lf++) nm[lf] = A.means[vsel * NE + (nbs[lf] >= 0 ? (int64_t)nbs[lf] : e)];
-#pragma unroll
-      for (int lf = 0; lf < NF; lf++) {
-        nbmask |= (nbs[lf] >= 0 ? 1 : 0) << lf;
-        mI = nm[lf] < mI ? nm[lf] : mI;
-        MI = nm[lf] > MI ? nm[lf] : MI;
-      }
-    }
-  }
+  const double mown = (active && lane < NEQ) ? A.means[lane * NE + e] : 0.0;
+  const int nbl = (active && ind_bj && lane < NF) ? A.nbr[lane * NE + e] : -1;
+  const double nml = nbl >= 0 ? A.means[vsel * NE + nbl] : 0.0;
+  const unsigned long long nbb = __ballot(nbl >= 0);
+  const int nbmask = (int)((nbb >> (team * T)) & ((1ull << NF) - 1ull));  // bit lf: face lf has a neighbour
   // Venkatakrishnan eps^2 = (K h)^3 with h = |element|^(1/dim): h^3 = det J
   // (3D) or det J^(3/2) (2D) for affine elements
   double eps2 = 0.0;
@@ -703,7 +690,27 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
   }
   tbr.store(reinterpret_cast<double*>(&tb), tid);
   str.store(s_u, tid);
+  if (lane < NEQ) s_mean[team][lane] = mown;
+  if (lane < NF) s_nbm[team][lane] = nml;
   __syncthreads();
+  double mean[NEQ];
+#pragma unroll
+  for (int v = 0; v < NEQ; v++) mean[v] = s_mean[team][v];
+  // stencil bounds (cell + face neighbours) of the deciding variable's means
+  double mI = 1e9, MI = -1e9;
+  if (ind_bj && active) {
+    const double mv = mean[vsel];
+    mI = mv;
+    MI = mv;
+#pragma unroll
+    for (int lf = 0; lf < NF; lf++) {
+      if ((nbmask >> lf) & 1) {
+        const double a = s_nbm[team][lf];
+        mI = a < mI ? a : mI;
+        MI = a > MI ? a : MI;
+      }
+    }
+  }
 
   // normal-direction traces of the deciding variable on every face
   if (ind_bj) {
@@ -722,16 +729,27 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
     }
   }
   if (g.limiter == RKDG_LIM_MULTI && g.last_hope) {
-    for (int it = lane; it < 2 * NFN; it += T) {
+    // sum-factorised vertex values, first two directions: one lane per
+    // (x side cx, variable, z node b) forms the x-face traces of its row and
+    // interpolates them to y = 0 and y = 1
+    for (int it = lane; it < 2 * NEQ * NB; it += T) {
       if (!active) break;
-      const int cx = it / NFN, fn = it - cx * NFN;
-      const int a = fn % N, b = fn / N;
+      const int cx = it / (NEQ * NB), rem = it - cx * (NEQ * NB);
+      const int v = rem / NB, b = rem - v * NB;
+      double tr[N];
 #pragma unroll
-      for (int v = 0; v < NEQ; v++) {
+      for (int a = 0; a < N; a++) {
         double t = 0.0;
 #pragma unroll
         for (int k = 0; k < N; k++) t += s_u[team][v][vnode<DIM, N>(0, k, a, b)] * tb.E[cx][k];
-        s_tx[team][cx][v][fn] = t;
+        tr[a] = t;
+      }
+#pragma unroll
+      for (int cy = 0; cy < 2; cy++) {
+        double tt = 0.0;
+#pragma unroll
+        for (int a = 0; a < N; a++) tt += tr[a] * tb.E[cy][a];
+        s_ty[team][cx][cy][v][b] = tt;
       }
     }
   }
@@ -821,7 +839,7 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
     if (g.last_hope) {
       bool bad_v = false;
       // the limited state at the reference vertices: mean + f (u(vertex) - mean),
-      // u(vertex) from the x-face traces (the nodal basis sums to one)
+      // u(vertex) from the two-direction partial sums (last direction here)
       for (int it = lane; it < NVERT; it += T) {
         if (!active) break;
         const int cx = it & 1, cy = (it >> 1) & 1, cz = (it >> 2) & 1;
@@ -830,10 +848,8 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
         for (int v = 0; v < NEQ; v++) {
           double s = 0.0;
 #pragma unroll
-          for (int b = 0; b < (DIM == 3 ? N : 1); b++) {
-            double tt = 0.0;
-#pragma unroll
-            for (int a = 0; a < N; a++) tt += s_tx[team][cx][v][a + N * b] * tb.E[cy][a];
+          for (int b = 0; b < NB; b++) {
+            const double tt = s_ty[team][cx][cy][v][b];
             s += DIM == 3 ? tt * tb.E[cz][b] : tt;
           }
           vs[v] = mean[v] + f * (s - mean[v]);
