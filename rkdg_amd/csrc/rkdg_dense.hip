@@ -332,6 +332,9 @@ __global__ __launch_bounds__(64, 4) void dense_limit_kernel(const LimitArgs A, c
 
   __shared__ DenseTab<GEOM, P> tb;
   __shared__ double s_u[EPB][NEQ][ND];
+  __shared__ double s_mean[EPB][NEQ];  // the element's means
+  __shared__ double s_nbm[EPB][NF];    // deciding variable's means of the face neighbours
+  __shared__ double s_area[EPB][ND];   // node mass weights (triangles: det J in [0])
 
   const int tid = threadIdx.x;
   const int team = tid / T, lane = tid % T;
@@ -347,45 +350,57 @@ __global__ __launch_bounds__(64, 4) void dense_limit_kernel(const LimitArgs A, c
   tbr.load(gtab, tid);
   StateRegs<EPB, NEQ, ND, BLOCK> str;
   str.load(A.x, NDOFS, e0, NE, tid);
+  // the element's means (lane v < NEQ), the deciding variable's means of the
+  // face neighbours (lane f < NF) and the node mass weights (lane k < ND): one
+  // load instruction each, combined through LDS after the barrier
+  static_assert(NEQ <= T && NF <= T && ND <= T, "one lane per variable, face and node");
+  const bool ind_bj = g.indicator == RKDG_IND_BJ;
+  const bool venkat = g.phi == RKDG_PHI_VENKAT && g.venkat_k > 0.0;
+  const double mown = (active && lane < NEQ) ? A.means[lane * NE + e] : 0.0;
+  const int nbl = (active && ind_bj && lane < NF) ? A.nbr[lane * NE + e] : -1;
+  const double nml = nbl >= 0 ? A.means[vsel * NE + nbl] : 0.0;
+  const unsigned long long nbb = __ballot(nbl >= 0);
+  const int nbmask = (int)((nbb >> (team * T)) & ((1ull << NF) - 1ull));  // bit f: face f has a neighbour
+  double arl = 0.0;
+  if (active && venkat) {
+    if (C::TRI) arl = lane == 0 ? A.geo[(int64_t)C::G_DET * NE + e] : 0.0;
+    else arl = lane < ND ? A.geo[(int64_t)(C::G_MW + lane) * NE + e] : 0.0;
+  }
+  tbr.store(reinterpret_cast<double*>(&tb), tid);
+  str.store(s_u, tid);
+  if (lane < NEQ) s_mean[team][lane] = mown;
+  if (lane < NF) s_nbm[team][lane] = nml;
+  if (lane < ND) s_area[team][lane] = arl;
+  __syncthreads();
   double mean[NEQ];
 #pragma unroll
-  for (int v = 0; v < NEQ; v++) mean[v] = active ? A.means[v * NE + e] : 0.0;
+  for (int v = 0; v < NEQ; v++) mean[v] = s_mean[team][v];
   double mI = 1e9, MI = -1e9;
-  const bool ind_bj = g.indicator == RKDG_IND_BJ;
-  int nbmask = 0;  // bit f set: face f has a neighbour element
   if (active && ind_bj) {
     const double mv = mean[vsel];
-    mI = mv < mI ? mv : mI;
-    MI = mv > MI ? mv : MI;
-    // branch-free, so the index loads and then the means loads are in flight together
-    int nbs[NF];
-#pragma unroll
-    for (int f = 0; f < NF; f++) nbs[f] = A.nbr[f * NE + e];
-    double nm[NF];
-#pragma unroll
-    for (int f = 0; f < NF; f++) nm[f] = A.means[vsel * NE + (nbs[f] >= 0 ? (int64_t)nbs[f] : e)];
+    mI = mv;
+    MI = mv;
 #pragma unroll
     for (int f = 0; f < NF; f++) {
-      nbmask |= (nbs[f] >= 0 ? 1 : 0) << f;
-      mI = nm[f] < mI ? nm[f] : mI;
-      MI = nm[f] > MI ? nm[f] : MI;
+      if ((nbmask >> f) & 1) {
+        const double a = s_nbm[team][f];
+        mI = a < mI ? a : mI;
+        MI = a > MI ? a : MI;
+      }
     }
   }
   double eps2 = 0.0;
-  if (active && g.phi == RKDG_PHI_VENKAT && g.venkat_k > 0.0) {
+  if (active && venkat) {
     // (K h)^3 with h = |element|^(1/2)
     double area = 0.0;
-    if (C::TRI) area = 0.5 * A.geo[(int64_t)C::G_DET * NE + e];
+    if (C::TRI) area = 0.5 * s_area[team][0];
     else
 #pragma unroll
-      for (int k = 0; k < ND; k++) area += A.geo[(int64_t)(C::G_MW + k) * NE + e];
+      for (int k = 0; k < ND; k++) area += s_area[team][k];
     const double h = sqrt(area);
     const double kh = g.venkat_k * h;
     eps2 = kh * kh * kh;
   }
-  tbr.store(reinterpret_cast<double*>(&tb), tid);
-  str.store(s_u, tid);
-  __syncthreads();
   double ylane = 1e9;  // this lane's minimum over its indicator points
   if (ind_bj) {
     for (int it = lane; it < NPT; it += T) {
