@@ -285,7 +285,7 @@ __device__ __forceinline__ void dense_rhs_block(const RhsArgs& A, const SchemePa
 
 // persistent: the grid strides over the work blocks
 template <int GEOM, int P, int RS, bool BC>
-__global__ __launch_bounds__(64, 4) void dense_rhs_kernel(const RhsArgs A, const SchemeParams g,
+__global__ __launch_bounds__(64, 3) void dense_rhs_kernel(const RhsArgs A, const SchemeParams g,
                                                        const double* __restrict__ gtab, int64_t nvb) {
   __shared__ DenseTab<GEOM, P> tb;
   stage_dense<GEOM, P>(tb, gtab);
