@@ -227,21 +227,43 @@ __device__ __forceinline__ void dense_rhs_block(const RhsArgs& A, const SchemePa
         mk = s_geo[team][C::G_MW + j];
       }
       const double dtm = A.dt_inst ? A.dt_inst[ins] : 1.0;
+      // every base is loaded before the first store (an output may be its own base)
+      const int64_t i0 = e * ND + j;
+      double u0[NEQ];
 #pragma unroll
-      for (int v = 0; v < NEQ; v++) {
-        const double yv = y[v];
-        const int64_t idx = v * NDOFS + e * ND + j;
-        if (A.k) A.k[idx] = yv;
-        outv[v] = 0.0;
+      for (int v = 0; v < NEQ; v++) u0[v] = s_u[team][v][j];
+      double base[4][NEQ];
 #pragma unroll
-        for (int m = 0; m < 4; m++) {
-          if (m < A.nout) {
-            const double base = A.outs[m].base ? A.outs[m].base[idx] : s_u[team][v][j];
-            const double o = base + (A.outs[m].coef * dtm) * yv;
-            A.outs[m].out[idx] = o;
+      for (int m = 0; m < 4; m++) {
+        const double* bp = m < A.nout ? A.outs[m].base : nullptr;
+        if (bp) {
+          bp += i0;
+#pragma unroll
+          for (int v = 0; v < NEQ; v++) base[m][v] = bp[v * NDOFS];
+        } else {
+#pragma unroll
+          for (int v = 0; v < NEQ; v++) base[m][v] = u0[v];
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < NEQ; v++) outv[v] = 0.0;
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        if (m < A.nout) {
+          double* op = A.outs[m].out + i0;
+          const double cf = A.outs[m].coef * dtm;
+#pragma unroll
+          for (int v = 0; v < NEQ; v++) {
+            const double o = base[m][v] + cf * y[v];
+            op[v * NDOFS] = o;
             if (m == A.means_of) outv[v] = o;
           }
         }
+      }
+      if (A.k) {
+        double* kp = A.k + i0;
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) kp[v * NDOFS] = y[v];
       }
     }
     if (A.means) {
