@@ -275,15 +275,34 @@ __global__ __launch_bounds__(64) void dense_limit_adj_a_kernel(const LimAdjArgs 
   StateRegs<EPB, NEQ, ND, BLOCK> sx, sub;
   sx.load(A.x, NDOFS, e0, NE, tid, 1.0);
   sub.load(A.ubar, NDOFS, e0, NE, tid, 0.0);
+  // the serial lane's element data, also loaded before the barrier and
+  // branch-free (the neighbours' density means in one round trip after the
+  // indices; a boundary face reads the element's own, which is ignored)
+  const bool serial = active && lane == 0;
+  double mean[NEQ];
+  int nbs[NF];
+  double nmd[NF], nml[NF];
+  double mw[C::TRI ? 1 : ND];
+#pragma unroll
+  for (int v = 0; v < NEQ; v++) mean[v] = serial ? A.means[v * NE + e] : 0.0;
+#pragma unroll
+  for (int fc = 0; fc < NF; fc++) nbs[fc] = serial ? A.nbr[fc * NE + e] : -1;
+#pragma unroll
+  for (int fc = 0; fc < NF; fc++) {
+    const int64_t ne = nbs[fc] >= 0 ? (int64_t)nbs[fc] : e;
+    nmd[fc] = serial ? A.means[ne] : 0.0;
+    nml[fc] = (serial && g.limiter == RKDG_LIM_FINDIFF) ? A.means[(NEQ - 1) * NE + ne] : 0.0;
+  }
+  if (C::TRI) mw[0] = serial ? A.geo[(int64_t)C::G_DET * NE + e] : 0.0;
+  else
+#pragma unroll
+    for (int k = 0; k < (C::TRI ? 1 : ND); k++) mw[k] = serial ? A.geo[(int64_t)(C::G_MW + k) * NE + e] : 0.0;
   tbr.store(reinterpret_cast<double*>(&tb), tid);
   sx.store(s_u, tid);
   sub.store(s_ub, tid);
   __syncthreads();
   // one lane per element does the (small) serial recomputation
-  if (active && lane == 0) {
-    double mean[NEQ];
-#pragma unroll
-    for (int v = 0; v < NEQ; v++) mean[v] = A.means[v * NE + e];
+  if (serial) {
     // objective on the limited state of this element (adds to ubar)
     double u[NEQ][ND];
     double alpha = 1.0, f = 1.0;
@@ -303,24 +322,25 @@ __global__ __launch_bounds__(64) void dense_limit_adj_a_kernel(const LimAdjArgs 
         if (a < mI) { mI = a; amin = e; }
         if (a > MI) { MI = a; amax = e; }
       }
+#pragma unroll
       for (int fc = 0; fc < NF; fc++) {
-        const int nb = A.nbr[fc * NE + e];
+        const int nb = nbs[fc];
         if (nb < 0) continue;
-        const double a = A.means[nb];
+        const double a = nmd[fc];
         if (a < mI) { mI = a; amin = nb; }
         if (a > MI) { MI = a; amax = nb; }
       }
       if (g.phi == RKDG_PHI_VENKAT && g.venkat_k > 0.0) {
         double area = 0.0;
-        if (C::TRI) area = 0.5 * A.geo[(int64_t)C::G_DET * NE + e];
+        if (C::TRI) area = 0.5 * mw[0];
         else
-          for (int k = 0; k < ND; k++) area += A.geo[(int64_t)(C::G_MW + k) * NE + e];
+          for (int k = 0; k < ND; k++) area += mw[k];
         const double kh = g.venkat_k * sqrt(area);
         eps2 = kh * kh * kh;
       }
       for (int it = 0; it < NPT; it++) {
         const int fc = it / 2, ip = it % 2;
-        if (A.nbr[fc * NE + e] < 0) continue;
+        if (nbs[fc] < 0) continue;
         double uu = 0.0;
 #pragma unroll
         for (int k = 0; k < ND; k++) uu += s_u[team][0][k] * tb.PhiI[fc][ip][k];
@@ -373,7 +393,7 @@ __global__ __launch_bounds__(64) void dense_limit_adj_a_kernel(const LimAdjArgs 
       if (ind_bj) {
         for (int it = 0; it < NPT; it++) {
           const int fc = it / 2, ip = it % 2;
-          if (A.nbr[fc * NE + e] < 0) continue;
+          if (nbs[fc] < 0) continue;
           double mIl = 1e9, MIl = -1e9;
           {
             const double a = mean[NEQ - 1];
@@ -381,9 +401,8 @@ __global__ __launch_bounds__(64) void dense_limit_adj_a_kernel(const LimAdjArgs 
             MIl = a > MIl ? a : MIl;
           }
           for (int f2 = 0; f2 < NF; f2++) {
-            const int nb = A.nbr[f2 * NE + e];
-            if (nb < 0) continue;
-            const double a = A.means[(NEQ - 1) * NE + nb];
+            if (nbs[f2] < 0) continue;
+            const double a = nml[f2];
             mIl = a < mIl ? a : mIl;
             MIl = a > MIl ? a : MIl;
           }
@@ -412,7 +431,7 @@ __global__ __launch_bounds__(64) void dense_limit_adj_a_kernel(const LimAdjArgs 
     // fused objective: ubar += coef * d/du of sum_q w|n| p(trace) on faces of obj_slot
     if (A.obj_slot >= 0) {
       for (int fc = 0; fc < NF; fc++) {
-        if (A.nbr[fc * NE + e] != -1 - A.obj_slot) continue;
+        if (nbs[fc] != -1 - A.obj_slot) continue;
         const double an0 = A.geo[(int64_t)(C::G_NRM + 2 * fc) * NE + e];
         const double an1 = A.geo[(int64_t)(C::G_NRM + 2 * fc + 1) * NE + e];
         const double normag = sqrt(an0 * an0 + an1 * an1);
@@ -502,11 +521,21 @@ __global__ __launch_bounds__(256) void dense_limit_adj_b_kernel(const LimAdjArgs
 #pragma unroll
   for (int v = 0; v < NEQ; v++) mb[v] = A.meanbar[v * NE + e];
   if (A.send_to[e] == e) mb[0] += A.send_val[e];
+  // branch-free: the indices, then every neighbour's record, in flight together
+  int nbs[NF];
+#pragma unroll
+  for (int fc = 0; fc < NF; fc++) nbs[fc] = A.nbr[fc * NE + e];
+  int64_t st[NF];
+  double sv[NF];
+#pragma unroll
   for (int fc = 0; fc < NF; fc++) {
-    const int nb = A.nbr[fc * NE + e];
-    if (nb < 0) continue;
-    if (A.send_to[nb] == e) mb[0] += A.send_val[nb];
+    const int64_t ne = nbs[fc] >= 0 ? (int64_t)nbs[fc] : e;
+    st[fc] = A.send_to[ne];
+    sv[fc] = A.send_val[ne];
   }
+#pragma unroll
+  for (int fc = 0; fc < NF; fc++)
+    if (nbs[fc] >= 0 && st[fc] == e) mb[0] += sv[fc];
   double mw[ND], ms = 0.0;
 #pragma unroll
   for (int k = 0; k < ND; k++) {
