@@ -169,33 +169,6 @@ RKDG_HD void inverse_rotate(double* s, const double* n) {
   }
 }
 
-// Toro two-rarefaction speeds (physics.cpp:309-350) or Einfeldt (:269-296) on
-// rotated states; lo/hi = lambdaF[0], lambdaF[dim+1].
-template <int DIM>
-RKDG_HD void wave_speeds(const double* s1, const double* s2, double p1, double p2,
-                         const SchemeParams& g, double& lo, double& hi) {
-  const double cL = sound_speed_p<DIM>(s1, p1, g), cR = sound_speed_p<DIM>(s2, p2, g);
-  const double uL = s1[1] / s1[0], uR = s2[1] / s2[0];
-  if (g.wave_speeds == RKDG_WS_EINFELDT) {
-    const double sL = sqrt(s1[0]), sR = sqrt(s2[0]);
-    const double sum = sL + sR;
-    const double eta2 = 0.5 * sL * sR / (sum * sum);
-    const double uav = (s1[1] / sL + s2[1] / sR) / sum;
-    const double cav = sqrt((sL * (cL * cL) + sR * (cR * cR)) / sum + eta2 * ((uR - uL) * (uR - uL)));
-    lo = uav - cav;
-    hi = uav + cav;
-    return;
-  }
-  const double gm = g.gamma;
-  const double z = 0.5 * (gm - 1.0) / gm;
-  const double pStar = pow((cL + cR - 0.5 * (gm - 1.0) * (uR - uL)) / (cL / pow(p1, z) + cR / pow(p2, z)),
-                           1.0 / z);
-  const double qL = pStar > p1 ? sqrt(1.0 + 0.5 * (gm + 1.0) * (pStar / p1 - 1.0) / gm) : 1.0;
-  const double qR = pStar > p2 ? sqrt(1.0 + 0.5 * (gm + 1.0) * (pStar / p2 - 1.0) / gm) : 1.0;
-  lo = uL - qL * cL;
-  hi = uR + qR * cR;
-}
-
 // Numerical flux through the unit normal n (rs_rusanov.cpp:6-40, rs_llf.cpp:6-37,
 // rs_hll.cpp:6-50, rs_hllc.cpp:71-161 with getFStar :39-67). The states are
 // taken by value (the reference rotates its inputs in place; nothing observes
@@ -233,6 +206,86 @@ RKDG_HD double sqrt_ws(double x) {
 #else
   return sqrt(x);
 #endif
+}
+
+// x^y for x > 0 on the device, for the two-rarefaction pressure estimate:
+// log x from frexp and the series log m = 2 atanh(s), s = (m-1)/(m+1), with m
+// in [1/sqrt2, sqrt2) (|s| < 0.172, terms to s^21), ln 2 split in a high part
+// with trailing zero bits and a low part; exp t by t = k ln2 + r, |r| <= ln2/2,
+// and the Taylor polynomial of degree 13. Relative error within ~1e-14 of the
+// library pow (the error of y log x grows with |y log x|); a third of its
+// instructions. The host build uses the library pow.
+RKDG_HD double pow_ws(double x, double y) {
+#ifdef __HIP_DEVICE_COMPILE__
+  int ex;
+  double m = frexp(x, &ex);  // x = m 2^ex, m in [0.5, 1)
+  if (m < 0.70710678118654752440) {
+    m *= 2.0;
+    ex -= 1;
+  }
+  const double s = (m - 1.0) * recip(m + 1.0);
+  const double s2 = s * s;
+  double p = 1.0 / 21.0;
+  p = fma(p, s2, 1.0 / 19.0);
+  p = fma(p, s2, 1.0 / 17.0);
+  p = fma(p, s2, 1.0 / 15.0);
+  p = fma(p, s2, 1.0 / 13.0);
+  p = fma(p, s2, 1.0 / 11.0);
+  p = fma(p, s2, 1.0 / 9.0);
+  p = fma(p, s2, 1.0 / 7.0);
+  p = fma(p, s2, 1.0 / 5.0);
+  p = fma(p, s2, 1.0 / 3.0);
+  const double lm = 2.0 * s + 2.0 * s * s2 * p;  // log m
+  const double LN2_HI = 6.93147180369123816490e-01, LN2_LO = 1.90821492927058770002e-10;
+  const double lx = fma((double)ex, LN2_HI, fma((double)ex, LN2_LO, lm));
+  const double t = y * lx;
+  const double k = rint(t * 1.44269504088896340736);  // t / ln 2
+  const double r = fma(-k, LN2_LO, fma(-k, LN2_HI, t));
+  double q = 1.0 / 6227020800.0;  // 1/13!
+  q = fma(q, r, 1.0 / 479001600.0);
+  q = fma(q, r, 1.0 / 39916800.0);
+  q = fma(q, r, 1.0 / 3628800.0);
+  q = fma(q, r, 1.0 / 362880.0);
+  q = fma(q, r, 1.0 / 40320.0);
+  q = fma(q, r, 1.0 / 5040.0);
+  q = fma(q, r, 1.0 / 720.0);
+  q = fma(q, r, 1.0 / 120.0);
+  q = fma(q, r, 1.0 / 24.0);
+  q = fma(q, r, 1.0 / 6.0);
+  q = fma(q, r, 0.5);
+  q = fma(q, r, 1.0);
+  q = fma(q, r, 1.0);
+  return ldexp(q, (int)k);
+#else
+  return pow(x, y);
+#endif
+}
+
+// Toro two-rarefaction speeds (physics.cpp:309-350) or Einfeldt (:269-296) on
+// rotated states; lo/hi = lambdaF[0], lambdaF[dim+1].
+template <int DIM>
+RKDG_HD void wave_speeds(const double* s1, const double* s2, double p1, double p2,
+                         const SchemeParams& g, double& lo, double& hi) {
+  const double cL = sound_speed_p<DIM>(s1, p1, g), cR = sound_speed_p<DIM>(s2, p2, g);
+  const double uL = s1[1] / s1[0], uR = s2[1] / s2[0];
+  if (g.wave_speeds == RKDG_WS_EINFELDT) {
+    const double sL = sqrt(s1[0]), sR = sqrt(s2[0]);
+    const double sum = sL + sR;
+    const double eta2 = 0.5 * sL * sR / (sum * sum);
+    const double uav = (s1[1] / sL + s2[1] / sR) / sum;
+    const double cav = sqrt((sL * (cL * cL) + sR * (cR * cR)) / sum + eta2 * ((uR - uL) * (uR - uL)));
+    lo = uav - cav;
+    hi = uav + cav;
+    return;
+  }
+  const double gm = g.gamma;
+  const double z = 0.5 * (gm - 1.0) / gm;
+  const double pStar = pow_ws((cL + cR - 0.5 * (gm - 1.0) * (uR - uL)) / (cL / pow_ws(p1, z) + cR / pow_ws(p2, z)),
+                              1.0 / z);
+  const double qL = pStar > p1 ? sqrt(1.0 + 0.5 * (gm + 1.0) * (pStar / p1 - 1.0) / gm) : 1.0;
+  const double qR = pStar > p2 ? sqrt(1.0 + 0.5 * (gm + 1.0) * (pStar / p2 - 1.0) / gm) : 1.0;
+  lo = uL - qL * cL;
+  hi = uR + qR * cR;
 }
 
 template <int DIM>
