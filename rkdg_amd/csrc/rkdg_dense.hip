@@ -75,7 +75,7 @@ __device__ __forceinline__ void dense_rhs_block(const RhsArgs& A, const SchemePa
 #pragma unroll
     for (int v = 0; v < NEQ; v++) st[v] = s_u[team][v][j];
     const SidePrim<2> q = side_prim<2>(st, g);
-    lmax = sqrt(q.m2) * q.inv + side_sound<2>(st, q, g);
+    lmax = sqrt_ws(q.m2) * q.inv + side_sound<2>(st, q, g);
     const double* adj = TRI ? &s_geo[team][C::G_ADJ] : &s_geo[team][C::G_ADJ + 4 * j];
 #pragma unroll
     for (int r = 0; r < 2; r++) {
@@ -115,7 +115,7 @@ __device__ __forceinline__ void dense_rhs_block(const RhsArgs& A, const SchemePa
       an[0] = A.geo[(int64_t)(C::G_NRM + 2 * fn) * NE + nb];
       an[1] = A.geo[(int64_t)(C::G_NRM + 2 * fn + 1) * NE + nb];
     }
-    const double normag = sqrt(an[0] * an[0] + an[1] * an[1]);
+    const double normag = sqrt_ws(an[0] * an[0] + an[1] * an[1]);
     const double inv = 1.0 / norml2_scaled<2>(an);
     const double nrm[2] = {an[0] * inv, an[1] * inv};
     if (!BC || nb >= 0) {

@@ -24,6 +24,94 @@ struct SchemeParams {
   int limiter, last_hope, pow_nd;
 };
 
+// 1/x on the device: v_rcp_f64 refined by two Newton steps (correct to about
+// one ulp). Per side, 1/rho is formed once and reused where the reference
+// divides by rho in each formula; the differences are rounding-level.
+RKDG_HD double recip(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return r;
+#else
+  return 1.0 / x;
+#endif
+}
+
+// Square root on the device (wave speeds, sound speeds, boundary states):
+// v_rsq_f64 refined by one Goldschmidt step (g = x r, h = r/2, e = 1/2 - h g)
+// and one Newton correction of g, about half the instructions of the correctly
+// rounded sqrt and within an ulp of it; sqrt_ws(0) = 0, negative input gives NaN as sqrt does. The host
+// build uses the library sqrt.
+RKDG_HD double sqrt_ws(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  const double r = __builtin_amdgcn_rsq(x);
+  double gq = x * r, h = 0.5 * r;
+  const double e = fma(-h, gq, 0.5);
+  gq = fma(gq, e, gq);
+  h = fma(h, e, h);
+  const double d = fma(-gq, gq, x);
+  gq = fma(d, h, gq);
+  return x == 0.0 ? 0.0 : gq;
+#else
+  return sqrt(x);
+#endif
+}
+
+// x^y for x > 0 on the device (the two-rarefaction pressure estimate, the
+// isentropic boundary states):
+// log x from frexp and the series log m = 2 atanh(s), s = (m-1)/(m+1), with m
+// in [1/sqrt2, sqrt2) (|s| < 0.172, terms to s^21), ln 2 split in a high part
+// with trailing zero bits and a low part; exp t by t = k ln2 + r, |r| <= ln2/2,
+// and the Taylor polynomial of degree 13. Relative error within ~1e-14 of the
+// library pow (the error of y log x grows with |y log x|); a third of its
+// instructions. The host build uses the library pow.
+RKDG_HD double pow_ws(double x, double y) {
+#ifdef __HIP_DEVICE_COMPILE__
+  int ex;
+  double m = frexp(x, &ex);  // x = m 2^ex, m in [0.5, 1)
+  if (m < 0.70710678118654752440) {
+    m *= 2.0;
+    ex -= 1;
+  }
+  const double s = (m - 1.0) * recip(m + 1.0);
+  const double s2 = s * s;
+  double p = 1.0 / 21.0;
+  p = fma(p, s2, 1.0 / 19.0);
+  p = fma(p, s2, 1.0 / 17.0);
+  p = fma(p, s2, 1.0 / 15.0);
+  p = fma(p, s2, 1.0 / 13.0);
+  p = fma(p, s2, 1.0 / 11.0);
+  p = fma(p, s2, 1.0 / 9.0);
+  p = fma(p, s2, 1.0 / 7.0);
+  p = fma(p, s2, 1.0 / 5.0);
+  p = fma(p, s2, 1.0 / 3.0);
+  const double lm = 2.0 * s + 2.0 * s * s2 * p;  // log m
+  const double LN2_HI = 6.93147180369123816490e-01, LN2_LO = 1.90821492927058770002e-10;
+  const double lx = fma((double)ex, LN2_HI, fma((double)ex, LN2_LO, lm));
+  const double t = y * lx;
+  const double k = rint(t * 1.44269504088896340736);  // t / ln 2
+  const double r = fma(-k, LN2_LO, fma(-k, LN2_HI, t));
+  double q = 1.0 / 6227020800.0;  // 1/13!
+  q = fma(q, r, 1.0 / 479001600.0);
+  q = fma(q, r, 1.0 / 39916800.0);
+  q = fma(q, r, 1.0 / 3628800.0);
+  q = fma(q, r, 1.0 / 362880.0);
+  q = fma(q, r, 1.0 / 40320.0);
+  q = fma(q, r, 1.0 / 5040.0);
+  q = fma(q, r, 1.0 / 720.0);
+  q = fma(q, r, 1.0 / 120.0);
+  q = fma(q, r, 1.0 / 24.0);
+  q = fma(q, r, 1.0 / 6.0);
+  q = fma(q, r, 0.5);
+  q = fma(q, r, 1.0);
+  q = fma(q, r, 1.0);
+  return ldexp(q, (int)k);
+#else
+  return pow(x, y);
+#endif
+}
+
 // physics.cpp:101-112 (covolume EoS; b == 0 skips the exact division by 1)
 template <int DIM>
 RKDG_HD double pressure(const double* s, const SchemeParams& g) {
@@ -41,7 +129,7 @@ template <int DIM>
 RKDG_HD double sound_speed_p(const double* s, double p, const SchemeParams& g) {
   double a = g.gamma * p / s[0];
   if (g.covolume != 0.0) a /= (1.0 - s[0] * g.covolume);
-  return sqrt(a);
+  return sqrt_ws(a);
 }
 
 // physics.cpp:53-97 (pressure check without covolume, :84)
@@ -78,7 +166,7 @@ RKDG_HD double max_char_speed(const double* s, const SchemeParams& g) {
   double p = (g.gamma - 1.0) * (s[1 + DIM] - 0.5 * v2);
   if (g.covolume != 0.0) p /= (1.0 - s[0] * g.covolume);
   double c = sound_speed_p<DIM>(s, p, g);
-  return sqrt(v2 / s[0]) + c;
+  return sqrt_ws(v2 / s[0]) + c;
 }
 
 // physics.cpp:143-168; f[k*DIM + d]
@@ -134,7 +222,7 @@ RKDG_HD void rotate(double* s, const double* n) {
     m[1] = my;
   } else {
     const double sp = n[2];
-    const double cp = sqrt(1 - sp * sp);
+    const double cp = sqrt_ws(1 - sp * sp);
     const double st = fabs(cp) > 1e-6 ? n[1] / cp : 1.0;
     const double ct = fabs(cp) > 1e-6 ? n[0] / cp : 0.0;
     double mx = cp * ct * m[0] + cp * st * m[1] + sp * m[2];
@@ -157,7 +245,7 @@ RKDG_HD void inverse_rotate(double* s, const double* n) {
     m[1] = my;
   } else {
     const double sp = n[2];
-    const double cp = sqrt(1 - sp * sp);
+    const double cp = sqrt_ws(1 - sp * sp);
     const double st = fabs(cp) > 1e-6 ? n[1] / cp : 1.0;
     const double ct = fabs(cp) > 1e-6 ? n[0] / cp : 0.0;
     double mx = cp * ct * m[0] - st * m[1] - sp * ct * m[2];
@@ -169,98 +257,6 @@ RKDG_HD void inverse_rotate(double* s, const double* n) {
   }
 }
 
-// Numerical flux through the unit normal n (rs_rusanov.cpp:6-40, rs_llf.cpp:6-37,
-// rs_hll.cpp:6-50, rs_hllc.cpp:71-161 with getFStar :39-67). The states are
-// taken by value (the reference rotates its inputs in place; nothing observes
-// it). Returns the solver's wave speed; ok=false flags an unphysical input.
-//
-// Per side, 1/rho is formed once and reused (the reference divides by rho in
-// each formula); on the device it is v_rcp_f64 refined by two Newton steps
-// (correct to about one ulp). The differences are rounding-level.
-RKDG_HD double recip(double x) {
-#ifdef __HIP_DEVICE_COMPILE__
-  double r = __builtin_amdgcn_rcp(x);
-  r = fma(fma(-x, r, 1.0), r, r);
-  r = fma(fma(-x, r, 1.0), r, r);
-  return r;
-#else
-  return 1.0 / x;
-#endif
-}
-
-// Square root for wave-speed estimates on the device: v_rsq_f64 refined by one
-// Goldschmidt step (g = x r, h = r/2, e = 1/2 - h g) and one Newton correction
-// of g, about half the instructions of the correctly rounded sqrt and within an
-// ulp of it; sqrt_ws(0) = 0, negative input gives NaN as sqrt does. The host
-// build uses the library sqrt.
-RKDG_HD double sqrt_ws(double x) {
-#ifdef __HIP_DEVICE_COMPILE__
-  const double r = __builtin_amdgcn_rsq(x);
-  double gq = x * r, h = 0.5 * r;
-  const double e = fma(-h, gq, 0.5);
-  gq = fma(gq, e, gq);
-  h = fma(h, e, h);
-  const double d = fma(-gq, gq, x);
-  gq = fma(d, h, gq);
-  return x == 0.0 ? 0.0 : gq;
-#else
-  return sqrt(x);
-#endif
-}
-
-// x^y for x > 0 on the device, for the two-rarefaction pressure estimate:
-// log x from frexp and the series log m = 2 atanh(s), s = (m-1)/(m+1), with m
-// in [1/sqrt2, sqrt2) (|s| < 0.172, terms to s^21), ln 2 split in a high part
-// with trailing zero bits and a low part; exp t by t = k ln2 + r, |r| <= ln2/2,
-// and the Taylor polynomial of degree 13. Relative error within ~1e-14 of the
-// library pow (the error of y log x grows with |y log x|); a third of its
-// instructions. The host build uses the library pow.
-RKDG_HD double pow_ws(double x, double y) {
-#ifdef __HIP_DEVICE_COMPILE__
-  int ex;
-  double m = frexp(x, &ex);  // x = m 2^ex, m in [0.5, 1)
-  if (m < 0.70710678118654752440) {
-    m *= 2.0;
-    ex -= 1;
-  }
-  const double s = (m - 1.0) * recip(m + 1.0);
-  const double s2 = s * s;
-  double p = 1.0 / 21.0;
-  p = fma(p, s2, 1.0 / 19.0);
-  p = fma(p, s2, 1.0 / 17.0);
-  p = fma(p, s2, 1.0 / 15.0);
-  p = fma(p, s2, 1.0 / 13.0);
-  p = fma(p, s2, 1.0 / 11.0);
-  p = fma(p, s2, 1.0 / 9.0);
-  p = fma(p, s2, 1.0 / 7.0);
-  p = fma(p, s2, 1.0 / 5.0);
-  p = fma(p, s2, 1.0 / 3.0);
-  const double lm = 2.0 * s + 2.0 * s * s2 * p;  // log m
-  const double LN2_HI = 6.93147180369123816490e-01, LN2_LO = 1.90821492927058770002e-10;
-  const double lx = fma((double)ex, LN2_HI, fma((double)ex, LN2_LO, lm));
-  const double t = y * lx;
-  const double k = rint(t * 1.44269504088896340736);  // t / ln 2
-  const double r = fma(-k, LN2_LO, fma(-k, LN2_HI, t));
-  double q = 1.0 / 6227020800.0;  // 1/13!
-  q = fma(q, r, 1.0 / 479001600.0);
-  q = fma(q, r, 1.0 / 39916800.0);
-  q = fma(q, r, 1.0 / 3628800.0);
-  q = fma(q, r, 1.0 / 362880.0);
-  q = fma(q, r, 1.0 / 40320.0);
-  q = fma(q, r, 1.0 / 5040.0);
-  q = fma(q, r, 1.0 / 720.0);
-  q = fma(q, r, 1.0 / 120.0);
-  q = fma(q, r, 1.0 / 24.0);
-  q = fma(q, r, 1.0 / 6.0);
-  q = fma(q, r, 0.5);
-  q = fma(q, r, 1.0);
-  q = fma(q, r, 1.0);
-  return ldexp(q, (int)k);
-#else
-  return pow(x, y);
-#endif
-}
-
 // Toro two-rarefaction speeds (physics.cpp:309-350) or Einfeldt (:269-296) on
 // rotated states; lo/hi = lambdaF[0], lambdaF[dim+1].
 template <int DIM>
@@ -269,11 +265,11 @@ RKDG_HD void wave_speeds(const double* s1, const double* s2, double p1, double p
   const double cL = sound_speed_p<DIM>(s1, p1, g), cR = sound_speed_p<DIM>(s2, p2, g);
   const double uL = s1[1] / s1[0], uR = s2[1] / s2[0];
   if (g.wave_speeds == RKDG_WS_EINFELDT) {
-    const double sL = sqrt(s1[0]), sR = sqrt(s2[0]);
+    const double sL = sqrt_ws(s1[0]), sR = sqrt_ws(s2[0]);
     const double sum = sL + sR;
     const double eta2 = 0.5 * sL * sR / (sum * sum);
     const double uav = (s1[1] / sL + s2[1] / sR) / sum;
-    const double cav = sqrt((sL * (cL * cL) + sR * (cR * cR)) / sum + eta2 * ((uR - uL) * (uR - uL)));
+    const double cav = sqrt_ws((sL * (cL * cL) + sR * (cR * cR)) / sum + eta2 * ((uR - uL) * (uR - uL)));
     lo = uav - cav;
     hi = uav + cav;
     return;
@@ -282,8 +278,8 @@ RKDG_HD void wave_speeds(const double* s1, const double* s2, double p1, double p
   const double z = 0.5 * (gm - 1.0) / gm;
   const double pStar = pow_ws((cL + cR - 0.5 * (gm - 1.0) * (uR - uL)) / (cL / pow_ws(p1, z) + cR / pow_ws(p2, z)),
                               1.0 / z);
-  const double qL = pStar > p1 ? sqrt(1.0 + 0.5 * (gm + 1.0) * (pStar / p1 - 1.0) / gm) : 1.0;
-  const double qR = pStar > p2 ? sqrt(1.0 + 0.5 * (gm + 1.0) * (pStar / p2 - 1.0) / gm) : 1.0;
+  const double qL = pStar > p1 ? sqrt_ws(1.0 + 0.5 * (gm + 1.0) * (pStar / p1 - 1.0) / gm) : 1.0;
+  const double qR = pStar > p2 ? sqrt_ws(1.0 + 0.5 * (gm + 1.0) * (pStar / p2 - 1.0) / gm) : 1.0;
   lo = uL - qL * cL;
   hi = uR + qR * cR;
 }
@@ -308,13 +304,6 @@ template <int DIM>
 RKDG_HD double side_sound(const double* s, const SidePrim<DIM>& q, const SchemeParams& g) {
   double a = g.gamma * q.p * q.inv;
   if (g.covolume != 0.0) a /= (1.0 - s[0] * g.covolume);
-  return sqrt(a);
-}
-// the same with sqrt_ws (Rusanov's and the nodal wave-speed estimates)
-template <int DIM>
-RKDG_HD double side_sound_ws(const double* s, const SidePrim<DIM>& q, const SchemeParams& g) {
-  double a = g.gamma * q.p * q.inv;
-  if (g.covolume != 0.0) a /= (1.0 - s[0] * g.covolume);
   return sqrt_ws(a);
 }
 // StateIsPhysicalSay (physics.cpp:53-97) from the side primitives
@@ -323,6 +312,10 @@ RKDG_HD bool side_ok(const double* s, const SidePrim<DIM>& q) {
   return (s[0] >= 0.0) && (s[1 + DIM] > 0.0) && (q.pid > 0.0);
 }
 
+// Numerical flux through the unit normal n (rs_rusanov.cpp:6-40, rs_llf.cpp:6-37,
+// rs_hll.cpp:6-50, rs_hllc.cpp:71-161 with getFStar :39-67). The states are
+// taken by value (the reference rotates its inputs in place; nothing observes
+// it). Returns the solver's wave speed; ok=false flags an unphysical input.
 // inm: 0.5 / |n| when the caller has it per face (Rusanov only; 0 = form it here)
 template <int DIM, int RS>
 RKDG_HD double riemann_t(const SchemeParams& g, const double* uL, const double* uR, const double* n,
@@ -337,8 +330,8 @@ RKDG_HD double riemann_t(const SchemeParams& g, const double* uL, const double* 
   if constexpr (RS == RKDG_RS_RUSANOV) {
     const SidePrim<DIM> q1 = side_prim<DIM>(s1, g), q2 = side_prim<DIM>(s2, g);
     ok = side_ok<DIM>(s1, q1) && side_ok<DIM>(s2, q2);
-    const double m1 = sqrt_ws(q1.m2) * q1.inv + side_sound_ws<DIM>(s1, q1, g);
-    const double m2 = sqrt_ws(q2.m2) * q2.inv + side_sound_ws<DIM>(s2, q2, g);
+    const double m1 = sqrt_ws(q1.m2) * q1.inv + side_sound<DIM>(s1, q1, g);
+    const double m2 = sqrt_ws(q2.m2) * q2.inv + side_sound<DIM>(s2, q2, g);
     const double mE = m1 > m2 ? m1 : m2;
     double vn1 = 0.0, vn2 = 0.0, nm = 0.0;
 #pragma unroll
@@ -358,7 +351,7 @@ RKDG_HD double riemann_t(const SchemeParams& g, const double* uL, const double* 
     f1[1 + DIM] = a1 * (s1[1 + DIM] + q1.p);
     f2[1 + DIM] = a2 * (s2[1 + DIM] + q2.p);
     // the unit normal's own norm (rs_rusanov.cpp:27-32); 1 up to rounding
-    const double inm = inm_face != 0.0 ? inm_face : 0.5 * recip(sqrt(nm));
+    const double inm = inm_face != 0.0 ? inm_face : 0.5 * recip(sqrt_ws(nm));
     const double hm = 0.5 * mE;
 #pragma unroll
     for (int i = 0; i < NE; i++) flux[i] = (f1[i] + f2[i]) * inm - hm * (s2[i] - s1[i]);
@@ -483,7 +476,7 @@ RKDG_HD void bc_state(const SchemeParams& g, int bi, const double* s1, const dou
       const double MTTotSq = rhoU2 / s2[0] / gm / R / TTot;
       const double MSq = MTTotSq / (1.0 - 0.5 * (gm - 1.0) * MTTotSq);
       const double p = g.bc_type[bi] == RKDG_BC_SUBSONIC_INLET_TOTAL_PRESSURE
-                           ? pTot * pow(1.0 + 0.5 * (gm - 1.0) * MSq, -gm / (gm - 1.0))
+                           ? pTot * pow_ws(1.0 + 0.5 * (gm - 1.0) * MSq, -gm / (gm - 1.0))
                            : pTot;
       const double T = TTot / (1.0 + 0.5 * (gm - 1.0) * MSq);
       const double rho = p / R / T;
@@ -518,8 +511,8 @@ RKDG_HD void bc_state(const SchemeParams& g, int bi, const double* s1, const dou
       const double epsIn = rhoN > 0 ? (s2[NE - 1] - 0.5 * r2U2) / s2[0] : R / (gm - 1.0) * TFix;
       double v2 = r2U2 / s2[0];
       double pin = pressure<DIM>(s2, g);
-      const double M = sqrt(v2 / s2[0]) / sound_speed_p<DIM>(s2, pin, g);
-      double pr = M > 1 ? pTotal * pow(1.0 + 0.5 * (gm - 1.0) * M * M, -gm / (gm - 1.0)) : pTotal;
+      const double M = sqrt_ws(v2 / s2[0]) / sound_speed_p<DIM>(s2, pin, g);
+      double pr = M > 1 ? pTotal * pow_ws(1.0 + 0.5 * (gm - 1.0) * M * M, -gm / (gm - 1.0)) : pTotal;
       const double rho = M > 1 ? pr / (gm - 1.0) / epsIn : pr / (0.5 * U2 + (gm - 1.0) * epsIn);
       pr = M > 1 ? pr : pr - 0.5 * U2 * rho;
       s2[NE - 1] = pr / (gm - 1.0) + 0.5 * rho * U2;
@@ -554,7 +547,7 @@ RKDG_HD double norml2_scaled(const double* v) {
       }
     }
   }
-  return scale * sqrt(sum);
+  return scale * sqrt_ws(sum);
 }
 
 // indicator_bj.cpp:49-79: ratio y from the stencil bounds, then the limiter

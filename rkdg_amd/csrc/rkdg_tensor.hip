@@ -273,7 +273,7 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
 #pragma unroll
       for (int v = 0; v < NEQ; v++) st[v] = s_u[team][v][j];
       const SidePrim<DIM> q = side_prim<DIM>(st, g);
-      if (!A.no_lambda) lmax = sqrt_ws(q.m2) * q.inv + side_sound_ws<DIM>(st, q, g);
+      if (!A.no_lambda) lmax = sqrt_ws(q.m2) * q.inv + side_sound<DIM>(st, q, g);
       double wj = tb.w[jj[0]] * tb.w[jj[1]];
       if (DIM == 3) wj *= tb.w[jj[2]];
       // contravariant fluxes G^r = w_j adj(J)(r,:) . F(u): with m.a_r = sum_d a_rd m_d,
