@@ -150,6 +150,7 @@ struct rkdg_ctx {
   double* d_lam = nullptr;    // per-stage wave speeds [8]
   long long* d_err = nullptr;
   double* d_means = nullptr;
+  int* d_order = nullptr;        // tensor contexts: work blocks in Morton order of their centroids
   TensorTables* d_tb = nullptr;
   int dense = 0;                 // 1: dense small-element kernels (triangles, bilinear quads)
   double* d_dtab = nullptr;      // dense reference tables
@@ -207,6 +208,7 @@ void free_ctx(rkdg_ctx* c) {
   (void)hipFree(c->d_lam);
   (void)hipFree(c->d_err);
   (void)hipFree(c->d_means);
+  (void)hipFree(c->d_order);
   (void)hipFree(c->d_tb);
   (void)hipFree(c->d_dtab);
   for (auto& p : c->d_stage) (void)hipFree(p);
@@ -882,6 +884,44 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
   HIP_TRY(hipMalloc(&c->d_err, sizeof(long long) * 4));
   HIP_TRY(hipMemset(c->d_err, 0, sizeof(long long) * 4));
   HIP_TRY(hipMalloc(&c->d_means, sizeof(double) * c->neq * NE));
+  {
+    // Traversal order of the RHS work blocks (EPB consecutive elements each):
+    // Morton order of the block centroids on a 2^10 (3D) / 2^16 (2D) grid of
+    // the bounding box, so that face neighbours are processed close in time.
+    const int epb = tensor_elems_per_block(dim, c->p);
+    const int64_t nb = (NE + epb - 1) / epb;
+    const int nv = c->nv;
+    std::vector<double> cen((size_t)nb * dim, 0.0);
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t b = 0; b < nb; b++) {
+      int cnt = 0;
+      for (int64_t e = b * epb; e < std::min<int64_t>(NE, (b + 1) * epb); e++, cnt++)
+        for (int v = 0; v < nv; v++)
+          for (int i = 0; i < dim; i++) cen[b * dim + i] += c->coords[(e * nv + v) * dim + i];
+      for (int i = 0; i < dim; i++) {
+        cen[b * dim + i] /= (double)(cnt * nv);
+        lo[i] = std::min(lo[i], cen[b * dim + i]);
+        hi[i] = std::max(hi[i], cen[b * dim + i]);
+      }
+    }
+    const int bits = dim == 3 ? 10 : 16;
+    const double cells = (double)((1u << bits) - 1);
+    std::vector<std::pair<uint64_t, int>> key((size_t)nb);
+    for (int64_t b = 0; b < nb; b++) {
+      uint64_t code = 0;
+      for (int i = 0; i < dim; i++) {
+        const double span = hi[i] > lo[i] ? hi[i] - lo[i] : 1.0;
+        const uint64_t q = (uint64_t)((cen[b * dim + i] - lo[i]) / span * cells + 0.5);
+        for (int k = 0; k < bits; k++) code |= ((q >> k) & 1ull) << (k * dim + i);
+      }
+      key[b] = {code, (int)b};
+    }
+    std::stable_sort(key.begin(), key.end());
+    std::vector<int> order((size_t)nb);
+    for (int64_t b = 0; b < nb; b++) order[b] = key[b].second;
+    HIP_TRY(hipMalloc(&c->d_order, sizeof(int) * nb));
+    HIP_TRY(hipMemcpy(c->d_order, order.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+  }
   HIP_TRY(hipMalloc(&c->d_tb, sizeof(TensorTables)));
   HIP_TRY(hipMemcpy(c->d_tb, &c->tb, sizeof(TensorTables), hipMemcpyHostToDevice));
   return RKDG_OK;
@@ -939,6 +979,7 @@ RhsArgs base_args(rkdg_ctx* c, const double* x) {
   a.inst = c->d_inst;
   a.bcp = c->d_bcp;
   a.grid_cap = c->grid_cap;
+  a.order = c->dense ? nullptr : c->d_order;
   return a;
 }
 

@@ -58,6 +58,10 @@ struct RhsArgs {
   const int* active;
   int grid_cap;  // dense kernel: persistent grid of at most this many workgroups (0: one per work block)
   int no_lambda; // 1: the wave speed of this launch is not needed (tensor kernel skips the nodal one)
+  // tensor kernel (nullable): work block processed by each workgroup (a block is
+  // the EPB elements from block * EPB), in an order that keeps face neighbours
+  // close in time so their gathered states are still in L2
+  const int* order;
 };
 
 struct LimitArgs {
@@ -148,6 +152,7 @@ hipError_t launch_tensor_means(int dim, int p, const MeansArgs& a, const TensorT
 hipError_t launch_fill(double* p, double v, int64_t n, hipStream_t s);
 hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
 bool tensor_supported(int dim, int p);
+int tensor_elems_per_block(int dim, int p);  // elements per workgroup of the tensor RHS
 
 // launchers (rkdg_dense.hip): affine triangles and bilinear quadrilaterals;
 // t points to the device copy of the dense reference tables

@@ -182,7 +182,8 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
   const int team = tid / T, lane = tid % T;
   const int64_t NE = A.NE;
   const int64_t NDOFS = NE * ND;
-  const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t blk0 = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t blk = A.order ? (int64_t)A.order[blk0] : blk0;
   const int64_t e0 = blk * EPB;
   const int64_t e = e0 + team;
   const bool active = e < NE;
@@ -911,6 +912,14 @@ bool tensor_supported(int dim, int p) {
   RKDG_TENSOR_CASES(X)
 #undef X
   return false;
+}
+
+int tensor_elems_per_block(int dim, int p) {
+#define X(D, PP) \
+  if (dim == D && p == PP) return TC<D, PP>::EPB;
+  RKDG_TENSOR_CASES(X)
+#undef X
+  return 1;
 }
 
 template <int D, int PP, int RS>
