@@ -271,3 +271,44 @@ def test_unphysical_state_reports_error(cuda):
     # the error word is cleared: a good state works again
     x[0, 5 * 8:6 * 8] = 1.0
     ctx.rhs(_dev(x.reshape(-1), cuda), yd)
+
+
+@pytest.mark.parametrize("rs", [S.RUSANOV, S.HLLC])
+def test_hex_odd_count_with_boundaries(rs, cuda):
+    """Q2 hex with an odd element count (the last two-element work block is
+    half empty) and boundary faces on every side: RHS, limiter (the stencil of
+    boundary elements has fewer neighbours) and two limited SSP-RK3 steps."""
+    import torch
+
+    m = M.structured_hex(5, 3, 3, periodic=(False, False, False))
+    assert m.num_elements % 2 == 1
+    bcs = [S.BC(1, S.BC_SLIP), S.BC(2, S.BC_OUTLET), S.BC(3, S.BC_SLIP), S.BC(4, S.BC_OUTLET),
+           S.BC(5, S.BC_SUPERSONIC_INLET, (1.1, 0.5, 0.05, 0.1, 1.0)), S.BC(6, S.BC_SLIP)]
+    sch = S.Scheme(riemann=rs, indicator=S.IND_BJ, phi=S.PHI_VENKAT, venkat_k=5.0, limiter=S.LIM_MULTI,
+                   last_hope=1, bcs=bcs)
+    y, y_ref, lam, lam_ref, neq = _rhs_pair(m, 2, sch, cuda, seed=21)
+    err = rel_linf(y, y_ref, neq)
+    print(f"[hex odd rs={rs}] rhs rel Linf {err:.3e}", flush=True)
+    assert err <= TOL
+    assert abs(lam - lam_ref) <= 1e-13 * lam_ref
+
+    ref = RefSolver(m, 2, sch)
+    x = _shocky_state(ref.node_coords(), seed=22)
+    x_ref, _, mv = ref.limit(x)
+    ctx = _ctx(m, 2, sch)
+    xd = _dev(x, cuda)
+    tr = torch.zeros(m.num_elements, dtype=torch.int8, device=cuda)
+    ctx.limit(xd, tr)
+    err = rel_linf(_host(xd), x_ref, 5)
+    print(f"[hex odd rs={rs}] limiter rel Linf {err:.3e} troubled {int((_host(tr) & 1).sum())}", flush=True)
+    assert err <= TOL
+
+    x = smooth_state(ref.node_coords(), seed=23, amp=1e-2)
+    xd = _dev(x, cuda)
+    for _ in range(2):
+        x, lam_ref = ref.step(x, 1e-3, S.TABLEAUS["ssprk3"])
+        lam = ctx.step(xd, 1e-3, S.TABLEAUS["ssprk3"])
+    err = rel_linf(_host(xd), x, 5)
+    print(f"[hex odd rs={rs}] 2 steps rel Linf {err:.3e}", flush=True)
+    assert err <= TOL
+    assert abs(lam - lam_ref) <= 1e-12 * lam_ref
