@@ -150,6 +150,7 @@ struct rkdg_ctx {
   double* d_lam = nullptr;    // per-stage wave speeds [8]
   long long* d_err = nullptr;
   double* d_means = nullptr;
+  long long* h_pin = nullptr;    // pinned host words: device error word [4], then the wave speeds [count]
   int* d_order = nullptr;        // tensor contexts: work blocks in Morton order of their centroids
   TensorTables* d_tb = nullptr;
   int dense = 0;                 // 1: dense small-element kernels (triangles, bilinear quads)
@@ -207,6 +208,7 @@ void free_ctx(rkdg_ctx* c) {
   (void)hipFree(c->d_fcode);
   (void)hipFree(c->d_lam);
   (void)hipFree(c->d_err);
+  if (c->h_pin) (void)hipHostFree(c->h_pin);
   (void)hipFree(c->d_means);
   (void)hipFree(c->d_order);
   (void)hipFree(c->d_tb);
@@ -983,9 +985,18 @@ RhsArgs base_args(rkdg_ctx* c, const double* x) {
   return a;
 }
 
+// pinned host words for the per-step readbacks (a copy to pageable memory is
+// staged by the runtime and costs tens of microseconds per step)
+hipError_t ensure_pinned(rkdg_ctx* c) {
+  if (c->h_pin) return hipSuccess;
+  return hipHostMalloc(reinterpret_cast<void**>(&c->h_pin), sizeof(long long) * 4 + sizeof(double) * c->count,
+                       hipHostMallocDefault);
+}
+
 int check_device_error(rkdg_ctx* c, int64_t* info) {
-  long long h[4];
-  HIP_TRY(hipMemcpyAsync(h, c->d_err, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(ensure_pinned(c));
+  long long* h = c->h_pin;
+  HIP_TRY(hipMemcpyAsync(h, c->d_err, sizeof(long long) * 4, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (h[0]) {
     if (info) {
@@ -1002,11 +1013,12 @@ int check_device_error(rkdg_ctx* c, int64_t* info) {
 
 // wave speed of stage slot; on a batch, the maximum over the instances
 int read_lambda(rkdg_ctx* c, int slot, double* lam) {
-  std::vector<double> h(c->count);
-  HIP_TRY(hipMemcpyAsync(h.data(), c->d_lam + (int64_t)slot * c->count, sizeof(double) * c->count,
+  HIP_TRY(ensure_pinned(c));
+  double* h = reinterpret_cast<double*>(c->h_pin + 4);
+  HIP_TRY(hipMemcpyAsync(h, c->d_lam + (int64_t)slot * c->count, sizeof(double) * c->count,
                          hipMemcpyDeviceToHost, c->stream));
   const int rc = check_device_error(c, nullptr);
-  *lam = *std::max_element(h.begin(), h.end());
+  *lam = *std::max_element(h, h + c->count);
   return rc;
 }
 
