@@ -300,61 +300,98 @@ __global__ __launch_bounds__(64) void dense_limit_adj_a_kernel(const LimAdjArgs 
   tbr.store(reinterpret_cast<double*>(&tb), tid);
   sx.store(s_u, tid);
   sub.store(s_ub, tid);
+  // stencil bounds of the density mean with the element that attains them
+  // (serial lane), shared through LDS with the team's lanes
+  __shared__ double s_st[EPB][4];  // mI, MI, eps2, mean of the density
+  __shared__ int s_nbok[EPB];      // bit fc: face fc has a neighbour
+  const bool ind_bj = g.indicator == RKDG_IND_BJ;
+  double mI = 1e9, MI = -1e9;
+  int64_t amin = -1, amax = -1;
+  double eps2 = 0.0;
+  if (serial && ind_bj) {
+    {
+      const double a = mean[0];
+      if (a < mI) { mI = a; amin = e; }
+      if (a > MI) { MI = a; amax = e; }
+    }
+    int ok = 0;
+#pragma unroll
+    for (int fc = 0; fc < NF; fc++) {
+      const int nb = nbs[fc];
+      if (nb < 0) continue;
+      ok |= 1 << fc;
+      const double a = nmd[fc];
+      if (a < mI) { mI = a; amin = nb; }
+      if (a > MI) { MI = a; amax = nb; }
+    }
+    if (g.phi == RKDG_PHI_VENKAT && g.venkat_k > 0.0) {
+      double area = 0.0;
+      if (C::TRI) area = 0.5 * mw[0];
+      else
+        for (int k = 0; k < ND; k++) area += mw[k];
+      const double kh = g.venkat_k * sqrt(area);
+      eps2 = kh * kh * kh;
+    }
+    s_st[team][0] = mI;
+    s_st[team][1] = MI;
+    s_st[team][2] = eps2;
+    s_st[team][3] = mean[0];
+    s_nbok[team] = ok;
+  }
   __syncthreads();
-  // one lane per element does the (small) serial recomputation
+  // the indicator points of interior faces, one lane each, and the team's
+  // argmin with the serial loop's tie-break (the first minimum in point order)
+  double ycl = 1e9, dl = 0.0, yl = 1.0;
+  int itl = NPT;
+  if (active && ind_bj) {
+    const double mIs = s_st[team][0], MIs = s_st[team][1], e2s = s_st[team][2], m0 = s_st[team][3];
+    const int ok = s_nbok[team];
+    for (int it = lane; it < NPT; it += T) {
+      const int fc = it / 2, ip = it % 2;
+      if (!((ok >> fc) & 1)) continue;
+      double uu = 0.0;
+#pragma unroll
+      for (int k = 0; k < ND; k++) uu += s_u[team][0][k] * tb.PhiI[fc][ip][k];
+      const double diff = uu - m0;
+      const double y = indicator_y(diff, m0, mIs, MIs);
+      double dy, dd;
+      const double yc = adj::limiter_phi(g, y, diff, e2s, &dy, &dd);
+      if (yc < ycl) {
+        ycl = yc;
+        dl = diff;
+        yl = y;
+        itl = it;
+      }
+    }
+  }
+#pragma unroll
+  for (int off = T / 2; off > 0; off >>= 1) {
+    const double oy = __shfl_xor(ycl, off, 64), od = __shfl_xor(dl, off, 64), oyy = __shfl_xor(yl, off, 64);
+    const int oi = __shfl_xor(itl, off, 64);
+    if (oy < ycl || (oy == ycl && oi < itl)) {
+      ycl = oy;
+      dl = od;
+      yl = oyy;
+      itl = oi;
+    }
+  }
+  // one lane per element does the rest of the (small) serial recomputation
   if (serial) {
     // objective on the limited state of this element (adds to ubar)
     double u[NEQ][ND];
     double alpha = 1.0, f = 1.0;
     bool reset = false;
-    // stencil bounds of the density mean with the element that attains them
-    double mI = 1e9, MI = -1e9;
-    int64_t amin = -1, amax = -1;
     double yMin0 = 1e9;
     int pstar = -1;
     double dstar = 0.0, ystar = 1.0;
     int branch = 0;  // +1: diff > 1e-3 (MI), -1: diff < -1e-3 (mI), 0: y = 1
-    double eps2 = 0.0;
-    const bool ind_bj = g.indicator == RKDG_IND_BJ;
     if (ind_bj) {
-      {
-        const double a = mean[0];
-        if (a < mI) { mI = a; amin = e; }
-        if (a > MI) { MI = a; amax = e; }
-      }
-#pragma unroll
-      for (int fc = 0; fc < NF; fc++) {
-        const int nb = nbs[fc];
-        if (nb < 0) continue;
-        const double a = nmd[fc];
-        if (a < mI) { mI = a; amin = nb; }
-        if (a > MI) { MI = a; amax = nb; }
-      }
-      if (g.phi == RKDG_PHI_VENKAT && g.venkat_k > 0.0) {
-        double area = 0.0;
-        if (C::TRI) area = 0.5 * mw[0];
-        else
-          for (int k = 0; k < ND; k++) area += mw[k];
-        const double kh = g.venkat_k * sqrt(area);
-        eps2 = kh * kh * kh;
-      }
-      for (int it = 0; it < NPT; it++) {
-        const int fc = it / 2, ip = it % 2;
-        if (nbs[fc] < 0) continue;
-        double uu = 0.0;
-#pragma unroll
-        for (int k = 0; k < ND; k++) uu += s_u[team][0][k] * tb.PhiI[fc][ip][k];
-        const double diff = uu - mean[0];
-        const double y = indicator_y(diff, mean[0], mI, MI);
-        double dy, dd;
-        const double yc = adj::limiter_phi(g, y, diff, eps2, &dy, &dd);
-        if (yc < yMin0) {
-          yMin0 = yc;
-          pstar = it;
-          dstar = diff;
-          ystar = y;
-          branch = diff > 1e-3 ? 1 : (diff < -1e-3 ? -1 : 0);
-        }
+      if (ycl < yMin0) {
+        yMin0 = ycl;
+        pstar = itl;
+        dstar = dl;
+        ystar = yl;
+        branch = dl > 1e-3 ? 1 : (dl < -1e-3 ? -1 : 0);
       }
     } else {
       yMin0 = g.indicator == RKDG_IND_NOWHERE ? 1.0 : 0.0;
