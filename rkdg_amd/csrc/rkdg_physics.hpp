@@ -41,18 +41,21 @@ RKDG_HD double recip(double x) {
 // Square root on the device (wave speeds, sound speeds, boundary states):
 // v_rsq_f64 refined by one Goldschmidt step (g = x r, h = r/2, e = 1/2 - h g)
 // and one Newton correction of g, about half the instructions of the correctly
-// rounded sqrt and within an ulp of it; sqrt_ws(0) = 0, negative input gives NaN as sqrt does. The host
-// build uses the library sqrt.
+// rounded sqrt and within an ulp of it; negative input gives NaN as sqrt does.
+// The seed is rsq(x + 2^-1022): for x >= 2^-969 the sum is x itself, and at
+// x = 0 the seed stays finite so every step below yields exactly 0 (no
+// select on x == 0). Positive x below 2^-969 gets a less accurate result. The
+// host build uses the library sqrt.
 RKDG_HD double sqrt_ws(double x) {
 #ifdef __HIP_DEVICE_COMPILE__
-  const double r = __builtin_amdgcn_rsq(x);
+  const double r = __builtin_amdgcn_rsq(x + 0x1p-1022);
   double gq = x * r, h = 0.5 * r;
   const double e = fma(-h, gq, 0.5);
   gq = fma(gq, e, gq);
   h = fma(h, e, h);
   const double d = fma(-gq, gq, x);
   gq = fma(d, h, gq);
-  return x == 0.0 ? 0.0 : gq;
+  return gq;
 #else
   return sqrt(x);
 #endif
