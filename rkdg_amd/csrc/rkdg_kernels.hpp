@@ -105,6 +105,17 @@ struct CopyRegs {
   }
 };
 
+// a wave-uniform address kept in scalar registers: passing it through
+// readfirstlane stops the compiler from re-associating the uniform part of an
+// address into per-lane 64-bit arithmetic
+template <typename Tp>
+__device__ __forceinline__ Tp* sbase(Tp* p) {
+  const uint64_t u = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+  return reinterpret_cast<Tp*>(((uint64_t)hi << 32) | lo);
+}
+
 // the NEQ-variable states of the EPB elements from e0 (byNODES layout) in one
 // pass: lane tid holds node tid % ND of element e0 + tid / ND; past the last
 // element the value is `fill`
@@ -116,7 +127,7 @@ struct StateRegs {
                                        double fill = 1.0) {
     const bool ok = tid < EPB * ND && e0 + tid / ND < NE;
 #pragma unroll
-    for (int q = 0; q < NEQ; q++) v[q] = ok ? x[q * NDOFS + e0 * ND + tid] : fill;
+    for (int q = 0; q < NEQ; q++) v[q] = ok ? sbase(x + (q * NDOFS + e0 * ND))[(unsigned)tid] : fill;
   }
   __device__ __forceinline__ void store(double (&su)[EPB][NEQ][ND], int tid) const {
     if (tid < EPB * ND) {

@@ -524,7 +524,10 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       if (DIM == 3) iw *= tb.invw[jj[2]];
       const double im = iw * recip(detJ);
       // every base is loaded before the first store (an output may be a base)
-      const int64_t i0 = e * ND + j;
+      // addresses: the workgroup's block of each variable (uniform, scalar
+      // registers) plus a small 32-bit lane offset
+      const int64_t eb = e0 * ND;
+      const unsigned lo = (unsigned)(team * ND + j);
       double u0[NEQ];  // the stage input at this node (the base of outputs without one)
 #pragma unroll
       for (int v = 0; v < NEQ; v++) u0[v] = s_u[team][v][j];
@@ -533,33 +536,31 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       for (int m = 0; m < 4; m++) {
         const double* bp = m < A.nout ? A.outs[m].base : nullptr;
         if (bp) {
-          bp += i0;
 #pragma unroll
-          for (int v = 0; v < NEQ; v++) base[m][v] = bp[v * NDOFS];
+          for (int v = 0; v < NEQ; v++) base[m][v] = sbase(bp + (v * NDOFS + eb))[lo];
         } else {
 #pragma unroll
           for (int v = 0; v < NEQ; v++) base[m][v] = u0[v];
         }
       }
-      double* kp = A.k ? A.k + i0 : nullptr;
 #pragma unroll
       for (int v = 0; v < NEQ; v++) outv[v] = 0.0;
 #pragma unroll
       for (int m = 0; m < 4; m++) {
         if (m < A.nout) {
-          double* op = A.outs[m].out + i0;
+          double* op = A.outs[m].out;
           const double cf = A.outs[m].coef;
 #pragma unroll
           for (int v = 0; v < NEQ; v++) {
             const double o = base[m][v] + cf * (z[v] * im);
-            op[v * NDOFS] = o;
+            sbase(op + (v * NDOFS + eb))[lo] = o;
             if (m == 0) outv[v] = o;  // means of outs[0] (the host puts the selected output first)
           }
         }
       }
-      if (kp) {
+      if (A.k) {
 #pragma unroll
-        for (int v = 0; v < NEQ; v++) kp[v * NDOFS] = z[v] * im;
+        for (int v = 0; v < NEQ; v++) sbase(A.k + (v * NDOFS + eb))[lo] = z[v] * im;
       }
     }
     if (A.means) {
