@@ -310,7 +310,7 @@ def main():
     ap.add_argument("--cfl", type=float, default=0.2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=48)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB read + 1 GiB write elementwise kernel (3 launches), "
