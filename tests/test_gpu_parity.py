@@ -336,13 +336,46 @@ def test_acoustic_pulse_workload_matches_oracle(cuda):
     ctx = _ctx(m, 2, sch)
     x = bench.pulse_state(ref.node_coords())
     xd = _dev(x, cuda)
+
+    # the RHS of the pulse: error against the RHS itself and against the
+    # volume term (the mass-scaled M^-1 (z - z_face)), the magnitude of the
+    # O(p/h) terms whose small difference the RHS is
+    import torch
+
+    y_ref, _ = ref.mult(x)
+    zf, _ = ref.face_mult(x)
+    ne, nd = m.num_elements, 27
+    minv = np.stack([ref.mass_inverse(e) for e in range(ne)])
+    vol = (y_ref.reshape(5, ne, nd) - np.einsum("eij,vej->vei", minv, zf.reshape(5, ne, nd))).reshape(5, -1)
+    yd = torch.empty_like(xd)
+    ctx.rhs(xd, yd)
+    ey = np.abs(_host(yd) - y_ref).reshape(5, -1).max(axis=1)
+    rel_rhs = ey / np.abs(y_ref).reshape(5, -1).max(axis=1)
+    rel_vol = ey / np.maximum(np.abs(vol).max(axis=1), 1e-300)
+    # density and energy have no volume term for a fluid at rest; their RHS is
+    # the face dissipation, lambda (u+ - u-) lifted, a difference of O(1)
+    # traces: the term scale lambda |x_v| / h bounds each of its terms
+    _, lam0 = ref.face_mult(x)
+    term = np.maximum(np.abs(vol).max(axis=1), lam0 * np.abs(x).reshape(5, -1).max(axis=1) / ref.hmin())
+    rel_term = ey / term
+    print(f"[pulse 16^3] RHS error per variable relative to the RHS {np.array2string(rel_rhs, precision=1)}, "
+          f"to the volume term {np.array2string(rel_vol, precision=1)}, "
+          f"to the term scale {np.array2string(rel_term, precision=1)}", flush=True)
+    assert rel_term.max() <= TOL
+
     cfl, hmin = 0.2, ref.hmin()
     assert abs(ctx.hmin() - hmin) <= 1e-14 * hmin
     _, lam_ref = ref.face_mult(x)
     lam = ctx.face_lambda(xd)
     assert abs(lam - lam_ref) <= 1e-13 * lam_ref
-    x, _, _ = ref.limit(x)
-    ctx.limit(xd)
+    x, _, mv = ref.limit(x)
+    tr = torch.zeros(ne, dtype=torch.int8, device=cuda)
+    ctx.limit(xd, tr)
+    near = (np.abs(mv - 1.0) < 1e-12) & (mv != 1.0)  # y = 1 exactly is not troubled either way
+    flags = _host(tr) & 1
+    print(f"[pulse 16^3] troubled cells {int(flags.sum())} (oracle {int((mv < 1.0).sum())}), "
+          f"within 1e-12 of y = 1 (y != 1): {int(near.sum())}", flush=True)
+    assert np.all((flags == (mv < 1.0)) | near)
     dt_ref, dt = cfl * hmin / lam_ref / 5.0, cfl * ctx.hmin() / lam / 5.0
     w = ref.means(x).reshape(5, -1)
     mass0, energy0 = w[0].sum(), w[4].sum()
