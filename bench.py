@@ -85,6 +85,36 @@ def measured_traffic(n):
     return None
 
 
+def host_cpu_info():
+    """CPU model, logical CPUs, sockets, physical cores and memory of this host."""
+    model, sockets, cores = "", set(), set()
+    phys = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and not model:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                    sockets.add(v)
+                elif k == "core id":
+                    cores.add((phys, v))
+    except OSError:
+        pass
+    mem_gb = None
+    try:
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemTotal:"):
+                    mem_gb = round(int(line.split()[1]) / 2**20, 1)
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(), "sockets": len(sockets) or None,
+            "physical_cores": len(cores) or None, "mem_gib": mem_gb}
+
+
 def cpu_baseline(n, steps, threads):
     from oracle.refcpu import RefSolver, lib
     from rkdg_amd import mesh as M
@@ -399,12 +429,20 @@ def main():
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         cv, cel, cne = cpu_baseline(args.cpu_n, args.cpu_steps, threads)
+        cv1, cel1, cne1 = cpu_baseline(24, 1, 1)
+        from oracle.refcpu import lib as _oracle_lib
+
+        _oracle_lib().ref_set_threads(threads)
         cpu = {"value": cv, "unit": "DOF-stage updates/s", "cores": threads, "kind": "port",
                "sample": f"oracle (CSR Aflux, dense inverse mass, per-point shape evaluation) on the "
                          f"same workload at {args.cpu_n}^3 elements ({cne} elements), "
                          f"{args.cpu_steps} SSP-RK3 steps after 1 warm-up step, {cel:.1f} s, "
-                         f"OpenMP {threads} threads"}
-        print(f"[bench] cpu baseline {cv:.3e} DOF-stage/s ({cel:.1f}s)", flush=True)
+                         f"OpenMP {threads} threads",
+               "host": host_cpu_info(),
+               "value_1_thread": cv1,
+               "sample_1_thread": f"24^3 elements ({cne1}), 1 step after 1 warm-up step, {cel1:.1f} s"}
+        print(f"[bench] cpu baseline {cv:.3e} DOF-stage/s ({cel:.1f}s, {threads} threads); "
+              f"1 thread {cv1:.3e} ({cel1:.1f}s)", flush=True)
 
     ens = None
     if not (args.no_ensemble and args.no_adjoint):
