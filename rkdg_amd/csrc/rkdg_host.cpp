@@ -201,42 +201,49 @@ struct rkdg_ctx {
 
 namespace {
 
+// hipFree of a null pointer is a no-op but still initialises the runtime: a
+// context torn down after a failed argument check touches no device
+template <typename Tp>
+void dfree(Tp* p) {
+  if (p) (void)hipFree((void*)p);
+}
+
 void free_ctx(rkdg_ctx* c) {
   if (!c) return;
-  (void)hipFree(c->d_geo);
-  (void)hipFree(c->d_nbr);
-  (void)hipFree(c->d_fcode);
-  (void)hipFree(c->d_lam);
-  (void)hipFree(c->d_err);
+  dfree(c->d_geo);
+  dfree(c->d_nbr);
+  dfree(c->d_fcode);
+  dfree(c->d_lam);
+  dfree(c->d_err);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
-  (void)hipFree(c->d_means);
-  (void)hipFree(c->d_order);
-  (void)hipFree(c->d_tb);
-  (void)hipFree(c->d_dtab);
-  for (auto& p : c->d_stage) (void)hipFree(p);
+  dfree(c->d_means);
+  dfree(c->d_order);
+  dfree(c->d_tb);
+  dfree(c->d_dtab);
+  for (auto& p : c->d_stage) dfree(p);
   for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
-  (void)hipFree(c->d_inst);
-  (void)hipFree(c->d_bcp);
-  (void)hipFree(c->d_active);
-  (void)hipFree(c->d_t);
-  (void)hipFree(c->d_dt);
-  (void)hipFree(c->d_dtr);
-  (void)hipFree(c->d_tf);
-  (void)hipFree(c->d_hmin);
-  (void)hipFree(c->d_steps);
-  (void)hipFree(c->d_nact);
+  dfree(c->d_inst);
+  dfree(c->d_bcp);
+  dfree(c->d_active);
+  dfree(c->d_t);
+  dfree(c->d_dt);
+  dfree(c->d_dtr);
+  dfree(c->d_tf);
+  dfree(c->d_hmin);
+  dfree(c->d_steps);
+  dfree(c->d_nact);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->bstream) (void)hipStreamDestroy(c->bstream);
-  (void)hipFree(c->d_ameans);
-  (void)hipFree(c->d_meanbar);
-  (void)hipFree(c->d_send_to);
-  (void)hipFree(c->d_send_val);
-  (void)hipFree(c->d_pbar);
-  (void)hipFree(c->d_J);
-  for (auto p : c->d_adj) (void)hipFree(p);
-  for (auto p : c->d_rec) (void)hipFree(p);
-  for (auto p : c->d_cp) (void)hipFree(p);
-  (void)hipFree(c->d_obj_elems);
+  dfree(c->d_ameans);
+  dfree(c->d_meanbar);
+  dfree(c->d_send_to);
+  dfree(c->d_send_val);
+  dfree(c->d_pbar);
+  dfree(c->d_J);
+  for (auto p : c->d_adj) dfree(p);
+  for (auto p : c->d_rec) dfree(p);
+  for (auto p : c->d_cp) dfree(p);
+  dfree(c->d_obj_elems);
   delete c;
 }
 
@@ -664,6 +671,19 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
   if (c->NE <= 0) return fail(RKDG_E_ARG, "empty mesh");
   if (c->NE > (int64_t)2147483647 / 8) return fail(RKDG_E_ARG, "too many elements for 32-bit neighbour ids");
   if (s->num_bc < 0 || s->num_bc > RKDG_MAX_BC) return fail(RKDG_E_ARG, "num_bc out of range");
+  // the scheme's plug-in choices (the reference's case manager stops on an
+  // unknown name, case_manager.cpp:523-612, 614-820)
+  if (s->riemann < RKDG_RS_RUSANOV || s->riemann > RKDG_RS_HLLC) return fail(RKDG_E_ARG, "unknown Riemann solver");
+  if (s->wave_speeds != RKDG_WS_TORO && s->wave_speeds != RKDG_WS_EINFELDT)
+    return fail(RKDG_E_ARG, "unknown wave-speed estimate");
+  if (s->indicator < RKDG_IND_NOWHERE || s->indicator > RKDG_IND_BJ) return fail(RKDG_E_ARG, "unknown indicator");
+  if (s->phi < RKDG_PHI_MICHALAK || s->phi > RKDG_PHI_VENKAT) return fail(RKDG_E_ARG, "unknown limiter function");
+  if (s->limiter < RKDG_LIM_NONE || s->limiter > RKDG_LIM_MULTI) return fail(RKDG_E_ARG, "unknown limiter");
+  if (!(s->gamma > 1.0)) return fail(RKDG_E_ARG, "gamma must exceed 1");
+  if (!(s->covolume >= 0.0)) return fail(RKDG_E_ARG, "covolume must be non-negative");
+  for (int i = 0; i < s->num_bc; i++)
+    if (s->bc_type[i] < RKDG_BC_SLIP || s->bc_type[i] > RKDG_BC_TOTAL_PRESSURE_OUTLET)
+      return fail(RKDG_E_ARG, "unknown boundary condition type in slot " + std::to_string(i));
   const int dim = c->dim, nv = c->nv, NE_ = (int)c->NE;
   const int64_t NE = c->NE;
   if (m->geom == RKDG_GEOM_TRI) c->nv = 3;

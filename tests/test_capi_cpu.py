@@ -50,3 +50,59 @@ def test_library_exports_selftest_symbols():
     assert names and set(names) == set(capi.SELFTEST_EXPORTS)
     for n in names:
         assert hasattr(L, n), n
+
+
+def _create_rc(mesh, order, scheme_desc):
+    L = capi.lib()
+    md, keep = capi._mesh_desc(mesh, order)
+    out = ctypes.c_void_p()
+    rc = L.rkdg_create(ctypes.byref(md), ctypes.byref(scheme_desc), ctypes.byref(out))
+    return rc, L.rkdg_last_error().decode(), out
+
+
+def test_create_rejects_bad_arguments_before_touching_the_device():
+    """Every argument check of rkdg_create runs on the host before any device
+    work, so it is exercised here without a GPU: unknown plug-in enums, bad gas
+    constants, unknown boundary types, empty meshes, unknown geometry and
+    orders without kernels all return RKDG_E_ARG / RKDG_E_UNSUPPORTED with a
+    message and no context."""
+    from rkdg_amd import mesh as M
+    from rkdg_amd import scheme as S
+
+    m = M.structured_hex(3, 3, 3)
+    cases = [
+        (dict(riemann=7), "Riemann"),
+        (dict(wave_speeds=2), "wave-speed"),
+        (dict(indicator=5), "indicator"),
+        (dict(phi=-1), "limiter function"),
+        (dict(limiter=3), "limiter"),
+        (dict(gamma=1.0), "gamma"),
+        (dict(covolume=-0.1), "covolume"),
+        (dict(bcs=[S.BC(1, 9)]), "boundary condition type"),
+    ]
+    for kw, word in cases:
+        rc, msg, out = _create_rc(m, 2, capi.scheme_desc(S.Scheme(**kw)))
+        assert rc == capi.RKDG_E_ARG and word in msg and not out.value, (kw, rc, msg)
+    good = capi.scheme_desc(S.Scheme())
+    empty = M.Mesh(M.GEOM_HEX, m.elem_coords[:0], m.elem_verts[:0])
+    rc, msg, _ = _create_rc(empty, 2, good)
+    assert rc == capi.RKDG_E_ARG and "empty" in msg
+    rc, msg, _ = _create_rc(m, 9, good)
+    assert rc == capi.RKDG_E_UNSUPPORTED and "order" in msg
+    md, keep = capi._mesh_desc(m, 2)
+    md.geom = 7
+    out = ctypes.c_void_p()
+    assert capi.lib().rkdg_create(ctypes.byref(md), ctypes.byref(good), ctypes.byref(out)) == capi.RKDG_E_ARG
+    assert "geometry" in capi.lib().rkdg_last_error().decode()
+
+
+def test_entry_points_reject_null_context():
+    L = capi.lib()
+    null = None
+    assert L.rkdg_rhs(null, null, null, null) == capi.RKDG_E_ARG
+    assert L.rkdg_limit(null, null, null) == capi.RKDG_E_ARG
+    assert L.rkdg_means(null, null, null) == capi.RKDG_E_ARG
+    assert L.rkdg_hmin(null, null) == capi.RKDG_E_ARG
+    assert L.rkdg_get_info(null, null) == capi.RKDG_E_ARG
+    assert L.rkdg_node_coords(null, null) == capi.RKDG_E_ARG
+    assert L.rkdg_destroy(null) == 0
