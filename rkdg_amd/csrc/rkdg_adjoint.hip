@@ -551,7 +551,12 @@ __global__ __launch_bounds__(256) void dense_limit_adj_b_kernel(const LimAdjArgs
   using C = DC<GEOM, P>;
   constexpr int ND = C::ND, NEQ = C::NEQ, NF = C::NF;
   const DenseTab<GEOM, P>* tb = reinterpret_cast<const DenseTab<GEOM, P>*>(gtab);
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  // one thread per (element, node): the element's mean adjoint is gathered by
+  // each of its node threads (cache hits after the first), so the node updates
+  // below are coalesced across the wave
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t e = gid / ND;
+  const int k = (int)(gid - e * ND);
   if (e >= A.NE) return;
   const int64_t NE = A.NE, NDOFS = NE * ND;
   double mb[NEQ];
@@ -575,15 +580,28 @@ __global__ __launch_bounds__(256) void dense_limit_adj_b_kernel(const LimAdjArgs
     if (nbs[fc] >= 0 && st[fc] == e) mb[0] += sv[fc];
   double mw[ND], ms = 0.0;
 #pragma unroll
-  for (int k = 0; k < ND; k++) {
-    mw[k] = C::TRI ? tb->mhat[k] * A.geo[(int64_t)C::G_DET * NE + e] : A.geo[(int64_t)(C::G_MW + k) * NE + e];
-    ms += mw[k];
+  for (int q = 0; q < ND; q++) {
+    mw[q] = C::TRI ? tb->mhat[q] * A.geo[(int64_t)C::G_DET * NE + e] : A.geo[(int64_t)(C::G_MW + q) * NE + e];
+    ms += mw[q];
   }
+  double mwk = mw[0];
+#pragma unroll
+  for (int q = 1; q < ND; q++) mwk = k == q ? mw[q] : mwk;
 #pragma unroll
   for (int v = 0; v < NEQ; v++) {
     const double c = mb[v] / ms;
+    const int64_t i = v * NDOFS + e * ND + k;
+    const double x = A.vbar[i] + c * mwk;
+    A.vbar[i] = x;
+    // the stage-combination adjoints fed by this vbar (the same update as
+    // axpy_multi_kernel, fused here to save a launch per stage)
 #pragma unroll
-    for (int k = 0; k < ND; k++) A.vbar[v * NDOFS + e * ND + k] += c * mw[k];
+    for (int t = 0; t < 4; t++) {
+      if (t < A.ax.n) {
+        double* y = A.ax.y[t];
+        y[i] = (A.ax.set[t] ? 0.0 : y[i]) + A.ax.c[t] * x;
+      }
+    }
   }
 }
 
@@ -687,7 +705,7 @@ hipError_t launch_dense_limit_adj(int geom, int p, const LimAdjArgs& a, const Sc
     dense_limit_adj_a_kernel<G, PP><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t);               \
     hipError_t err = hipGetLastError();                                                       \
     if (err != hipSuccess) return err;                                                        \
-    dense_limit_adj_b_kernel<G, PP><<<(unsigned)((a.NE + 255) / 256), 256, 0, s>>>(a, t);    \
+    dense_limit_adj_b_kernel<G, PP><<<(unsigned)((a.NE * C::ND + 255) / 256), 256, 0, s>>>(a, t); \
     return hipGetLastError();                                                                 \
   }
   RKDG_DENSE_CASES(X)

@@ -1167,14 +1167,15 @@ int rhs_adj_launch(rkdg_ctx* c, const double* x, const double* kbar, double* xba
 // vbar = (dL/dv)^T ubar (+ objective of the limited state when obj_slot >= 0);
 // means: element means of v (computed here when null)
 int limit_adj_launch(rkdg_ctx* c, const double* v, const double* means, const double* ubar, double* vbar,
-                     int obj_slot, double obj_coef) {
+                     int obj_slot, double obj_coef, const AxpyTargets* ax = nullptr) {
   if (!means) {
     MeansArgs ma{v, c->d_ameans, c->d_geo, c->NE};
     HIP_TRY(launch_dense_means(c->geom, c->p, ma, c->d_dtab, c->stream));
     means = c->d_ameans;
   }
   LimAdjArgs a{v, means, ubar, vbar, c->d_meanbar, c->d_send_to, c->d_send_val, c->d_geo, c->d_nbr, c->d_fcode,
-               c->NE, obj_slot, obj_coef};
+               c->NE, obj_slot, obj_coef, AxpyTargets{}};
+  if (ax) a.ax = *ax;
   HIP_TRY(launch_dense_limit_adj(c->geom, c->p, a, c->sp, c->d_dtab, c->stream));
   return RKDG_OK;
 }
@@ -1257,30 +1258,31 @@ int step_adjoint(rkdg_ctx* c, const StepRec& r, double dt, int s, const double* 
   double* kb[4] = {c->d_adj[1], c->d_adj[2], c->d_adj[3], c->d_adj[4]};
   double* vb = c->d_adj[5];
   double* Wb = c->d_adj[6];
-  int rc = limit_adj_launch(c, r.v[s - 1], r.mean[s - 1], ybar, vb, obj_slot, obj_coef);
-  if (rc) return rc;
+  // the adjoint of each limiter feeds the stage combinations through the
+  // targets its pass B updates (u0bar, kbar_i)
   {
     // u0bar = vb; kbar_i = dt b_i vb
     AxpyTargets t{};
     t.n = 0;
     t.y[t.n] = u0bar; t.c[t.n] = 1.0; t.set[t.n] = 1; t.n++;
     for (int i = 0; i < s && t.n < 4; i++) { t.y[t.n] = kb[i]; t.c[t.n] = dt * b[i]; t.set[t.n] = 1; t.n++; }
-    HIP_TRY(launch_axpy_multi(vb, sz, t, c->stream));
+    int rc = limit_adj_launch(c, r.v[s - 1], r.mean[s - 1], ybar, vb, obj_slot, obj_coef, &t);
+    if (rc) return rc;
     if (s == 4) {
       AxpyTargets t2{};
       t2.n = 1; t2.y[0] = kb[3]; t2.c[0] = dt * b[3]; t2.set[0] = 1;
       HIP_TRY(launch_axpy_multi(vb, sz, t2, c->stream));
     }
   }
+  int rc = RKDG_OK;
   for (int j = s - 1; j >= 1; j--) {
     rc = rhs_adj_launch(c, r.w[j - 1], kb[j], Wb, nullptr, c->d_pbar);
-    if (rc) return rc;
-    rc = limit_adj_launch(c, r.v[j - 1], r.mean[j - 1], Wb, vb, -1, 0.0);
     if (rc) return rc;
     AxpyTargets t{};
     t.y[t.n] = u0bar; t.c[t.n] = 1.0; t.set[t.n] = 0; t.n++;
     for (int i = 0; i < j; i++) { t.y[t.n] = kb[i]; t.c[t.n] = dt * A_(j, i); t.set[t.n] = 0; t.n++; }
-    HIP_TRY(launch_axpy_multi(vb, sz, t, c->stream));
+    rc = limit_adj_launch(c, r.v[j - 1], r.mean[j - 1], Wb, vb, -1, 0.0, &t);
+    if (rc) return rc;
   }
   return rhs_adj_launch(c, r.x, kb[0], xbar, u0bar, c->d_pbar);
 }

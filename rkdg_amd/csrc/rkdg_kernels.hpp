@@ -188,6 +188,14 @@ struct RhsAdjArgs {
   const int* fcode;
   int64_t NE;
 };
+// y_k = (set_k ? 0 : y_k) + c_k v for up to four targets (adjoint of the
+// stage combinations)
+struct AxpyTargets {
+  int n;
+  double* y[4];
+  double c[4];
+  int set[4];
+};
 struct LimAdjArgs {
   const double* x;      // pre-limit state
   const double* means;  // its element means [v*NE + e]
@@ -202,6 +210,7 @@ struct LimAdjArgs {
   int64_t NE;
   int obj_slot;         // >= 0: add obj_coef * d(objective)/du of the limited state to ubar first
   double obj_coef;
+  AxpyTargets ax;       // applied with v = the final vbar by pass B (ax.n = 0: none)
 };
 struct ObjArgs {
   const double* x;
@@ -214,12 +223,6 @@ struct ObjArgs {
   int64_t NE;
   const int* elems;  // the elements with a face on the slot, ascending
   int nelems;
-};
-struct AxpyTargets {
-  int n;
-  double* y[4];
-  double c[4];
-  int set[4];
 };
 hipError_t launch_dense_rhs_adj(int geom, int p, const RhsAdjArgs& a, const SchemeParams& g, const double* t,
                                 bool has_bdr, hipStream_t s);
