@@ -85,6 +85,29 @@ def measured_traffic(n):
     return None
 
 
+def copy_bandwidth(dev, n=1 << 27, reps=5):
+    """Attainable HBM bandwidth of a plain streaming kernel on this GPU: a
+    1 GiB read + 1 GiB write elementwise op (torch), timed with events on
+    torch's stream, best of `reps`; GB/s counting both directions."""
+    import torch
+
+    src = torch.ones(n, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    torch.mul(src, 1.0, out=dst)
+    best = None
+    for _ in range(reps):
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        torch.mul(src, 1.0, out=dst)
+        t1.record()
+        t1.synchronize()
+        ms = t0.elapsed_time(t1)
+        best = ms if best is None else min(best, ms)
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * 8 * n / (best * 1e-3) / 1e9
+
+
 def host_cpu_info():
     """CPU model, logical CPUs, sockets, physical cores and memory of this host."""
     model, sockets, cores = "", set(), set()
@@ -410,6 +433,7 @@ def main():
     ctx.check_error()
     lam = ctx.face_lambda(x)
 
+    copy_gbs = copy_bandwidth(dev)
     dofs = info.state_size
     ms_step = el * 1e3 / args.steps
     value = dofs * 3 * args.steps / el
@@ -421,7 +445,7 @@ def main():
     lim_bytes = 2 * 8 * dofs + 8 * 5 * info.num_elements * 7
     traffic = measured_traffic(n)
     print(f"[bench] stage kernel avg {rhs_avg_ms:.3f} ms over {rhs_n} launches "
-          f"({achieved:.0f} GB/s algorithmic), limiter avg {lim_avg_ms:.3f} ms "
+          f"({achieved:.0f} GB/s algorithmic; plain copy {copy_gbs:.0f} GB/s), limiter avg {lim_avg_ms:.3f} ms "
           f"({lim_bytes / (lim_avg_ms * 1e-3) / 1e9:.0f} GB/s), step {ms_step:.3f} ms, "
           f"lambda {lam:.6f}", flush=True)
 
@@ -486,6 +510,8 @@ def main():
             "kernel": "tensor_rhs_kernel<3,2> (fused RHS + SSP-RK3 stage update)",
             "avg_launch_ms": rhs_avg_ms,
             "algorithmic_bytes_per_launch": bytes_per_launch,
+            "copy_gbs": copy_gbs,
+            "frac_of_copy": achieved / copy_gbs,
         },
         "cpu_baseline": cpu,
         "ensemble": ens,
