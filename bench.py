@@ -85,6 +85,14 @@ def measured_traffic(n):
     return None
 
 
+# FP64 work of the Q2 hex right-hand side per element: 64 lanes x (2 FMA + MUL +
+# ADD) per wave from the SQ_INSTS_VALU_{FMA,MUL,ADD}_F64 counters of the plain
+# RHS at 96^3, two elements per wave (idle lanes included: an upper bound on
+# useful FLOPs; the stage epilogue's few axpy FLOPs per DOF are left out)
+RHS_FLOP_PER_ELEMENT = 45952
+FP64_PEAK_TFLOPS = 78.6
+
+
 def copy_bandwidth(dev, n=1 << 27, reps=5):
     """Attainable HBM bandwidth of a plain streaming kernel on this GPU: a
     1 GiB read + 1 GiB write elementwise op (torch), timed with events on
@@ -155,6 +163,13 @@ def cpu_baseline(n, steps, threads):
         x, lam = ref.step(x, dt, S.TABLEAUS["ssprk3"])
     el = time.perf_counter() - t0
     dofs = 5 * m.num_elements * 27
+    # the right-hand side alone: median of 5 Mult calls after the warm-up above
+    mt = []
+    for _ in range(5):
+        t1 = time.perf_counter()
+        ref.mult(x)
+        mt.append(time.perf_counter() - t1)
+    cpu_baseline.mult_median_s = sorted(mt)[2]
     return dofs * 3 * steps / el, el, m.num_elements
 
 
@@ -374,6 +389,9 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--oracle-steps", type=int, default=200)
+    ap.add_argument("--profile-rhs", action="store_true",
+                    help="only run the plain right-hand side 3 times and the limiter once on the 96^3 "
+                         "workload (the driver of the counter profiles), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -405,6 +423,15 @@ def main():
     t0 = time.perf_counter()
     m = M.structured_hex(n, n, n)
     ctx = Context(m, 2, scheme())
+    if args.profile_rhs:
+        x = torch.from_numpy(pulse_state(ctx.node_coords())).to(dev)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            ctx.rhs(x, y, want_lambda=False)
+        ctx.limit(x)
+        torch.cuda.synchronize()
+        print(f"[bench] profile driver: 3 right-hand sides and 1 limiter pass at {n}^3", flush=True)
+        return
     info = ctx.info
     x0 = pulse_state(ctx.node_coords())
     print(f"[bench] mesh {n}^3 Q2 hex: NE={info.num_elements} DOF={info.state_size} "
@@ -453,6 +480,7 @@ def main():
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         cv, cel, cne = cpu_baseline(args.cpu_n, args.cpu_steps, threads)
+        cpu_mult_s = cpu_baseline.mult_median_s
         cv1, cel1, cne1 = cpu_baseline(24, 1, 1)
         from oracle.refcpu import lib as _oracle_lib
 
@@ -463,6 +491,8 @@ def main():
                          f"{args.cpu_steps} SSP-RK3 steps after 1 warm-up step, {cel:.1f} s, "
                          f"OpenMP {threads} threads",
                "host": host_cpu_info(),
+               "mult_median_s": cpu_mult_s,
+               "mult_dof_per_s": 5 * cne * 27 / cpu_mult_s,
                "value_1_thread": cv1,
                "sample_1_thread": f"24^3 elements ({cne1}), 1 step after 1 warm-up step, {cel1:.1f} s"}
         print(f"[bench] cpu baseline {cv:.3e} DOF-stage/s ({cel:.1f}s, {threads} threads); "
@@ -512,6 +542,12 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "copy_gbs": copy_gbs,
             "frac_of_copy": achieved / copy_gbs,
+            "fp64": {
+                "flop_per_element": RHS_FLOP_PER_ELEMENT,
+                "achieved_tflops": RHS_FLOP_PER_ELEMENT * info.num_elements / (rhs_avg_ms * 1e-3) / 1e12,
+                "peak_tflops": FP64_PEAK_TFLOPS,
+                "frac": RHS_FLOP_PER_ELEMENT * info.num_elements / (rhs_avg_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+            },
         },
         "cpu_baseline": cpu,
         "ensemble": ens,
