@@ -177,7 +177,9 @@ def test_dense_conservation_and_hmin(cuda):
     ctx.rhs(xd, yd)
     means = torch.empty(4 * m.num_elements, dtype=torch.float64, device=cuda)
     ctx.means(yd, means)
-    area = 0.5 * np.abs(np.cross(m.elem_coords[:, 1] - m.elem_coords[:, 0], m.elem_coords[:, 2] - m.elem_coords[:, 0]))
+    a = m.elem_coords[:, 1] - m.elem_coords[:, 0]
+    b = m.elem_coords[:, 2] - m.elem_coords[:, 0]
+    area = 0.5 * np.abs(a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0])
     tot = (_host(means).reshape(4, -1) * area).sum(axis=1)
     scale = np.abs(_host(yd)).reshape(4, -1).max(axis=1) * area.sum()
     assert abs(tot[0]) <= 1e-13 * scale[0] and abs(tot[3]) <= 1e-13 * scale[3]
