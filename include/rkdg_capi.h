@@ -71,6 +71,9 @@ extern "C" {
 #define RKDG_BC_SUBSONIC_INLET_FIXED_PRESSURE 4
 #define RKDG_BC_FIXED_PRESSURE_OUTLET 5
 #define RKDG_BC_TOTAL_PRESSURE_OUTLET 6
+/* charOutlet (boundary_integrator_char_outlet.cpp:14-235): characteristic ghost state,
+ * physical flux of the ghost (no Riemann solver); params (rhoI, UI[0], UI[1], UI[2], pI) */
+#define RKDG_BC_CHAR_OUTLET 7
 
 #define RKDG_IND_NOWHERE 0
 #define RKDG_IND_EVERYWHERE 1

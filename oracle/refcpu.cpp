@@ -740,9 +740,51 @@ void bc_state(const Scheme& sc, int dim, int bi, const double* s1, const double*
       s2[0] = rho;
       break;
     }
+    case REF_BC_CHAR_OUTLET: {  // boundary_integrator_char_outlet.cpp:14-51 with its ctor :6-12
+      // the fixed state as case_manager.cpp:781-799 builds it (energy without w^2)
+      double fs[5] = {0, 0, 0, 0, 0};
+      fs[0] = P[0];
+      fs[1] = P[0] * P[1];
+      fs[2] = P[0] * P[2];
+      if (dim == 3) fs[3] = P[0] * P[3];
+      fs[ne - 1] = P[4] / (gm - 1.) + 0.5 * P[0] * (P[1] * P[1] + P[2] * P[2]);
+      const double cOut = sound_speed(fs, dim, g);
+      const double UOut = fs[1] / fs[0];
+      const double cIn = sound_speed(s1, dim, g);
+      // state2 starts as state1 (the reference rotates it uninitialised, Q10)
+      rotate(s2, nor, dim);
+      const double UIn = s2[1] / s2[0];
+      const double UB = 0.5 * (UIn + UOut) - (cOut - cIn) / (gm - 1.0);
+      const double cB = -0.25 * (gm - 1.0) * (UOut - UIn) + 0.5 * (cOut + cIn);
+      double sB;
+      if (s2[1] < 0)  // inflow
+        sB = cOut * cOut / std::pow(fs[0], gm - 1.0) / gm;
+      else  // outflow
+        sB = cIn * cIn / std::pow(s2[0], gm - 1.0) / gm;
+      const double rhoB = std::pow(cB * cB / sB / gm, 1.0 / (gm - 1.0));
+      const double pB = rhoB * cB * cB / gm;
+      s2[0] = rhoB;
+      // the normal momentum rhoB UB (the reference stores the velocity UB in
+      // the momentum slot); the tangential momenta stay, so the tangential
+      // velocities in the energy are those momenta over rhoB, as there
+      s2[1] = rhoB * UB;
+      const double vt = s2[2] / s2[0], wt = dim == 3 ? s2[3] / s2[0] : 0.0;
+      s2[ne - 1] = pB / (gm - 1.0) * (1.0 - rhoB * g.b) + 0.5 * rhoB * (UB * UB + vt * vt + wt * wt);
+      inverse_rotate(s2, nor, dim);
+      break;
+    }
     default:
       break;
   }
+}
+
+// boundary_integrator_char_outlet.cpp:153-189: the characteristic outlet's flux
+// is the physical flux of its ghost state (no Riemann solver); returns |u|+c of
+// the ghost
+double char_outlet_flux(const Scheme& sc, int dim, const double* s2, const double* nor, double* fl) {
+  const Gas g = sc.gas();
+  flux_dot_n(s2, nor, dim, g, fl);
+  return max_char_speed(s2, dim, g);
 }
 
 // ----------------------------------------------------------------------------
@@ -1067,7 +1109,8 @@ int Ref::face_mult(const double* x, double* z, double& mcs, int64_t* bad) {
         double inv = 1.0 / n2;
         for (int i = 0; i < dim; i++) nor[i] *= inv;
         if (isb) bc_state(sc, dim, bi, u1, nor, u2);
-        double m = riemann(sc, dim, u1, u2, nor, fl);
+        double m = (isb && sc.s.bc_type[bi] == REF_BC_CHAR_OUTLET) ? char_outlet_flux(sc, dim, u2, nor, fl)
+                                                                   : riemann(sc, dim, u1, u2, nor, fl);
         if (m < 0) {
 #pragma omp critical
           {

@@ -65,6 +65,7 @@ extern "C" {
 #define REF_BC_SUBSONIC_INLET_FIXED_PRESSURE 4
 #define REF_BC_FIXED_PRESSURE_OUTLET 5
 #define REF_BC_TOTAL_PRESSURE_OUTLET 6
+#define REF_BC_CHAR_OUTLET 7
 
 #define REF_IND_NOWHERE 0
 #define REF_IND_EVERYWHERE 1

@@ -142,7 +142,8 @@ __device__ __forceinline__ void dense_rhs_block(const RhsArgs& A, const SchemePa
       if (slot < g.num_bc) {
         const double* Pover = A.bcp ? A.bcp + ((int64_t)ins * RKDG_MAX_BC + slot) * 8 : nullptr;
         bc_state<2>(g, slot, um, nrm, up, Pover);
-        lam = riemann_t<2, RS>(g, um, up, nrm, fl, ok);
+        if (g.bc_type[slot] == RKDG_BC_CHAR_OUTLET) lam = char_outlet_flux<2>(g, up, nrm, fl);
+        else lam = riemann_t<2, RS>(g, um, up, nrm, fl, ok);
       } else {
 #pragma unroll
         for (int v = 0; v < NEQ; v++) fl[v] = 0.0;

@@ -682,7 +682,7 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
   if (!(s->gamma > 1.0)) return fail(RKDG_E_ARG, "gamma must exceed 1");
   if (!(s->covolume >= 0.0)) return fail(RKDG_E_ARG, "covolume must be non-negative");
   for (int i = 0; i < s->num_bc; i++)
-    if (s->bc_type[i] < RKDG_BC_SLIP || s->bc_type[i] > RKDG_BC_TOTAL_PRESSURE_OUTLET)
+    if (s->bc_type[i] < RKDG_BC_SLIP || s->bc_type[i] > RKDG_BC_CHAR_OUTLET)
       return fail(RKDG_E_ARG, "unknown boundary condition type in slot " + std::to_string(i));
   const int dim = c->dim, nv = c->nv, NE_ = (int)c->NE;
   const int64_t NE = c->NE;

@@ -525,9 +525,48 @@ RKDG_HD void bc_state(const SchemeParams& g, int bi, const double* s1, const dou
       s2[0] = rho;
       break;
     }
+    case RKDG_BC_CHAR_OUTLET: {  // boundary_integrator_char_outlet.cpp:14-51 with its ctor :6-12
+      // the fixed state as case_manager.cpp:781-799 builds it (energy without w^2)
+      double fs[NE];
+      fs[0] = P[0];
+      fs[1] = P[0] * P[1];
+      fs[2] = P[0] * P[2];
+      if (DIM == 3) fs[3] = P[0] * P[3];
+      fs[NE - 1] = P[4] / (gm - 1.) + 0.5 * P[0] * (P[1] * P[1] + P[2] * P[2]);
+      const double cOut = sound_speed_p<DIM>(fs, pressure<DIM>(fs, g), g);
+      const double UOut = fs[1] / fs[0];
+      const double cIn = sound_speed_p<DIM>(s1, pressure<DIM>(s1, g), g);
+      // state2 starts as state1 (the reference rotates it uninitialised)
+      rotate<DIM>(s2, n);
+      const double UIn = s2[1] / s2[0];
+      const double UB = 0.5 * (UIn + UOut) - (cOut - cIn) / (gm - 1.0);
+      const double cB = -0.25 * (gm - 1.0) * (UOut - UIn) + 0.5 * (cOut + cIn);
+      const double sB = s2[1] < 0 ? cOut * cOut / pow_ws(fs[0], gm - 1.0) / gm   // inflow
+                                  : cIn * cIn / pow_ws(s2[0], gm - 1.0) / gm;    // outflow
+      const double rhoB = pow_ws(cB * cB / sB / gm, 1.0 / (gm - 1.0));
+      const double pB = rhoB * cB * cB / gm;
+      s2[0] = rhoB;
+      // normal momentum rhoB UB (the reference stores the velocity UB in the
+      // momentum slot); the tangential momenta stay, so the tangential
+      // velocities in the energy are those momenta over rhoB, as there
+      s2[1] = rhoB * UB;
+      const double vt = s2[2] / s2[0], wt = DIM == 3 ? s2[3] / s2[0] : 0.0;
+      s2[NE - 1] = pB / (gm - 1.0) * (1.0 - rhoB * g.covolume) + 0.5 * rhoB * (UB * UB + vt * vt + wt * wt);
+      inverse_rotate<DIM>(s2, n);
+      break;
+    }
     default:
       break;
   }
+}
+
+// boundary_integrator_char_outlet.cpp:153-189: the characteristic outlet's flux
+// is the physical flux of its ghost state through the unit normal (no Riemann
+// solver); returns |u|+c of the ghost
+template <int DIM>
+RKDG_HD double char_outlet_flux(const SchemeParams& g, const double* s2, const double* n, double* fl) {
+  flux_dot_n<DIM>(s2, n, pressure<DIM>(s2, g), g, fl);
+  return max_char_speed<DIM>(s2, g);
 }
 
 // Euclidean norm by the scaled (hypot-style) recurrence of MFEM Vector::Norml2,

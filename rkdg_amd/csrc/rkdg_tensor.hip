@@ -454,7 +454,8 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
         const int slot = -1 - nb;
         if (slot < g.num_bc) {
           bc_state<DIM>(g, slot, um, nrm, up);
-          lam = riemann_t<DIM, RS>(g, um, up, nrm, fl, ok, inm);
+          if (g.bc_type[slot] == RKDG_BC_CHAR_OUTLET) lam = char_outlet_flux<DIM>(g, up, nrm, fl);
+          else lam = riemann_t<DIM, RS>(g, um, up, nrm, fl, ok, inm);
         } else {
 #pragma unroll
           for (int v = 0; v < NEQ; v++) fl[v] = 0.0;

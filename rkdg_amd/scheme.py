@@ -26,6 +26,7 @@ BC_SUBSONIC_INLET_TOTAL_PRESSURE = 3
 BC_SUBSONIC_INLET_FIXED_PRESSURE = 4
 BC_FIXED_PRESSURE_OUTLET = 5
 BC_TOTAL_PRESSURE_OUTLET = 6
+BC_CHAR_OUTLET = 7
 
 IND_NOWHERE, IND_EVERYWHERE, IND_BJ = 0, 1, 2
 PHI_MICHALAK, PHI_BJ, PHI_VENKAT = 0, 1, 2
@@ -42,6 +43,7 @@ BC_NAMES = {
     "subsonicInletFixedPressure": BC_SUBSONIC_INLET_FIXED_PRESSURE,
     "fixedPressureOutlet": BC_FIXED_PRESSURE_OUTLET,
     "totalPressureFixedTemperatureOutlet": BC_TOTAL_PRESSURE_OUTLET,
+    "charOutlet": BC_CHAR_OUTLET,
 }
 
 
@@ -51,7 +53,8 @@ class BC:
 
     Parameter order per type: supersonicInlet (rhoI, UI[0], UI[1], UI[2], pI);
     subsonicInletTotalPressure (pTot, TTot); subsonicInletFixedPressure (pFix, TTot);
-    fixedPressureOutlet (pFix,); totalPressureFixedTemperatureOutlet (pTot, TFix).
+    fixedPressureOutlet (pFix,); totalPressureFixedTemperatureOutlet (pTot, TFix);
+    charOutlet (rhoI, UI[0], UI[1], UI[2], pI).
     """
 
     attr: int

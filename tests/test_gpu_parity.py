@@ -396,3 +396,38 @@ def test_acoustic_pulse_workload_matches_oracle(cuda):
           f"energy drift {abs(w[4].sum() - energy0) / energy0:.2e}", flush=True)
     assert abs(w[0].sum() - mass0) <= 1e-13 * mass0
     assert abs(w[4].sum() - energy0) <= 1e-13 * energy0
+
+
+@pytest.mark.parametrize("kind", ["quad_q2", "tri_p2", "hex_q1"])
+def test_char_outlet_boundary(kind, cuda):
+    """The characteristic outlet (ghost from the Riemann invariants, physical
+    flux of the ghost instead of a Riemann solver) on the tensor path (Q2
+    quads, Q1 hexes) and the dense path (P2 triangles): RHS, wave speed and two
+    limited SSP-RK3 steps against the oracle."""
+    fixed = (1.1, 0.3, 0.05, 0.0, 0.9)
+    if kind == "quad_q2":
+        m, p = M.structured_quad(8, 4, periodic=(False, True)), 2
+        bcs = [S.BC(2, S.BC_CHAR_OUTLET, fixed), S.BC(4, S.BC_OUTLET)]
+    elif kind == "tri_p2":
+        m, p = M.jittered_delaunay_tri(6, 3, seed=11), 2
+        bcs = [S.BC(1, S.BC_SLIP), S.BC(2, S.BC_CHAR_OUTLET, fixed), S.BC(3, S.BC_SLIP), S.BC(4, S.BC_OUTLET)]
+    else:
+        m, p = M.structured_hex(4, 3, 3, periodic=(False, True, True)), 1
+        bcs = [S.BC(a, S.BC_CHAR_OUTLET if a == 3 else S.BC_OUTLET, fixed) for a in sorted(set(m.bdr_attr.tolist()))]
+    sch = S.Scheme(riemann=S.HLLC, indicator=S.IND_BJ, phi=S.PHI_BJ, limiter=S.LIM_MULTI, bcs=bcs)
+    y, y_ref, lam, lam_ref, neq = _rhs_pair(m, p, sch, cuda, seed=31)
+    err = rel_linf(y, y_ref, neq)
+    print(f"[char outlet {kind}] rhs rel Linf {err:.3e} lam {lam} ref {lam_ref}", flush=True)
+    assert err <= TOL
+    assert abs(lam - lam_ref) <= 1e-13 * lam_ref
+    ref = RefSolver(m, p, sch)
+    ctx = _ctx(m, p, sch)
+    x = smooth_state(ref.node_coords(), seed=32)
+    xd = _dev(x, cuda)
+    for _ in range(2):
+        x, lam_ref = ref.step(x, 1e-3, S.TABLEAUS["ssprk3"])
+        lam = ctx.step(xd, 1e-3, S.TABLEAUS["ssprk3"])
+    err = rel_linf(_host(xd), x, neq)
+    print(f"[char outlet {kind}] 2 steps rel Linf {err:.3e}", flush=True)
+    assert err <= TOL
+    assert abs(lam - lam_ref) <= 1e-12 * lam_ref

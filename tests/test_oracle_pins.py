@@ -510,3 +510,45 @@ def test_subsonic_inlet_fixed_pressure_ghost_invariants():
     np.testing.assert_allclose(vel, np.array(u[1:3]) / u[0], rtol=1e-14)
     T = p / (rho * Rg)
     np.testing.assert_allclose(T * (1 + 0.5 * (g - 1) * (vel @ vel) / (g * Rg * T)), TTot, rtol=1e-13)
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.5236, 2.0])
+@pytest.mark.parametrize("flow", ["out", "in"])
+def test_char_outlet_ghost_keeps_the_characteristic_invariants(theta, flow):
+    """charOutlet (boundary_integrator_char_outlet.cpp:14-51): in the face's
+    normal frame the ghost carries the interior's outgoing Riemann invariant
+    u + 2c/(g-1), the fixed state's incoming one U_out - 2c_out/(g-1) (U_out is
+    the fixed state's x velocity, as in the reference), the interior's entropy
+    p/rho^g on outflow and the fixed state's on inflow, and the interior's
+    tangential momentum."""
+    g = 1.4
+    n = np.array([np.cos(theta), np.sin(theta)])
+    t = np.array([-n[1], n[0]])
+    fixed = (1.2, 0.3, 0.1, 0.0, 0.9)  # rhoI, UI[0], UI[1], UI[2], pI
+    sch = S.Scheme(bcs=[S.BC(2, S.BC_CHAR_OUTLET, fixed)])
+    un = 0.25 if flow == "out" else -0.2
+    vel = un * n + 0.15 * t
+    s1 = np.array(conservative(1.05, list(vel), 1.1))
+    gh = R.bc_state(sch, 2, 0, s1, n)
+    rho, v, p = _prim2(gh)
+    c = math.sqrt(g * p / rho)
+    rho1, v1, p1 = _prim2(s1)
+    c1 = math.sqrt(g * p1 / rho1)
+    rf, uf, pf = fixed[0], fixed[1], fixed[4]
+    cf = math.sqrt(g * pf / rf)
+    assert abs((v @ n + 2 * c / (g - 1)) - (v1 @ n + 2 * c1 / (g - 1))) < 1e-13
+    assert abs((v @ n - 2 * c / (g - 1)) - (uf - 2 * cf / (g - 1))) < 1e-13
+    s_ref = p1 / rho1 ** g if flow == "out" else pf / rf ** g
+    assert abs(p / rho ** g - s_ref) < 1e-13 * s_ref
+    assert abs(rho * (v @ t) - rho1 * (v1 @ t)) < 1e-14
+
+
+def test_char_outlet_ghost_of_the_fixed_state_is_itself():
+    """Interior equal to the fixed state, flowing out along the x normal: the
+    ghost is that state (this pins the ghost's normal momentum rho_B U_B; the
+    reference stores U_B in the momentum slot)."""
+    fixed = (1.3, 0.4, -0.1, 0.0, 0.8)
+    sch = S.Scheme(bcs=[S.BC(2, S.BC_CHAR_OUTLET, fixed)])
+    s1 = np.array(conservative(1.3, [0.4, -0.1], 0.8))
+    gh = R.bc_state(sch, 2, 0, s1, np.array([1.0, 0.0]))
+    np.testing.assert_allclose(gh, s1, rtol=1e-13, atol=1e-15)
