@@ -51,6 +51,9 @@ extern "C" {
 #define RKDG_E_UNPHYSICAL (-2) /* a Riemann solver saw an unphysical state (reference: exit(1)) */
 #define RKDG_E_HIP (-3)
 #define RKDG_E_UNSUPPORTED (-4)
+/* a Riemann solver's input held a NaN or an infinity (reference: StateIsPhysicalSay
+ * fails and the integrator calls exit(1), face_integrator.cpp:214-218) */
+#define RKDG_E_NAN (-5)
 
 #define RKDG_GEOM_QUAD 0
 #define RKDG_GEOM_TRI 1
@@ -152,7 +155,8 @@ int rkdg_means(rkdg_ctx* ctx, const double* x, double* means);
 int rkdg_hmin(rkdg_ctx* ctx, double* hmin);
 /* Physical coordinates of the nodes, HOST [NE][nd][dim] (for initial conditions). */
 int rkdg_node_coords(rkdg_ctx* ctx, double* xyz);
-/* Synchronize and report a deferred device error (RKDG_E_UNPHYSICAL with the
+/* Synchronize and report a deferred device error (RKDG_E_UNPHYSICAL, or
+ * RKDG_E_NAN when an input of the failing flux was not finite, with the
  * element pair in info[0], info[1]); clears it. */
 int rkdg_check_error(rkdg_ctx* ctx, int64_t* info);
 

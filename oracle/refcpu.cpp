@@ -15,32 +15,43 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+
+// The arithmetic type of the restatement. The default build is double, as the
+// reference computes; REF_REAL=long double builds a wider copy of the same
+// operator (librefcpu_ld.so), used to measure how far each double evaluation
+// is from the operator evaluated with more precision. The C interface stays
+// double in both.
+#ifndef REF_REAL
+#define REF_REAL double
+#endif
+using real = REF_REAL;
 
 namespace {
 
 // ----------------------------------------------------------------------------
 // 1D Gauss-Legendre rule on [0,1] (MFEM QuadratureFunctions1D::GaussLegendre):
 // Newton iteration on P_n, points ascending, weights sum to 1.
-void gauss_legendre(int n, double* x, double* w) {
+void gauss_legendre(int n, real* x, real* w) {
   for (int i = 0; i < (n + 1) / 2; i++) {
-    double z = std::cos(M_PI * (i + 0.75) / (n + 0.5));
-    double p1 = 0, p0 = 0, dp = 0;
+    real z = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+    real p1 = 0, p0 = 0, dp = 0;
     for (int it = 0; it < 100; it++) {
       p0 = 1.0;
       p1 = z;
       if (n == 1) p1 = z;
       for (int k = 2; k <= n; k++) {
-        double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+        real p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
         p0 = p1;
         p1 = p2;
       }
       if (n == 1) { p1 = z; p0 = 1.0; }
       dp = n * (z * p1 - p0) / (z * z - 1.0);
-      double dz = p1 / dp;
+      real dz = p1 / dp;
       z -= dz;
       if (std::fabs(dz) < 1e-17) break;
     }
@@ -48,13 +59,13 @@ void gauss_legendre(int n, double* x, double* w) {
     p0 = 1.0;
     p1 = z;
     for (int k = 2; k <= n; k++) {
-      double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+      real p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
       p0 = p1;
       p1 = p2;
     }
     if (n == 1) { p1 = z; p0 = 1.0; }
     dp = n * (z * p1 - p0) / (z * z - 1.0);
-    double wi = 1.0 / ((1.0 - z * z) * dp * dp);  // half of the [-1,1] weight
+    real wi = 1.0 / ((1.0 - z * z) * dp * dp);  // half of the [-1,1] weight
     x[i] = 0.5 * (1.0 - z);
     x[n - 1 - i] = 0.5 * (1.0 + z);
     w[i] = wi;
@@ -66,21 +77,21 @@ void gauss_legendre(int n, double* x, double* w) {
 // Lagrange basis on given nodes (product form: exact Kronecker delta at nodes).
 struct Lagrange1D {
   int n = 0;
-  std::vector<double> x;
-  void eval(double y, double* u) const {
+  std::vector<real> x;
+  void eval(real y, real* u) const {
     for (int i = 0; i < n; i++) {
-      double v = 1.0;
+      real v = 1.0;
       for (int j = 0; j < n; j++)
         if (j != i) v *= (y - x[j]) / (x[i] - x[j]);
       u[i] = v;
     }
   }
-  void deriv(double y, double* d) const {
+  void deriv(real y, real* d) const {
     for (int i = 0; i < n; i++) {
-      double s = 0.0;
+      real s = 0.0;
       for (int m = 0; m < n; m++) {
         if (m == i) continue;
-        double t = 1.0 / (x[i] - x[m]);
+        real t = 1.0 / (x[i] - x[m]);
         for (int j = 0; j < n; j++)
           if (j != i && j != m) t *= (y - x[j]) / (x[i] - x[j]);
         s += t;
@@ -90,9 +101,9 @@ struct Lagrange1D {
   }
 };
 
-bool invert_dense(int n, const double* A, double* Ainv) {
+bool invert_dense(int n, const real* A, real* Ainv) {
   // Gauss-Jordan with partial pivoting (stands in for MFEM DenseMatrixInverse LU)
-  std::vector<double> M(A, A + n * n), I(n * n, 0.0);
+  std::vector<real> M(A, A + n * n), I(n * n, 0.0);
   for (int i = 0; i < n; i++) I[i * n + i] = 1.0;
   for (int c = 0; c < n; c++) {
     int piv = c;
@@ -104,14 +115,14 @@ bool invert_dense(int n, const double* A, double* Ainv) {
         std::swap(M[c * n + k], M[piv * n + k]);
         std::swap(I[c * n + k], I[piv * n + k]);
       }
-    double inv = 1.0 / M[c * n + c];
+    real inv = 1.0 / M[c * n + c];
     for (int k = 0; k < n; k++) {
       M[c * n + k] *= inv;
       I[c * n + k] *= inv;
     }
     for (int r = 0; r < n; r++) {
       if (r == c) continue;
-      double f = M[r * n + c];
+      real f = M[r * n + c];
       if (f == 0.0) continue;
       for (int k = 0; k < n; k++) {
         M[r * n + k] -= f * M[c * n + k];
@@ -119,15 +130,15 @@ bool invert_dense(int n, const double* A, double* Ainv) {
       }
     }
   }
-  std::memcpy(Ainv, I.data(), sizeof(double) * n * n);
+  std::memcpy(Ainv, I.data(), sizeof(real) * n * n);
   return true;
 }
 
 // ----------------------------------------------------------------------------
 // Reference-element data (MFEM Geometry tables).
-const double kQuadVerts[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
-const double kTriVerts[3][2] = {{0, 0}, {1, 0}, {0, 1}};
-const double kHexVerts[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+const real kQuadVerts[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+const real kTriVerts[3][2] = {{0, 0}, {1, 0}, {0, 1}};
+const real kHexVerts[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
                                 {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
 const int kQuadEdges[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
 const int kTriEdges[3][2] = {{0, 1}, {1, 2}, {2, 0}};
@@ -139,8 +150,8 @@ struct RefFE {
   Lagrange1D b1;
   // triangle P_p: monomial exponents and coefficient matrix (coef[m*nd+k])
   std::vector<std::array<int, 2>> mono;
-  std::vector<double> coef;
-  std::vector<double> nodes;  // nd*dim
+  std::vector<real> coef;
+  std::vector<real> nodes;  // nd*dim
 
   void init(int g, int order) {
     geom = g;
@@ -148,7 +159,7 @@ struct RefFE {
     if (g == REF_GEOM_QUAD) { dim = 2; nv = 4; nf = 4; nvf = 2; nd = (p + 1) * (p + 1); }
     else if (g == REF_GEOM_HEX) { dim = 3; nv = 8; nf = 6; nvf = 4; nd = (p + 1) * (p + 1) * (p + 1); }
     else { dim = 2; nv = 3; nf = 3; nvf = 2; nd = (p + 1) * (p + 2) / 2; }
-    std::vector<double> gx(p + 1), gw(p + 1);
+    std::vector<real> gx(p + 1), gw(p + 1);
     gauss_legendre(p + 1, gx.data(), gw.data());
     b1.n = p + 1;
     b1.x = gx;
@@ -175,14 +186,14 @@ struct RefFE {
       int o = 0;
       for (int j = 0; j <= p; j++)
         for (int i = 0; i + j <= p; i++) {
-          double w = gx[i] + gx[j] + gx[p - i - j];
+          real w = gx[i] + gx[j] + gx[p - i - j];
           nodes[o * 2 + 0] = gx[i] / w;
           nodes[o * 2 + 1] = gx[j] / w;
           o++;
         }
       for (int a = 0; a <= p; a++)
         for (int b = 0; a + b <= p; b++) mono.push_back({a, b});
-      std::vector<double> V(nd * nd);
+      std::vector<real> V(nd * nd);
       for (int k = 0; k < nd; k++)
         for (int m = 0; m < nd; m++)
           V[k * nd + m] = std::pow(nodes[k * 2], mono[m][0]) * std::pow(nodes[k * 2 + 1], mono[m][1]);
@@ -191,18 +202,18 @@ struct RefFE {
     }
   }
 
-  void shape(const double* xi, double* phi) const {
+  void shape(const real* xi, real* phi) const {
     if (geom == REF_GEOM_TRI) {
-      double mv[64];
+      real mv[64];
       for (int m = 0; m < nd; m++) mv[m] = std::pow(xi[0], mono[m][0]) * std::pow(xi[1], mono[m][1]);
       for (int k = 0; k < nd; k++) {
-        double s = 0;
+        real s = 0;
         for (int m = 0; m < nd; m++) s += mv[m] * coef[m * nd + k];
         phi[k] = s;
       }
       return;
     }
-    double ux[16], uy[16], uz[16];
+    real ux[16], uy[16], uz[16];
     b1.eval(xi[0], ux);
     b1.eval(xi[1], uy);
     int n = p + 1;
@@ -217,13 +228,13 @@ struct RefFE {
     }
   }
   // dphi[k*dim + r] = d phi_k / d xi_r
-  void dshape(const double* xi, double* dphi) const {
+  void dshape(const real* xi, real* dphi) const {
     if (geom == REF_GEOM_TRI) {
       for (int k = 0; k < nd; k++) {
-        double dx = 0, dy = 0;
+        real dx = 0, dy = 0;
         for (int m = 0; m < nd; m++) {
           int a = mono[m][0], b = mono[m][1];
-          double c = coef[m * nd + k];
+          real c = coef[m * nd + k];
           if (a > 0) dx += c * a * std::pow(xi[0], a - 1) * std::pow(xi[1], b);
           if (b > 0) dy += c * b * std::pow(xi[0], a) * std::pow(xi[1], b - 1);
         }
@@ -233,7 +244,7 @@ struct RefFE {
       return;
     }
     int n = p + 1;
-    double ux[16], uy[16], uz[16], dx[16], dy[16], dz[16];
+    real ux[16], uy[16], uz[16], dx[16], dy[16], dz[16];
     b1.eval(xi[0], ux);
     b1.eval(xi[1], uy);
     b1.deriv(xi[0], dx);
@@ -258,7 +269,7 @@ struct RefFE {
           }
     }
   }
-  const double* vert(int v) const {
+  const real* vert(int v) const {
     if (geom == REF_GEOM_QUAD) return kQuadVerts[v];
     if (geom == REF_GEOM_TRI) return kTriVerts[v];
     return kHexVerts[v];
@@ -281,7 +292,7 @@ struct RefFE {
 // exact rule gives MFEM's values up to rounding since integrands are polynomial).
 struct Rule {
   int n = 0, dim = 1;
-  std::vector<double> pts, w;
+  std::vector<real> pts, w;
 };
 Rule seg_rule(int order) {
   Rule r;
@@ -317,7 +328,7 @@ Rule geom_rule(int geom, int order) {
   } else {
     r.dim = 2;
     int n = (order + 3) / 2;
-    std::vector<double> x(n), w(n);
+    std::vector<real> x(n), w(n);
     gauss_legendre(n, x.data(), w.data());
     for (int i = 0; i < n; i++)
       for (int j = 0; j < n; j++) {
@@ -340,20 +351,20 @@ Rule face_rule(int geom, int order) {
 // hex, affine triangle). J[i*dim + r] = dx_i / dxi_r.
 struct Trans {
   const RefFE* fe;
-  const double* X;  // nv*dim vertex coordinates
-  void jacobian(const double* xi, double* J) const {
+  const real* X;  // nv*dim vertex coordinates
+  void jacobian(const real* xi, real* J) const {
     int d = fe->dim;
     std::fill(J, J + d * d, 0.0);
     for (int v = 0; v < fe->nv; v++) {
-      double dN[3];
+      real dN[3];
       if (fe->geom == REF_GEOM_TRI) {
         if (v == 0) { dN[0] = -1; dN[1] = -1; }
         else if (v == 1) { dN[0] = 1; dN[1] = 0; }
         else { dN[0] = 0; dN[1] = 1; }
       } else {
-        const double* R = fe->vert(v);
+        const real* R = fe->vert(v);
         for (int r = 0; r < d; r++) {
-          double t = R[r] > 0.5 ? 1.0 : -1.0;
+          real t = R[r] > 0.5 ? 1.0 : -1.0;
           for (int s = 0; s < d; s++)
             if (s != r) t *= R[s] > 0.5 ? xi[s] : 1.0 - xi[s];
           dN[r] = t;
@@ -363,14 +374,14 @@ struct Trans {
         for (int r = 0; r < d; r++) J[i * d + r] += X[v * d + i] * dN[r];
     }
   }
-  void transform(const double* xi, double* x) const {
+  void transform(const real* xi, real* x) const {
     int d = fe->dim;
     std::fill(x, x + d, 0.0);
     for (int v = 0; v < fe->nv; v++) {
-      double N;
+      real N;
       if (fe->geom == REF_GEOM_TRI) N = v == 0 ? 1 - xi[0] - xi[1] : xi[v - 1];
       else {
-        const double* R = fe->vert(v);
+        const real* R = fe->vert(v);
         N = 1.0;
         for (int s = 0; s < d; s++) N *= R[s] > 0.5 ? xi[s] : 1.0 - xi[s];
       }
@@ -378,13 +389,13 @@ struct Trans {
     }
   }
 };
-double det(int d, const double* J) {
+real det(int d, const real* J) {
   if (d == 2) return J[0] * J[3] - J[1] * J[2];
   return J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
          J[2] * (J[3] * J[7] - J[4] * J[6]);
 }
 // adj(J): J * adj(J) = det(J) I   (MFEM CalcAdjugate)
-void adjugate(int d, const double* J, double* A) {
+void adjugate(int d, const real* J, real* A) {
   if (d == 2) {
     A[0] = J[3];
     A[1] = -J[1];
@@ -406,158 +417,158 @@ void adjugate(int d, const double* J, double* A) {
 // ----------------------------------------------------------------------------
 // Physics (src/physics/physics.cpp). State layout: rho, rho*u[dim], rho*E.
 struct Gas {
-  double gamma, R, b;
+  real gamma, R, b;
 };
 // physics.cpp:101-112
-double pressure(const double* s, int dim, const Gas& g) {
-  double v2 = 0;
+real pressure(const real* s, int dim, const Gas& g) {
+  real v2 = 0;
   for (int d = 0; d < dim; d++) v2 += s[1 + d] * s[1 + d];
   v2 /= s[0];
   return (g.gamma - 1.0) * (s[1 + dim] - 0.5 * v2) / (1.0 - s[0] * g.b);
 }
 // physics.cpp:134-140
-double sound_speed(const double* s, int dim, const Gas& g) {
-  double p = pressure(s, dim, g);
+real sound_speed(const real* s, int dim, const Gas& g) {
+  real p = pressure(s, dim, g);
   return std::sqrt(g.gamma * p / s[0] / (1.0 - s[0] * g.b));
 }
 // physics.cpp:6-50
-bool state_is_physical(const double* s, int dim, const Gas& g) {
-  double den = s[0], E = s[1 + dim];
+bool state_is_physical(const real* s, int dim, const Gas& g) {
+  real den = s[0], E = s[1 + dim];
   if (den < 0 || den != den) return false;
   if (E <= 0 || E != E) return false;
-  double p = pressure(s, dim, g);
+  real p = pressure(s, dim, g);
   if (p <= 0 || p != p) return false;
   return true;
 }
 // physics.cpp:53-97 (pressure check ignores the covolume, :84)
-bool state_is_physical_say(const double* s, int dim, const Gas& g) {
-  double den = s[0], E = s[1 + dim];
+bool state_is_physical_say(const real* s, int dim, const Gas& g) {
+  real den = s[0], E = s[1 + dim];
   if (den < 0 || den != den) return false;
   if (E <= 0 || E != E) return false;
-  double v2 = 0;
+  real v2 = 0;
   for (int i = 0; i < dim; i++) v2 += s[1 + i] * s[1 + i];
   v2 /= den;
-  double p = (g.gamma - 1.0) * (E - 0.5 * v2);
+  real p = (g.gamma - 1.0) * (E - 0.5 * v2);
   if (p <= 0 || p != p) return false;
   return true;
 }
 // physics.cpp:143-168; f[k*dim + d]
-void flux_tensor(const double* s, int dim, const Gas& g, double* f) {
-  double den = s[0];
-  double p = pressure(s, dim, g);
+void flux_tensor(const real* s, int dim, const Gas& g, real* f) {
+  real den = s[0];
+  real p = pressure(s, dim, g);
   for (int d = 0; d < dim; d++) {
     f[0 * dim + d] = s[1 + d];
     for (int i = 0; i < dim; i++) f[(1 + i) * dim + d] = s[1 + i] * s[1 + d] / den;
     f[(1 + d) * dim + d] += p;
   }
-  double H = (s[1 + dim] + p) / den;
+  real H = (s[1 + dim] + p) / den;
   for (int d = 0; d < dim; d++) f[(1 + dim) * dim + d] = s[1 + d] * H;
 }
 // physics.cpp:171-195
-void flux_dot_n(const double* s, const double* nor, int dim, const Gas& g, double* fn) {
-  double den = s[0];
-  double p = pressure(s, dim, g);
-  double vn = 0;
+void flux_dot_n(const real* s, const real* nor, int dim, const Gas& g, real* fn) {
+  real den = s[0];
+  real p = pressure(s, dim, g);
+  real vn = 0;
   for (int d = 0; d < dim; d++) vn += s[1 + d] * nor[d];
   fn[0] = vn;
   for (int d = 0; d < dim; d++) fn[1 + d] = vn * s[1 + d] / den + p * nor[d];
-  double H = (s[1 + dim] + p) / den;
+  real H = (s[1 + dim] + p) / den;
   fn[1 + dim] = vn * H;
 }
 // physics.cpp:198-231 (x-direction flux of a rotated state)
-void flux_f(const double* s, int dim, const Gas& g, double* f) {
-  double p = pressure(s, dim, g);
+void flux_f(const real* s, int dim, const Gas& g, real* f) {
+  real p = pressure(s, dim, g);
   f[0] = s[1];
   f[1] = s[1] * s[1] / s[0] + p;
   for (int d = 1; d < dim; d++) f[d + 1] = s[1] * s[d + 1] / s[0];
-  double H = (s[1 + dim] + p) / s[0];
+  real H = (s[1 + dim] + p) / s[0];
   f[1 + dim] = s[1] * H;
 }
 // physics.cpp:252-266
-double max_char_speed(const double* s, int dim, const Gas& g) {
-  double v2 = 0;
+real max_char_speed(const real* s, int dim, const Gas& g) {
+  real v2 = 0;
   for (int d = 0; d < dim; d++) v2 += s[1 + d] * s[1 + d];
   v2 /= s[0];
-  double c = sound_speed(s, dim, g);
-  double vel = std::sqrt(v2 / s[0]);
+  real c = sound_speed(s, dim, g);
+  real vel = std::sqrt(v2 / s[0]);
   return vel + c;
 }
 // physics.cpp:235-249
-double mach(const double* s, int dim, const Gas& g) {
-  double v2 = 0;
+real mach(const real* s, int dim, const Gas& g) {
+  real v2 = 0;
   for (int d = 0; d < dim; d++) v2 += s[1 + d] * s[1 + d];
   v2 /= s[0];
-  double c = sound_speed(s, dim, g);
+  real c = sound_speed(s, dim, g);
   return std::sqrt(v2 / s[0]) / c;
 }
 // physics.cpp:309-350 (Toro two-rarefaction estimate)
-void toro_speeds(const double* s1, const double* s2, int dim, const Gas& g, double* lam) {
-  double cL = sound_speed(s1, dim, g), cR = sound_speed(s2, dim, g);
-  double uL = s1[1] / s1[0], uR = s2[1] / s2[0];
-  double pL = pressure(s1, dim, g), pR = pressure(s2, dim, g);
-  double gm = g.gamma;
-  double z = 0.5 * (gm - 1.0) / gm;
-  double pStar = std::pow((cL + cR - 0.5 * (gm - 1.0) * (uR - uL)) /
+void toro_speeds(const real* s1, const real* s2, int dim, const Gas& g, real* lam) {
+  real cL = sound_speed(s1, dim, g), cR = sound_speed(s2, dim, g);
+  real uL = s1[1] / s1[0], uR = s2[1] / s2[0];
+  real pL = pressure(s1, dim, g), pR = pressure(s2, dim, g);
+  real gm = g.gamma;
+  real z = 0.5 * (gm - 1.0) / gm;
+  real pStar = std::pow((cL + cR - 0.5 * (gm - 1.0) * (uR - uL)) /
                               (cL / std::pow(pL, z) + cR / std::pow(pR, z)),
                           1.0 / z);
-  double qL = pStar > pL ? std::sqrt(1.0 + 0.5 * (gm + 1.0) * (pStar / pL - 1.0) / gm) : 1.0;
-  double qR = pStar > pR ? std::sqrt(1.0 + 0.5 * (gm + 1.0) * (pStar / pR - 1.0) / gm) : 1.0;
+  real qL = pStar > pL ? std::sqrt(1.0 + 0.5 * (gm + 1.0) * (pStar / pL - 1.0) / gm) : 1.0;
+  real qR = pStar > pR ? std::sqrt(1.0 + 0.5 * (gm + 1.0) * (pStar / pR - 1.0) / gm) : 1.0;
   lam[0] = uL - qL * cL;
   lam[dim + 1] = uR + qR * cR;
   for (int i = 1; i < dim + 1; i++) lam[i] = uL;
 }
 // physics.cpp:269-296 (Einfeldt averaged speeds; unused by the reference solvers)
-void einfeldt_speeds(const double* s1, const double* s2, int dim, const Gas& g, double* lam) {
-  double cL = sound_speed(s1, dim, g), cR = sound_speed(s2, dim, g);
-  double ruL = s1[1], ruR = s2[1];
-  double uL = ruL / s1[0], uR = ruR / s2[0];
-  double sL = std::sqrt(s1[0]), sR = std::sqrt(s2[0]);
-  double sum = sL + sR;
-  double eta2 = 0.5 * sL * sR / ((sL + sR) * (sL + sR));
-  double uav = (ruL / sL + ruR / sR) / sum;
-  double cav = std::sqrt((sL * (cL * cL) + sR * (cR * cR)) / sum + eta2 * ((uR - uL) * (uR - uL)));
+void einfeldt_speeds(const real* s1, const real* s2, int dim, const Gas& g, real* lam) {
+  real cL = sound_speed(s1, dim, g), cR = sound_speed(s2, dim, g);
+  real ruL = s1[1], ruR = s2[1];
+  real uL = ruL / s1[0], uR = ruR / s2[0];
+  real sL = std::sqrt(s1[0]), sR = std::sqrt(s2[0]);
+  real sum = sL + sR;
+  real eta2 = 0.5 * sL * sR / ((sL + sR) * (sL + sR));
+  real uav = (ruL / sL + ruR / sR) / sum;
+  real cav = std::sqrt((sL * (cL * cL) + sR * (cR * cR)) / sum + eta2 * ((uR - uL) * (uR - uL)));
   lam[0] = uav - cav;
   lam[dim + 1] = uav + cav;
   for (int i = 1; i < dim + 1; i++) lam[i] = uav;
 }
 
 // rs_basic.cpp:11-38
-void rotate(double* s, const double* nor, int dim) {
-  double* m = s + 1;
+void rotate(real* s, const real* nor, int dim) {
+  real* m = s + 1;
   if (dim == 2) {
-    double mx = nor[0] * m[0] + nor[1] * m[1];
-    double my = -nor[1] * m[0] + nor[0] * m[1];
+    real mx = nor[0] * m[0] + nor[1] * m[1];
+    real my = -nor[1] * m[0] + nor[0] * m[1];
     m[0] = mx;
     m[1] = my;
   } else {
-    double sp = nor[2];
-    double cp = std::sqrt(1 - sp * sp);
-    double st = std::fabs(cp) > 1e-6 ? nor[1] / cp : 1.0;
-    double ct = std::fabs(cp) > 1e-6 ? nor[0] / cp : 0.0;
-    double mx = cp * ct * m[0] + cp * st * m[1] + sp * m[2];
-    double my = -st * m[0] + ct * m[1];
-    double mz = -sp * ct * m[0] - sp * st * m[1] + cp * m[2];
+    real sp = nor[2];
+    real cp = std::sqrt(1 - sp * sp);
+    real st = std::fabs(cp) > 1e-6 ? nor[1] / cp : 1.0;
+    real ct = std::fabs(cp) > 1e-6 ? nor[0] / cp : 0.0;
+    real mx = cp * ct * m[0] + cp * st * m[1] + sp * m[2];
+    real my = -st * m[0] + ct * m[1];
+    real mz = -sp * ct * m[0] - sp * st * m[1] + cp * m[2];
     m[0] = mx;
     m[1] = my;
     m[2] = mz;
   }
 }
 // rs_basic.cpp:40-67
-void inverse_rotate(double* s, const double* nor, int dim) {
-  double* m = s + 1;
+void inverse_rotate(real* s, const real* nor, int dim) {
+  real* m = s + 1;
   if (dim == 2) {
-    double mx = nor[0] * m[0] - nor[1] * m[1];
-    double my = nor[1] * m[0] + nor[0] * m[1];
+    real mx = nor[0] * m[0] - nor[1] * m[1];
+    real my = nor[1] * m[0] + nor[0] * m[1];
     m[0] = mx;
     m[1] = my;
   } else {
-    double sp = nor[2];
-    double cp = std::sqrt(1 - sp * sp);
-    double st = std::fabs(cp) > 1e-6 ? nor[1] / cp : 1.0;
-    double ct = std::fabs(cp) > 1e-6 ? nor[0] / cp : 0.0;
-    double mx = cp * ct * m[0] - st * m[1] - sp * ct * m[2];
-    double my = cp * st * m[0] + ct * m[1] - sp * st * m[2];
-    double mz = sp * m[0] + cp * m[2];
+    real sp = nor[2];
+    real cp = std::sqrt(1 - sp * sp);
+    real st = std::fabs(cp) > 1e-6 ? nor[1] / cp : 1.0;
+    real ct = std::fabs(cp) > 1e-6 ? nor[0] / cp : 0.0;
+    real mx = cp * ct * m[0] - st * m[1] - sp * ct * m[2];
+    real my = cp * st * m[0] + ct * m[1] - sp * st * m[2];
+    real mz = sp * m[0] + cp * m[2];
     m[0] = mx;
     m[1] = my;
     m[2] = mz;
@@ -572,19 +583,19 @@ struct Scheme {
 // Riemann solvers. The inputs are rotated in place as the reference does
 // (rs_llf.cpp:18-19, rs_hll.cpp:14-15, rs_hllc.cpp:79-80). Returns -1 on an
 // unphysical input state.
-double riemann(const Scheme& sc, int dim, double* s1, double* s2, const double* nor, double* flux) {
+real riemann(const Scheme& sc, int dim, real* s1, real* s2, const real* nor, real* flux) {
   const Gas g = sc.gas();
   const int ne = dim + 2;
   if (!state_is_physical_say(s1, dim, g)) return -1;
   if (!state_is_physical_say(s2, dim, g)) return -1;
-  double f1[5], f2[5], lam[5];
+  real f1[5], f2[5], lam[5];
   switch (sc.s.riemann) {
     case REF_RS_RUSANOV: {  // rs_rusanov.cpp:6-40
-      double m1 = max_char_speed(s1, dim, g), m2 = max_char_speed(s2, dim, g);
-      double mE = std::max(m1, m2);
+      real m1 = max_char_speed(s1, dim, g), m2 = max_char_speed(s2, dim, g);
+      real mE = std::max(m1, m2);
       flux_dot_n(s1, nor, dim, g, f1);
       flux_dot_n(s2, nor, dim, g, f2);
-      double nm = 0;
+      real nm = 0;
       for (int i = 0; i < dim; i++) nm += nor[i] * nor[i];
       nm = std::sqrt(nm);
       for (int i = 0; i < ne; i++) flux[i] = 0.5 * (f1[i] + f2[i]) / nm - 0.5 * mE * (s2[i] - s1[i]);
@@ -597,7 +608,7 @@ double riemann(const Scheme& sc, int dim, double* s1, double* s2, const double* 
       flux_f(s2, dim, g, f2);
       if (sc.s.wave_speeds == REF_WS_EINFELDT) einfeldt_speeds(s1, s2, dim, g, lam);
       else toro_speeds(s1, s2, dim, g, lam);
-      double mE = std::max(std::fabs(lam[0]), std::fabs(lam[dim + 1]));
+      real mE = std::max(std::fabs(lam[0]), std::fabs(lam[dim + 1]));
       for (int i = 0; i < ne; i++) flux[i] = 0.5 * (f1[i] + f2[i]) - 0.5 * mE * (s2[i] - s1[i]);
       inverse_rotate(flux, nor, dim);
       return mE;
@@ -609,8 +620,8 @@ double riemann(const Scheme& sc, int dim, double* s1, double* s2, const double* 
       flux_f(s2, dim, g, f2);
       if (sc.s.wave_speeds == REF_WS_EINFELDT) einfeldt_speeds(s1, s2, dim, g, lam);
       else toro_speeds(s1, s2, dim, g, lam);
-      double mE = std::max(std::fabs(lam[0]), std::fabs(lam[dim + 1]));
-      double SL = lam[0], SR = lam[dim + 1];
+      real mE = std::max(std::fabs(lam[0]), std::fabs(lam[dim + 1]));
+      real SL = lam[0], SR = lam[dim + 1];
       if (SL >= 0)
         for (int i = 0; i < ne; i++) flux[i] = f1[i];
       else if (SR <= 0)
@@ -628,24 +639,24 @@ double riemann(const Scheme& sc, int dim, double* s1, double* s2, const double* 
       flux_f(s2, dim, g, f2);
       if (sc.s.wave_speeds == REF_WS_EINFELDT) einfeldt_speeds(s1, s2, dim, g, lam);
       else toro_speeds(s1, s2, dim, g, lam);
-      double SL = lam[0], SR = lam[dim + 1];
-      double mE = sc.s.hllc_abs ? std::max(std::fabs(SL), std::fabs(SR)) : std::max(SL, SR);
+      real SL = lam[0], SR = lam[dim + 1];
+      real mE = sc.s.hllc_abs ? std::max(std::fabs(SL), std::fabs(SR)) : std::max(SL, SR);
       if (SL >= 0)
         for (int i = 0; i < ne; i++) flux[i] = f1[i];
       else if (SR <= 0)
         for (int i = 0; i < ne; i++) flux[i] = f2[i];
       else {
-        double pL = pressure(s1, dim, g), pR = pressure(s2, dim, g);
-        double cL = SL - s1[1] / s1[0];
-        double cR = SR - s2[1] / s2[0];
-        double sStar = (pR - pL + s1[1] * cL - s2[1] * cR) / (s1[0] * cL - s2[0] * cR);
-        const double* st = sStar > 0.0 ? s1 : s2;
-        const double* fK = sStar > 0.0 ? f1 : f2;
-        double pK = sStar > 0.0 ? pL : pR;
-        double SK = sStar > 0.0 ? SL : SR;
-        double cK = sStar > 0.0 ? cL : cR;
-        double rhoK = st[0];
-        double D[5] = {0, 0, 0, 0, 0};
+        real pL = pressure(s1, dim, g), pR = pressure(s2, dim, g);
+        real cL = SL - s1[1] / s1[0];
+        real cR = SR - s2[1] / s2[0];
+        real sStar = (pR - pL + s1[1] * cL - s2[1] * cR) / (s1[0] * cL - s2[0] * cR);
+        const real* st = sStar > 0.0 ? s1 : s2;
+        const real* fK = sStar > 0.0 ? f1 : f2;
+        real pK = sStar > 0.0 ? pL : pR;
+        real SK = sStar > 0.0 ? SL : SR;
+        real cK = sStar > 0.0 ? cL : cR;
+        real rhoK = st[0];
+        real D[5] = {0, 0, 0, 0, 0};
         D[1] = 1.0;
         D[dim + 1] = sStar;
         for (int i = 0; i < ne; i++)
@@ -659,11 +670,12 @@ double riemann(const Scheme& sc, int dim, double* s1, double* s2, const double* 
 }
 
 // Ghost states: src/bc/boundary_integrator_*.cpp computeRightState.
-void bc_state(const Scheme& sc, int dim, int bi, const double* s1, const double* nor, double* s2) {
+void bc_state(const Scheme& sc, int dim, int bi, const real* s1, const real* nor, real* s2) {
   const Gas g = sc.gas();
   const int ne = dim + 2;
-  const double* P = sc.s.bc_param[bi];
-  const double gm = g.gamma, R = g.R;
+  real P[8];
+  for (int k = 0; k < 8; k++) P[k] = sc.s.bc_param[bi][k];
+  const real gm = g.gamma, R = g.R;
   for (int i = 0; i < ne; i++) s2[i] = s1[i];
   switch (sc.s.bc_type[bi]) {
     case REF_BC_SLIP:  // boundary_integrator_slip.cpp:7-17
@@ -675,8 +687,8 @@ void bc_state(const Scheme& sc, int dim, int bi, const double* s1, const double*
       break;
     case REF_BC_SUPERSONIC_INLET: {  // boundary_integrator_supersonic_inlet.cpp:8-19;
       // state from case_manager.cpp:736-750 (energy omits w^2, :749)
-      double rho = P[0], u = P[1], v = P[2], w = P[3], p = P[4];
-      double st[5];
+      real rho = P[0], u = P[1], v = P[2], w = P[3], p = P[4];
+      real st[5];
       st[0] = rho;
       st[1] = rho * u;
       st[2] = rho * v;
@@ -687,17 +699,17 @@ void bc_state(const Scheme& sc, int dim, int bi, const double* s1, const double*
     }
     case REF_BC_SUBSONIC_INLET_TOTAL_PRESSURE:  // ..._subsonic_inlet_total_pressure.cpp:8-38
     case REF_BC_SUBSONIC_INLET_FIXED_PRESSURE: {  // ..._subsonic_inlet_fixed_pressure.cpp:8-38
-      double pTot = P[0], TTot = P[1];
-      double rhoU2 = s2[1] * s2[1] + s2[2] * s2[2];
+      real pTot = P[0], TTot = P[1];
+      real rhoU2 = s2[1] * s2[1] + s2[2] * s2[2];
       if (dim == 3) rhoU2 += s2[3] * s2[3];
       rhoU2 /= s2[0];
-      double MTTotSq = rhoU2 / s2[0] / gm / R / TTot;
-      double MSq = MTTotSq / (1.0 - 0.5 * (gm - 1.0) * MTTotSq);
-      double p = sc.s.bc_type[bi] == REF_BC_SUBSONIC_INLET_TOTAL_PRESSURE
+      real MTTotSq = rhoU2 / s2[0] / gm / R / TTot;
+      real MSq = MTTotSq / (1.0 - 0.5 * (gm - 1.0) * MTTotSq);
+      real p = sc.s.bc_type[bi] == REF_BC_SUBSONIC_INLET_TOTAL_PRESSURE
                      ? pTot * std::pow(1.0 + 0.5 * (gm - 1.0) * MSq, -gm / (gm - 1.0))
                      : pTot;
-      double T = TTot / (1.0 + 0.5 * (gm - 1.0) * MSq);
-      double rho = p / R / T;
+      real T = TTot / (1.0 + 0.5 * (gm - 1.0) * MSq);
+      real rho = p / R / T;
       s2[ne - 1] = p / (gm - 1.0) + 0.5 * rhoU2 / s2[0] * rho;
       s2[1] = s2[1] / s2[0] * rho;
       s2[2] = s2[2] / s2[0] * rho;
@@ -706,13 +718,13 @@ void bc_state(const Scheme& sc, int dim, int bi, const double* s1, const double*
       break;
     }
     case REF_BC_FIXED_PRESSURE_OUTLET: {  // boundary_integrator_open_fixed_pressure.cpp:9-32
-      double pFix = P[0];
-      double rhoU2 = s2[1] * s2[1] + s2[2] * s2[2];
+      real pFix = P[0];
+      real rhoU2 = s2[1] * s2[1] + s2[2] * s2[2];
       if (dim == 3) rhoU2 += s2[3] * s2[3];
-      double U2 = rhoU2 / s2[0] / s2[0];
-      double epsIn = (s2[ne - 1] - 0.5 * rhoU2) / s2[0];
-      double rho = pFix / (gm - 1.0) / epsIn;
-      double rhoE = pFix / (gm - 1.0) + 0.5 * rho * U2;
+      real U2 = rhoU2 / s2[0] / s2[0];
+      real epsIn = (s2[ne - 1] - 0.5 * rhoU2) / s2[0];
+      real rho = pFix / (gm - 1.0) / epsIn;
+      real rhoE = pFix / (gm - 1.0) + 0.5 * rho * U2;
       s2[ne - 1] = rhoE;
       s2[1] = s1[1] / s1[0] * rho;
       s2[2] = s1[2] / s1[0] * rho;
@@ -721,18 +733,18 @@ void bc_state(const Scheme& sc, int dim, int bi, const double* s1, const double*
       break;
     }
     case REF_BC_TOTAL_PRESSURE_OUTLET: {  // ..._open_total_pressure.cpp:9-42, normal index fixed (nor[0..dim-1])
-      double pTotal = P[0], TFix = P[1];
-      double rho2U2 = s2[1] * s2[1] + s2[2] * s2[2];
+      real pTotal = P[0], TFix = P[1];
+      real rho2U2 = s2[1] * s2[1] + s2[2] * s2[2];
       if (dim == 3) rho2U2 += s2[3] * s2[3];
-      double rhoN = s2[1] * nor[0] + s2[2] * nor[1];
+      real rhoN = s2[1] * nor[0] + s2[2] * nor[1];
       if (dim == 3) rhoN += s2[3] * nor[2];
-      double U2 = rho2U2 / s2[0] / s2[0];
-      double epsIn = rhoN > 0 ? (s2[ne - 1] - 0.5 * rho2U2) / s2[0] : R / (gm - 1.0) * TFix;
-      double M = mach(s2, dim, g);
-      double pr = M > 1 ? pTotal * std::pow(1.0 + 0.5 * (gm - 1.0) * M * M, -gm / (gm - 1.0)) : pTotal;
-      double rho = M > 1 ? pr / (gm - 1.0) / epsIn : pr / (0.5 * U2 + (gm - 1.0) * epsIn);
+      real U2 = rho2U2 / s2[0] / s2[0];
+      real epsIn = rhoN > 0 ? (s2[ne - 1] - 0.5 * rho2U2) / s2[0] : R / (gm - 1.0) * TFix;
+      real M = mach(s2, dim, g);
+      real pr = M > 1 ? pTotal * std::pow(1.0 + 0.5 * (gm - 1.0) * M * M, -gm / (gm - 1.0)) : pTotal;
+      real rho = M > 1 ? pr / (gm - 1.0) / epsIn : pr / (0.5 * U2 + (gm - 1.0) * epsIn);
       pr = M > 1 ? pr : pr - 0.5 * U2 * rho;
-      double rhoE = pr / (gm - 1.0) + 0.5 * rho * U2;
+      real rhoE = pr / (gm - 1.0) + 0.5 * rho * U2;
       s2[ne - 1] = rhoE;
       s2[1] *= rho / s2[0];
       s2[2] *= rho / s2[0];
@@ -742,33 +754,33 @@ void bc_state(const Scheme& sc, int dim, int bi, const double* s1, const double*
     }
     case REF_BC_CHAR_OUTLET: {  // boundary_integrator_char_outlet.cpp:14-51 with its ctor :6-12
       // the fixed state as case_manager.cpp:781-799 builds it (energy without w^2)
-      double fs[5] = {0, 0, 0, 0, 0};
+      real fs[5] = {0, 0, 0, 0, 0};
       fs[0] = P[0];
       fs[1] = P[0] * P[1];
       fs[2] = P[0] * P[2];
       if (dim == 3) fs[3] = P[0] * P[3];
       fs[ne - 1] = P[4] / (gm - 1.) + 0.5 * P[0] * (P[1] * P[1] + P[2] * P[2]);
-      const double cOut = sound_speed(fs, dim, g);
-      const double UOut = fs[1] / fs[0];
-      const double cIn = sound_speed(s1, dim, g);
+      const real cOut = sound_speed(fs, dim, g);
+      const real UOut = fs[1] / fs[0];
+      const real cIn = sound_speed(s1, dim, g);
       // state2 starts as state1 (the reference rotates it uninitialised, Q10)
       rotate(s2, nor, dim);
-      const double UIn = s2[1] / s2[0];
-      const double UB = 0.5 * (UIn + UOut) - (cOut - cIn) / (gm - 1.0);
-      const double cB = -0.25 * (gm - 1.0) * (UOut - UIn) + 0.5 * (cOut + cIn);
-      double sB;
+      const real UIn = s2[1] / s2[0];
+      const real UB = 0.5 * (UIn + UOut) - (cOut - cIn) / (gm - 1.0);
+      const real cB = -0.25 * (gm - 1.0) * (UOut - UIn) + 0.5 * (cOut + cIn);
+      real sB;
       if (s2[1] < 0)  // inflow
         sB = cOut * cOut / std::pow(fs[0], gm - 1.0) / gm;
       else  // outflow
         sB = cIn * cIn / std::pow(s2[0], gm - 1.0) / gm;
-      const double rhoB = std::pow(cB * cB / sB / gm, 1.0 / (gm - 1.0));
-      const double pB = rhoB * cB * cB / gm;
+      const real rhoB = std::pow(cB * cB / sB / gm, 1.0 / (gm - 1.0));
+      const real pB = rhoB * cB * cB / gm;
       s2[0] = rhoB;
       // the normal momentum rhoB UB (the reference stores the velocity UB in
       // the momentum slot); the tangential momenta stay, so the tangential
       // velocities in the energy are those momenta over rhoB, as there
       s2[1] = rhoB * UB;
-      const double vt = s2[2] / s2[0], wt = dim == 3 ? s2[3] / s2[0] : 0.0;
+      const real vt = s2[2] / s2[0], wt = dim == 3 ? s2[3] / s2[0] : 0.0;
       s2[ne - 1] = pB / (gm - 1.0) * (1.0 - rhoB * g.b) + 0.5 * rhoB * (UB * UB + vt * vt + wt * wt);
       inverse_rotate(s2, nor, dim);
       break;
@@ -781,7 +793,7 @@ void bc_state(const Scheme& sc, int dim, int bi, const double* s1, const double*
 // boundary_integrator_char_outlet.cpp:153-189: the characteristic outlet's flux
 // is the physical flux of its ghost state (no Riemann solver); returns |u|+c of
 // the ghost
-double char_outlet_flux(const Scheme& sc, int dim, const double* s2, const double* nor, double* fl) {
+real char_outlet_flux(const Scheme& sc, int dim, const real* s2, const real* nor, real* fl) {
   const Gas g = sc.gas();
   flux_dot_n(s2, nor, dim, g, fl);
   return max_char_speed(s2, dim, g);
@@ -799,7 +811,7 @@ struct Face {
 struct CSR {
   std::vector<int64_t> I;
   std::vector<int64_t> J;
-  std::vector<double> A;
+  std::vector<real> A;
 };
 
 struct Ref {
@@ -807,7 +819,7 @@ struct Ref {
   Scheme sc;
   int dim = 2, nv = 4, nd = 4;
   int64_t NE = 0, NDofs = 0;
-  std::vector<double> X;       // NE*nv*dim
+  std::vector<real> X;       // NE*nv*dim
   std::vector<int64_t> topo;   // NE*nv
   std::vector<Face> faces;     // all faces in face-id order
   std::vector<int64_t> interior;  // interior face ids (ascending)
@@ -816,11 +828,11 @@ struct Ref {
   std::vector<std::vector<int64_t>> elem_faces;  // local face -> face id
   std::vector<std::vector<int64_t>> el2el;       // face neighbours
   CSR Aflux;
-  std::vector<double> Me_inv;  // NE*nd*nd (row-major)
-  std::vector<double> avgs;    // nv_eq*NE  (byNODES: [v*NE + e])
+  std::vector<real> Me_inv;  // NE*nd*nd (row-major)
+  std::vector<real> avgs;    // nv_eq*NE  (byNODES: [v*NE + e])
   // per-element face-contribution order for deterministic parallel accumulation
   std::vector<std::vector<std::pair<int64_t, int>>> contrib;  // (slot, side)
-  std::vector<double> face_buf;  // slot * 2 * nd * ne
+  std::vector<real> face_buf;  // slot * 2 * nd * ne
   int64_t n_slots = 0;
 
   std::vector<char> affine;    // element map is affine (parallelogram/parallelepiped/triangle)
@@ -829,8 +841,8 @@ struct Ref {
   Trans trans(int64_t e) const { return Trans{&fe, X.data() + e * nv * dim}; }
   // exact Jacobian of an affine element: J(:,r) = X(v_r) - X(v_0) with v_r the
   // vertex at the unit point of reference axis r
-  void affine_jacobian(int64_t e, double* J) const {
-    const double* Xe = X.data() + e * nv * dim;
+  void affine_jacobian(int64_t e, real* J) const {
+    const real* Xe = X.data() + e * nv * dim;
     int cols[3] = {1, fe.geom == REF_GEOM_TRI ? 2 : 3, 4};
     for (int i = 0; i < dim; i++)
       for (int r = 0; r < dim; r++) J[i * dim + r] = Xe[cols[r] * dim + i] - Xe[i];
@@ -839,15 +851,15 @@ struct Ref {
     affine.assign(NE, 1);
     if (fe.geom == REF_GEOM_TRI) return;
     for (int64_t e = 0; e < NE; e++) {
-      double J[9];
+      real J[9];
       affine_jacobian(e, J);
-      const double* Xe = X.data() + e * nv * dim;
-      double scale = 0;
+      const real* Xe = X.data() + e * nv * dim;
+      real scale = 0;
       for (int i = 0; i < dim * dim; i++) scale = std::max(scale, std::fabs(J[i]));
       for (int v = 0; v < nv; v++) {
-        const double* R = fe.vert(v);
+        const real* R = fe.vert(v);
         for (int i = 0; i < dim; i++) {
-          double xv = Xe[i];
+          real xv = Xe[i];
           for (int r = 0; r < dim; r++) xv += J[i * dim + r] * R[r];
           if (std::fabs(xv - Xe[v * dim + i]) > 1e-10 * scale) affine[e] = 0;
         }
@@ -856,8 +868,8 @@ struct Ref {
   }
 
   // reference coordinates of face point s (face param) in element e's local face
-  void face_to_elem(const Face& f, int side, const double* s, double* xi) const {
-    double N[4];
+  void face_to_elem(const Face& f, int side, const real* s, real* xi) const {
+    real N[4];
     if (dim == 2) { N[0] = 1 - s[0]; N[1] = s[0]; }
     else { N[0] = (1 - s[0]) * (1 - s[1]); N[1] = s[0] * (1 - s[1]); N[2] = s[0] * s[1]; N[3] = (1 - s[0]) * s[1]; }
     int nvf = fe.nvf;
@@ -875,7 +887,7 @@ struct Ref {
               if (fe.face_vert(f.lf2, mm) == v) lv = v;
           }
       }
-      const double* R = fe.vert(lv);
+      const real* R = fe.vert(lv);
       for (int d = 0; d < dim; d++) xi[d] += N[m] * R[d];
     }
   }
@@ -886,26 +898,26 @@ struct Ref {
   // Exactness matters: the reference 3D rotation (rs_basic.cpp:21-28) loses
   // orthogonality when n_z = 1 - ulp, so last-bit noise in an axis-aligned
   // unit normal would perturb rotated fluxes at the 1e-8 level.
-  void face_normal(const Face& f, const double* s, double* nor) const {
-    double xi[3];
+  void face_normal(const Face& f, const real* s, real* nor) const {
+    real xi[3];
     face_to_elem(f, 0, s, xi);
-    double J[9];
+    real J[9];
     if (affine[f.e1]) affine_jacobian(f.e1, J);
     else trans(f.e1).jacobian(xi, J);
-    const double* R[4];
+    const real* R[4];
     for (int m = 0; m < fe.nvf; m++) R[m] = fe.vert(fe.face_vert(f.lf1, m));
     if (dim == 2) {
-      double D[2] = {R[1][0] - R[0][0], R[1][1] - R[0][1]};
-      double Jf[2] = {J[0] * D[0] + J[1] * D[1], J[2] * D[0] + J[3] * D[1]};
+      real D[2] = {R[1][0] - R[0][0], R[1][1] - R[0][1]};
+      real Jf[2] = {J[0] * D[0] + J[1] * D[1], J[2] * D[0] + J[3] * D[1]};
       nor[0] = Jf[1];
       nor[1] = -Jf[0];
     } else {
-      double Ds[3], Dt[3];
+      real Ds[3], Dt[3];
       for (int d = 0; d < 3; d++) {
         Ds[d] = R[1][d] - R[0][d];
         Dt[d] = R[3][d] - R[0][d];
       }
-      double a[3], b[3];
+      real a[3], b[3];
       for (int i = 0; i < 3; i++) {
         a[i] = J[i * 3 + 0] * Ds[0] + J[i * 3 + 1] * Ds[1] + J[i * 3 + 2] * Ds[2];
         b[i] = J[i * 3 + 0] * Dt[0] + J[i * 3 + 1] * Dt[1] + J[i * 3 + 2] * Dt[2];
@@ -925,11 +937,11 @@ struct Ref {
   bool build(std::string& err);
   void assemble_aflux();
   void assemble_mass();
-  int face_mult(const double* x, double* z, double& mcs, int64_t* bad);
-  int mult(const double* x, double* y, double& mcs, int64_t* bad);
-  void compute_means(const double* x);
-  void limiter_update(double* x, double* ind_values, double* min_values);
-  double hmin() const;
+  int face_mult(const real* x, real* z, real& mcs, int64_t* bad);
+  int mult(const real* x, real* y, real& mcs, int64_t* bad);
+  void compute_means(const real* x);
+  void limiter_update(real* x, real* ind_values, real* min_values);
+  real hmin() const;
 };
 
 bool Ref::build(std::string& err) {
@@ -983,19 +995,19 @@ void Ref::assemble_aflux() {
   for (int64_t r = 0; r <= rows; r++) Aflux.I[r] = r * rowlen;
 #pragma omp parallel for schedule(static)
   for (int64_t e = 0; e < NE; e++) {
-    std::vector<double> elmat(nd * rowlen, 0.0), shape(nd), dshr(nd * dim), dshx(nd * dim);
+    std::vector<real> elmat(nd * rowlen, 0.0), shape(nd), dshr(nd * dim), dshx(nd * dim);
     Trans T = trans(e);
     for (int q = 0; q < ir.n; q++) {
-      const double* ip = &ir.pts[q * dim];
+      const real* ip = &ir.pts[q * dim];
       fe.shape(ip, shape.data());
       for (int k = 0; k < nd; k++) shape[k] *= ir.w[q];
-      double J[9], A[9];
+      real J[9], A[9];
       T.jacobian(ip, J);
       adjugate(dim, J, A);
       fe.dshape(ip, dshr.data());
       for (int j = 0; j < nd; j++)
         for (int d = 0; d < dim; d++) {
-          double s = 0;
+          real s = 0;
           for (int r = 0; r < dim; r++) s += dshr[j * dim + r] * A[r * dim + d];
           dshx[j * dim + d] = s;
         }
@@ -1021,13 +1033,13 @@ void Ref::assemble_mass() {
   Me_inv.assign(NE * nd * nd, 0.0);
 #pragma omp parallel for schedule(static)
   for (int64_t e = 0; e < NE; e++) {
-    std::vector<double> M(nd * nd, 0.0), shape(nd);
+    std::vector<real> M(nd * nd, 0.0), shape(nd);
     Trans T = trans(e);
     for (int q = 0; q < ir.n; q++) {
-      const double* ip = &ir.pts[q * dim];
-      double J[9];
+      const real* ip = &ir.pts[q * dim];
+      real J[9];
       T.jacobian(ip, J);
-      double w = det(dim, J) * ir.w[q];
+      real w = det(dim, J) * ir.w[q];
       fe.shape(ip, shape.data());
       for (int i = 0; i < nd; i++)
         for (int j = 0; j < nd; j++) M[i * nd + j] += w * shape[i] * shape[j];
@@ -1041,7 +1053,7 @@ void Ref::assemble_mass() {
 // (boundary_integrator.cpp:12-194). Per-face element vectors are computed in
 // parallel; each element then adds them in face-id order (interior faces, then
 // boundary elements), which is the reference's serial accumulation order.
-int Ref::face_mult(const double* x, double* z, double& mcs, int64_t* bad) {
+int Ref::face_mult(const real* x, real* z, real& mcs, int64_t* bad) {
   const int NEQ = ne();
   const int stride = 2 * nd * NEQ;
   Rule ir = face_rule(fe.geom, face_intorder());
@@ -1049,18 +1061,18 @@ int Ref::face_mult(const double* x, double* z, double& mcs, int64_t* bad) {
   face_buf.assign((size_t)nslots * stride, 0.0);
   int err = 0;
   int64_t bad1 = -1, bad2 = -1;
-  double gmax = 0.0;
+  real gmax = 0.0;
 #pragma omp parallel
   {
-    std::vector<double> sh1(nd), sh2(nd);
-    double lmax = 0.0;
+    std::vector<real> sh1(nd), sh2(nd);
+    real lmax = 0.0;
 #pragma omp for schedule(static)
     for (int64_t slot = 0; slot < nslots; slot++) {
       bool isb = slot >= (int64_t)interior.size();
       int64_t fid = isb ? bdr_faces[slot - interior.size()] : interior[slot];
       const Face& f = faces[fid];
-      double* ev1 = &face_buf[(size_t)slot * stride];
-      double* ev2 = ev1 + nd * NEQ;
+      real* ev1 = &face_buf[(size_t)slot * stride];
+      real* ev2 = ev1 + nd * NEQ;
       int bi = -1;
       if (isb) {
         for (int b = 0; b < sc.s.num_bc; b++)
@@ -1068,12 +1080,12 @@ int Ref::face_mult(const double* x, double* z, double& mcs, int64_t* bad) {
         if (bi < 0) continue;  // unmarked boundary attribute: no integrator
       }
       for (int q = 0; q < ir.n; q++) {
-        const double* s = &ir.pts[q * ir.dim];
-        double xi1[3], xi2[3], nor[3], u1[5], u2[5], fl[5];
+        const real* s = &ir.pts[q * ir.dim];
+        real xi1[3], xi2[3], nor[3], u1[5], u2[5], fl[5];
         face_to_elem(f, 0, s, xi1);
         fe.shape(xi1, sh1.data());
         for (int k = 0; k < NEQ; k++) {
-          double v = 0;
+          real v = 0;
           for (int i = 0; i < nd; i++) v += x[k * NDofs + f.e1 * nd + i] * sh1[i];
           u1[k] = v;
         }
@@ -1081,35 +1093,35 @@ int Ref::face_mult(const double* x, double* z, double& mcs, int64_t* bad) {
           face_to_elem(f, 1, s, xi2);
           fe.shape(xi2, sh2.data());
           for (int k = 0; k < NEQ; k++) {
-            double v = 0;
+            real v = 0;
             for (int i = 0; i < nd; i++) v += x[k * NDofs + f.e2 * nd + i] * sh2[i];
             u2[k] = v;
           }
         }
         face_normal(f, s, nor);
-        double normag = 0;
+        real normag = 0;
         for (int i = 0; i < dim; i++) normag += nor[i] * nor[i];
         normag = std::sqrt(normag);
         // Vector::Norml2 (scaled hypot-style) then nor *= 1/norm
-        double scale = 0.0, sum = 0.0;
+        real scale = 0.0, sum = 0.0;
         for (int i = 0; i < dim; i++) {
           if (nor[i] != 0.0) {
-            double a = std::fabs(nor[i]);
+            real a = std::fabs(nor[i]);
             if (scale <= a) {
-              double r = scale / a;
+              real r = scale / a;
               sum = 1.0 + sum * (r * r);
               scale = a;
             } else {
-              double r = a / scale;
+              real r = a / scale;
               sum += r * r;
             }
           }
         }
-        double n2 = scale * std::sqrt(sum);
-        double inv = 1.0 / n2;
+        real n2 = scale * std::sqrt(sum);
+        real inv = 1.0 / n2;
         for (int i = 0; i < dim; i++) nor[i] *= inv;
         if (isb) bc_state(sc, dim, bi, u1, nor, u2);
-        double m = (isb && sc.s.bc_type[bi] == REF_BC_CHAR_OUTLET) ? char_outlet_flux(sc, dim, u2, nor, fl)
+        real m = (isb && sc.s.bc_type[bi] == REF_BC_CHAR_OUTLET) ? char_outlet_flux(sc, dim, u2, nor, fl)
                                                                    : riemann(sc, dim, u1, u2, nor, fl);
         if (m < 0) {
 #pragma omp critical
@@ -1137,7 +1149,7 @@ int Ref::face_mult(const double* x, double* z, double& mcs, int64_t* bad) {
     for (int k = 0; k < NEQ; k++)
       for (int i = 0; i < nd; i++) z[k * NDofs + e * nd + i] = 0.0;
     for (auto& c : contrib[e]) {
-      const double* ev = &face_buf[(size_t)c.first * stride] + c.second * nd * NEQ;
+      const real* ev = &face_buf[(size_t)c.first * stride] + c.second * nd * NEQ;
       for (int k = 0; k < NEQ; k++)
         for (int i = 0; i < nd; i++) z[k * NDofs + e * nd + i] += ev[k * nd + i];
     }
@@ -1150,33 +1162,33 @@ int Ref::face_mult(const double* x, double* z, double& mcs, int64_t* bad) {
 }
 
 // fe_evolution.cpp:102-216 (Mult) and :219-245 (GetFlux)
-int Ref::mult(const double* x, double* y, double& mcs, int64_t* bad) {
+int Ref::mult(const real* x, real* y, real& mcs, int64_t* bad) {
   const int NEQ = ne();
   mcs = 0.0;  // :105
-  std::vector<double> z(NEQ * NDofs);
+  std::vector<real> z(NEQ * NDofs);
   int rc = face_mult(x, z.data(), mcs, bad);
   // GetFlux: flux(i, d, k) stored at i + NDofs*(d + dim*k)
-  std::vector<double> flux((size_t)NDofs * dim * NEQ);
-  double gmax = 0.0;
+  std::vector<real> flux((size_t)NDofs * dim * NEQ);
+  real gmax = 0.0;
 #pragma omp parallel for schedule(static) reduction(max : gmax)
   for (int64_t i = 0; i < NDofs; i++) {
-    double st[5], f[15];
+    real st[5], f[15];
     for (int k = 0; k < NEQ; k++) st[k] = x[k * NDofs + i];
     Gas g = sc.gas();
     flux_tensor(st, dim, g, f);
     for (int d = 0; d < dim; d++)
       for (int k = 0; k < NEQ; k++) flux[i + NDofs * (d + dim * k)] = f[k * dim + d];
-    double m = max_char_speed(st, dim, g);
+    real m = max_char_speed(st, dim, g);
     if (m > gmax) gmax = m;
   }
   if (gmax > mcs) mcs = gmax;
   // Aflux->AddMult(fk, zk) per equation (SparseMatrix::AddMult: row sum then add)
   for (int k = 0; k < NEQ; k++) {
-    const double* fk = &flux[(size_t)k * dim * NDofs];
-    double* zk = &z[(size_t)k * NDofs];
+    const real* fk = &flux[(size_t)k * dim * NDofs];
+    real* zk = &z[(size_t)k * NDofs];
 #pragma omp parallel for schedule(static)
     for (int64_t r = 0; r < NDofs; r++) {
-      double d = 0.0;
+      real d = 0.0;
       for (int64_t j = Aflux.I[r]; j < Aflux.I[r + 1]; j++) d += Aflux.A[j] * fk[Aflux.J[j]];
       zk[r] += d;
     }
@@ -1184,10 +1196,10 @@ int Ref::mult(const double* x, double* y, double& mcs, int64_t* bad) {
   // y_e = Me_inv[e] * z_e  (mfem::Mult dense)
 #pragma omp parallel for schedule(static)
   for (int64_t e = 0; e < NE; e++) {
-    const double* Mi = &Me_inv[e * nd * nd];
+    const real* Mi = &Me_inv[e * nd * nd];
     for (int k = 0; k < NEQ; k++)
       for (int i = 0; i < nd; i++) {
-        double s = 0.0;
+        real s = 0.0;
         for (int j = 0; j < nd; j++) s += Mi[i * nd + j] * z[k * NDofs + e * nd + j];
         y[k * NDofs + e * nd + i] = s;
       }
@@ -1197,26 +1209,26 @@ int Ref::mult(const double* x, double* y, double& mcs, int64_t* bad) {
 
 // averager.cpp:84-109 -> GridFunction::GetElementAverages: (sum_k m_k u_k)/(sum_k m_k),
 // m_k = integral of phi_k with MassIntegrator order p + OrderW.
-void Ref::compute_means(const double* x) {
+void Ref::compute_means(const real* x) {
   const int NEQ = ne();
   Rule ir = geom_rule(fe.geom, fe.p + fe.order_w());
   avgs.assign(NEQ * NE, 0.0);
 #pragma omp parallel for schedule(static)
   for (int64_t e = 0; e < NE; e++) {
-    std::vector<double> m(nd, 0.0), shape(nd);
+    std::vector<real> m(nd, 0.0), shape(nd);
     Trans T = trans(e);
     for (int q = 0; q < ir.n; q++) {
-      const double* ip = &ir.pts[q * dim];
-      double J[9];
+      const real* ip = &ir.pts[q * dim];
+      real J[9];
       T.jacobian(ip, J);
-      double w = det(dim, J) * ir.w[q];
+      real w = det(dim, J) * ir.w[q];
       fe.shape(ip, shape.data());
       for (int k = 0; k < nd; k++) m[k] += w * 1.0 * shape[k];
     }
-    double ipsi = 0;
+    real ipsi = 0;
     for (int k = 0; k < nd; k++) ipsi += m[k] * 1.0;
     for (int v = 0; v < NEQ; v++) {
-      double s = 0;
+      real s = 0;
       for (int k = 0; k < nd; k++) s += m[k] * x[v * NDofs + e * nd + k];
       avgs[v * NE + e] = s / ipsi;
     }
@@ -1226,22 +1238,22 @@ void Ref::compute_means(const double* x) {
 // Limiter::update (limiter.cpp:22-94), IndicatorBJ::checkDiscontinuity
 // (indicator_bj.cpp:88-200) with updateYMin (:15-86), LimiterMultiplier::limit
 // (limiter_multiplier.cpp:12-99), LimiterFinDiff::limit (limiter_findiff.cpp:3-33).
-void Ref::limiter_update(double* x, double* ind_values, double* min_values) {
+void Ref::limiter_update(real* x, real* ind_values, real* min_values) {
   const int NEQ = ne();
   const Gas g = sc.gas();
-  std::vector<double> xNew(x, x + NEQ * NDofs);
+  std::vector<real> xNew(x, x + NEQ * NDofs);
   compute_means(x);
   Rule fr = face_rule(fe.geom, 2);  // indicator_bj.cpp:150 (intorder 2)
 #pragma omp parallel for schedule(static)
   for (int64_t c = 0; c < NE; c++) {
-    double yMin[5], mI[5], MI[5], mean[5];
-    std::vector<double> el(nd * NEQ), shape(nd);
+    real yMin[5], mI[5], MI[5], mean[5];
+    std::vector<real> el(nd * NEQ), shape(nd);
     for (int k = 0; k < NEQ; k++)
       for (int i = 0; i < nd; i++) el[k * nd + i] = x[k * NDofs + c * nd + i];
     for (int v = 0; v < NEQ; v++) mean[v] = avgs[v * NE + c];
     // --- indicator
     if (sc.s.indicator == REF_IND_NOWHERE || sc.s.indicator == REF_IND_EVERYWHERE) {
-      double val = sc.s.indicator == REF_IND_NOWHERE ? 1.0 : 0.0;
+      real val = sc.s.indicator == REF_IND_NOWHERE ? 1.0 : 0.0;
       for (int v = 0; v < NEQ; v++) yMin[v] = val;
     } else {
       for (int v = 0; v < NEQ; v++) { mI[v] = 1e9; MI[v] = -1e9; yMin[v] = 1e9; }
@@ -1250,18 +1262,18 @@ void Ref::limiter_update(double* x, double* ind_values, double* min_values) {
       for (int64_t nb : el2el[c]) cells.push_back(nb);
       for (int64_t k : cells)
         for (int v = 0; v < NEQ; v++) {
-          double a = avgs[v * NE + k];
+          real a = avgs[v * NE + k];
           mI[v] = a < mI[v] ? a : mI[v];
           MI[v] = a > MI[v] ? a : MI[v];
         }
-      double hK = 0.0;
+      real hK = 0.0;
       if (sc.s.phi == REF_PHI_VENKAT && sc.s.venkat_k > 0) {
         // element size (volume^(1/dim)) for eps^2 = (K h)^3
         Rule vr2 = geom_rule(fe.geom, 2);
-        double vol = 0;
+        real vol = 0;
         Trans T = trans(c);
         for (int q = 0; q < vr2.n; q++) {
-          double J[9];
+          real J[9];
           T.jacobian(&vr2.pts[q * dim], J);
           vol += det(dim, J) * vr2.w[q];
         }
@@ -1273,17 +1285,17 @@ void Ref::limiter_update(double* x, double* ind_values, double* min_values) {
         if (f.e2 < 0) continue;  // boundary faces are not in the stencil
         int side = f.e1 == c ? 0 : 1;
         for (int q = 0; q < fr.n; q++) {
-          double xi[3], u[5];
+          real xi[3], u[5];
           face_to_elem(f, side, &fr.pts[q * fr.dim], xi);
           fe.shape(xi, shape.data());
           for (int k = 0; k < NEQ; k++) {
-            double s = 0;
+            real s = 0;
             for (int i = 0; i < nd; i++) s += el[k * nd + i] * shape[i];
             u[k] = s;
           }
           for (int i = 0; i < NEQ; i++) {
-            double diff = u[i] - mean[i];
-            double y;
+            real diff = u[i] - mean[i];
+            real y;
             if (diff > 1e-3) y = (MI[i] - mean[i]) / diff;
             else if (diff < -1e-3) y = (mI[i] - mean[i]) / diff;
             else y = 1.0;
@@ -1291,19 +1303,19 @@ void Ref::limiter_update(double* x, double* ind_values, double* min_values) {
               yMin[i] = y < yMin[i] ? y : yMin[i];
               yMin[i] = yMin[i] > 1.0 ? 1.0 : yMin[i];
             } else if (sc.s.phi == REF_PHI_VENKAT) {  // :70-72 (commented in reference)
-              double yCur;
+              real yCur;
               if (sc.s.venkat_k > 0) {
-                double eps2 = std::pow(sc.s.venkat_k * hK, 3.0);
-                double d2 = diff * diff;
+                real eps2 = std::pow(sc.s.venkat_k * hK, 3.0);
+                real d2 = diff * diff;
                 yCur = (d2 * (y * y + 2.0 * y) + eps2) / (d2 * (y * y + y + 2.0) + eps2);
               } else {
                 yCur = (y * y + 2.0 * y) / (y * y + y + 2.0);
               }
               yMin[i] = yCur < yMin[i] ? yCur : yMin[i];
             } else {  // michalak, :75-79 (active in the reference)
-              double yStar = 1.5;
-              double yRel = y / yStar;
-              double yCur = y < 1.0 ? y + (3.0 - 2.0 * yStar) * yRel * yRel + (yStar - 2.0) * yRel * yRel * yRel : 1.0;
+              real yStar = 1.5;
+              real yRel = y / yStar;
+              real yCur = y < 1.0 ? y + (3.0 - 2.0 * yStar) * yRel * yRel + (yStar - 2.0) * yRel * yRel * yRel : 1.0;
               yMin[i] = yCur < yMin[i] ? yCur : yMin[i];
             }
           }
@@ -1311,7 +1323,7 @@ void Ref::limiter_update(double* x, double* ind_values, double* min_values) {
       }
     }
     for (int v = 0; v < NEQ; v++) if (ind_values) ind_values[v * NE + c] = yMin[v];
-    double minV = yMin[0];
+    real minV = yMin[0];
     if (min_values) min_values[c] = minV;
     // --- limit
     if (sc.s.limiter == REF_LIM_MULTI) {
@@ -1322,9 +1334,9 @@ void Ref::limiter_update(double* x, double* ind_values, double* min_values) {
       if (sc.s.last_hope) {
         for (int v = 0; v < fe.nv; v++) {
           fe.shape(fe.vert(v), shape.data());
-          double u[5];
+          real u[5] = {};
           for (int k = 0; k < NEQ; k++) {
-            double s = 0;
+            real s = 0;
             for (int i = 0; i < nd; i++) s += el[k * nd + i] * shape[i];
             u[k] = s;
           }
@@ -1344,22 +1356,22 @@ void Ref::limiter_update(double* x, double* ind_values, double* min_values) {
     for (int k = 0; k < NEQ; k++)
       for (int i = 0; i < nd; i++) xNew[k * NDofs + c * nd + i] = el[k * nd + i];
   }
-  std::memcpy(x, xNew.data(), sizeof(double) * NEQ * NDofs);
+  std::memcpy(x, xNew.data(), sizeof(real) * NEQ * NDofs);
 }
 
 // Mesh::GetElementSize(i, 1): smallest singular value of the (perfect-geometry)
 // Jacobian at the element centre; main.cpp:191-201 takes the minimum.
-double Ref::hmin() const {
-  double hm = 1e300;
+real Ref::hmin() const {
+  real hm = 1e300;
   for (int64_t e = 0; e < NE; e++) {
-    double xc[3] = {0.5, 0.5, 0.5};
+    real xc[3] = {0.5, 0.5, 0.5};
     if (fe.geom == REF_GEOM_TRI) { xc[0] = 1.0 / 3.0; xc[1] = 1.0 / 3.0; }
-    double J[9];
+    real J[9];
     trans(e).jacobian(xc, J);
     if (fe.geom == REF_GEOM_TRI) {
       // multiply by the Jacobian mapping the equilateral triangle onto the reference one
-      double P[4] = {1.0, -1.0 / std::sqrt(3.0), 0.0, 2.0 / std::sqrt(3.0)};
-      double Q[4];
+      real P[4] = {1.0, -1.0 / std::sqrt(3.0), 0.0, 2.0 / std::sqrt(3.0)};
+      real Q[4];
       Q[0] = J[0] * P[0] + J[1] * P[2];
       Q[1] = J[0] * P[1] + J[1] * P[3];
       Q[2] = J[2] * P[0] + J[3] * P[2];
@@ -1368,37 +1380,37 @@ double Ref::hmin() const {
     }
     int d = dim;
     // smallest singular value: sqrt of smallest eigenvalue of J^T J
-    double A[9] = {0};
+    real A[9] = {0};
     for (int i = 0; i < d; i++)
       for (int j = 0; j < d; j++) {
-        double s = 0;
+        real s = 0;
         for (int k = 0; k < d; k++) s += J[k * d + i] * J[k * d + j];
         A[i * d + j] = s;
       }
-    double lmin;
+    real lmin;
     if (d == 2) {
-      double tr = A[0] + A[3], dt = A[0] * A[3] - A[1] * A[2];
-      double disc = std::sqrt(std::max(0.0, tr * tr / 4 - dt));
+      real tr = A[0] + A[3], dt = A[0] * A[3] - A[1] * A[2];
+      real disc = std::sqrt(std::max(real(0), tr * tr / 4 - dt));
       lmin = tr / 2 - disc;
     } else {
       // symmetric 3x3 eigenvalues (trigonometric method)
-      double p1 = A[1] * A[1] + A[2] * A[2] + A[5] * A[5];
-      double q = (A[0] + A[4] + A[8]) / 3;
+      real p1 = A[1] * A[1] + A[2] * A[2] + A[5] * A[5];
+      real q = (A[0] + A[4] + A[8]) / 3;
       if (p1 == 0) lmin = std::min(A[0], std::min(A[4], A[8]));
       else {
-        double p2 = (A[0] - q) * (A[0] - q) + (A[4] - q) * (A[4] - q) + (A[8] - q) * (A[8] - q) + 2 * p1;
-        double p = std::sqrt(p2 / 6);
-        double B[9];
+        real p2 = (A[0] - q) * (A[0] - q) + (A[4] - q) * (A[4] - q) + (A[8] - q) * (A[8] - q) + 2 * p1;
+        real p = std::sqrt(p2 / 6);
+        real B[9];
         for (int i = 0; i < 9; i++) B[i] = (A[i] - (i % 4 == 0 ? q : 0)) / p;
-        double r = det(3, B) / 2;
-        double phi = r <= -1 ? M_PI / 3 : (r >= 1 ? 0 : std::acos(r) / 3);
-        double e1 = q + 2 * p * std::cos(phi);
-        double e3 = q + 2 * p * std::cos(phi + 2 * M_PI / 3);
+        real r = det(3, B) / 2;
+        real phi = r <= -1 ? M_PI / 3 : (r >= 1 ? 0 : std::acos(r) / 3);
+        real e1 = q + 2 * p * std::cos(phi);
+        real e3 = q + 2 * p * std::cos(phi + 2 * M_PI / 3);
         lmin = e3;
         (void)e1;
       }
     }
-    double h = std::sqrt(std::max(0.0, lmin));
+    real h = std::sqrt(std::max(real(0), lmin));
     if (h < hm) hm = h;
   }
   return hm;
@@ -1407,6 +1419,40 @@ double Ref::hmin() const {
 }  // namespace
 
 // ============================================================================
+// The C interface is double; when the restatement is built wider (REF_REAL),
+// inputs are widened on the way in and results rounded on the way out. In the
+// double build these are pass-throughs without copies.
+namespace {
+const real* widen(const double* x, size_t n, std::vector<real>& tmp) {
+  if constexpr (std::is_same<real, double>::value) {
+    (void)n;
+    (void)tmp;
+    return reinterpret_cast<const real*>(x);
+  } else {
+    tmp.assign(x, x + n);
+    return tmp.data();
+  }
+}
+real* widen_out(double* y, size_t n, std::vector<real>& tmp) {
+  if constexpr (std::is_same<real, double>::value) {
+    (void)n;
+    (void)tmp;
+    return reinterpret_cast<real*>(y);
+  } else {
+    tmp.assign(n, real(0));
+    return tmp.data();
+  }
+}
+void narrow(const std::vector<real>& tmp, double* y) {
+  if constexpr (!std::is_same<real, double>::value) {
+    for (size_t i = 0; i < tmp.size(); i++) y[i] = (double)tmp[i];
+  } else {
+    (void)tmp;
+    (void)y;
+  }
+}
+}  // namespace
+
 extern "C" {
 
 void* ref_create(int geom, int order, int64_t num_elements, const double* elem_coords,
@@ -1420,7 +1466,7 @@ void* ref_create(int geom, int order, int64_t num_elements, const double* elem_c
   r->nd = r->fe.nd;
   r->NE = num_elements;
   r->NDofs = num_elements * r->nd;
-  r->X.assign(elem_coords, elem_coords + num_elements * r->nv * r->dim);
+  r->X.assign(elem_coords, elem_coords + num_elements * r->nv * r->dim);  // widened element-wise
   r->topo.assign(elem_verts, elem_verts + num_elements * r->nv);
   std::string e;
   if (!r->build(e)) {
@@ -1477,37 +1523,54 @@ void ref_node_coords(void* h, double* xyz) {
   for (int64_t e = 0; e < r->NE; e++) {
     Trans T = r->trans(e);
     for (int i = 0; i < r->nd; i++) {
-      double x[3];
+      real x[3];
       T.transform(&r->fe.nodes[i * r->dim], x);
-      for (int d = 0; d < r->dim; d++) xyz[(e * r->nd + i) * r->dim + d] = x[d];
+      for (int d = 0; d < r->dim; d++) xyz[(e * r->nd + i) * r->dim + d] = (double)x[d];
     }
   }
 }
 
 int ref_mult(void* h, const double* x, double* y, double* mcs, int64_t* bad) {
   Ref* r = (Ref*)h;
-  double m = 0;
-  int rc = r->mult(x, y, m, bad);
-  if (mcs) *mcs = m;
+  const size_t n = (size_t)r->ne() * r->NDofs;
+  std::vector<real> xt, yt;
+  real m = 0;
+  int rc = r->mult(widen(x, n, xt), widen_out(y, n, yt), m, bad);
+  narrow(yt, y);
+  if (mcs) *mcs = (double)m;
   return rc;
 }
 
 int ref_face_mult(void* h, const double* x, double* z, double* mcs, int64_t* bad) {
   Ref* r = (Ref*)h;
-  double m = 0;
-  int rc = r->face_mult(x, z, m, bad);
-  if (mcs) *mcs = m;
+  const size_t n = (size_t)r->ne() * r->NDofs;
+  std::vector<real> xt, zt;
+  real m = 0;
+  int rc = r->face_mult(widen(x, n, xt), widen_out(z, n, zt), m, bad);
+  narrow(zt, z);
+  if (mcs) *mcs = (double)m;
   return rc;
 }
 
 void ref_means(void* h, const double* x, double* means) {
   Ref* r = (Ref*)h;
-  r->compute_means(x);
-  std::memcpy(means, r->avgs.data(), sizeof(double) * r->avgs.size());
+  std::vector<real> xt;
+  r->compute_means(widen(x, (size_t)r->ne() * r->NDofs, xt));
+  for (size_t i = 0; i < r->avgs.size(); i++) means[i] = (double)r->avgs[i];
 }
 
 void ref_limiter_update(void* h, double* x, double* ind, double* mv) {
-  ((Ref*)h)->limiter_update(x, ind, mv);
+  Ref* r = (Ref*)h;
+  const size_t n = (size_t)r->ne() * r->NDofs;
+  std::vector<real> xt, it, mt;
+  real* xr = widen_out(x, n, xt);
+  if constexpr (!std::is_same<real, double>::value) std::copy(x, x + n, xr);
+  real* ir = ind ? widen_out(ind, (size_t)r->ne() * r->NE, it) : nullptr;
+  real* mr = mv ? widen_out(mv, (size_t)r->NE, mt) : nullptr;
+  r->limiter_update(xr, ir, mr);
+  narrow(xt, x);
+  if (ind) narrow(it, ind);
+  if (mv) narrow(mt, mv);
 }
 
 // ExplicitRKLimitedSolver::Step (rk_explicit_limited.cpp:30-82): packed a[l++]
@@ -1518,16 +1581,20 @@ int ref_step(void* h, double* x, double t, double dt, int s, const double* a, co
   (void)c;
   Ref* r = (Ref*)h;
   int64_t n = r->ne() * r->NDofs;
-  std::vector<std::vector<double>> k(s, std::vector<double>(n));
-  std::vector<double> y(n, 0.0);
-  double m = 0;
-  int rc = r->mult(x, k[0].data(), m, bad);
+  std::vector<std::vector<real>> k(s, std::vector<real>(n));
+  std::vector<real> y(n, 0.0);
+  std::vector<real> xt;
+  real* xr = widen_out(x, (size_t)n, xt);
+  if constexpr (!std::is_same<real, double>::value) std::copy(x, x + n, xr);
+  real m = 0;
+  int rc = r->mult(xr, k[0].data(), m, bad);
+  const real dtr = dt;
   for (int l = 0, i = 1; i < s; i++) {
-    double a0 = a[l++] * dt;
+    real a0 = a[l++] * dtr;
 #pragma omp parallel for schedule(static)
-    for (int64_t j = 0; j < n; j++) y[j] = x[j] + a0 * k[0][j];
+    for (int64_t j = 0; j < n; j++) y[j] = xr[j] + a0 * k[0][j];
     for (int jj = 1; jj < i; jj++) {
-      double aj = a[l++] * dt;
+      real aj = a[l++] * dtr;
 #pragma omp parallel for schedule(static)
       for (int64_t j = 0; j < n; j++) y[j] += aj * k[jj][j];
     }
@@ -1536,20 +1603,21 @@ int ref_step(void* h, double* x, double t, double dt, int s, const double* a, co
     if (rc2) rc = rc2;
   }
   for (int i = 0; i < s; i++) {
-    double bi = b[i] * dt;
+    real bi = b[i] * dtr;
 #pragma omp parallel for schedule(static)
-    for (int64_t j = 0; j < n; j++) x[j] += bi * k[i][j];
+    for (int64_t j = 0; j < n; j++) xr[j] += bi * k[i][j];
   }
-  r->limiter_update(x, nullptr, nullptr);
-  if (mcs) *mcs = m;
+  r->limiter_update(xr, nullptr, nullptr);
+  narrow(xt, x);
+  if (mcs) *mcs = (double)m;
   return rc;
 }
 
-double ref_hmin(void* h) { return ((Ref*)h)->hmin(); }
+double ref_hmin(void* h) { return (double)((Ref*)h)->hmin(); }
 
 void ref_element_mass_inverse(void* h, int64_t e, double* out) {
   Ref* r = (Ref*)h;
-  std::memcpy(out, &r->Me_inv[e * r->nd * r->nd], sizeof(double) * r->nd * r->nd);
+  for (int i = 0; i < r->nd * r->nd; i++) out[i] = (double)r->Me_inv[e * r->nd * r->nd + i];
 }
 
 void ref_set_threads(int n) {
@@ -1564,25 +1632,49 @@ double ref_riemann_eval(const ref_scheme* s, int dim, const double* s1, const do
                         const double* nor, double* flux) {
   Scheme sc;
   sc.s = *s;
-  double a[5], b[5];
-  std::memcpy(a, s1, sizeof(double) * (dim + 2));
-  std::memcpy(b, s2, sizeof(double) * (dim + 2));
-  return riemann(sc, dim, a, b, nor, flux);
+  real a[5] = {}, b[5] = {}, n[3] = {}, f[5] = {};
+  for (int i = 0; i < dim + 2; i++) {
+    a[i] = s1[i];
+    b[i] = s2[i];
+  }
+  for (int i = 0; i < dim; i++) n[i] = nor[i];
+  const real lam = riemann(sc, dim, a, b, n, f);
+  for (int i = 0; i < dim + 2; i++) flux[i] = (double)f[i];
+  return (double)lam;
 }
 
 void ref_bc_state(const ref_scheme* s, int dim, int bi, const double* s1, const double* nor, double* s2) {
   Scheme sc;
   sc.s = *s;
-  bc_state(sc, dim, bi, s1, nor, s2);
+  real a[5] = {}, n[3] = {}, g[5] = {};
+  for (int i = 0; i < dim + 2; i++) a[i] = s1[i];
+  for (int i = 0; i < dim; i++) n[i] = nor[i];
+  bc_state(sc, dim, bi, a, n, g);
+  for (int i = 0; i < dim + 2; i++) s2[i] = (double)g[i];
 }
 
-void ref_gl_rule(int n, double* x, double* w) { gauss_legendre(n, x, w); }
+void ref_gl_rule(int n, double* x, double* w) {
+  real xr[64], wr[64];
+  gauss_legendre(n, xr, wr);
+  for (int i = 0; i < n; i++) {
+    x[i] = (double)xr[i];
+    w[i] = (double)wr[i];
+  }
+}
 
 void ref_shape(int geom, int order, const double* xi, double* phi, double* dphi) {
   RefFE fe;
   fe.init(geom, order);
-  if (phi) fe.shape(xi, phi);
-  if (dphi) fe.dshape(xi, dphi);
+  real xr[3] = {xi[0], xi[1], fe.dim == 3 ? xi[2] : 0.0};
+  std::vector<real> ph(fe.nd), dph((size_t)fe.nd * fe.dim);
+  if (phi) {
+    fe.shape(xr, ph.data());
+    for (int i = 0; i < fe.nd; i++) phi[i] = (double)ph[i];
+  }
+  if (dphi) {
+    fe.dshape(xr, dph.data());
+    for (size_t i = 0; i < dph.size(); i++) dphi[i] = (double)dph[i];
+  }
 }
 
 }  // extern "C"

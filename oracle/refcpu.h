@@ -21,6 +21,12 @@
  * IntRules point counts, face Loc1/Loc2 maps, CalcOrtho normals, MassIntegrator,
  * GetElementAverages, GetElementSize).
  *
+ * The same source builds a second library with REF_REAL = long double
+ * (librefcpu_ld.so): the operator evaluated with a 64-bit significand, against
+ * which the rounding error of the double oracle and of the GPU path are
+ * measured on ill-conditioned workloads (a fluid at rest, whose RHS is a small
+ * difference of large flux terms).
+ *
  * The reference cannot be built here (MFEM, hypre, METIS and rapidyaml are
  * absent) and holds no test vectors, so the restatement is pinned by closed
  * forms, exact solutions, invariants and symmetries (tests/test_oracle_pins.py).

@@ -12,7 +12,12 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "librefcpu.so")
+_LIB_PATHS = {
+    "double": os.path.join(_HERE, "_build", "librefcpu.so"),
+    # the same restatement built with REF_REAL = long double (x87 80-bit, 64-bit
+    # significand): how far a double evaluation lies from the operator
+    "long double": os.path.join(_HERE, "_build", "librefcpu_ld.so"),
+}
 MAX_BC = 16
 
 
@@ -31,15 +36,15 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+def lib(precision="double"):
+    if precision not in _libs:
+        path = _LIB_PATHS[precision]
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(_LIB_PATH)
+        L = C.CDLL(path)
         P = C.c_void_p
         dp = C.POINTER(C.c_double)
         ip = C.POINTER(C.c_int64)
@@ -67,8 +72,8 @@ def lib():
         L.ref_bc_state.argtypes = [C.POINTER(RefScheme), C.c_int, C.c_int, dp, dp, dp]
         L.ref_gl_rule.argtypes = [C.c_int, dp, dp]
         L.ref_shape.argtypes = [C.c_int, C.c_int, dp, dp, dp]
-        _lib = L
-    return _lib
+        _libs[precision] = L
+    return _libs[precision]
 
 
 def _dp(a):
@@ -105,8 +110,11 @@ def ref_scheme(s) -> RefScheme:
 class RefSolver:
     """The oracle for one mesh + scheme: FE_Evolution::Mult, Limiter::update, Step."""
 
-    def __init__(self, mesh, order, scheme, threads=None):
-        L = lib()
+    def __init__(self, mesh, order, scheme, threads=None, precision="double"):
+        """precision: "double" (the reference's arithmetic) or "long double"
+        (the same operator evaluated wider, for measuring rounding errors)."""
+        self.L = L = lib(precision)
+        self.precision = precision
         if threads is not None:
             L.ref_set_threads(int(threads))
         self.mesh = mesh
@@ -128,12 +136,12 @@ class RefSolver:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().ref_destroy(self.h)
+            self.L.ref_destroy(self.h)
             self.h = None
 
     def node_coords(self):
         out = np.zeros((self.mesh.num_elements, self.nd, self.mesh.dim))
-        lib().ref_node_coords(self.h, _dp(out))
+        self.L.ref_node_coords(self.h, _dp(out))
         return out
 
     def mult(self, x):
@@ -141,7 +149,7 @@ class RefSolver:
         y = np.zeros_like(x)
         m = np.zeros(1)
         bad = np.full(2, -1, np.int64)
-        rc = lib().ref_mult(self.h, _dp(x), _dp(y), _dp(m), _ip(bad))
+        rc = self.L.ref_mult(self.h, _dp(x), _dp(y), _dp(m), _ip(bad))
         if rc:
             raise FloatingPointError(f"unphysical state at faces between elements {bad[0]} {bad[1]}")
         return y, float(m[0])
@@ -151,7 +159,7 @@ class RefSolver:
         z = np.zeros_like(x)
         m = np.zeros(1)
         bad = np.full(2, -1, np.int64)
-        rc = lib().ref_face_mult(self.h, _dp(x), _dp(z), _dp(m), _ip(bad))
+        rc = self.L.ref_face_mult(self.h, _dp(x), _dp(z), _dp(m), _ip(bad))
         if rc:
             raise FloatingPointError(f"unphysical state at faces between elements {bad[0]} {bad[1]}")
         return z, float(m[0])
@@ -159,14 +167,14 @@ class RefSolver:
     def means(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)
         out = np.zeros(self.neq * self.mesh.num_elements)
-        lib().ref_means(self.h, _dp(x), _dp(out))
+        self.L.ref_means(self.h, _dp(x), _dp(out))
         return out
 
     def limit(self, x):
         x = np.array(x, dtype=np.float64, copy=True)
         ind = np.zeros(self.neq * self.mesh.num_elements)
         mv = np.zeros(self.mesh.num_elements)
-        lib().ref_limiter_update(self.h, _dp(x), _dp(ind), _dp(mv))
+        self.L.ref_limiter_update(self.h, _dp(x), _dp(ind), _dp(mv))
         return x, ind, mv
 
     def step(self, x, dt, tableau, t=0.0):
@@ -177,13 +185,13 @@ class RefSolver:
         c = np.ascontiguousarray(c + [0.0], dtype=np.float64)
         m = np.zeros(1)
         bad = np.full(2, -1, np.int64)
-        rc = lib().ref_step(self.h, _dp(x), t, dt, s, _dp(a), _dp(b), _dp(c), _dp(m), _ip(bad))
+        rc = self.L.ref_step(self.h, _dp(x), t, dt, s, _dp(a), _dp(b), _dp(c), _dp(m), _ip(bad))
         if rc:
             raise FloatingPointError(f"unphysical state at faces between elements {bad[0]} {bad[1]}")
         return x, float(m[0])
 
     def hmin(self):
-        return lib().ref_hmin(self.h)
+        return self.L.ref_hmin(self.h)
 
     def patch_pressure(self, x, attr, gamma=1.4):
         """Integral of the interior-trace pressure over the boundary faces with
@@ -223,26 +231,57 @@ class RefSolver:
 
     def mass_inverse(self, e):
         out = np.zeros((self.nd, self.nd))
-        lib().ref_element_mass_inverse(self.h, e, _dp(out))
+        self.L.ref_element_mass_inverse(self.h, e, _dp(out))
         return out
 
 
-def riemann_eval(scheme, dim, s1, s2, nor):
+def run(ref, x, t_final, cfl, tableau, t0=0.0, max_steps=None, callback=None):
+    """The reference time loop (main.cpp:265, 467-553, 701, 733) on the oracle:
+    limit the projected state, first dt = cfl hmin / lambda_faces / (2p+1)
+    from the face wave speeds, then per step dt_real = min(dt, t_final - t),
+    Step(x, dt_real), dt = cfl hmin / lambda_last_stage / (2p+1), and t += dt
+    (the recomputed dt, quirk Q7), until t >= t_final - 1e-8 dt. Returns the
+    final state and a dict with steps, t, dt, the list of dt_real and their
+    sum (the time the state has actually been advanced to)."""
+    x, _, _ = ref.limit(x)
+    hmin = ref.hmin()
+    p = ref.order
+    _, lam = ref.face_mult(x)
+    dt = cfl * hmin / lam / (2 * p + 1)
+    t = t0
+    dts = []
+    steps = 0
+    while True:
+        dt_real = min(dt, t_final - t)
+        x, lam = ref.step(x, dt_real, tableau, t)
+        dts.append(dt_real)
+        dt = cfl * hmin / lam / (2 * p + 1)
+        t += dt
+        steps += 1
+        if callback is not None:
+            callback(steps, t, dt, x)
+        if t >= t_final - 1e-8 * dt or (max_steps is not None and steps >= max_steps):
+            break
+    return x, {"steps": steps, "t": t, "dt": dt, "dt_real": dts, "advanced": t0 + float(np.sum(dts)),
+               "max_char_speed": lam}
+
+
+def riemann_eval(scheme, dim, s1, s2, nor, precision="double"):
     rs = ref_scheme(scheme)
     s1 = np.ascontiguousarray(s1, np.float64)
     s2 = np.ascontiguousarray(s2, np.float64)
     nor = np.ascontiguousarray(nor, np.float64)
     f = np.zeros(dim + 2)
-    lam = lib().ref_riemann_eval(C.byref(rs), dim, _dp(s1), _dp(s2), _dp(nor), _dp(f))
+    lam = lib(precision).ref_riemann_eval(C.byref(rs), dim, _dp(s1), _dp(s2), _dp(nor), _dp(f))
     return f, lam
 
 
-def bc_state(scheme, dim, bc_index, s1, nor):
+def bc_state(scheme, dim, bc_index, s1, nor, precision="double"):
     rs = ref_scheme(scheme)
     s1 = np.ascontiguousarray(s1, np.float64)
     nor = np.ascontiguousarray(nor, np.float64)
     s2 = np.zeros(dim + 2)
-    lib().ref_bc_state(C.byref(rs), dim, bc_index, _dp(s1), _dp(nor), _dp(s2))
+    lib(precision).ref_bc_state(C.byref(rs), dim, bc_index, _dp(s1), _dp(nor), _dp(s2))
     return s2
 
 

@@ -22,6 +22,7 @@ RKDG_E_ARG = -1
 RKDG_E_UNPHYSICAL = -2
 RKDG_E_HIP = -3
 RKDG_E_UNSUPPORTED = -4
+RKDG_E_NAN = -5
 
 
 class MeshDesc(C.Structure):
@@ -146,9 +147,15 @@ class UnphysicalStateError(RKDGError):
     pass
 
 
+class NonFiniteStateError(UnphysicalStateError):
+    """A flux input held a NaN or an infinity (RKDG_E_NAN)."""
+
+
 def _check(rc):
     if rc != RKDG_OK:
         msg = lib().rkdg_last_error().decode()
+        if rc == RKDG_E_NAN:
+            raise NonFiniteStateError(rc, msg)
         if rc == RKDG_E_UNPHYSICAL:
             raise UnphysicalStateError(rc, msg)
         raise RKDGError(rc, msg)
@@ -180,9 +187,46 @@ def scheme_desc(s: S.Scheme) -> SchemeDesc:
 def _ptr(t):
     if t is None:
         return None
-    if hasattr(t, "data_ptr"):
-        return C.c_void_p(t.data_ptr())
-    return C.c_void_p(int(t))
+    return C.c_void_p(t.data_ptr())
+
+
+def check_buffer(t, n, name, dtype="float64", nullable=False):
+    """Validates a device buffer before its pointer reaches the kernels: a torch
+    tensor on the GPU, of the given dtype (float64 states, int8 flags),
+    contiguous, with exactly n elements. Raises ValueError otherwise (a short,
+    strided or host buffer would be read or written out of bounds on the
+    device). Returns the tensor."""
+    if t is None:
+        if nullable:
+            return None
+        raise ValueError(f"{name}: a device buffer is required")
+    import torch
+
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name}: expected a torch tensor on the GPU, got {type(t).__name__}")
+    if not t.is_cuda:
+        raise ValueError(f"{name}: tensor is on {t.device}, the kernels need GPU memory")
+    if t.dtype != getattr(torch, dtype):
+        raise ValueError(f"{name}: dtype {t.dtype}, expected torch.{dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: tensor is not contiguous")
+    if t.numel() != n:
+        raise ValueError(f"{name}: {t.numel()} elements, expected {n}")
+    return t
+
+
+def check_tableau(tableau):
+    """(s, a, b, c) with 1 <= s <= 4, at least the s(s-1)/2 entries of a that
+    the packed a[l++] rule reads (the reference's literal tableaus hold more,
+    case_manager.cpp:829-884) and s weights b."""
+    s, a, b, c = tableau
+    if not 1 <= int(s) <= 4:
+        raise ValueError(f"tableau: {s} stages, 1 to 4 supported")
+    if len(a) < s * (s - 1) // 2:
+        raise ValueError(f"tableau: {len(a)} a entries, the packed rule reads s(s-1)/2 = {s * (s - 1) // 2}")
+    if len(b) != s:
+        raise ValueError(f"tableau: {len(b)} b entries, expected {s}")
+    return tableau
 
 
 def _mesh_desc(mesh, order, md=None):
@@ -202,7 +246,7 @@ def _mesh_desc(mesh, order, md=None):
 
 
 def _tableau_arrays(tableau):
-    s, a, b, c = tableau
+    s, a, b, c = check_tableau(tableau)
     a = (C.c_double * max(1, len(a)))(*a) if a else (C.c_double * 1)(0.0)
     b = (C.c_double * len(b))(*b)
     c = (C.c_double * max(1, len(c)))(*c) if c else (C.c_double * 1)(0.0)
@@ -249,23 +293,34 @@ class Context:
     def set_stream(self, stream_ptr):
         _check(lib().rkdg_set_stream(self.h, C.c_void_p(stream_ptr)))
 
+    def _st(self, t, name, nullable=False):
+        return check_buffer(t, self.info.state_size, name, nullable=nullable)
+
     def rhs(self, x, y, want_lambda=True):
+        self._st(x, "x")
+        self._st(y, "y")
         lam = C.c_double(0.0)
         _check(lib().rkdg_rhs(self.h, _ptr(x), _ptr(y), C.byref(lam) if want_lambda else None))
         return lam.value
 
     def face_lambda(self, x):
+        self._st(x, "x")
         lam = C.c_double(0.0)
         _check(lib().rkdg_face_lambda(self.h, _ptr(x), C.byref(lam)))
         return lam.value
 
     def limit(self, x, troubled=None):
+        self._st(x, "x")
+        check_buffer(troubled, self.info.num_elements, "troubled", dtype="int8", nullable=True)
         _check(lib().rkdg_limit(self.h, _ptr(x), _ptr(troubled)))
 
     def means(self, x, out):
+        self._st(x, "x")
+        check_buffer(out, self.info.num_equations * self.info.num_elements, "means")
         _check(lib().rkdg_means(self.h, _ptr(x), _ptr(out)))
 
     def step(self, x, dt, tableau, t=0.0, want_lambda=True):
+        self._st(x, "x")
         s, a, b, c = _tableau_arrays(tableau)
         lam = C.c_double(0.0)
         _check(lib().rkdg_step(self.h, _ptr(x), t, dt, s, a, b, c,
@@ -296,6 +351,7 @@ class Context:
         """Autotuner (rkdg_tune_grid): times the right-hand side of x for the
         persistent grid sizes 0 (one workgroup per work block) and candidates,
         keeps the fastest; returns (chosen cap, {cap: ms per launch})."""
+        self._st(x, "x")
         cand = (C.c_int * len(candidates))(*candidates)
         ms = (C.c_double * (len(candidates) + 1))()
         best = C.c_int(0)
@@ -312,15 +368,26 @@ class Context:
         _check(lib().rkdg_set_bc_params(self.h, int(slot), p.ctypes.data_as(C.POINTER(C.c_double))))
 
     def rhs_adjoint(self, x, kbar, xbar, addend=None, pbar=None):
+        for t, nm in ((x, "x"), (kbar, "kbar"), (xbar, "xbar")):
+            self._st(t, nm)
+        self._st(addend, "addend", nullable=True)
+        check_buffer(pbar, S.MAX_BC * 8, "pbar", nullable=True)
         _check(lib().rkdg_rhs_adjoint(self.h, _ptr(x), _ptr(kbar), _ptr(xbar), _ptr(addend), _ptr(pbar)))
 
     def limit_adjoint(self, x, ubar, xbar):
+        for t, nm in ((x, "x"), (ubar, "ubar"), (xbar, "xbar")):
+            self._st(t, nm)
         _check(lib().rkdg_limit_adjoint(self.h, _ptr(x), _ptr(ubar), _ptr(xbar)))
 
     def objective(self, x, slot, coef, J=None, xbar=None):
+        self._st(x, "x")
+        check_buffer(J, 1, "J", nullable=True)
+        self._st(xbar, "xbar", nullable=True)
         _check(lib().rkdg_objective(self.h, _ptr(x), int(slot), float(coef), _ptr(J), _ptr(xbar)))
 
     def step_adjoint(self, x, dt, tableau, ybar, xbar):
+        for t, nm in ((x, "x"), (ybar, "ybar"), (xbar, "xbar")):
+            self._st(t, nm)
         s, a, b, c = _tableau_arrays(tableau)
         pbar = np.zeros(S.MAX_BC * 8)
         pd = C.POINTER(C.c_double)
@@ -329,6 +396,7 @@ class Context:
         return pbar.reshape(S.MAX_BC, 8)
 
     def forward_solve(self, x, dts, tableau, obj_slot=-1):
+        self._st(x, "x")
         s, a, b, c = _tableau_arrays(tableau)
         d = np.ascontiguousarray(dts, np.float64)
         J = C.c_double(0.0)
@@ -339,6 +407,9 @@ class Context:
 
     def adjoint_solve(self, x0, dts, tableau, obj_slot, x0bar, every=0, xN=None):
         """Returns (J, bc_param_bar[MAX_BC, 8]); x0bar receives dJ/dx0."""
+        self._st(x0, "x0")
+        self._st(x0bar, "x0bar")
+        self._st(xN, "xN", nullable=True)
         s, a, b, c = _tableau_arrays(tableau)
         d = np.ascontiguousarray(dts, np.float64)
         J = C.c_double(0.0)
@@ -384,6 +455,7 @@ class BatchContext(Context):
         self.offsets = np.array(off[:], dtype=np.int64)
 
     def init(self, x, t_final, cfl, t0=0.0, max_steps=0):
+        self._st(x, "x")
         tf = np.broadcast_to(np.asarray(t_final, np.float64), (self.count,)).copy()
         t0 = np.broadcast_to(np.asarray(t0, np.float64), (self.count,)).copy()
         pd = C.POINTER(C.c_double)
@@ -392,6 +464,7 @@ class BatchContext(Context):
 
     def advance(self, x, tableau, nsteps=1):
         """Returns the number of instances still running."""
+        self._st(x, "x")
         s, a, b, c = _tableau_arrays(tableau)
         na = C.c_int(0)
         _check(lib().rkdg_batch_advance(self.h, _ptr(x), s, a, b, c, int(nsteps), C.byref(na)))

@@ -153,11 +153,8 @@ __device__ __forceinline__ void dense_rhs_block(const RhsArgs& A, const SchemePa
       if (!bad) {
         const int64_t other = nb >= 0 ? (int64_t)nb : -1;
         const bool mine = factor < 0.0;
-        if (atomicCAS(reinterpret_cast<unsigned long long*>(A.err), 0ull, 1ull) == 0ull) {
-          A.err[1] = mine ? e : other;
-          A.err[2] = mine ? other : e;
-          A.err[3] = f;
-        }
+        flag_error(A.err, mine ? e : other, mine ? other : e, f,
+                   nonfinite<NEQ>(um, up) ? kErrNonFinite : kErrUnphysical);
       }
       bad = true;
     } else {

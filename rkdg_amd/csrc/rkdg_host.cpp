@@ -951,6 +951,18 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
 
 constexpr size_t kMaxProfiled = 8192;
 
+// Restores a context field on every exit path of an entry point that switches
+// it temporarily (the batch stream, the profiling flag, the tuner's grid cap).
+template <typename Tp>
+struct Restore {
+  Tp& ref;
+  Tp saved;
+  explicit Restore(Tp& r) : ref(r), saved(r) {}
+  Restore(const Restore&) = delete;
+  Restore& operator=(const Restore&) = delete;
+  ~Restore() { ref = saved; }
+};
+
 // returns (start, stop) event indices or (-1, -1) when profiling is off/full
 std::pair<int, int> prof_begin(rkdg_ctx* c, std::vector<std::pair<int, int>>& list) {
   if (!c->prof || list.size() >= kMaxProfiled) return {-1, -1};
@@ -1019,14 +1031,18 @@ int check_device_error(rkdg_ctx* c, int64_t* info) {
   HIP_TRY(hipMemcpyAsync(h, c->d_err, sizeof(long long) * 4, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (h[0]) {
+    const long long kind = h[0], e1 = h[1], e2 = h[2];
     if (info) {
-      info[0] = h[1];
-      info[1] = h[2];
+      info[0] = e1;
+      info[1] = e2;
     }
-    HIP_TRY(hipMemsetAsync(c->d_err, 0, sizeof(long long) * 4, c->stream));
+    HIP_TRY(launch_fill(reinterpret_cast<double*>(c->d_err), 0.0, 4, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    return fail(RKDG_E_UNPHYSICAL, "unphysical state at a face between elements " + std::to_string(h[1]) +
-                                       " and " + std::to_string(h[2]));
+    if (kind == (long long)kErrNonFinite)
+      return fail(RKDG_E_NAN, "non-finite state at a face between elements " + std::to_string(e1) + " and " +
+                                  std::to_string(e2));
+    return fail(RKDG_E_UNPHYSICAL, "unphysical state at a face between elements " + std::to_string(e1) +
+                                       " and " + std::to_string(e2));
   }
   return RKDG_OK;
 }
@@ -1061,6 +1077,50 @@ int ensure_stage_buffers(rkdg_ctx* c, int n) {
   const int64_t sz = (int64_t)c->neq * c->ndofs;
   for (int i = 0; i < n; i++)
     if (!c->d_stage[i]) HIP_TRY(hipMalloc(&c->d_stage[i], sizeof(double) * sz));
+  return RKDG_OK;
+}
+
+// ExplicitRKLimitedSolver::Step (rk_explicit_limited.cpp:30-82). Stage i reads
+// its (limited) input and produces, in the epilogue of the RHS kernel, every
+// pending combination that needs k_i: W_j += (a_ji dt) k_i for later stages j
+// and acc += (b_i dt) k_i, in the reference's association order. W_1 (stage 0)
+// and the final x are the only vectors written whole; the means of the next
+// stage input come out of the same epilogue for the limiter. buf[0..s-2] hold
+// W_1..W_{s-1}, buf[3] the accumulator; every one is written before it is read
+// (stage 0 has no bases), so nothing needs clearing. The last stage's wave
+// speed lands in d_lam[s-1] when want_lambda.
+int step_impl(rkdg_ctx* c, double* x, double dt, int s, const double* a, const double* b, double* const* buf,
+              bool want_lambda) {
+  double* W[4] = {nullptr, buf[0], buf[1], buf[2]};
+  double* acc = buf[3];
+  HIP_TRY(launch_fill(c->d_lam, 0.0, 8 * c->count, c->stream));
+  auto A_ = [&](int i, int j) { return a[i * (i - 1) / 2 + j]; };
+  for (int i = 0; i < s; i++) {
+    const double* in = i == 0 ? x : W[i];
+    RhsArgs ra = base_args(c, in);
+    ra.lam = c->d_lam + (int64_t)i * c->count;
+    int n = 0;
+    for (int j = i + 1; j < s; j++) {
+      ra.outs[n].out = W[j];
+      ra.outs[n].base = i == 0 ? nullptr : W[j];
+      ra.outs[n].coef = A_(j, i) * dt;
+      n++;
+    }
+    const bool last = i == s - 1;
+    ra.no_lambda = (!last || !want_lambda) ? 1 : 0;  // only the last stage's wave speed is read
+    ra.outs[n].out = last ? (s == 1 ? acc : x) : acc;
+    ra.outs[n].base = i == 0 ? nullptr : acc;
+    ra.outs[n].coef = b[i] * dt;
+    n++;
+    ra.nout = n;
+    ra.means = c->d_means;
+    ra.means_of = last ? n - 1 : 0;  // next stage input W_{i+1} is outs[0]; final is the last
+    HIP_TRY(rhs_launch(c, ra));
+    if (last && s == 1) HIP_TRY(launch_copy(x, acc, (int64_t)c->neq * c->ndofs, c->stream));
+    double* lim = last ? x : W[i + 1];
+    int rc = limit_impl(c, lim, nullptr, true);
+    if (rc) return rc;
+  }
   return RKDG_OK;
 }
 
@@ -1399,12 +1459,7 @@ int rkdg_means(rkdg_ctx* c, const double* x, double* means) {
   return RKDG_OK;
 }
 
-// ExplicitRKLimitedSolver::Step (rk_explicit_limited.cpp:30-82). Stage i reads
-// its (limited) input and produces, in the epilogue of the RHS kernel, every
-// pending combination that needs k_i: W_j += (a_ji dt) k_i for later stages j
-// and acc += (b_i dt) k_i, in the reference's association order. W_1 (stage 0)
-// and the final x are the only vectors written whole; the means of the next
-// stage input come out of the same epilogue for the limiter.
+// ExplicitRKLimitedSolver::Step (rk_explicit_limited.cpp:30-82), see step_impl.
 int rkdg_step(rkdg_ctx* c, double* x, double t, double dt, int s, const double* a, const double* b,
               const double* cc, double* lambda_max) {
   (void)t;
@@ -1414,36 +1469,8 @@ int rkdg_step(rkdg_ctx* c, double* x, double t, double dt, int s, const double* 
   // buffers: d_stage[0..s-2] = W_1..W_{s-1}, d_stage[3] = accumulator
   int rc = ensure_stage_buffers(c, 4);
   if (rc) return rc;
-  double* W[4] = {nullptr, c->d_stage[0], c->d_stage[1], c->d_stage[2]};
-  double* acc = c->d_stage[3];
-  HIP_TRY(launch_fill(c->d_lam, 0.0, 8 * c->count, c->stream));
-  auto A_ = [&](int i, int j) { return a[i * (i - 1) / 2 + j]; };
-  for (int i = 0; i < s; i++) {
-    const double* in = i == 0 ? x : W[i];
-    RhsArgs ra = base_args(c, in);
-    ra.lam = c->d_lam + (int64_t)i * c->count;
-    int n = 0;
-    for (int j = i + 1; j < s; j++) {
-      ra.outs[n].out = W[j];
-      ra.outs[n].base = i == 0 ? nullptr : W[j];
-      ra.outs[n].coef = A_(j, i) * dt;
-      n++;
-    }
-    const bool last = i == s - 1;
-    ra.no_lambda = (!last || !lambda_max) ? 1 : 0;  // only the last stage's wave speed is read
-    ra.outs[n].out = last ? (s == 1 ? acc : x) : acc;
-    ra.outs[n].base = i == 0 ? nullptr : acc;
-    ra.outs[n].coef = b[i] * dt;
-    n++;
-    ra.nout = n;
-    ra.means = c->d_means;
-    ra.means_of = last ? n - 1 : 0;  // next stage input W_{i+1} is outs[0]; final is the last
-    HIP_TRY(rhs_launch(c, ra));
-    if (last && s == 1) HIP_TRY(launch_copy(x, acc, (int64_t)c->neq * c->ndofs, c->stream));
-    double* lim = last ? x : W[i + 1];
-    rc = limit_impl(c, lim, nullptr, true);
-    if (rc) return rc;
-  }
+  rc = step_impl(c, x, dt, s, a, b, c->d_stage, lambda_max != nullptr);
+  if (rc) return rc;
   if (lambda_max) return read_lambda(c, s - 1, lambda_max);
   return RKDG_OK;
 }
@@ -1811,23 +1838,20 @@ int rkdg_batch_init(rkdg_ctx* c, double* x, const double* t0, const double* t_fi
     c->gexec = nullptr;
     c->gkey.clear();
   }
-  hipStream_t keep = c->stream;
+  Restore<hipStream_t> keep_stream(c->stream);
+  HIP_TRY(hipStreamSynchronize(c->stream));
   c->stream = c->bstream;
-  HIP_TRY(hipStreamSynchronize(keep));
   HIP_TRY(hipMemcpyAsync(c->d_tf, t_final, sizeof(double) * c->count, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpyAsync(c->d_t, t0, sizeof(double) * c->count, hipMemcpyHostToDevice, c->stream));
   int rc = limit_impl(c, x, nullptr, false);
-  if (rc == RKDG_OK) {
-    HIP_TRY(launch_fill(c->d_lam, 0.0, c->count, c->stream));
-    RhsArgs a = base_args(c, x);
-    a.face_only = 1;
-    HIP_TRY(rhs_launch(c, a));
-    BatchTime b = batch_time(c, c->d_lam);
-    HIP_TRY(launch_batch_init(b, c->stream));
-    rc = check_device_error(c, nullptr);
-  }
-  c->stream = keep;
-  return rc;
+  if (rc) return rc;
+  HIP_TRY(launch_fill(c->d_lam, 0.0, c->count, c->stream));
+  RhsArgs a = base_args(c, x);
+  a.face_only = 1;
+  HIP_TRY(rhs_launch(c, a));
+  BatchTime b = batch_time(c, c->d_lam);
+  HIP_TRY(launch_batch_init(b, c->stream));
+  return check_device_error(c, nullptr);
 }
 
 // Advances every active instance by up to nsteps steps (one HIP graph launch
@@ -1841,10 +1865,10 @@ int rkdg_batch_advance(rkdg_ctx* c, double* x, int s, const double* a, const dou
   if (s < 2 || s > 4) return fail(RKDG_E_ARG, "2 <= s <= 4 stages supported for batches");
   int rc = ensure_stage_buffers(c, 4);
   if (rc) return rc;
-  hipStream_t keep = c->stream;
-  HIP_TRY(hipStreamSynchronize(keep));
+  Restore<hipStream_t> keep_stream(c->stream);
+  Restore<bool> keep_prof(c->prof);
+  HIP_TRY(hipStreamSynchronize(c->stream));
   c->stream = c->bstream;
-  const bool prof_keep = c->prof;
   std::vector<double> key;
   key.push_back((double)(uintptr_t)x);
   key.push_back(s);
@@ -1858,10 +1882,9 @@ int rkdg_batch_advance(rkdg_ctx* c, double* x, int s, const double* a, const dou
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     rc = batch_step_launch(c, x, s, a, b);
     hipError_t ce = hipStreamEndCapture(c->stream, &graph);
-    c->prof = prof_keep;
+    c->prof = keep_prof.saved;
     if (rc) {
       if (graph) (void)hipGraphDestroy(graph);
-      c->stream = keep;
       return rc;
     }
     HIP_TRY(ce);
@@ -1875,16 +1898,12 @@ int rkdg_batch_advance(rkdg_ctx* c, double* x, int s, const double* a, const dou
       HIP_TRY(hipGraphLaunch(c->gexec, c->stream));
     } else {
       rc = batch_step_launch(c, x, s, a, b);
-      if (rc) {
-        c->stream = keep;
-        return rc;
-      }
+      if (rc) return rc;
     }
   }
   int h = 0;
   HIP_TRY(hipMemcpyAsync(&h, c->d_nact, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   rc = check_device_error(c, nullptr);
-  c->stream = keep;
   if (n_active) *n_active = h;
   return rc;
 }
@@ -1929,34 +1948,40 @@ int rkdg_tune_grid(rkdg_ctx* c, const double* x, const int* candidates, int ncan
   int rc = ensure_stage_buffers(c, 4);
   if (rc) return rc;
   reps = std::max(reps, 1);
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
-  const int keep_cap = c->grid_cap;
   double best_ms = 1e300;
   int best_cap = 0;
-  for (int k = -1; k < ncand; k++) {
-    const int cap = k < 0 ? 0 : candidates[k];
-    c->grid_cap = cap;
-    RhsArgs ra = base_args(c, x);
-    ra.k = c->d_stage[3];
-    HIP_TRY(rhs_launch(c, ra));  // warm-up
-    HIP_TRY(hipEventRecord(e0, c->stream));
-    for (int r = 0; r < reps; r++) HIP_TRY(rhs_launch(c, ra));
-    HIP_TRY(hipEventRecord(e1, c->stream));
-    HIP_TRY(hipEventSynchronize(e1));
-    float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-    const double per = (double)ms / reps;
-    if (ms_out) ms_out[k + 1] = per;
-    if (per < best_ms) {
-      best_ms = per;
-      best_cap = cap;
+  {
+    // events freed and the grid cap put back on every exit path
+    struct Events {
+      hipEvent_t e[2] = {nullptr, nullptr};
+      ~Events() {
+        for (auto ev : e)
+          if (ev) (void)hipEventDestroy(ev);
+      }
+    } ev;
+    Restore<int> keep_cap(c->grid_cap);
+    HIP_TRY(hipEventCreate(&ev.e[0]));
+    HIP_TRY(hipEventCreate(&ev.e[1]));
+    for (int k = -1; k < ncand; k++) {
+      const int cap = k < 0 ? 0 : candidates[k];
+      c->grid_cap = cap;
+      RhsArgs ra = base_args(c, x);
+      ra.k = c->d_stage[3];
+      HIP_TRY(rhs_launch(c, ra));  // warm-up
+      HIP_TRY(hipEventRecord(ev.e[0], c->stream));
+      for (int r = 0; r < reps; r++) HIP_TRY(rhs_launch(c, ra));
+      HIP_TRY(hipEventRecord(ev.e[1], c->stream));
+      HIP_TRY(hipEventSynchronize(ev.e[1]));
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
+      const double per = (double)ms / reps;
+      if (ms_out) ms_out[k + 1] = per;
+      if (per < best_ms) {
+        best_ms = per;
+        best_cap = cap;
+      }
     }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  c->grid_cap = keep_cap;
   rc = check_device_error(c, nullptr);
   if (rc) return rc;
   if (best) *best = best_cap;

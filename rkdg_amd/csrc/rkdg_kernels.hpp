@@ -150,6 +150,28 @@ __device__ __forceinline__ void atomic_max_pos(double* addr, double v) {
   if (bits > cur) atomicMax(p, bits);
 }
 
+// Device error word [4]: kind, first element, second element (-1 at a
+// boundary), local face. The first face to fail records itself; later ones
+// leave the word alone until the host reads and clears it.
+constexpr unsigned long long kErrUnphysical = 1;  // a Riemann solver saw an unphysical state
+constexpr unsigned long long kErrNonFinite = 2;   // ... and one of its inputs was NaN or infinite
+__device__ __forceinline__ void flag_error(long long* err, int64_t e1, int64_t e2, int lf,
+                                           unsigned long long kind) {
+  if (atomicCAS(reinterpret_cast<unsigned long long*>(err), 0ull, kind) == 0ull) {
+    err[1] = e1;
+    err[2] = e2;
+    err[3] = lf;
+  }
+}
+// true if any of the two traces holds a NaN or an infinity (x - x is not 0)
+template <int NEQ>
+__device__ __forceinline__ bool nonfinite(const double* a, const double* b) {
+  bool r = false;
+#pragma unroll
+  for (int v = 0; v < NEQ; v++) r = r || !(a[v] - a[v] == 0.0) || !(b[v] - b[v] == 0.0);
+  return r;
+}
+
 // launchers (rkdg_tensor.hip)
 // t: device copy of the tables, th: the host copy (its packed tables also go to
 // the kernel as an argument, for reads at wave-uniform indices from scalar

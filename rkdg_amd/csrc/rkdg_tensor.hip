@@ -112,14 +112,6 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nb) {
 }
 
 
-__device__ __forceinline__ void flag_error(long long* err, int64_t e1, int64_t e2, int lf) {
-  if (atomicCAS(reinterpret_cast<unsigned long long*>(err), 0ull, 1ull) == 0ull) {
-    err[1] = e1;
-    err[2] = e2;
-    err[3] = lf;
-  }
-}
-
 // the part of TensorTables an element of order N-1 uses, packed for LDS
 template <int N, int NQ>
 struct TT {
@@ -462,7 +454,9 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
         }
       }
       if (!ok || lam < 0.0) {
-        if (!bad) flag_error(A.err, factor < 0.0 ? e : nb, factor < 0.0 ? (nb >= 0 ? nb : -1) : e, lf);
+        if (!bad)
+          flag_error(A.err, factor < 0.0 ? e : nb, factor < 0.0 ? (nb >= 0 ? nb : -1) : e, lf,
+                     nonfinite<NEQ>(um, up) ? kErrNonFinite : kErrUnphysical);
         bad = true;
       } else {
         lmax = lam > lmax ? lam : lmax;

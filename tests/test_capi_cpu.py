@@ -106,3 +106,71 @@ def test_entry_points_reject_null_context():
     assert L.rkdg_get_info(null, null) == capi.RKDG_E_ARG
     assert L.rkdg_node_coords(null, null) == capi.RKDG_E_ARG
     assert L.rkdg_destroy(null) == 0
+
+
+def test_error_codes_match_header():
+    txt = open(os.path.join(ROOT, "include", "rkdg_capi.h")).read()
+    codes = dict((k, int(v)) for k, v in re.findall(r"#define (RKDG_E_[A-Z]+) \((-?\d+)\)", txt))
+    for name in ("RKDG_E_ARG", "RKDG_E_UNPHYSICAL", "RKDG_E_HIP", "RKDG_E_UNSUPPORTED", "RKDG_E_NAN"):
+        assert getattr(capi, name) == codes[name], name
+    assert len(set(codes.values())) == len(codes)
+
+
+def test_buffer_validation_rejects_bad_buffers_on_the_host():
+    """Every device buffer is checked before its pointer reaches the kernels:
+    host tensors, wrong dtypes, strided views, wrong sizes and non-tensors are
+    refused with ValueError (no GPU needed: the checks run first)."""
+    import pytest
+    import torch
+
+    x = torch.zeros(10, dtype=torch.float64)
+    with pytest.raises(ValueError, match="on cpu"):
+        capi.check_buffer(x, 10, "x")
+    with pytest.raises(ValueError, match="torch tensor"):
+        capi.check_buffer(12345, 10, "x")
+    with pytest.raises(ValueError, match="required"):
+        capi.check_buffer(None, 10, "x")
+    assert capi.check_buffer(None, 10, "x", nullable=True) is None
+
+    class FakeCuda:
+        """A stand-in with the tensor attributes the check reads."""
+
+        def __init__(self, t):
+            self.t = t
+            self.is_cuda = True
+            self.dtype = t.dtype
+
+        def is_contiguous(self):
+            return self.t.is_contiguous()
+
+        def numel(self):
+            return self.t.numel()
+
+    import unittest.mock as um
+
+    with um.patch.object(torch, "Tensor", FakeCuda):
+        with pytest.raises(ValueError, match="dtype"):
+            capi.check_buffer(FakeCuda(torch.zeros(10, dtype=torch.float32)), 10, "x")
+        with pytest.raises(ValueError, match="contiguous"):
+            capi.check_buffer(FakeCuda(torch.zeros(20, dtype=torch.float64)[::2]), 10, "x")
+        with pytest.raises(ValueError, match="expected 10"):
+            capi.check_buffer(FakeCuda(torch.zeros(9, dtype=torch.float64)), 10, "x")
+        with pytest.raises(ValueError, match="int8"):
+            capi.check_buffer(FakeCuda(torch.zeros(10, dtype=torch.float64)), 10, "troubled", dtype="int8")
+        ok = FakeCuda(torch.zeros(10, dtype=torch.int8))
+        assert capi.check_buffer(ok, 10, "troubled", dtype="int8") is ok
+
+
+def test_tableau_validation():
+    import pytest
+
+    from rkdg_amd import scheme as S
+
+    for name, tab in S.TABLEAUS.items():
+        assert capi.check_tableau(tab) is tab, name
+    with pytest.raises(ValueError, match="stages"):
+        capi.check_tableau((5, [0.0] * 10, [0.2] * 5, []))
+    with pytest.raises(ValueError, match="a entries"):
+        capi.check_tableau((3, [1.0, 0.25], [1 / 6, 1 / 6, 2 / 3], [1.0, 0.5]))
+    with pytest.raises(ValueError, match="b entries"):
+        capi.check_tableau((3, [1.0, 0.25, 0.25], [0.5, 0.5], [1.0, 0.5]))
