@@ -30,15 +30,12 @@
  * The reference cannot be built here (MFEM, hypre, METIS and rapidyaml are
  * absent) and holds no test vectors, so the restatement is pinned by closed
  * forms, exact solutions, invariants and symmetries (tests/test_oracle_pins.py).
- * Parity unpinned (restated from the reference and MFEM's published behaviour
- * with no independent check):
+ * Parity unpinned: the MFEM behaviour the reference relies on and that only
+ * MFEM itself could confirm (restated from its published algorithms):
  *   - the node positions of L2_TriangleElement for p >= 2 (only the Lagrange
  *     property at its own nodes is checked);
  *   - the face and volume quadrature point counts derived from OrderW;
- *   - GetElementSize(i, 1) (hmin): smallest singular value at the centre;
  *   - Vector::Norml2 rounding beyond vectors with one non-zero entry;
- *   - the ghost state of totalPressureFixedTemperatureOutlet
- *     (boundary_integrator_open_total_pressure.cpp, normal index fixed);
  *   - MFEM's face numbering and Elem1 choice (results depend on them only
  *     through rounding, which the orientation-invariance pin bounds).
  */

@@ -236,6 +236,22 @@ def jittered_delaunay_tri(nx, ny, x0=0.0, x1=4.0, y0=0.0, y1=1.0, jitter=0.3, se
     return Mesh(GEOM_TRI, coords, simp, bv, ba, name=f"tri{len(simp)}")
 
 
+def relabel_boundary(m: Mesh, rule) -> Mesh:
+    """A copy of m whose boundary element k gets attribute rule(attr_k,
+    midpoint_k) (midpoint of its vertices), e.g. to split one side of a box
+    into two boundary conditions."""
+    vx = {}
+    for e in range(m.num_elements):
+        for v in range(m.elem_verts.shape[1]):
+            vx[int(m.elem_verts[e, v])] = m.elem_coords[e, v]
+    ba = m.bdr_attr.copy()
+    for k in range(len(ba)):
+        mid = np.mean([vx[int(v)] for v in m.bdr_verts[k]], axis=0)
+        ba[k] = rule(int(ba[k]), mid)
+    out = Mesh(m.geom, m.elem_coords, m.elem_verts, m.bdr_verts, ba, name=m.name)
+    return out
+
+
 def nozzle_quad(num_elements, area_ratio, length=1.0, inlet_half_height=0.5) -> Mesh:
     """Converging-diverging planar nozzle on a structured quad grid with about
     ``num_elements`` cells (aspect kept near length:height). The half-height is

@@ -431,3 +431,50 @@ def test_char_outlet_boundary(kind, cuda):
     print(f"[char outlet {kind}] 2 steps rel Linf {err:.3e}", flush=True)
     assert err <= TOL
     assert abs(lam - lam_ref) <= 1e-12 * lam_ref
+
+
+@pytest.mark.parametrize("kind", ["hex_q2", "quad_q2_dense", "tri_p2"])
+def test_last_hope_reset_matches_oracle(kind, cuda):
+    """A cell whose nodes are all physical but whose state extrapolated to a
+    reference vertex has negative density: the multiplier limiter's last hope
+    (limiter_multiplier.cpp:41-95) resets it to its mean. The GPU sets bit 1 of
+    its troubled flag for exactly that cell and its limited state matches the
+    oracle's, on the tensor hex path and the dense quadrilateral and triangle
+    paths."""
+    import torch
+
+    from rkdg_amd.capi import Context
+
+    if kind == "hex_q2":
+        m, p, dense = M.structured_hex(3, 3, 3), 2, False
+    elif kind == "quad_q2_dense":
+        m, p, dense = M.structured_quad(4, 3, periodic=(True, True)), 2, True
+    else:
+        m, p, dense = M.jittered_delaunay_tri(4, 3, 0.0, 1.0, 0.0, 1.0, seed=3), 2, False
+    dim, neq = m.dim, m.dim + 2
+    sch = S.Scheme(indicator=S.IND_NOWHERE, limiter=S.LIM_MULTI, last_hope=1)
+    ref = RefSolver(m, p, sch)
+    xyz = ref.node_coords()
+    ne, nd = xyz.shape[:2]
+    bad = 4
+    rho = np.ones((ne, nd))
+    # density 0.05 at the node nearest to the element's first vertex, 1 elsewhere
+    k = int(np.argmin(((xyz[bad] - m.elem_coords[bad, 0]) ** 2).sum(axis=1)))
+    rho[bad, k] = 0.05
+    z = np.zeros((ne, nd))
+    from tests.helpers import conservative
+
+    x = np.concatenate([u.ravel() for u in conservative(rho, [z] * dim, np.ones((ne, nd)))])
+    xl, _, _ = ref.limit(x)
+    mean = ref.means(x).reshape(neq, -1)[:, bad]
+    u1 = xl.reshape(neq, ne, nd)[:, bad]
+    assert np.abs(u1 - mean[:, None]).max() <= 1e-15  # the oracle reset the cell
+    ctx = Context(m, p, sch, dense=dense)
+    xd = _dev(x, cuda)
+    tr = torch.zeros(ne, dtype=torch.int8, device=cuda)
+    ctx.limit(xd, tr)
+    flags = _host(tr)
+    print(f"[last hope {kind}] flags of the reset cell {int(flags[bad])}, cells flagged "
+          f"{np.flatnonzero(flags).tolist()}", flush=True)
+    assert flags[bad] & 2 and np.count_nonzero(flags & 2) == 1
+    assert rel_linf(_host(xd), xl, neq) <= TOL

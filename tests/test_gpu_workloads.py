@@ -1,0 +1,220 @@
+"""The workloads of the benchmark configurations run end to end on the GPU
+against the oracle's own time loop (the reference's dt rule, main.cpp:467-553,
+701, 733):
+
+* the Sod tube (200x10 Q1 quads, HLLC, Barth-Jespersen, SSP-RK3, CFL 0.2, to
+  t = 0.2) through ``solver.run``: the step count, t and the dt sequence, a
+  one-step comparison from the oracle's state every 50 steps, and both final
+  states against Toro's exact solution;
+* the double Mach reflection (P2 jittered-Delaunay triangles, HLLC with
+  Einfeldt speeds, Michalak, post-shock inflow on the bottom for x < 1/6) at a
+  reduced size: RHS, limiter and 5 steps of the dt loop;
+* the Mach 3 forward-facing step at the reference inflow (v = 0) on a reduced
+  mesh: the forward solve and its objective against the oracle's steps, and
+  the adjoint's dJ/dMach against central differences of the oracle's forward
+  solve.
+"""
+import numpy as np
+import pytest
+
+from oracle import refcpu as R
+from oracle.refcpu import RefSolver
+from rkdg_amd import cases
+from rkdg_amd import scheme as S
+from tests.exact_riemann import sample
+
+TOL = 1e-12
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(x, cuda):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(x)).to(cuda)
+
+
+def _host(t):
+    import torch
+
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def _rel_per_var(a, b, neq):
+    a, b = np.asarray(a).reshape(neq, -1), np.asarray(b).reshape(neq, -1)
+    den = np.abs(b).max(axis=1)
+    return np.abs(a - b).max(axis=1) / np.where(den > 0, den, 1.0)
+
+
+def test_sod_tube_c1_full_run_matches_oracle_loop(cuda):
+    from rkdg_amd.solver import ExplicitRKLimitedSolver, FE_Evolution, Limiter, run
+
+    cs = cases.sod2d()
+    tab = S.TABLEAUS[cs.tableau]
+    ref = RefSolver(cs.mesh, cs.order, cs.scheme)
+    x0 = cs.initial_state(ref.node_coords())
+    saved = {}
+    otrace = []
+
+    def keep(k, t, dt, x):
+        otrace.append((t, dt))
+        if k % 50 == 0:
+            saved[k] = x.copy()
+
+    xr, st = R.run(ref, x0, cs.t_final, cs.cfl, tab, callback=keep)
+
+    f = FE_Evolution(cs.mesh, cs.order, cs.scheme)
+    lim = Limiter(f)
+    sol = ExplicitRKLimitedSolver(*tab, limiter=lim)
+    xg = _dev(x0, cuda)
+    trace = []
+    stats = run(f, sol, lim, xg, cs.t_final, cs.cfl, callback=lambda k, t, dt, x: trace.append((t, dt)))
+    print(f"[sod C1] oracle {st['steps']} steps t {st['t']:.15g}, GPU {stats.steps} steps t {stats.t:.15g}",
+          flush=True)
+    assert stats.steps == st["steps"]
+    assert abs(stats.t - st["t"]) <= 1e-12 * st["t"]
+    # t and the recomputed dt after every step
+    tg, to = np.array(trace), np.array(otrace)
+    assert np.all(np.abs(tg - to) <= 1e-12 * np.abs(to)), np.abs(tg - to).max(axis=0)
+    xgh = _host(xg)
+    print(f"[sod C1] final states differ by {np.array2string(_rel_per_var(xgh, xr, 4), precision=2)} "
+          f"(relative per variable)", flush=True)
+
+    # one step from the oracle's state every 50 steps, GPU vs oracle
+    worst = np.zeros(4)
+    for k, xs in sorted(saved.items()):
+        dt = st["dt_real"][k] if k < len(st["dt_real"]) else st["dt_real"][-1]
+        xo, lo = ref.step(xs, dt, tab)
+        xd = _dev(xs, cuda)
+        lg = f.ctx.step(xd, dt, tab)
+        e = _rel_per_var(_host(xd), xo, 4)
+        worst = np.maximum(worst, e)
+        assert abs(lg - lo) <= 1e-13 * lo, k
+    print(f"[sod C1] one step from the oracle's state at steps {sorted(saved)}: worst relative error per "
+          f"variable {np.array2string(worst, precision=2)}", flush=True)
+    assert worst.max() <= TOL
+
+    # both against the exact solution at the time the state was advanced to
+    nx = 200
+    xc = (np.arange(nx) + 0.5) / nx
+    tend = st["advanced"]
+    rho_ex = np.array([sample((xx - 0.5) / tend, 1.0, 0.0, 1.0, 0.125, 0.0, 0.1)[0] for xx in xc])
+
+    def l1(x):
+        rho = ref.means(x).reshape(4, -1)[0].reshape(10, nx).mean(axis=0)
+        return np.abs(rho - rho_ex).mean()
+
+    l1r, l1g = l1(xr), l1(xgh)
+    print(f"[sod C1] density L1 error vs the exact solution: oracle {l1r:.6e}, GPU {l1g:.6e}", flush=True)
+    assert l1r < 0.01 and abs(l1g - l1r) <= 1e-6 * l1r
+
+
+def test_double_mach_c2_reduced_matches_oracle(cuda):
+    """At 40x10 (800 P2 triangles). With the discontinuous data interpolated at
+    the nodes, as configured, the oracle stops in the first step on an
+    unphysical face state (the reference's exit(1)); the GPU must report the
+    same (RKDG_E_UNPHYSICAL) after the same RHS and limiter. With the shock
+    blended over one cell width the scheme runs, and 5 steps of the dt loop
+    are compared."""
+    import torch
+
+    from rkdg_amd.capi import Context, UnphysicalStateError
+
+    neq = 4
+    for width in (0.0, 0.1):
+        cs = cases.double_mach(nx=40, ny=10, shock_width=width)
+        assert 5 in set(cs.mesh.bdr_attr.tolist()) and 1 in set(cs.mesh.bdr_attr.tolist())
+        tab = S.TABLEAUS[cs.tableau]
+        ref = RefSolver(cs.mesh, cs.order, cs.scheme)
+        ctx = Context(cs.mesh, cs.order, cs.scheme)
+        x = cs.initial_state(ref.node_coords())
+        xd = _dev(x, cuda)
+        xl, _, mv = ref.limit(x)
+        tr = torch.zeros(cs.mesh.num_elements, dtype=torch.int8, device=cuda)
+        ctx.limit(xd, tr)
+        flags = _host(tr) & 1
+        near = (np.abs(mv - 1.0) < 1e-12) & (mv != 1.0)
+        print(f"[DMR C2 {cs.mesh.num_elements} P2 triangles, shock width {width}] troubled {int(flags.sum())} "
+              f"(oracle {int((mv < 1).sum())}), near threshold {int(near.sum())}", flush=True)
+        assert np.all((flags == (mv < 1.0)) | near)
+        e = _rel_per_var(_host(xd), xl, neq)
+        assert e.max() <= TOL, e
+
+        y_ref, lam_ref = ref.mult(xl)
+        yd = torch.empty_like(xd)
+        lam = ctx.rhs(xd, yd)
+        e = _rel_per_var(_host(yd), y_ref, neq)
+        print(f"[DMR C2] RHS of the limited state rel per variable {np.array2string(e, precision=2)}, "
+              f"lambda {lam:.15g} (oracle {lam_ref:.15g})", flush=True)
+        assert e.max() <= TOL and abs(lam - lam_ref) <= 1e-13 * lam_ref
+
+        hmin = ref.hmin()
+        assert abs(ctx.hmin() - hmin) <= 1e-14 * hmin
+        _, lr = ref.face_mult(xl)
+        lg = ctx.face_lambda(xd)
+        assert abs(lg - lr) <= 1e-13 * lr
+        dt_r, dt_g = cs.cfl * hmin / lr / 5.0, cs.cfl * ctx.hmin() / lg / 5.0
+        x = xl
+        if width == 0.0:
+            with pytest.raises(FloatingPointError):
+                ref.step(x, dt_r, tab)
+            with pytest.raises(UnphysicalStateError):
+                ctx.step(xd, dt_g, tab)
+            print("[DMR C2] sharp shock: oracle and GPU both stop on an unphysical face state in step 1",
+                  flush=True)
+            continue
+        for k in range(5):
+            x, lr = ref.step(x, dt_r, tab)
+            lg = ctx.step(xd, dt_g, tab)
+            e = _rel_per_var(_host(xd), x, neq)
+            print(f"[DMR C2] step {k + 1}: rel per variable {np.array2string(e, precision=2)}", flush=True)
+            assert e.max() <= TOL, (k, e)
+            assert abs(dt_g - dt_r) <= 1e-12 * dt_r
+            dt_r, dt_g = cs.cfl * hmin / lr / 5.0, cs.cfl * ctx.hmin() / lg / 5.0
+        ctx.check_error()
+
+
+def test_forward_step_c5_forward_and_gradient_against_oracle(cuda):
+    import torch
+
+    from rkdg_amd.adjoint import FFSProblem, ffs_scheme
+
+    nsteps = 100
+    P = FFSProblem(60, 20)
+    prob = P.prob
+    tab = prob.tableau
+    dts = P.schedule(nsteps, device=cuda)
+    mach = torch.tensor(3.0, dtype=torch.float64, device=cuda)
+    delta = torch.from_numpy(P.perturbation()).to(cuda)
+    prob.set_bc(P.inlet_slot, P.inlet_params(mach).cpu().numpy())
+    x0 = prob.limit(P.initial_state(mach, delta))
+    xN = x0.clone()
+    J = prob.ctx.forward_solve(xN, dts, tab, prob.obj_slot)
+
+    def oracle_run(m, steps):
+        ref = RefSolver(P.mesh, 1, ffs_scheme(m))
+        xi = P.initial_state(torch.tensor(m, dtype=torch.float64), delta.cpu()).numpy()
+        x = ref.limit(xi)[0]
+        total = 0.0
+        for n in range(steps):
+            x, _ = ref.step(x, dts[n], tab)
+            total += dts[n] * ref.patch_pressure(x, 5)
+        return x, total
+
+    xr, Jr = oracle_run(3.0, nsteps)
+    e = _rel_per_var(_host(xN), xr, 4)
+    print(f"[FFS C5 {P.mesh.num_elements} P1 quads, v = 0] {nsteps} steps: state rel per variable "
+          f"{np.array2string(e, precision=2)}; J {J:.15g} (oracle {Jr:.15g})", flush=True)
+    assert e.max() <= TOL
+    assert abs(J - Jr) <= 1e-12 * abs(Jr)
+
+    h = 1e-6
+    fd = (oracle_run(3.0 + h, nsteps)[1] - oracle_run(3.0 - h, nsteps)[1]) / (2 * h)
+    mr = mach.clone().requires_grad_(True)
+    Jt = P.J(mr, delta, dts)
+    Jt.backward()
+    g = float(mr.grad)
+    print(f"[FFS C5] dJ/dMach adjoint {g:.12e}, oracle central difference (h = 1e-6) {fd:.12e}, "
+          f"rel {abs(g - fd) / abs(fd):.2e}", flush=True)
+    assert abs(g - fd) <= 1e-6 * abs(fd)

@@ -548,3 +548,222 @@ def test_char_outlet_ghost_of_the_fixed_state_is_itself():
     s1 = np.array(conservative(1.3, [0.4, -0.1], 0.8))
     gh = R.bc_state(sch, 2, 0, s1, np.array([1.0, 0.0]))
     np.testing.assert_allclose(gh, s1, rtol=1e-13, atol=1e-15)
+
+
+# ---------------------------------------------------------------- wave speeds and dissipation
+def _prim_to_cons(rho, vel, p, g=1.4):
+    return np.array(conservative(rho, list(vel), p, g), float)
+
+
+@pytest.mark.skipif(mpmath is None, reason="mpmath missing")
+@pytest.mark.parametrize("rs", [S.HLL, S.HLLC])
+def test_einfeldt_speeds_against_the_published_hlle_estimate(rs):
+    """Einfeldt's (1988) HLLE speeds: the Roe-averaged velocity
+    (sqrt(rhoL) uL + sqrt(rhoR) uR) / (sqrt(rhoL) + sqrt(rhoR)) and
+    c^2 = (sqrt(rhoL) cL^2 + sqrt(rhoR) cR^2) / (sqrt(rhoL) + sqrt(rhoR))
+    + eta2 (uR - uL)^2, eta2 = sqrt(rhoL) sqrt(rhoR) / (2 (sqrt(rhoL) + sqrt(rhoR))^2),
+    written here in that (Roe-average) form and evaluated in 40 digits. The
+    solvers return max(|S_L|, |S_R|); for a subsonic pair the HLL flux is
+    (S_R F_L - S_L F_R + S_L S_R (U_R - U_L)) / (S_R - S_L). Equal states give
+    S = u -+ c exactly, and mirroring the pair (swap sides, negate the normal
+    velocity) swaps and negates the speeds."""
+    mpmath.mp.dps = 40
+    g = 1.4
+    sch = S.Scheme(riemann=rs, wave_speeds=S.EINFELDT)
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        rL, rR = rng.uniform(0.2, 3.0, 2)
+        uL, uR = rng.uniform(-0.4, 0.4, 2)
+        vL, vR = rng.uniform(-0.5, 0.5, 2)
+        pL, pR = rng.uniform(0.3, 3.0, 2)
+        UL, UR = _prim_to_cons(rL, (uL, vL), pL, g), _prim_to_cons(rR, (uR, vR), pR, g)
+        f, lam = R.riemann_eval(sch, 2, UL, UR, np.array([1.0, 0.0]))
+        mp = mpmath.mpf
+        sL, sR = mpmath.sqrt(mp(rL)), mpmath.sqrt(mp(rR))
+        ubar = (sL * mp(uL) + sR * mp(uR)) / (sL + sR)
+        cL2, cR2 = mp(g) * mp(pL) / mp(rL), mp(g) * mp(pR) / mp(rR)
+        eta2 = sL * sR / (2 * (sL + sR) ** 2)
+        cbar = mpmath.sqrt((sL * cL2 + sR * cR2) / (sL + sR) + eta2 * (mp(uR) - mp(uL)) ** 2)
+        SL, SR = ubar - cbar, ubar + cbar
+        assert abs(lam - float(max(abs(SL), abs(SR)))) <= 4e-15 * float(cbar)
+        if rs == S.HLL and SL < 0 < SR:
+            def F(r, u, v, p):
+                E = mp(p) / (g - 1) + mp(r) * (mp(u) ** 2 + mp(v) ** 2) / 2
+                return [mp(r) * u, mp(r) * u * u + p, mp(r) * u * v, u * (E + p)], [mp(r), mp(r) * u, mp(r) * v, E]
+            FL, QL = F(rL, uL, vL, pL)
+            FR, QR = F(rR, uR, vR, pR)
+            want = [(SR * a - SL * b + SL * SR * (qr - ql)) / (SR - SL) for a, b, ql, qr in zip(FL, FR, QL, QR)]
+            np.testing.assert_allclose(f, [float(w) for w in want], rtol=0, atol=2e-14 * float(abs(FL[1]) + 1))
+    # equal states: S = u -+ c, so lambda = |u| + c
+    U = _prim_to_cons(1.3, (0.7, -0.2), 2.1, g)
+    _, lam = R.riemann_eval(sch, 2, U, U, np.array([1.0, 0.0]))
+    assert abs(lam - (0.7 + math.sqrt(g * 2.1 / 1.3))) <= 1e-15 * lam
+    # mirror symmetry
+    UL, UR = _prim_to_cons(0.9, (0.3, 0.1), 1.2, g), _prim_to_cons(1.7, (-0.1, 0.2), 0.8, g)
+    mirror = lambda U_: U_ * np.array([1.0, -1.0, 1.0, 1.0])
+    _, l1 = R.riemann_eval(sch, 2, UL, UR, np.array([1.0, 0.0]))
+    _, l2 = R.riemann_eval(sch, 2, mirror(UR), mirror(UL), np.array([1.0, 0.0]))
+    assert abs(l1 - l2) <= 1e-15 * l1
+
+
+@pytest.mark.parametrize("rs", [S.RUSANOV, S.LLF])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_dissipation_coefficient_fixed_by_a_stationary_contact(rs, dim):
+    """A stationary contact (equal p, zero velocity, different densities) has
+    equal physical fluxes (0, p n, 0) on both sides, so the mass flux is the
+    dissipation alone: -lambda/2 (rhoR - rhoL), with lambda = max(cL, cR) for
+    Rusanov (max |u| + c) and for LLF (the two-rarefaction speeds reduce to
+    -cL, +cR when p and u are equal). The returned speed is that lambda."""
+    g, p = 1.4, 1.3
+    sch = S.Scheme(riemann=rs)
+    rL, rR = 0.6, 2.2
+    z = [0.0] * dim
+    UL, UR = _prim_to_cons(rL, z, p, g), _prim_to_cons(rR, z, p, g)
+    n = np.array([0.6, 0.8] if dim == 2 else [0.48, 0.6, 0.64])
+    f, lam = R.riemann_eval(sch, dim, UL, UR, n)
+    lam_exact = max(math.sqrt(g * p / rL), math.sqrt(g * p / rR))
+    assert abs(lam - lam_exact) <= 2e-15 * lam_exact
+    assert abs(f[0] - (-0.5 * lam_exact * (rR - rL))) <= 2e-15 * lam_exact
+    np.testing.assert_allclose(f[1:1 + dim], p * n, rtol=0, atol=1e-14)
+    assert abs(f[-1]) <= 1e-14
+
+
+# ---------------------------------------------------------------- hmin (GetElementSize(i, 1))
+def _one_element(geom, X):
+    X = np.asarray(X, float)
+    return M.Mesh(geom, X[None], np.arange(len(X))[None])
+
+
+def test_hmin_box_parallelepiped_and_equilateral_triangle():
+    """GetElementSize(e, 1) is the smallest singular value of the Jacobian at
+    the element centre, taken against the 'perfect' element (the equilateral
+    triangle for triangles, the unit square/cube otherwise): a box gives its
+    smallest side, a sheared parallelepiped the smallest singular value
+    numpy.linalg.svd gives for its edge matrix, an equilateral triangle of side
+    a gives a, and a right triangle the SVD of its edge matrix times the inverse
+    of the equilateral map."""
+    sch = S.Scheme()
+    box = M.structured_quad(7, 3, 0.0, 0.7, 0.0, 0.45)
+    assert abs(R.RefSolver(box, 1, sch).hmin() - 0.1) < 1e-15
+    hx = M.structured_hex(3, 4, 5, box=((0.0, 0.6), (0.0, 1.2), (0.0, 0.5)), periodic=(False, False, False))
+    assert abs(R.RefSolver(hx, 1, sch).hmin() - 0.1) < 1e-15
+    # sheared parallelepiped: X = X0 + J xi
+    J = np.array([[0.3, 0.1, 0.05], [0.02, 0.25, -0.07], [0.04, 0.03, 0.2]])
+    ref_verts = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+    X = np.array([0.2, -0.1, 0.3]) + ref_verts @ J.T
+    h = R.RefSolver(_one_element(M.GEOM_HEX, X), 1, sch).hmin()
+    assert abs(h - np.linalg.svd(J, compute_uv=False).min()) < 1e-15
+    # parallelogram
+    J2 = np.array([[0.4, 0.15], [-0.05, 0.22]])
+    X = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], float) @ J2.T
+    h = R.RefSolver(_one_element(M.GEOM_QUAD, X), 1, sch).hmin()
+    assert abs(h - np.linalg.svd(J2, compute_uv=False).min()) < 1e-15
+    # triangles against the equilateral map E = [[1, 1/2], [0, sqrt(3)/2]]
+    a = 0.37
+    X = np.array([[0, 0], [a, 0], [a / 2, a * math.sqrt(3) / 2]])
+    assert abs(R.RefSolver(_one_element(M.GEOM_TRI, X), 1, sch).hmin() - a) < 1e-15
+    X = np.array([[0.1, 0.2], [0.5, 0.25], [0.15, 0.6]])
+    Jt = np.array([X[1] - X[0], X[2] - X[0]]).T
+    E = np.array([[1.0, 0.5], [0.0, math.sqrt(3) / 2]])
+    want = np.linalg.svd(Jt @ np.linalg.inv(E), compute_uv=False).min()
+    assert abs(R.RefSolver(_one_element(M.GEOM_TRI, X), 1, sch).hmin() - want) < 1e-15
+
+
+# ---------------------------------------------------------------- totalPressureFixedTemperatureOutlet
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("case", ["out_subsonic", "out_supersonic", "in"])
+def test_total_pressure_outlet_ghost_invariants(dim, case):
+    """The ghost of totalPressureFixedTemperatureOutlet keeps the interior
+    velocity; its specific internal energy e = p / ((gamma-1) rho) is the
+    interior one as the reference forms it, (rho E - |rho u|^2 / 2) / rho, for
+    outflow (rho u . n > 0) and R TFix / (gamma-1) for inflow
+    (so T = p / (rho R) = TFix); subsonic (M <= 1) it satisfies
+    p + rho |u|^2 / 2 = pTotal, supersonic p = pTotal (1 + (gamma-1)/2 M^2)^(-gamma/(gamma-1))
+    with the interior Mach number. The normal's components are read from index
+    0 (the reference reads 1..dim, off by one): a normal whose two readings
+    disagree about the flow direction would fail the inflow/outflow branch."""
+    g, Rg, pTot, TFix = 1.4, 0.8, 2.5, 1.9
+    sch = S.Scheme(bcs=[S.BC(1, S.BC_TOTAL_PRESSURE_OUTLET, (pTot, TFix))], gas_constant=Rg, gamma=g)
+    rho_i, p_i = 1.2, 1.5
+    c_i = math.sqrt(g * p_i / rho_i)
+    speed = {"out_subsonic": 0.4 * c_i, "out_supersonic": 1.6 * c_i, "in": 0.5 * c_i}[case]
+    n = np.array([1.0, 0.0, 0.0][:dim])
+    # velocity mostly along +n for outflow, -n for inflow, with a tangential part
+    d = np.array([1.0, -0.3, 0.2][:dim]) if case != "in" else np.array([-1.0, 0.3, -0.2][:dim])
+    vel = speed * d / np.linalg.norm(d)
+    U = _prim_to_cons(rho_i, vel, p_i, g)
+    G = R.bc_state(sch, dim, 0, U, n)
+    rho_g = G[0]
+    vel_g = G[1:1 + dim] / rho_g
+    p_g = (g - 1.0) * (G[-1] - 0.5 * rho_g * vel_g @ vel_g)
+    np.testing.assert_allclose(vel_g, vel, rtol=1e-14, atol=1e-15)
+    e_g = p_g / ((g - 1.0) * rho_g)
+    if case == "in":
+        assert abs(p_g / (rho_g * Rg) - TFix) <= 1e-14 * TFix
+    else:
+        # the reference's interior energy (rho E - |rho u|^2 / 2) / rho, with the
+        # momentum squared where rho |u|^2 belongs, as in the fixed-pressure
+        # outlet (boundary_integrator_open_total_pressure.cpp:25); it equals
+        # the specific internal energy only when rho = 1
+        m = U[1:1 + dim]
+        assert abs(e_g - (U[-1] - 0.5 * m @ m) / U[0]) <= 1e-14 * e_g
+    mach = speed / c_i
+    if mach <= 1.0:
+        assert abs(p_g + 0.5 * rho_g * vel @ vel - pTot) <= 1e-14 * pTot
+    else:
+        assert abs(p_g - pTot * (1.0 + 0.5 * (g - 1.0) * mach * mach) ** (-g / (g - 1.0))) <= 1e-14 * pTot
+
+
+# ---------------------------------------------------------------- last hope
+def _last_hope_state(m, order, dim, bad_elem):
+    """A uniform state at rest whose density in one element, along the first
+    axis, is the quadratic (or cubic) through the nodal values (0.05, 1, 1):
+    every node positive, the extrapolation to the x = 0 vertices negative."""
+    ref = R.RefSolver(m, order, S.Scheme())
+    xyz = ref.node_coords()
+    ne, nd = xyz.shape[:2]
+    rho = np.ones((ne, nd))
+    x1 = R.gl_rule(order + 1)[0]
+    for i in range(nd):
+        if order == 2:
+            k = i % 3
+            rho[bad_elem, i] = (0.05, 1.0, 1.0)[k]
+    p = np.ones((ne, nd))
+    z = np.zeros((ne, nd))
+    U = conservative(rho, [z] * dim, p)
+    return np.concatenate([u.ravel() for u in U]), x1
+
+
+@pytest.mark.parametrize("geom", ["quad", "hex"])
+def test_last_hope_resets_a_cell_with_a_negative_vertex_to_its_mean(geom):
+    """limiter_multiplier.cpp:41-95: after the multiplier, the state is checked
+    at the reference vertices and a cell with an unphysical vertex state
+    becomes its mean. With the Nowhere indicator the multiplier is the
+    identity, so only the last hope acts: the bad cell's nodes all equal its
+    mean (which is preserved), every other cell is untouched; with last_hope
+    off nothing changes."""
+    if geom == "quad":
+        m, dim = M.structured_quad(4, 3, periodic=(True, True)), 2
+    else:
+        m, dim = M.structured_hex(3, 3, 3), 3
+    bad = 4
+    x, x1 = _last_hope_state(m, 2, dim, bad)
+    # the density's quadratic through (x1, (0.05, 1, 1)) is negative at 0
+    L = np.polyfit(x1, [0.05, 1.0, 1.0], 2)
+    assert np.polyval(L, 0.0) < 0.0
+    neq = dim + 2
+    for last_hope in (1, 0):
+        sch = S.Scheme(indicator=S.IND_NOWHERE, limiter=S.LIM_MULTI, last_hope=last_hope)
+        ref = R.RefSolver(m, 2, sch)
+        mean0 = ref.means(x).reshape(neq, -1)
+        xl, _, _ = ref.limit(x)
+        u0, u1 = x.reshape(neq, m.num_elements, -1), xl.reshape(neq, m.num_elements, -1)
+        others = np.arange(m.num_elements) != bad
+        # y = 1: u <- mean + (u - mean), the identity up to rounding
+        np.testing.assert_allclose(u1[:, others], u0[:, others], rtol=0, atol=1e-14)
+        if last_hope:
+            for v in range(neq):
+                np.testing.assert_allclose(u1[v, bad], mean0[v, bad], rtol=0, atol=1e-15)
+            np.testing.assert_allclose(ref.means(xl).reshape(neq, -1), mean0, rtol=0, atol=1e-15)
+        else:
+            np.testing.assert_allclose(u1, u0, rtol=0, atol=1e-14)
