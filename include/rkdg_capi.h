@@ -150,6 +150,16 @@ int rkdg_limit(rkdg_ctx* ctx, double* x, int8_t* troubled);
 /* One RK step, Butcher tableau consumed with the packed a[l++] rule. s <= 4. */
 int rkdg_step(rkdg_ctx* ctx, double* x, double t, double dt, int s, const double* a,
               const double* b, const double* c, double* lambda_max);
+/* rkdg_step that allocates nothing. No counterpart in the reference: it is
+ * for callers that cannot allocate device or pinned memory during a step (a
+ * loop under HIP graph capture, a tight serving loop). The stage vectors live
+ * in a static per-device scratch array of 128 MiB (2^24 doubles), so the
+ * state may hold at most 2^22 doubles; larger contexts get RKDG_E_ARG. The
+ * last stage's wave speed is copied to lambda_dev, a DEVICE double (nullable),
+ * and there is no host synchronisation: deferred errors are read with
+ * rkdg_check_error. Calls that overlap on one GPU share the array and race. */
+int rkdg_step_noalloc(rkdg_ctx* ctx, double* x, double t, double dt, int s, const double* a,
+                      const double* b, const double* c, double* lambda_dev);
 /* Element means, layout [v*NE + e] (DEVICE pointer). */
 int rkdg_means(rkdg_ctx* ctx, const double* x, double* means);
 int rkdg_hmin(rkdg_ctx* ctx, double* hmin);

@@ -62,7 +62,7 @@ EXPORTS = [
     "rkdg_create_ex", "rkdg_batch_create", "rkdg_batch_offsets", "rkdg_batch_init",
     "rkdg_batch_advance", "rkdg_batch_status", "rkdg_batch_set_options", "rkdg_tune_grid",
     "rkdg_set_bc_params", "rkdg_rhs_adjoint", "rkdg_limit_adjoint", "rkdg_objective",
-    "rkdg_step_adjoint", "rkdg_forward_solve", "rkdg_adjoint_solve",
+    "rkdg_step_adjoint", "rkdg_forward_solve", "rkdg_adjoint_solve", "rkdg_step_noalloc",
 ]
 
 # include/rkdg_selftest.h (host-only)
@@ -97,6 +97,8 @@ def lib():
     L.rkdg_limit.argtypes = [P, dp, dp]
     L.rkdg_step.argtypes = [P, dp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double),
                             C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.rkdg_step_noalloc.argtypes = [P, dp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double), C.POINTER(C.c_double), dp]
     L.rkdg_means.argtypes = [P, dp, dp]
     L.rkdg_hmin.argtypes = [P, C.POINTER(C.c_double)]
     L.rkdg_node_coords.argtypes = [P, C.POINTER(C.c_double)]
@@ -326,6 +328,14 @@ class Context:
         _check(lib().rkdg_step(self.h, _ptr(x), t, dt, s, a, b, c,
                                C.byref(lam) if want_lambda else None))
         return lam.value
+
+    def step_noalloc(self, x, dt, tableau, lam_dev=None, t=0.0):
+        """rkdg_step_noalloc: the step with its stage vectors in the static
+        scratch array; the wave speed goes to the device double lam_dev."""
+        self._st(x, "x")
+        check_buffer(lam_dev, 1, "lam_dev", nullable=True)
+        s, a, b, c = _tableau_arrays(tableau)
+        _check(lib().rkdg_step_noalloc(self.h, _ptr(x), t, dt, s, a, b, c, _ptr(lam_dev)))
 
     def hmin(self):
         v = C.c_double(0.0)

@@ -157,6 +157,7 @@ struct rkdg_ctx {
   double* d_dtab = nullptr;      // dense reference tables
   std::vector<double> ref_nodes; // reference node coordinates [nd][dim] (dense path)
   double* d_stage[4] = {nullptr, nullptr, nullptr, nullptr};
+  double* scratch = nullptr;     // this device's static scratch array (rkdg_step_noalloc)
   std::vector<double> coords;  // [NE][nv][dim]
   std::vector<double> geo_h;   // [g][NE]
   int64_t n_int = 0, n_bdr = 0;
@@ -1388,6 +1389,10 @@ int rkdg_create_ex(const rkdg_mesh_desc* mesh, const rkdg_scheme_desc* scheme, i
   rkdg_ctx* c = new rkdg_ctx();
   c->force_dense = (flags & RKDG_CREATE_DENSE) ? 1 : 0;
   int rc = build_ctx(c, mesh, scheme);
+  if (rc == RKDG_OK) {
+    const hipError_t se = noalloc_scratch(&c->scratch);
+    if (se != hipSuccess) rc = fail(RKDG_E_HIP, std::string("scratch array: ") + hipGetErrorString(se));
+  }
   if (rc != RKDG_OK) {
     std::string keep = g_last_error;
     free_ctx(c);
@@ -1472,6 +1477,33 @@ int rkdg_step(rkdg_ctx* c, double* x, double t, double dt, int s, const double* 
   rc = step_impl(c, x, dt, s, a, b, c->d_stage, lambda_max != nullptr);
   if (rc) return rc;
   if (lambda_max) return read_lambda(c, s - 1, lambda_max);
+  return RKDG_OK;
+}
+
+// rkdg_step without any allocation: the stage vectors live in this device's
+// static scratch array (rkdg_noalloc.hip), located once at rkdg_create. The
+// wave speed stays on the device (copied to lambda_dev, a device double, when
+// given) and no host synchronisation or pinned staging happens, so a caller
+// that may not allocate (a loop under HIP graph capture, a serving loop) can
+// call it. Calls that overlap on one GPU share the array and race.
+int rkdg_step_noalloc(rkdg_ctx* c, double* x, double t, double dt, int s, const double* a, const double* b,
+                      const double* cc, double* lambda_dev) {
+  (void)t;
+  (void)cc;
+  if (!c || !x || !a || !b) return fail(RKDG_E_ARG, "null argument");
+  if (s < 1 || s > 4) return fail(RKDG_E_ARG, "1 <= s <= 4 stages supported");
+  if (!c->scratch) return fail(RKDG_E_HIP, "no scratch array on this context's device");
+  const int64_t sz = (int64_t)c->neq * c->ndofs;
+  const int64_t nvec = 4;
+  if (sz > (int64_t)(kScratchDoubles / nvec))
+    return fail(RKDG_E_ARG, "state of " + std::to_string(sz) + " doubles exceeds the allocation-free limit of " +
+                                std::to_string(kScratchDoubles / nvec) + " (4 stage vectors in " +
+                                std::to_string(kScratchDoubles * 8 >> 20) + " MiB of static scratch)");
+  double* W[4];
+  for (int i = 0; i < 4; i++) W[i] = c->scratch + i * sz;
+  int rc = step_impl(c, x, dt, s, a, b, W, lambda_dev != nullptr);
+  if (rc) return rc;
+  if (lambda_dev) HIP_TRY(launch_copy(lambda_dev, c->d_lam + (int64_t)(s - 1) * c->count, 1, c->stream));
   return RKDG_OK;
 }
 

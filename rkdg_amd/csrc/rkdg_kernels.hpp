@@ -183,6 +183,11 @@ hipError_t launch_tensor_limit(int dim, int p, const LimitArgs& a, const SchemeP
 hipError_t launch_tensor_means(int dim, int p, const MeansArgs& a, const TensorTables* t,
                                hipStream_t s);
 hipError_t launch_fill(double* p, double v, int64_t n, hipStream_t s);
+// the static scratch array of rkdg_step_noalloc (rkdg_noalloc.hip): its
+// address on the current device, looked up once per device
+constexpr int64_t kScratchDoubles = int64_t(1) << 24;
+constexpr int kMaxDevices = 64;
+hipError_t noalloc_scratch(double** out);
 hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
 bool tensor_supported(int dim, int p);
 int tensor_elems_per_block(int dim, int p);  // elements per workgroup of the tensor RHS
