@@ -146,30 +146,38 @@ def host_cpu_info():
             "physical_cores": len(cores) or None, "mem_gib": mem_gb}
 
 
-def cpu_baseline(n, steps, threads):
+def cpu_baseline(n, steps, threads, warmup=1, mults=5):
+    """The oracle on the same workload at n^3: ``steps`` SSP-RK3 steps after
+    ``warmup`` steps, then the median of ``mults`` Mult calls. Returns
+    (DOF-stage/s, seconds of the timed steps, elements); the Mult median and
+    the setup time are left on the function object."""
     from oracle.refcpu import RefSolver, lib
     from rkdg_amd import mesh as M
     from rkdg_amd import scheme as S
 
     lib().ref_set_threads(threads)
+    t0 = time.perf_counter()
     m = M.structured_hex(n, n, n)
     ref = RefSolver(m, 2, scheme(), threads=threads)
     x = pulse_state(ref.node_coords())
+    cpu_baseline.setup_s = time.perf_counter() - t0
     _, lam = ref.face_mult(x)
     dt = 0.2 * ref.hmin() / lam / 5.0
-    x, lam = ref.step(x, dt, S.TABLEAUS["ssprk3"])  # warm-up
+    for _ in range(warmup):
+        x, lam = ref.step(x, dt, S.TABLEAUS["ssprk3"])
     t0 = time.perf_counter()
     for _ in range(steps):
         x, lam = ref.step(x, dt, S.TABLEAUS["ssprk3"])
     el = time.perf_counter() - t0
     dofs = 5 * m.num_elements * 27
-    # the right-hand side alone: median of 5 Mult calls after the warm-up above
+    # the right-hand side alone: median of the Mult calls after the steps above
     mt = []
-    for _ in range(5):
+    for _ in range(mults):
         t1 = time.perf_counter()
         ref.mult(x)
         mt.append(time.perf_counter() - t1)
-    cpu_baseline.mult_median_s = sorted(mt)[2]
+    cpu_baseline.mult_median_s = sorted(mt)[len(mt) // 2]
+    del ref
     return dofs * 3 * steps / el, el, m.num_elements
 
 
@@ -253,6 +261,66 @@ def ensemble_bench(dev, count=256, steps=20, warmup=3, seed=12345):
     print(f"[ensemble] batched {out['batched_ms_per_step']:.2f} ms/step, sequential "
           f"{out['sequential_ms_per_step']:.2f} ms/step, ratio {out['ratio']:.2f}, "
           f"identical {identical}, grid {best}", flush=True)
+    return out
+
+
+def dmr_bench(dev, reps=20, steps=5):
+    """The double Mach reflection mesh as configured (about 120k P2 triangles
+    from a jittered Delaunay grid of 490x123 cells; HLLC with Einfeldt speeds,
+    Michalak, SSP-RK3, CFL 0.1): the plain right-hand side timed with HIP
+    events on the context's stream (reps launches) against its roofline, and
+    steps of the reference dt loop. The shock is blended over one cell width
+    (with the discontinuous nodal data the scheme stops on an unphysical face
+    state in its first step, as the reference does)."""
+    import torch
+
+    from rkdg_amd import cases
+    from rkdg_amd import scheme as S
+    from rkdg_amd.capi import Context
+
+    t0 = time.perf_counter()
+    cs = cases.double_mach(shock_width=4.0 / 490)
+    ctx = Context(cs.mesh, cs.order, cs.scheme)
+    ne = cs.mesh.num_elements
+    x = torch.from_numpy(cs.initial_state(ctx.node_coords())).to(dev)
+    ctx.limit(x)
+    dof = ctx.info.state_size
+    y = torch.empty_like(x)
+    ctx.rhs(x, y, want_lambda=False)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    ctx.profile(True)
+    for _ in range(reps):
+        ctx.rhs(x, y, want_lambda=False)
+    ms, n, _, _ = ctx.profile_read()
+    ctx.profile(False)
+    avg = ms / max(1, n)
+    # 16 B per DOF + the affine-triangle record (adj J, det J, 3 area normals:
+    # 11 doubles) and the neighbour/face-code tables (3 faces x 2 int32)
+    nbytes = 16 * dof + (11 * 8 + 24) * ne
+    tab = S.TABLEAUS["ssprk3"]
+    hmin = ctx.hmin()
+    lam = ctx.face_lambda(x)
+    dt = cs.cfl * hmin / lam / 5.0
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        lam = ctx.step(x, dt, tab)
+        dt = cs.cfl * hmin / lam / 5.0
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t1
+    ctx.check_error()
+    ctx.close()
+    out = {"workload": f"double Mach reflection mesh, {ne} P2 triangles ({dof} DOF), HLLC + Einfeldt, "
+                       f"Michalak, SSP-RK3, CFL 0.1",
+           "elements": ne, "dof": dof, "rhs_avg_launch_ms": avg, "rhs_launches": n,
+           "rhs_algorithmic_bytes": nbytes, "rhs_achieved_gbs": nbytes / (avg * 1e-3) / 1e9,
+           "rhs_frac_of_hbm": nbytes / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "rhs_dof_per_s": dof / (avg * 1e-3),
+           "steps": steps, "ms_per_step": el * 1e3 / steps, "dof_stage_per_s": dof * 3 * steps / el,
+           "setup_s": setup}
+    print(f"[dmr] {ne} P2 triangles: RHS {avg:.3f} ms ({out['rhs_achieved_gbs']:.0f} GB/s, "
+          f"{out['rhs_frac_of_hbm'] * 100:.1f} % of HBM), step {out['ms_per_step']:.3f} ms", flush=True)
     return out
 
 
@@ -377,8 +445,12 @@ def main():
     ap.add_argument("--n", type=int, default=96)
     ap.add_argument("--cfl", type=float, default=0.2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=48)
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-n", type=int, default=96,
+                    help="mesh of the oracle's timed sample (96: the benchmark mesh itself, about 40 GB of host "
+                         "memory for the oracle's CSR Aflux and dense inverse masses; 64 when the host has less)")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-n2", type=int, default=48, help="second oracle sample (0: none)")
+    ap.add_argument("--cpu-steps2", type=int, default=8)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB read + 1 GiB write elementwise kernel (3 launches), "
@@ -386,6 +458,7 @@ def main():
     ap.add_argument("--ensemble-only", action="store_true")
     ap.add_argument("--ensemble-count", type=int, default=256)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-dmr", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--oracle-steps", type=int, default=200)
@@ -459,6 +532,18 @@ def main():
     ctx.profile(False)
     ctx.check_error()
     lam = ctx.face_lambda(x)
+    # the plain right-hand side (rkdg_rhs, FE_Evolution::Mult): read x, write y,
+    # HIP events on the context's stream around each of 10 launches
+    y = torch.empty_like(x)
+    ctx.rhs(x, y, want_lambda=False)
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    for _ in range(10):
+        ctx.rhs(x, y, want_lambda=False)
+    plain_ms, plain_n, _, _ = ctx.profile_read()
+    ctx.profile(False)
+    del y
+    plain_avg_ms = plain_ms / max(1, plain_n)
 
     copy_gbs = copy_bandwidth(dev)
     dofs = info.state_size
@@ -470,39 +555,58 @@ def main():
     lim_avg_ms = lim_ms / max(1, lim_n)
     achieved = bytes_per_launch / (rhs_avg_ms * 1e-3) / 1e9
     lim_bytes = 2 * 8 * dofs + 8 * 5 * info.num_elements * 7
+    rhs_bytes = 16 * dofs + (80 + 48) * info.num_elements
     traffic = measured_traffic(n)
     print(f"[bench] stage kernel avg {rhs_avg_ms:.3f} ms over {rhs_n} launches "
           f"({achieved:.0f} GB/s algorithmic; plain copy {copy_gbs:.0f} GB/s), limiter avg {lim_avg_ms:.3f} ms "
           f"({lim_bytes / (lim_avg_ms * 1e-3) / 1e9:.0f} GB/s), step {ms_step:.3f} ms, "
-          f"lambda {lam:.6f}", flush=True)
+          f"lambda {lam:.6f}; plain RHS {plain_avg_ms:.3f} ms ({rhs_bytes / (plain_avg_ms * 1e-3) / 1e9:.0f} GB/s, "
+          f"{rhs_bytes / (plain_avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS * 100:.1f} % of HBM)", flush=True)
 
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        cv, cel, cne = cpu_baseline(args.cpu_n, args.cpu_steps, threads)
-        cpu_mult_s = cpu_baseline.mult_median_s
+        host = host_cpu_info()
+        cn = args.cpu_n
+        if cn == 96 and (host["mem_gib"] or 0) < 64:
+            cn = 64  # the oracle's 96^3 structures need about 40 GB
+        cv, cel, cne = cpu_baseline(cn, args.cpu_steps, threads, warmup=0)
+        cpu_mult_s, cpu_setup_s = cpu_baseline.mult_median_s, cpu_baseline.setup_s
+        print(f"[bench] cpu baseline at {cn}^3: {cv:.3e} DOF-stage/s ({cel:.1f}s, {threads} threads), Mult "
+              f"median {cpu_mult_s:.2f}s, setup {cpu_setup_s:.0f}s", flush=True)
+        second = None
+        if args.cpu_n2:
+            cv2, cel2, cne2 = cpu_baseline(args.cpu_n2, args.cpu_steps2, threads)
+            second = {"value": cv2, "elements": cne2, "mult_median_s": cpu_baseline.mult_median_s,
+                      "sample": f"{args.cpu_n2}^3 elements, {args.cpu_steps2} SSP-RK3 steps after 1 warm-up "
+                                f"step, {cel2:.1f} s, OpenMP {threads} threads"}
         cv1, cel1, cne1 = cpu_baseline(24, 1, 1)
         from oracle.refcpu import lib as _oracle_lib
 
         _oracle_lib().ref_set_threads(threads)
         cpu = {"value": cv, "unit": "DOF-stage updates/s", "cores": threads, "kind": "port",
                "sample": f"oracle (CSR Aflux, dense inverse mass, per-point shape evaluation) on the "
-                         f"same workload at {args.cpu_n}^3 elements ({cne} elements), "
-                         f"{args.cpu_steps} SSP-RK3 steps after 1 warm-up step, {cel:.1f} s, "
-                         f"OpenMP {threads} threads",
-               "host": host_cpu_info(),
+                         f"benchmark workload at {cn}^3 elements ({cne} elements, {5 * cne * 27} DOF): "
+                         f"{args.cpu_steps} SSP-RK3 step(s) of 3 stages (limiter + Mult + update), "
+                         f"{cel:.1f} s, OpenMP {threads} threads; then the median of 5 Mult calls",
+               "host": host,
                "mult_median_s": cpu_mult_s,
                "mult_dof_per_s": 5 * cne * 27 / cpu_mult_s,
+               "setup_s": cpu_setup_s,
+               "second_sample": second,
                "value_1_thread": cv1,
                "sample_1_thread": f"24^3 elements ({cne1}), 1 step after 1 warm-up step, {cel1:.1f} s"}
         print(f"[bench] cpu baseline {cv:.3e} DOF-stage/s ({cel:.1f}s, {threads} threads); "
               f"1 thread {cv1:.3e} ({cel1:.1f}s)", flush=True)
 
     ens = None
-    if not (args.no_ensemble and args.no_adjoint):
+    if not (args.no_ensemble and args.no_adjoint and args.no_dmr):
         del x
         ctx.close()
         torch.cuda.empty_cache()
+    dmr = None
+    if not args.no_dmr:
+        dmr = dmr_bench(dev)
     if not args.no_ensemble:
         ens = ensemble_bench(dev, args.ensemble_count)
     adj = None
@@ -548,8 +652,30 @@ def main():
                 "peak_tflops": FP64_PEAK_TFLOPS,
                 "frac": RHS_FLOP_PER_ELEMENT * info.num_elements / (rhs_avg_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
             },
+            # the plain rkdg_rhs (FE_Evolution::Mult) the north star's 60 % target
+            # names: 16 B per DOF (read x, write y) + geometry 80 B and
+            # connectivity 48 B per element, timed with HIP events
+            "rhs": {
+                "kernel": "tensor_rhs_kernel<3,2> as rkdg_rhs (y = RHS(x))",
+                "avg_launch_ms": plain_avg_ms,
+                "algorithmic_bytes_per_launch": rhs_bytes,
+                "achieved": rhs_bytes / (plain_avg_ms * 1e-3) / 1e9,
+                "frac": rhs_bytes / (plain_avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            },
+            # the whole step against the stage ceilings: 24 B per DOF-stage (the
+            # low-storage stage of the survey) and the Butcher form as run here
+            # (12 stage vectors + 6 limiter vectors per 3-stage step = 48 B)
+            "step": {
+                "dof_stage_per_s": value,
+                "ceiling_24B": HBM_PEAK_GBS * 1e9 / 24.0,
+                "frac_of_24B_ceiling": value / (HBM_PEAK_GBS * 1e9 / 24.0),
+                "ceiling_butcher_48B": HBM_PEAK_GBS * 1e9 / 48.0,
+                "frac_of_butcher_ceiling": value / (HBM_PEAK_GBS * 1e9 / 48.0),
+                "limiter_avg_ms": lim_avg_ms,
+            },
         },
         "cpu_baseline": cpu,
+        "dmr": dmr,
         "ensemble": ens,
         "adjoint": adj,
     }

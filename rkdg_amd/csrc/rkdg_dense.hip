@@ -153,8 +153,8 @@ __device__ __forceinline__ void dense_rhs_block(const RhsArgs& A, const SchemePa
       if (!bad) {
         const int64_t other = nb >= 0 ? (int64_t)nb : -1;
         const bool mine = factor < 0.0;
-        flag_error(A.err, mine ? e : other, mine ? other : e, f,
-                   nonfinite<NEQ>(um, up) ? kErrNonFinite : kErrUnphysical);
+        const bool nf = nonfinite_nodes(A.x, NDOFS, e, ND, NEQ) || nonfinite_nodes(A.x, NDOFS, other, ND, NEQ);
+        flag_error(A.err, mine ? e : other, mine ? other : e, f, nf ? kErrNonFinite : kErrUnphysical);
       }
       bad = true;
     } else {

@@ -163,12 +163,19 @@ __device__ __forceinline__ void flag_error(long long* err, int64_t e1, int64_t e
     err[3] = lf;
   }
 }
-// true if any of the two traces holds a NaN or an infinity (x - x is not 0)
-template <int NEQ>
-__device__ __forceinline__ bool nonfinite(const double* a, const double* b) {
+// Error path only: true if a nodal value of element e (of a byNODES state with
+// ndofs = NE * nd scalar nodes) is a NaN or an infinity (x - x is not 0).
+// Reading the nodes from memory here, instead of testing the traces the
+// Riemann solver saw, keeps the check out of the hot path's registers.
+__device__ __attribute__((noinline)) bool nonfinite_nodes(const double* x, int64_t ndofs, int64_t e, int nd,
+                                                           int neq) {
+  if (e < 0) return false;
   bool r = false;
-#pragma unroll
-  for (int v = 0; v < NEQ; v++) r = r || !(a[v] - a[v] == 0.0) || !(b[v] - b[v] == 0.0);
+  for (int v = 0; v < neq; v++)
+    for (int i = 0; i < nd; i++) {
+      const double a = x[v * ndofs + e * nd + i];
+      r = r || !(a - a == 0.0);
+    }
   return r;
 }
 

@@ -454,9 +454,11 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
         }
       }
       if (!ok || lam < 0.0) {
-        if (!bad)
+        if (!bad) {
+          const bool nf = nonfinite_nodes(A.x, NDOFS, e, ND, NEQ) || nonfinite_nodes(A.x, NDOFS, nb, ND, NEQ);
           flag_error(A.err, factor < 0.0 ? e : nb, factor < 0.0 ? (nb >= 0 ? nb : -1) : e, lf,
-                     nonfinite<NEQ>(um, up) ? kErrNonFinite : kErrUnphysical);
+                     nf ? kErrNonFinite : kErrUnphysical);
+        }
         bad = true;
       } else {
         lmax = lam > lmax ? lam : lmax;
