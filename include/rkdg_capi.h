@@ -132,6 +132,7 @@ typedef struct {
   int64_t state_size;                  /* num_equations*num_dofs */
   int64_t num_interior_faces, num_boundary_faces;
   int affine;                          /* 1 if every element is affine */
+  int two_colour;                      /* 1 if the right-hand side runs as a solver/receiver pair of kernels */
 } rkdg_info;
 
 int rkdg_create(const rkdg_mesh_desc* mesh, const rkdg_scheme_desc* scheme, rkdg_ctx** out);
@@ -184,6 +185,12 @@ int rkdg_profile_read(rkdg_ctx* ctx, double* rhs_ms, int64_t* rhs_launches, doub
  * (the kernels a batch uses, so a single instance can be compared with its
  * batch bit for bit). Same counterpart as rkdg_create. */
 #define RKDG_CREATE_DENSE 1
+/* RKDG_CREATE_TWO_COLOUR: on tensor meshes whose face graph is two-colourable
+ * and has no boundary faces (periodic meshes of even extents), run the
+ * right-hand side as a solver/receiver pair of kernels that solves each face
+ * once; the receivers' share of the face fluxes goes through a buffer of the
+ * size of one state vector. Other meshes keep the one-kernel form. */
+#define RKDG_CREATE_TWO_COLOUR 2
 int rkdg_create_ex(const rkdg_mesh_desc* mesh, const rkdg_scheme_desc* scheme, int flags, rkdg_ctx** out);
 
 /* Batched ensembles. No counterpart in the reference, which runs one case per

@@ -50,7 +50,7 @@ class Info(C.Structure):
         ("dim", C.c_int), ("num_equations", C.c_int), ("nodes_per_element", C.c_int),
         ("faces_per_element", C.c_int), ("num_elements", C.c_int64), ("num_dofs", C.c_int64),
         ("state_size", C.c_int64), ("num_interior_faces", C.c_int64),
-        ("num_boundary_faces", C.c_int64), ("affine", C.c_int),
+        ("num_boundary_faces", C.c_int64), ("affine", C.c_int), ("two_colour", C.c_int),
     ]
 
 
@@ -73,6 +73,7 @@ SELFTEST_EXPORTS = [
 ]
 
 RKDG_CREATE_DENSE = 1
+RKDG_CREATE_TWO_COLOUR = 2
 
 _lib = None
 
@@ -257,16 +258,19 @@ def _tableau_arrays(tableau):
 
 class Context:
     """One rkdg_ctx: a flattened mesh + scheme resident on the GPU. dense=True
-    runs quadrilaterals on the dense bilinear kernels (RKDG_CREATE_DENSE)."""
+    runs quadrilaterals on the dense bilinear kernels (RKDG_CREATE_DENSE);
+    two_colour=True runs the right-hand side as a solver/receiver pair of
+    kernels where the mesh allows it (RKDG_CREATE_TWO_COLOUR)."""
 
-    def __init__(self, mesh, order: int, scheme: S.Scheme, dense: bool = False, _handle=None):
+    def __init__(self, mesh, order: int, scheme: S.Scheme, dense: bool = False, _handle=None,
+                 two_colour: bool = False):
         L = lib()
         if _handle is None:
             md, self._keep = _mesh_desc(mesh, order)
             self._sd = scheme_desc(scheme)
             h = C.c_void_p()
-            _check(L.rkdg_create_ex(C.byref(md), C.byref(self._sd), RKDG_CREATE_DENSE if dense else 0,
-                                    C.byref(h)))
+            flags = (RKDG_CREATE_DENSE if dense else 0) | (RKDG_CREATE_TWO_COLOUR if two_colour else 0)
+            _check(L.rkdg_create_ex(C.byref(md), C.byref(self._sd), flags, C.byref(h)))
         else:
             h = _handle
         self.h = h

@@ -152,6 +152,15 @@ struct rkdg_ctx {
   double* d_means = nullptr;
   long long* h_pin = nullptr;    // pinned host words: device error word [4], then the wave speeds [count]
   int* d_order = nullptr;        // tensor contexts: work blocks in Morton order of their centroids
+  // two-coloured tensor meshes (bipartite face graph, no boundary faces): the
+  // solver and receiver element lists (each in Morton order of the element
+  // centroids), each element's receiver slot and the receivers' face fluxes
+  int rb = 0;
+  int64_t n_solve = 0, n_recv = 0;
+  int* d_elist_solve = nullptr;
+  int* d_elist_recv = nullptr;
+  int* d_fslot = nullptr;
+  double* d_fbuf = nullptr;
   TensorTables* d_tb = nullptr;
   int dense = 0;                 // 1: dense small-element kernels (triangles, bilinear quads)
   double* d_dtab = nullptr;      // dense reference tables
@@ -170,6 +179,7 @@ struct rkdg_ctx {
   // batched ensembles: count disjoint instances in one union mesh
   int count = 1;
   int force_dense = 0;
+  int two_colour = 0;            // the solver/receiver form where the mesh allows it
   int grid_cap = 0;              // persistent grid of the dense right-hand-side kernel
   std::vector<int64_t> inst_off; // [count+1] element offsets
   int* d_inst = nullptr;         // [NE] element -> instance
@@ -219,6 +229,10 @@ void free_ctx(rkdg_ctx* c) {
   if (c->h_pin) (void)hipHostFree(c->h_pin);
   dfree(c->d_means);
   dfree(c->d_order);
+  dfree(c->d_elist_solve);
+  dfree(c->d_elist_recv);
+  dfree(c->d_fslot);
+  dfree(c->d_fbuf);
   dfree(c->d_tb);
   dfree(c->d_dtab);
   for (auto& p : c->d_stage) dfree(p);
@@ -653,6 +667,81 @@ int build_dense(rkdg_ctx* c, const rkdg_mesh_desc* m) {
   return RKDG_OK;
 }
 
+// Two-colouring of the face graph of a tensor mesh without boundary faces
+// (breadth-first, every face joining the two colours). On success the lists of
+// the tensor kernel's two-colour form are built and uploaded; a mesh with an
+// odd cycle (e.g. a periodic direction of odd length) keeps the one-kernel form.
+int build_two_colour(rkdg_ctx* c, const std::vector<int>& nbr) {
+  const int64_t NE = c->NE;
+  const int nf = c->nf;
+  std::vector<int8_t> col((size_t)NE, -1);
+  std::vector<int64_t> queue;
+  queue.reserve((size_t)NE);
+  for (int64_t s0 = 0; s0 < NE; s0++) {
+    if (col[s0] >= 0) continue;
+    col[s0] = 0;
+    queue.clear();
+    queue.push_back(s0);
+    for (size_t q = 0; q < queue.size(); q++) {
+      const int64_t e = queue[q];
+      for (int lf = 0; lf < nf; lf++) {
+        const int n = nbr[(size_t)lf * NE + e];
+        if (n < 0) return RKDG_OK;  // a boundary face: not this form
+        if (col[n] < 0) {
+          col[n] = (int8_t)(1 - col[e]);
+          queue.push_back(n);
+        } else if (col[n] == col[e]) {
+          return RKDG_OK;  // odd cycle
+        }
+      }
+    }
+  }
+  // Morton order of the element centroids (as the work blocks of the one-kernel form)
+  const int dim = c->dim, nv = c->nv;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  std::vector<double> cen((size_t)NE * dim, 0.0);
+  for (int64_t e = 0; e < NE; e++)
+    for (int i = 0; i < dim; i++) {
+      double m = 0.0;
+      for (int v = 0; v < nv; v++) m += c->coords[(e * nv + v) * dim + i];
+      cen[e * dim + i] = m / nv;
+      lo[i] = std::min(lo[i], cen[e * dim + i]);
+      hi[i] = std::max(hi[i], cen[e * dim + i]);
+    }
+  const int bits = dim == 3 ? 10 : 16;
+  const double cells = (double)((1u << bits) - 1);
+  std::vector<std::pair<uint64_t, int>> key[2];
+  for (int64_t e = 0; e < NE; e++) {
+    uint64_t code = 0;
+    for (int i = 0; i < dim; i++) {
+      const double span = hi[i] > lo[i] ? hi[i] - lo[i] : 1.0;
+      const uint64_t q = (uint64_t)((cen[e * dim + i] - lo[i]) / span * cells + 0.5);
+      for (int k = 0; k < bits; k++) code |= ((q >> k) & 1ull) << (k * dim + i);
+    }
+    key[col[e]].push_back({code, (int)e});
+  }
+  for (auto& k : key) std::stable_sort(k.begin(), k.end());
+  std::vector<int> solve(key[0].size()), recv(key[1].size()), slot((size_t)NE, -1);
+  for (size_t i = 0; i < key[0].size(); i++) solve[i] = key[0][i].second;
+  for (size_t i = 0; i < key[1].size(); i++) {
+    recv[i] = key[1][i].second;
+    slot[recv[i]] = (int)i;
+  }
+  c->n_solve = (int64_t)solve.size();
+  c->n_recv = (int64_t)recv.size();
+  const int N = c->p + 1;
+  const int64_t fb = (int64_t)nf * c->neq * (dim == 3 ? N * N : N);
+  HIP_TRY(hipMalloc(&c->d_elist_solve, sizeof(int) * std::max<size_t>(1, solve.size())));
+  HIP_TRY(hipMemcpy(c->d_elist_solve, solve.data(), sizeof(int) * solve.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMalloc(&c->d_elist_recv, sizeof(int) * std::max<size_t>(1, recv.size())));
+  HIP_TRY(hipMemcpy(c->d_elist_recv, recv.data(), sizeof(int) * recv.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMalloc(&c->d_fslot, sizeof(int) * NE));
+  HIP_TRY(hipMemcpy(c->d_fslot, slot.data(), sizeof(int) * NE, hipMemcpyHostToDevice));
+  HIP_TRY(hipMalloc(&c->d_fbuf, sizeof(double) * std::max<int64_t>(1, c->n_recv * fb)));
+  c->rb = 1;
+  return RKDG_OK;
+}
+
 int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
   if (m->geom != RKDG_GEOM_QUAD && m->geom != RKDG_GEOM_HEX && m->geom != RKDG_GEOM_TRI)
     return fail(RKDG_E_ARG, "unknown geometry");
@@ -947,6 +1036,10 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
   }
   HIP_TRY(hipMalloc(&c->d_tb, sizeof(TensorTables)));
   HIP_TRY(hipMemcpy(c->d_tb, &c->tb, sizeof(TensorTables), hipMemcpyHostToDevice));
+  if (c->n_bdr == 0 && c->two_colour) {
+    int rc = build_two_colour(c, nbr);
+    if (rc) return rc;
+  }
   return RKDG_OK;
 }
 
@@ -991,8 +1084,27 @@ hipError_t rhs_launch(rkdg_ctx* c, const RhsArgs& a0) {
     a.means_of = 0;
   }
   auto pr = prof_begin(c, c->ev_rhs);
-  hipError_t e = c->dense ? launch_dense_rhs(c->geom, c->p, a, c->sp, c->d_dtab, c->n_bdr > 0, c->stream)
-                          : launch_tensor_rhs(c->dim, c->p, a, c->sp, c->d_tb, c->tb, c->n_bdr > 0, c->stream);
+  hipError_t e;
+  if (c->dense) {
+    e = launch_dense_rhs(c->geom, c->p, a, c->sp, c->d_dtab, c->n_bdr > 0, c->stream);
+  } else if (c->rb) {
+    // two-colour form: the solver elements (every face flux once, the
+    // receiver's share to fbuf), then the receivers (their share read back)
+    a.fslot = c->d_fslot;
+    a.fbuf = c->d_fbuf;
+    a.role = 1;
+    a.elist = c->d_elist_solve;
+    a.nlist = c->n_solve;
+    e = launch_tensor_rhs(c->dim, c->p, a, c->sp, c->d_tb, c->tb, false, c->stream);
+    if (e == hipSuccess && !a.face_only) {
+      a.role = 2;
+      a.elist = c->d_elist_recv;
+      a.nlist = c->n_recv;
+      e = launch_tensor_rhs(c->dim, c->p, a, c->sp, c->d_tb, c->tb, false, c->stream);
+    }
+  } else {
+    e = launch_tensor_rhs(c->dim, c->p, a, c->sp, c->d_tb, c->tb, c->n_bdr > 0, c->stream);
+  }
   prof_end(c, pr);
   return e;
 }
@@ -1388,6 +1500,7 @@ int rkdg_create_ex(const rkdg_mesh_desc* mesh, const rkdg_scheme_desc* scheme, i
   *out = nullptr;
   rkdg_ctx* c = new rkdg_ctx();
   c->force_dense = (flags & RKDG_CREATE_DENSE) ? 1 : 0;
+  c->two_colour = (flags & RKDG_CREATE_TWO_COLOUR) ? 1 : 0;
   int rc = build_ctx(c, mesh, scheme);
   if (rc == RKDG_OK) {
     const hipError_t se = noalloc_scratch(&c->scratch);
@@ -1423,6 +1536,7 @@ int rkdg_get_info(rkdg_ctx* c, rkdg_info* info) {
   info->num_interior_faces = c->n_int;
   info->num_boundary_faces = c->n_bdr;
   info->affine = c->affine;
+  info->two_colour = c->rb;
   return RKDG_OK;
 }
 

@@ -62,6 +62,17 @@ struct RhsArgs {
   // the EPB elements from block * EPB), in an order that keeps face neighbours
   // close in time so their gathered states are still in L2
   const int* order;
+  // Two-colour (red-black) form of the tensor kernel, for meshes whose face
+  // graph is bipartite and has no boundary faces: every face joins a "solver"
+  // element and a "receiver" element. role 1 runs over the solver elements,
+  // solves each of their faces once and writes the receiver's share of the
+  // lifted face flux into fbuf; role 2 runs over the receivers, which add it
+  // instead of solving faces. role 0: every element solves its own faces.
+  int role;
+  const int* elist;    // roles 1, 2: the elements of work block b are elist[b*EPB .. b*EPB+EPB-1]
+  int64_t nlist;       // number of elements in elist
+  const int* fslot;    // [NE]: receiver slot of each element (its index in the receivers' elist)
+  double* fbuf;        // [receivers][NF][NEQ][face nodes]: lifted face fluxes in the receiver's face frame
 };
 
 struct LimitArgs {
@@ -128,6 +139,22 @@ struct StateRegs {
     const bool ok = tid < EPB * ND && e0 + tid / ND < NE;
 #pragma unroll
     for (int q = 0; q < NEQ; q++) v[q] = ok ? sbase(x + (q * NDOFS + e0 * ND))[(unsigned)tid] : fill;
+  }
+  // the same for the elements elist[li0 .. li0+EPB-1] (entries from nlist on:
+  // `fill`); the list entries are read at wave-uniform addresses
+  __device__ __forceinline__ void load_list(const double* x, int64_t NDOFS, const int* elist, int64_t li0,
+                                            int64_t nlist, int tid, double fill = 1.0) {
+    const int te = tid / ND, tn = tid - te * ND;
+    int el[EPB];
+#pragma unroll
+    for (int t = 0; t < EPB; t++) el[t] = li0 + t < nlist ? elist[li0 + t] : -1;
+    int me = el[0];
+#pragma unroll
+    for (int t = 1; t < EPB; t++) me = te == t ? el[t] : me;
+    const bool ok = tid < EPB * ND && me >= 0;
+#pragma unroll
+    for (int q = 0; q < NEQ; q++)
+      v[q] = ok ? sbase(x + q * NDOFS)[(unsigned)me * (unsigned)ND + (unsigned)tn] : fill;
   }
   __device__ __forceinline__ void store(double (&su)[EPB][NEQ][ND], int tid) const {
     if (tid < EPB * ND) {

@@ -135,8 +135,12 @@ struct TArg {
 };
 
 // ---------------------------------------------------------------------------
-template <int DIM, int P, int RS, bool BC>
-__global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const SchemeParams g,
+// ROLE 0: every element solves its faces. ROLE 1 (solver elements of a
+// two-coloured mesh): as ROLE 0, and the receiver's share of each lifted face
+// flux is written to A.fbuf. ROLE 2 (receivers): the face fluxes are read from
+// A.fbuf instead of being solved; no traces, Riemann solves or lifts.
+template <int DIM, int P, int RS, bool BC, int ROLE>
+__global__ __launch_bounds__(64, ROLE == 1 ? 4 : 1) void tensor_rhs_kernel(const RhsArgs A, const SchemeParams g,
                                                         const TensorTables* __restrict__ gtb,
                                                         const TArg<DIM, P + 1, TC<DIM, P>::NQ> ka) {
   const TT<P + 1, TC<DIM, P>::NQ>& kt = ka.t;
@@ -157,18 +161,23 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
   constexpr bool LIFT_IN_WORK = 2 * NEQ * N <= T && 2 * NFP <= T;
   constexpr int LIFT = 2 * NEQ * NFN;
 
+  constexpr bool SOLVE = ROLE != 2;    // this kernel solves face fluxes
+  constexpr int ES = SOLVE ? EPB : 1;  // extent of the arrays only the face solves use
+  constexpr int FB = NF * NEQ * NFN;   // lifted face fluxes of one receiver element
   __shared__ TT<C::N, C::NQ> tb;
   __shared__ double s_u[EPB][NEQ][ND];
   // traces interpolated along the first tangential direction to the face Gauss
   // abscissae: [face c][variable][qi][b], own and neighbour (its face frame)
-  __shared__ double s_to[EPB][2][NEQ][NQ][NB];
-  __shared__ double s_tn[EPB][2][NEQ][NQ][NB];
+  __shared__ double s_to[ES][2][NEQ][NQ][NB];
+  __shared__ double s_tn[ES][2][NEQ][NQ][NB];
   __shared__ double s_work[EPB][WORK > LIFT ? WORK : LIFT];
-  __shared__ double s_lift[LIFT_IN_WORK ? 1 : EPB][LIFT_IN_WORK ? 1 : LIFT];  // face fluxes lifted to face nodes
-  __shared__ double s_nrm[EPB][NF][DIM + 2];   // unit normal, flux sign * |n|, 0.5 / |unit normal|
+  __shared__ double s_lift[LIFT_IN_WORK || !SOLVE ? 1 : EPB][LIFT_IN_WORK || !SOLVE ? 1 : LIFT];  // face fluxes lifted to face nodes
+  __shared__ double s_nrm[ES][NF][DIM + 2];   // unit normal, flux sign * |n|, 0.5 / |unit normal|
   __shared__ double s_geo[EPB][GD];
-  __shared__ int s_nb[EPB][NF];                // by MFEM local face
-  __shared__ int s_code[EPB][NF];
+  __shared__ int s_nb[ES][NF];                // by MFEM local face
+  __shared__ int s_code[ES][NF];
+  __shared__ int s_slot[ROLE == 1 ? EPB : 1][NF];      // receiver slot of each face's neighbour
+  __shared__ double s_fb[ROLE == 2 ? EPB : 1][ROLE == 2 ? FB : 1];  // receivers: [lf][v][face node]
 
   const int tid = threadIdx.x;
   const int team = tid / T, lane = tid % T;
@@ -176,9 +185,10 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
   const int64_t NDOFS = NE * ND;
   const int64_t blk0 = xcd_remap(blockIdx.x, gridDim.x);
   const int64_t blk = A.order ? (int64_t)A.order[blk0] : blk0;
-  const int64_t e0 = blk * EPB;
-  const int64_t e = e0 + team;
-  const bool active = e < NE;
+  const int64_t e0 = blk * EPB;  // ROLE 0: the block's first element
+  const int64_t li = blk0 * EPB + team;  // ROLES 1, 2: this team's index in the element list
+  const int64_t e = ROLE == 0 ? e0 + team : (li < A.nlist ? (int64_t)A.elist[li] : (int64_t)0);
+  const bool active = ROLE == 0 ? e < NE : li < A.nlist;
 
   // ---- phase 0: tables, element state, connectivity, normals and geometry
   // into LDS. Every load is issued before the first LDS store: one memory round
@@ -187,16 +197,32 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
   CopyRegs<sizeof(TT<N, NQ>) / sizeof(double), BLOCK> tbr;
   tbr.load(gtb->packed, tid);
   StateRegs<EPB, NEQ, ND, BLOCK> str;
-  str.load(A.x, NDOFS, e0, NE, tid);
+  if constexpr (ROLE == 0) str.load(A.x, NDOFS, e0, NE, tid);
+  else str.load_list(A.x, NDOFS, A.elist, blk0 * EPB, A.nlist, tid);
+  // receivers: the block's lifted face fluxes, contiguous in A.fbuf
+  constexpr int FBP = ROLE == 2 ? (EPB * FB + BLOCK - 1) / BLOCK : 1;
+  double fbr[FBP];
+  if constexpr (ROLE == 2) {
+    const int64_t f0 = blk0 * EPB * (int64_t)FB;
+    const int64_t fend = (A.nlist < blk0 * EPB + EPB ? A.nlist : blk0 * EPB + EPB) * (int64_t)FB;
+#pragma unroll
+    for (int p = 0; p < FBP; p++) {
+      const int64_t i = f0 + tid + p * BLOCK;
+      fbr[p] = (tid + p * BLOCK < EPB * FB && i < fend) ? A.fbuf[i] : 0.0;
+    }
+  }
   constexpr int GDP = (GD + T - 1) / T;
   double gv[GDP];
 #pragma unroll
   for (int p = 0; p < GDP; p++) gv[p] = (active && lane + p * T < GD) ? A.geo[(lane + p * T) * NE + e] : 0.0;
   // connectivity of local face i = lane and Elem1's unit normal on it (the
   // neighbour's geometry when the neighbour is Elem1)
-  const bool face_lane = active && lane < NF;
+  const bool face_lane = SOLVE && active && lane < NF;
   const int nb0 = face_lane ? A.nbr[lane * NE + e] : -1;
   const int cd0 = face_lane ? A.fcode[lane * NE + e] : 0;
+  if constexpr (ROLE == 1) {
+    if (face_lane) s_slot[team][lane] = nb0 >= 0 ? A.fslot[nb0] : -1;
+  }
   if (face_lane) {
     const int i = lane;
     const int nb = nb0;
@@ -238,6 +264,11 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
 #pragma unroll
   for (int p = 0; p < GDP; p++)
     if (active && lane + p * T < GD) s_geo[team][lane + p * T] = gv[p];
+  if constexpr (ROLE == 2) {
+#pragma unroll
+    for (int p = 0; p < FBP; p++)
+      if (tid + p * BLOCK < EPB * FB) (&s_fb[0][0])[tid + p * BLOCK] = fbr[p];
+  }
   __syncthreads();
 
   const int j = lane;
@@ -306,208 +337,245 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
     }
   }
 
-  // ---- face terms, one axis at a time. Phases per axis r: points(r) |
-  // lift(r) + traces(r+1) | node accumulation(r) + points(r+1) ..., so each
-  // axis costs two barriers.
-  bool bad = false;
-  // Table values that depend on the lane but not on the axis, read from LDS
-  // once (the phases below are one pass of the team when ONE_PASS; otherwise
-  // they read the tables per item).
-  constexpr bool ONE_PASS = 2 * NEQ * N <= T && 2 * NFP <= T;
-  double ec[N];         // traces: E[c][k] of the lane's face side
-  double bqj[N];        // points: B[qj][b] of the lane's point
-  double bqa[NQ];       // lift: B[qi][a] of the lane's row
-  double wq2 = 0.0;     // points: the lane's face weight
-  if (ONE_PASS) {
-    const int ct = lane / (NEQ * NB);
-    const int at = (lane - (lane / (NEQ * N)) * (NEQ * N)) % N;
-    const int qt = lane % NFP, qit = qt % NQ, qjt = DIM == 3 ? qt / NQ : 0;
-#pragma unroll
-    for (int k = 0; k < N; k++) ec[k] = tb.E[ct < 2 ? ct : 0][k];
-#pragma unroll
-    for (int b = 0; b < N; b++) bqj[b] = tb.B[qjt][b];
-#pragma unroll
-    for (int q = 0; q < NQ; q++) bqa[q] = tb.B[q][at];
-    wq2 = DIM == 3 ? tb.wq[qit] * tb.wq[qjt] : tb.wq[qit];
-  }
-  // own normal-direction traces on faces (r,0), (r,1) and the neighbour's
-  // traces in its own face frame (read from global memory), then interpolated
-  // along the first tangential direction a to the Gauss abscissae qi. One lane
-  // per (face, variable, second tangential index b).
-  auto traces = [&](const int r) {
-    for (int it = lane; it < 2 * NEQ * NB; it += T) {
-      if (!active) break;
-      const int c = it / (NEQ * NB), rem = it - c * (NEQ * NB);
-      const int v = rem / NB, b = rem - v * NB;
-      const int lf = lf_of<DIM>(r, c);
-      const int nb = s_nb[team][lf];
-      double vals[N][N];
-      int cn = 0;
-      if (nb >= 0) {
-        int rn, sk, sa, sb;
-        rc_of<DIM>(s_code[team][lf] & 7, rn, cn);
-        vstrides<DIM, N>(rn, sk, sa, sb);
-        const double* xn = A.x + v * NDOFS + (int64_t)nb * ND + b * sb;
-#pragma unroll
-        for (int a = 0; a < N; a++)
-#pragma unroll
-          for (int k = 0; k < N; k++) vals[a][k] = xn[a * sa + k * sk];
-      }
-      double tr[N];
-#pragma unroll
-      for (int a = 0; a < N; a++) {
-        double t = 0.0;
-#pragma unroll
-        for (int k = 0; k < N; k++) t += s_u[team][v][vnode<DIM, N>(r, k, a, b)] * (ONE_PASS ? ec[k] : tb.E[c][k]);
-        tr[a] = t;
-      }
-#pragma unroll
-      for (int qi = 0; qi < NQ; qi++) {
-        double tt = 0.0;
-#pragma unroll
-        for (int a = 0; a < N; a++) tt += tr[a] * kt.B[qi][a];
-        s_to[team][c][v][qi][b] = tt;
-      }
-      if (nb >= 0) {
-#pragma unroll
+  if constexpr (SOLVE) {
+    // ---- face terms, one axis at a time. Phases per axis r: points(r) |
+    // lift(r) + traces(r+1) | node accumulation(r) + points(r+1) ..., so each
+    // axis costs two barriers.
+    bool bad = false;
+    // Table values that depend on the lane but not on the axis, read from LDS
+    // once (the phases below are one pass of the team when ONE_PASS; otherwise
+    // they read the tables per item).
+    constexpr bool ONE_PASS = 2 * NEQ * N <= T && 2 * NFP <= T;
+    double ec[N];         // traces: E[c][k] of the lane's face side
+    double bqj[N];        // points: B[qj][b] of the lane's point
+    double bqa[NQ];       // lift: B[qi][a] of the lane's row
+    double wq2 = 0.0;     // points: the lane's face weight
+    if (ONE_PASS) {
+      const int ct = lane / (NEQ * NB);
+      const int at = (lane - (lane / (NEQ * N)) * (NEQ * N)) % N;
+      const int qt = lane % NFP, qit = qt % NQ, qjt = DIM == 3 ? qt / NQ : 0;
+  #pragma unroll
+      for (int k = 0; k < N; k++) ec[k] = tb.E[ct < 2 ? ct : 0][k];
+  #pragma unroll
+      for (int b = 0; b < N; b++) bqj[b] = tb.B[qjt][b];
+  #pragma unroll
+      for (int q = 0; q < NQ; q++) bqa[q] = tb.B[q][at];
+      wq2 = DIM == 3 ? tb.wq[qit] * tb.wq[qjt] : tb.wq[qit];
+    }
+    // own normal-direction traces on faces (r,0), (r,1) and the neighbour's
+    // traces in its own face frame (read from global memory), then interpolated
+    // along the first tangential direction a to the Gauss abscissae qi. One lane
+    // per (face, variable, second tangential index b).
+    auto traces = [&](const int r) {
+      for (int it = lane; it < 2 * NEQ * NB; it += T) {
+        if (!active) break;
+        const int c = it / (NEQ * NB), rem = it - c * (NEQ * NB);
+        const int v = rem / NB, b = rem - v * NB;
+        const int lf = lf_of<DIM>(r, c);
+        const int nb = s_nb[team][lf];
+        double vals[N][N];
+        int cn = 0;
+        if (nb >= 0) {
+          int rn, sk, sa, sb;
+          rc_of<DIM>(s_code[team][lf] & 7, rn, cn);
+          vstrides<DIM, N>(rn, sk, sa, sb);
+          const double* xn = A.x + v * NDOFS + (int64_t)nb * ND + b * sb;
+  #pragma unroll
+          for (int a = 0; a < N; a++)
+  #pragma unroll
+            for (int k = 0; k < N; k++) vals[a][k] = xn[a * sa + k * sk];
+        }
+        double tr[N];
+  #pragma unroll
         for (int a = 0; a < N; a++) {
           double t = 0.0;
-#pragma unroll
-          for (int k = 0; k < N; k++) t += vals[a][k] * tb.E[cn][k];
+  #pragma unroll
+          for (int k = 0; k < N; k++) t += s_u[team][v][vnode<DIM, N>(r, k, a, b)] * (ONE_PASS ? ec[k] : tb.E[c][k]);
           tr[a] = t;
         }
-#pragma unroll
+  #pragma unroll
         for (int qi = 0; qi < NQ; qi++) {
           double tt = 0.0;
-#pragma unroll
+  #pragma unroll
           for (int a = 0; a < N; a++) tt += tr[a] * kt.B[qi][a];
-          s_tn[team][c][v][qi][b] = tt;
+          s_to[team][c][v][qi][b] = tt;
+        }
+        if (nb >= 0) {
+  #pragma unroll
+          for (int a = 0; a < N; a++) {
+            double t = 0.0;
+  #pragma unroll
+            for (int k = 0; k < N; k++) t += vals[a][k] * tb.E[cn][k];
+            tr[a] = t;
+          }
+  #pragma unroll
+          for (int qi = 0; qi < NQ; qi++) {
+            double tt = 0.0;
+  #pragma unroll
+            for (int a = 0; a < N; a++) tt += tr[a] * kt.B[qi][a];
+            s_tn[team][c][v][qi][b] = tt;
+          }
         }
       }
-    }
-  };
-  traces(0);
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < DIM; r++) {
-    // face points: traces, Riemann solve, weighted flux
-    for (int it = lane; it < 2 * NFP; it += T) {
-      if (!active) break;
-      const int c = it / NFP, q = it - c * NFP;
-      const int qi = q % NQ, qj = DIM == 3 ? q / NQ : 0;
-      const int f = 2 * r + c;
-      const int lf = lf_of<DIM>(r, c);
-      const int nb = s_nb[team][lf];
-      double um[NEQ], up[NEQ];
-#pragma unroll
-      for (int v = 0; v < NEQ; v++) {
-        double t = 0.0;
-        if (DIM == 3) {
-#pragma unroll
-          for (int b = 0; b < N; b++) t += s_to[team][c][v][qi][b] * (ONE_PASS ? bqj[b] : tb.B[qj][b]);
-        } else {
-          t = s_to[team][c][v][qi][0];
-        }
-        um[v] = t;
-      }
-      double nrm[DIM];
-#pragma unroll
-      for (int d = 0; d < DIM; d++) nrm[d] = s_nrm[team][f][d];
-      const double snm = s_nrm[team][f][DIM];  // -|n| on Elem1's side, +|n| on Elem2's
-      const double inm = s_nrm[team][f][DIM + 1];
-      double fl[NEQ];
-      double factor = -1.0;
-      bool ok = true;
-      double lam = 0.0;
-      if (!BC || nb >= 0) {
-        const int q2 = map_point<DIM, NQ>(s_code[team][lf], qi, qj);
-        const int pi = q2 % NQ, pj = DIM == 3 ? q2 / NQ : 0;
-#pragma unroll
+    };
+    traces(0);
+    __syncthreads();
+  #pragma unroll
+    for (int r = 0; r < DIM; r++) {
+      // face points: traces, Riemann solve, weighted flux
+      for (int it = lane; it < 2 * NFP; it += T) {
+        if (!active) break;
+        const int c = it / NFP, q = it - c * NFP;
+        const int qi = q % NQ, qj = DIM == 3 ? q / NQ : 0;
+        const int f = 2 * r + c;
+        const int lf = lf_of<DIM>(r, c);
+        const int nb = s_nb[team][lf];
+        double um[NEQ], up[NEQ];
+  #pragma unroll
         for (int v = 0; v < NEQ; v++) {
           double t = 0.0;
           if (DIM == 3) {
-#pragma unroll
-            for (int b = 0; b < N; b++) t += s_tn[team][c][v][pi][b] * tb.B[pj][b];
+  #pragma unroll
+            for (int b = 0; b < N; b++) t += s_to[team][c][v][qi][b] * (ONE_PASS ? bqj[b] : tb.B[qj][b]);
           } else {
-            t = s_tn[team][c][v][pi][0];
+            t = s_to[team][c][v][qi][0];
           }
-          up[v] = t;
+          um[v] = t;
         }
-        factor = snm < 0.0 ? -1.0 : 1.0;
-        const bool elem1 = factor < 0.0;
-        double sL[NEQ], sR[NEQ];
-#pragma unroll
-        for (int v = 0; v < NEQ; v++) {
-          sL[v] = elem1 ? um[v] : up[v];
-          sR[v] = elem1 ? up[v] : um[v];
+        double nrm[DIM];
+  #pragma unroll
+        for (int d = 0; d < DIM; d++) nrm[d] = s_nrm[team][f][d];
+        const double snm = s_nrm[team][f][DIM];  // -|n| on Elem1's side, +|n| on Elem2's
+        const double inm = s_nrm[team][f][DIM + 1];
+        double fl[NEQ];
+        double factor = -1.0;
+        bool ok = true;
+        double lam = 0.0;
+        if (!BC || nb >= 0) {
+          const int q2 = map_point<DIM, NQ>(s_code[team][lf], qi, qj);
+          const int pi = q2 % NQ, pj = DIM == 3 ? q2 / NQ : 0;
+  #pragma unroll
+          for (int v = 0; v < NEQ; v++) {
+            double t = 0.0;
+            if (DIM == 3) {
+  #pragma unroll
+              for (int b = 0; b < N; b++) t += s_tn[team][c][v][pi][b] * tb.B[pj][b];
+            } else {
+              t = s_tn[team][c][v][pi][0];
+            }
+            up[v] = t;
+          }
+          factor = snm < 0.0 ? -1.0 : 1.0;
+          const bool elem1 = factor < 0.0;
+          double sL[NEQ], sR[NEQ];
+  #pragma unroll
+          for (int v = 0; v < NEQ; v++) {
+            sL[v] = elem1 ? um[v] : up[v];
+            sR[v] = elem1 ? up[v] : um[v];
+          }
+          lam = riemann_t<DIM, RS>(g, sL, sR, nrm, fl, ok, inm);
+        } else if constexpr (BC) {
+          const int slot = -1 - nb;
+          if (slot < g.num_bc) {
+            bc_state<DIM>(g, slot, um, nrm, up);
+            if (g.bc_type[slot] == RKDG_BC_CHAR_OUTLET) lam = char_outlet_flux<DIM>(g, up, nrm, fl);
+            else lam = riemann_t<DIM, RS>(g, um, up, nrm, fl, ok, inm);
+          } else {
+  #pragma unroll
+            for (int v = 0; v < NEQ; v++) fl[v] = 0.0;
+          }
         }
-        lam = riemann_t<DIM, RS>(g, sL, sR, nrm, fl, ok, inm);
-      } else if constexpr (BC) {
-        const int slot = -1 - nb;
-        if (slot < g.num_bc) {
-          bc_state<DIM>(g, slot, um, nrm, up);
-          if (g.bc_type[slot] == RKDG_BC_CHAR_OUTLET) lam = char_outlet_flux<DIM>(g, up, nrm, fl);
-          else lam = riemann_t<DIM, RS>(g, um, up, nrm, fl, ok, inm);
+        if (!ok || lam < 0.0) {
+          if (!bad) {
+            const bool nf = nonfinite_nodes(A.x, NDOFS, e, ND, NEQ) || nonfinite_nodes(A.x, NDOFS, nb, ND, NEQ);
+            flag_error(A.err, factor < 0.0 ? e : nb, factor < 0.0 ? (nb >= 0 ? nb : -1) : e, lf,
+                       nf ? kErrNonFinite : kErrUnphysical);
+          }
+          bad = true;
         } else {
-#pragma unroll
-          for (int v = 0; v < NEQ; v++) fl[v] = 0.0;
+          lmax = lam > lmax ? lam : lmax;
         }
+        const double wgt = (ONE_PASS ? wq2 : (DIM == 3 ? tb.wq[qi] * tb.wq[qj] : tb.wq[qi])) * snm;  // flux sign * w_q * |n|
+  #pragma unroll
+        for (int v = 0; v < NEQ; v++) work[(c * NEQ + v) * RW + q] = fl[v] * wgt;
       }
-      if (!ok || lam < 0.0) {
-        if (!bad) {
-          const bool nf = nonfinite_nodes(A.x, NDOFS, e, ND, NEQ) || nonfinite_nodes(A.x, NDOFS, nb, ND, NEQ);
-          flag_error(A.err, factor < 0.0 ? e : nb, factor < 0.0 ? (nb >= 0 ? nb : -1) : e, lf,
-                     nf ? kErrNonFinite : kErrUnphysical);
-        }
-        bad = true;
-      } else {
-        lmax = lam > lmax ? lam : lmax;
-      }
-      const double wgt = (ONE_PASS ? wq2 : (DIM == 3 ? tb.wq[qi] * tb.wq[qj] : tb.wq[qi])) * snm;  // flux sign * w_q * |n|
-#pragma unroll
-      for (int v = 0; v < NEQ; v++) work[(c * NEQ + v) * RW + q] = fl[v] * wgt;
-    }
-    __syncthreads();
-    // lift to face nodes, sum-factorised: one lane per (face, variable, a)
-    // forms W(qj) = sum_qi F(qi, qj) B(qi, a) once and the N nodes (a, b) from it
-    for (int it = lane; it < 2 * NEQ * N; it += T) {
-      if (!active) break;
-      const int c = it / (NEQ * N), rem = it - c * (NEQ * N);
-      const int v = rem / N, a = rem - v * N;
-      const double* w = work + (c * NEQ + v) * RW;
-      if (DIM == 3) {
-        double W[NQ];
-#pragma unroll
-        for (int qj = 0; qj < NQ; qj++) {
-          double tt = 0.0;
-#pragma unroll
-          for (int qi = 0; qi < NQ; qi++) tt += w[qi + NQ * qj] * (ONE_PASS ? bqa[qi] : tb.B[qi][a]);
-          W[qj] = tt;
-        }
-#pragma unroll
-        for (int b = 0; b < N; b++) {
+      __syncthreads();
+      // lift to face nodes, sum-factorised: one lane per (face, variable, a)
+      // forms W(qj) = sum_qi F(qi, qj) B(qi, a) once and the N nodes (a, b) from it
+      for (int it = lane; it < 2 * NEQ * N; it += T) {
+        if (!active) break;
+        const int c = it / (NEQ * N), rem = it - c * (NEQ * N);
+        const int v = rem / N, a = rem - v * N;
+        const double* w = work + (c * NEQ + v) * RW;
+        if (DIM == 3) {
+          double W[NQ];
+  #pragma unroll
+          for (int qj = 0; qj < NQ; qj++) {
+            double tt = 0.0;
+  #pragma unroll
+            for (int qi = 0; qi < NQ; qi++) tt += w[qi + NQ * qj] * (ONE_PASS ? bqa[qi] : tb.B[qi][a]);
+            W[qj] = tt;
+          }
+  #pragma unroll
+          for (int b = 0; b < N; b++) {
+            double L = 0.0;
+  #pragma unroll
+            for (int qj = 0; qj < NQ; qj++) L += W[qj] * kt.B[qj][b];
+            lift[(c * NEQ + v) * NFN + a + N * b] = L;
+            if constexpr (ROLE == 1) {
+              // the receiver's share: the same flux with the opposite sign,
+              // at the same face node in the receiver's face frame
+              if (!A.face_only) {
+                const int lf = lf_of<DIM>(r, c);
+                const int code = s_code[team][lf];
+                A.fbuf[((int64_t)s_slot[team][lf] * NF + (code & 7)) * (NEQ * NFN) + v * NFN +
+                       map_point<DIM, N>(code, a, b)] = -L;
+              }
+            }
+          }
+        } else {
           double L = 0.0;
-#pragma unroll
-          for (int qj = 0; qj < NQ; qj++) L += W[qj] * kt.B[qj][b];
-          lift[(c * NEQ + v) * NFN + a + N * b] = L;
+  #pragma unroll
+          for (int qi = 0; qi < NQ; qi++) L += w[qi] * tb.B[qi][a];
+          lift[(c * NEQ + v) * NFN + a] = L;
+          if constexpr (ROLE == 1) {
+            if (!A.face_only) {
+              const int lf = lf_of<DIM>(r, c);
+              const int code = s_code[team][lf];
+              A.fbuf[((int64_t)s_slot[team][lf] * NF + (code & 7)) * (NEQ * NFN) + v * NFN +
+                     map_point<DIM, N>(code, a, 0)] = -L;
+            }
+          }
         }
-      } else {
-        double L = 0.0;
-#pragma unroll
-        for (int qi = 0; qi < NQ; qi++) L += w[qi] * tb.B[qi][a];
-        lift[(c * NEQ + v) * NFN + a] = L;
+      }
+      if (r + 1 < DIM) traces(r + 1);
+      __syncthreads();
+      if (node_lane) {
+        const int k = jj[r];
+        const int a = r == 0 ? jj[1] : jj[0];
+        const int b = DIM == 3 ? (r == 2 ? jj[1] : jj[2]) : 0;
+        const int fn = DIM == 3 ? a + N * b : a;
+        const double e0v = tb.E[0][k], e1v = tb.E[1][k];
+  #pragma unroll
+        for (int v = 0; v < NEQ; v++) z[v] += lift[v * NFN + fn] * e0v + lift[(NEQ + v) * NFN + fn] * e1v;
       }
     }
-    if (r + 1 < DIM) traces(r + 1);
-    __syncthreads();
+  } else {
+    // receivers: the lifted face fluxes of the solver elements, axis by axis
+    // as a solver element adds its own (the same formula and order)
     if (node_lane) {
-      const int k = jj[r];
-      const int a = r == 0 ? jj[1] : jj[0];
-      const int b = DIM == 3 ? (r == 2 ? jj[1] : jj[2]) : 0;
-      const int fn = DIM == 3 ? a + N * b : a;
-      const double e0v = tb.E[0][k], e1v = tb.E[1][k];
 #pragma unroll
-      for (int v = 0; v < NEQ; v++) z[v] += lift[v * NFN + fn] * e0v + lift[(NEQ + v) * NFN + fn] * e1v;
+      for (int r = 0; r < DIM; r++) {
+        const int k = jj[r];
+        const int a = r == 0 ? jj[1] : jj[0];
+        const int b = DIM == 3 ? (r == 2 ? jj[1] : jj[2]) : 0;
+        const int fn = DIM == 3 ? a + N * b : a;
+        const double e0v = tb.E[0][k], e1v = tb.E[1][k];
+        const double* l0 = &s_fb[team][lf_of<DIM>(r, 0) * (NEQ * NFN) + fn];
+        const double* l1 = &s_fb[team][lf_of<DIM>(r, 1) * (NEQ * NFN) + fn];
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) z[v] += l0[v * NFN] * e0v + l1[v * NFN] * e1v;
+      }
     }
   }
 
@@ -522,9 +590,10 @@ __global__ __launch_bounds__(64) void tensor_rhs_kernel(const RhsArgs A, const S
       const double im = iw * recip(detJ);
       // every base is loaded before the first store (an output may be a base)
       // addresses: the workgroup's block of each variable (uniform, scalar
-      // registers) plus a small 32-bit lane offset
-      const int64_t eb = e0 * ND;
-      const unsigned lo = (unsigned)(team * ND + j);
+      // registers) plus a small 32-bit lane offset; with an element list the
+      // lane offset is the element's own node index
+      const int64_t eb = ROLE == 0 ? e0 * ND : 0;
+      const unsigned lo = ROLE == 0 ? (unsigned)(team * ND + j) : (unsigned)e * (unsigned)ND + (unsigned)j;
       double u0[NEQ];  // the stage input at this node (the base of outputs without one)
 #pragma unroll
       for (int v = 0; v < NEQ; v++) u0[v] = s_u[team][v][j];
@@ -935,8 +1004,17 @@ hipError_t launch_rhs_rs(const RhsArgs& a, const SchemeParams& g, const TensorTa
     ka.wk[k] = w;
     ka.wsum += w;
   }
-  if (bc) tensor_rhs_kernel<D, PP, RS, true><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t, ka);
-  else tensor_rhs_kernel<D, PP, RS, false><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t, ka);
+  if (a.role != 0) {
+    // two-coloured meshes have no boundary faces (the host checks)
+    if (bc) return hipErrorInvalidValue;
+    const int64_t nl = (a.nlist + C::EPB - 1) / C::EPB;
+    if (nl == 0) return hipSuccess;
+    if (a.role == 1) tensor_rhs_kernel<D, PP, RS, false, 1><<<(unsigned)nl, C::BLOCK, 0, s>>>(a, g, t, ka);
+    else tensor_rhs_kernel<D, PP, RS, false, 2><<<(unsigned)nl, C::BLOCK, 0, s>>>(a, g, t, ka);
+    return hipGetLastError();
+  }
+  if (bc) tensor_rhs_kernel<D, PP, RS, true, 0><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t, ka);
+  else tensor_rhs_kernel<D, PP, RS, false, 0><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t, ka);
   return hipGetLastError();
 }
 

@@ -33,12 +33,12 @@ def test_struct_layouts_match_header(tmp_path):
         '#include <stdio.h>\n#include <stddef.h>\n#include "rkdg_capi.h"\n'
         'int main(){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(rkdg_mesh_desc), sizeof(rkdg_scheme_desc),'
         ' sizeof(rkdg_info), offsetof(rkdg_scheme_desc, bc_param), offsetof(rkdg_scheme_desc, pow_nd),'
-        ' offsetof(rkdg_info, affine)); return 0;}\n')
+        ' offsetof(rkdg_info, two_colour)); return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     want = [ctypes.sizeof(capi.MeshDesc), ctypes.sizeof(capi.SchemeDesc), ctypes.sizeof(capi.Info),
-            capi.SchemeDesc.bc_param.offset, capi.SchemeDesc.pow_nd.offset, capi.Info.affine.offset]
+            capi.SchemeDesc.bc_param.offset, capi.SchemeDesc.pow_nd.offset, capi.Info.two_colour.offset]
     assert got == want
 
 
