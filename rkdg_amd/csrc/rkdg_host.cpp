@@ -180,6 +180,7 @@ struct rkdg_ctx {
   int count = 1;
   int force_dense = 0;
   int two_colour = 0;            // the solver/receiver form where the mesh allows it
+  int aligned = 0;               // tensor mesh: every interior face joins (r, c) to (r, 1-c), identity orientation
   int grid_cap = 0;              // persistent grid of the dense right-hand-side kernel
   std::vector<int64_t> inst_off; // [count+1] element offsets
   int* d_inst = nullptr;         // [NE] element -> instance
@@ -983,6 +984,22 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
     c->n_bdr++;
   }
   (void)NE_;
+  // aligned (structured) meshes: every interior face is the neighbour's
+  // opposite face of the same axis with the identity orientation
+  c->aligned = 1;
+  for (int64_t e = 0; e < NE && c->aligned; e++)
+    for (int lf = 0; lf < nf; lf++) {
+      if (nbr[(size_t)lf * NE + e] < 0) continue;
+      int r, cc;
+      rc_of_lf(dim, lf, r, cc);
+      int rn, cn;
+      const int code = fcode[(size_t)lf * NE + e];
+      rc_of_lf(dim, code & 7, rn, cn);
+      if (rn != r || cn != 1 - cc || ((code >> 3) & 7) != 0) {
+        c->aligned = 0;
+        break;
+      }
+    }
 
   HIP_TRY(hipMalloc(&c->d_geo, sizeof(double) * G * NE));
   HIP_TRY(hipMemcpy(c->d_geo, c->geo_h.data(), sizeof(double) * G * NE, hipMemcpyHostToDevice));
@@ -1127,6 +1144,7 @@ RhsArgs base_args(rkdg_ctx* c, const double* x) {
   a.bcp = c->d_bcp;
   a.grid_cap = c->grid_cap;
   a.order = c->dense ? nullptr : c->d_order;
+  a.aligned = c->dense ? 0 : c->aligned;
   return a;
 }
 

@@ -73,6 +73,7 @@ struct RhsArgs {
   int64_t nlist;       // number of elements in elist
   const int* fslot;    // [NE]: receiver slot of each element (its index in the receivers' elist)
   double* fbuf;        // [receivers][NF][NEQ][face nodes]: lifted face fluxes in the receiver's face frame
+  int aligned;         // tensor kernel: every interior face joins faces (r, c) and (r, 1-c), identity orientation
 };
 
 struct LimitArgs {

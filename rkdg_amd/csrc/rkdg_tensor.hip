@@ -139,7 +139,10 @@ struct TArg {
 // two-coloured mesh): as ROLE 0, and the receiver's share of each lifted face
 // flux is written to A.fbuf. ROLE 2 (receivers): the face fluxes are read from
 // A.fbuf instead of being solved; no traces, Riemann solves or lifts.
-template <int DIM, int P, int RS, bool BC, int ROLE>
+// AL: every interior face joins local face (r, c) to the neighbour's (r, 1-c)
+// with the identity orientation (structured meshes), so the neighbour's face,
+// node strides and point map are known at compile time.
+template <int DIM, int P, int RS, bool BC, int ROLE, bool AL>
 __global__ __launch_bounds__(64, ROLE == 1 ? 4 : 1) void tensor_rhs_kernel(const RhsArgs A, const SchemeParams g,
                                                         const TensorTables* __restrict__ gtb,
                                                         const TArg<DIM, P + 1, TC<DIM, P>::NQ> ka) {
@@ -377,7 +380,12 @@ __global__ __launch_bounds__(64, ROLE == 1 ? 4 : 1) void tensor_rhs_kernel(const
         int cn = 0;
         if (nb >= 0) {
           int rn, sk, sa, sb;
-          rc_of<DIM>(s_code[team][lf] & 7, rn, cn);
+          if constexpr (AL) {
+            rn = r;
+            cn = 1 - c;
+          } else {
+            rc_of<DIM>(s_code[team][lf] & 7, rn, cn);
+          }
           vstrides<DIM, N>(rn, sk, sa, sb);
           const double* xn = A.x + v * NDOFS + (int64_t)nb * ND + b * sb;
   #pragma unroll
@@ -452,14 +460,14 @@ __global__ __launch_bounds__(64, ROLE == 1 ? 4 : 1) void tensor_rhs_kernel(const
         bool ok = true;
         double lam = 0.0;
         if (!BC || nb >= 0) {
-          const int q2 = map_point<DIM, NQ>(s_code[team][lf], qi, qj);
+          const int q2 = AL ? q : map_point<DIM, NQ>(s_code[team][lf], qi, qj);
           const int pi = q2 % NQ, pj = DIM == 3 ? q2 / NQ : 0;
   #pragma unroll
           for (int v = 0; v < NEQ; v++) {
             double t = 0.0;
             if (DIM == 3) {
   #pragma unroll
-              for (int b = 0; b < N; b++) t += s_tn[team][c][v][pi][b] * tb.B[pj][b];
+              for (int b = 0; b < N; b++) t += s_tn[team][c][v][pi][b] * ((AL && ONE_PASS) ? bqj[b] : tb.B[pj][b]);
             } else {
               t = s_tn[team][c][v][pi][0];
             }
@@ -1009,12 +1017,17 @@ hipError_t launch_rhs_rs(const RhsArgs& a, const SchemeParams& g, const TensorTa
     if (bc) return hipErrorInvalidValue;
     const int64_t nl = (a.nlist + C::EPB - 1) / C::EPB;
     if (nl == 0) return hipSuccess;
-    if (a.role == 1) tensor_rhs_kernel<D, PP, RS, false, 1><<<(unsigned)nl, C::BLOCK, 0, s>>>(a, g, t, ka);
-    else tensor_rhs_kernel<D, PP, RS, false, 2><<<(unsigned)nl, C::BLOCK, 0, s>>>(a, g, t, ka);
+    if (a.role == 1) tensor_rhs_kernel<D, PP, RS, false, 1, false><<<(unsigned)nl, C::BLOCK, 0, s>>>(a, g, t, ka);
+    else tensor_rhs_kernel<D, PP, RS, false, 2, false><<<(unsigned)nl, C::BLOCK, 0, s>>>(a, g, t, ka);
     return hipGetLastError();
   }
-  if (bc) tensor_rhs_kernel<D, PP, RS, true, 0><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t, ka);
-  else tensor_rhs_kernel<D, PP, RS, false, 0><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t, ka);
+  if (a.aligned) {
+    if (bc) tensor_rhs_kernel<D, PP, RS, true, 0, true><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t, ka);
+    else tensor_rhs_kernel<D, PP, RS, false, 0, true><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t, ka);
+    return hipGetLastError();
+  }
+  if (bc) tensor_rhs_kernel<D, PP, RS, true, 0, false><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t, ka);
+  else tensor_rhs_kernel<D, PP, RS, false, 0, false><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t, ka);
   return hipGetLastError();
 }
 
