@@ -151,6 +151,15 @@ int rkdg_limit(rkdg_ctx* ctx, double* x, int8_t* troubled);
 /* One RK step, Butcher tableau consumed with the packed a[l++] rule. s <= 4. */
 int rkdg_step(rkdg_ctx* ctx, double* x, double t, double dt, int s, const double* a,
               const double* b, const double* c, double* lambda_max);
+/* SSP-RK3 in the Shu-Osher form with the limiter applied to each stage value
+ * (the classical limited RKDG step): u1 = L(x + dt R(x)),
+ * u2 = L(3/4 x + 1/4 (u1 + dt R(u1))), x = L(1/3 x + 2/3 (u2 + dt R(u2))), with R
+ * the right-hand side and L the limiter; 8 vector passes per step. No
+ * counterpart in the reference, whose ExplicitRKLimitedSolver consumes Butcher
+ * tableaus: with the limiter inactive this equals rkdg_step with the SSP-RK3
+ * tableau to rounding; with it active the forms differ (the Butcher form limits
+ * stage inputs it does not reuse). lambda_max: HOST pointer or NULL. */
+int rkdg_step_ssprk3(rkdg_ctx* ctx, double* x, double t, double dt, double* lambda_max);
 /* rkdg_step that allocates nothing. No counterpart in the reference: it is
  * for callers that cannot allocate device or pinned memory during a step (a
  * loop under HIP graph capture, a tight serving loop). The stage vectors live

@@ -63,6 +63,7 @@ EXPORTS = [
     "rkdg_batch_advance", "rkdg_batch_status", "rkdg_batch_set_options", "rkdg_tune_grid",
     "rkdg_set_bc_params", "rkdg_rhs_adjoint", "rkdg_limit_adjoint", "rkdg_objective",
     "rkdg_step_adjoint", "rkdg_forward_solve", "rkdg_adjoint_solve", "rkdg_step_noalloc",
+    "rkdg_step_ssprk3",
 ]
 
 # include/rkdg_selftest.h (host-only)
@@ -98,6 +99,7 @@ def lib():
     L.rkdg_limit.argtypes = [P, dp, dp]
     L.rkdg_step.argtypes = [P, dp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double),
                             C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.rkdg_step_ssprk3.argtypes = [P, dp, C.c_double, C.c_double, C.POINTER(C.c_double)]
     L.rkdg_step_noalloc.argtypes = [P, dp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double),
                                     C.POINTER(C.c_double), C.POINTER(C.c_double), dp]
     L.rkdg_means.argtypes = [P, dp, dp]
@@ -331,6 +333,14 @@ class Context:
         lam = C.c_double(0.0)
         _check(lib().rkdg_step(self.h, _ptr(x), t, dt, s, a, b, c,
                                C.byref(lam) if want_lambda else None))
+        return lam.value
+
+    def step_ssprk3(self, x, dt, t=0.0, want_lambda=True):
+        """rkdg_step_ssprk3: one SSP-RK3 step in the Shu-Osher form, each stage
+        value limited (the classical limited RKDG step)."""
+        self._st(x, "x")
+        lam = C.c_double(0.0)
+        _check(lib().rkdg_step_ssprk3(self.h, _ptr(x), t, dt, C.byref(lam) if want_lambda else None))
         return lam.value
 
     def step_noalloc(self, x, dt, tableau, lam_dev=None, t=0.0):

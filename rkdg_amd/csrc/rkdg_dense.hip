@@ -250,9 +250,10 @@ __device__ __forceinline__ void dense_rhs_block(const RhsArgs& A, const SchemePa
         if (m < A.nout) {
           double* op = A.outs[m].out + i0;
           const double cf = A.outs[m].coef * dtm;
+          const double wb = A.outs[m].wb, wi = A.outs[m].wi;
 #pragma unroll
           for (int v = 0; v < NEQ; v++) {
-            const double o = base[m][v] + cf * y[v];
+            const double o = wi == 0.0 ? base[m][v] + cf * y[v] : wb * base[m][v] + wi * (u0[v] + cf * y[v]);
             op[v * NDOFS] = o;
             if (m == A.means_of) outv[v] = o;
           }

@@ -27,11 +27,13 @@ struct TensorTables {
   double packed[kMaxPacked];
 };
 
-// out = (base ? base : stage input) + coef * k, element-wise
+// Butcher combination (wi == 0): out = (base ? base : stage input) + coef * k;
+// Shu-Osher combination (wi != 0): out = wb * base + wi * (stage input + coef * k)
 struct StageOut {
   double* out;
   const double* base;
   double coef;
+  double wb, wi;
 };
 
 struct RhsArgs {

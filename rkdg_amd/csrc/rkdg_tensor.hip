@@ -624,9 +624,10 @@ __global__ __launch_bounds__(64, ROLE == 1 ? 4 : 1) void tensor_rhs_kernel(const
         if (m < A.nout) {
           double* op = A.outs[m].out;
           const double cf = A.outs[m].coef;
+          const double wb = A.outs[m].wb, wi = A.outs[m].wi;
 #pragma unroll
           for (int v = 0; v < NEQ; v++) {
-            const double o = base[m][v] + cf * (z[v] * im);
+            const double o = wi == 0.0 ? base[m][v] + cf * (z[v] * im) : wb * base[m][v] + wi * (u0[v] + cf * (z[v] * im));
             sbase(op + (v * NDOFS + eb))[lo] = o;
             if (m == 0) outv[v] = o;  // means of outs[0] (the host puts the selected output first)
           }

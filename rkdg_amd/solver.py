@@ -103,6 +103,21 @@ class ExplicitRKLimitedSolver:
         return t, dt
 
 
+class SSPRK3LimitedSolver(ExplicitRKLimitedSolver):
+    """SSP-RK3 in the Shu-Osher form with the limiter applied to each stage
+    value (the classical limited RKDG step; no reference counterpart). Without
+    limiting it equals ExplicitRKLimitedSolver with the SSP-RK3 tableau to
+    rounding; with limiting the two forms differ (the Butcher form limits
+    stage inputs it does not reuse)."""
+
+    def __init__(self, limiter: Limiter):
+        super().__init__(*S.TABLEAUS["ssprk3"], limiter=limiter)
+
+    def Step(self, x, t: float, dt: float):
+        self.f.max_char_speed = self.f.ctx.step_ssprk3(x, dt, t=t)
+        return t, dt
+
+
 @dataclass
 class RunStats:
     steps: int

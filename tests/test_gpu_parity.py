@@ -478,3 +478,56 @@ def test_last_hope_reset_matches_oracle(kind, cuda):
           f"{np.flatnonzero(flags).tolist()}", flush=True)
     assert flags[bad] & 2 and np.count_nonzero(flags & 2) == 1
     assert rel_linf(_host(xd), xl, neq) <= TOL
+
+
+@pytest.mark.parametrize("kind", ["hex_q2_aligned", "hex_q2_scrambled", "quad_q1_bc", "tri_p2_bc", "quad_q1_dense_bc"])
+def test_ssprk3_shu_osher_step_matches_oracle(kind, cuda):
+    """rkdg_step_ssprk3, SSP-RK3 in the Shu-Osher form with the limiter applied
+    to each stage value (the classical limited RKDG step), against the same
+    step composed from the oracle's Mult and Limiter::update, and, with the
+    limiter switched off (Nowhere indicator), against the oracle's Butcher-form
+    step (the same method then, equal to rounding). With an active limiter the
+    two forms differ: the Butcher form limits stage inputs it does not reuse."""
+    from rkdg_amd.capi import Context
+
+    bcs = [S.BC(1, S.BC_SLIP), S.BC(3, S.BC_SLIP), S.BC(2, S.BC_OUTLET), S.BC(4, S.BC_OUTLET)]
+    dense = False
+    if kind == "hex_q2_aligned":
+        m, p, bcs = M.structured_hex(4, 4, 4), 2, []
+    elif kind == "hex_q2_scrambled":
+        m, p, bcs = scramble_orientations(M.structured_hex(4, 3, 3), seed=2), 2, []
+    elif kind == "quad_q1_bc":
+        m, p = M.structured_quad(10, 4), 1
+    elif kind == "tri_p2_bc":
+        m, p = M.jittered_delaunay_tri(6, 3, 0.0, 1.0, 0.0, 0.5, seed=8), 2
+    else:
+        m, p, dense = M.structured_quad(10, 4), 1, True
+    neq = m.dim + 2
+    dt = 1e-3
+    for ind in (S.IND_BJ, S.IND_NOWHERE):
+        sch = S.Scheme(riemann=S.HLLC, indicator=ind, phi=S.PHI_MICHALAK, limiter=S.LIM_MULTI, bcs=bcs)
+        ref = RefSolver(m, p, sch)
+        ctx = Context(m, p, sch, dense=dense)
+        x = smooth_state(ref.node_coords(), seed=21)
+        xa = _dev(x, cuda)
+        xs = x.copy()
+        for _ in range(3):
+            k, _ = ref.mult(xs)
+            u1 = ref.limit(xs + dt * k)[0]
+            k, _ = ref.mult(u1)
+            u2 = ref.limit(0.75 * xs + 0.25 * (u1 + dt * k))[0]
+            k, l_ref = ref.mult(u2)
+            xs = ref.limit(xs / 3.0 + 2.0 / 3.0 * (u2 + dt * k))[0]
+            la = ctx.step_ssprk3(xa, dt)
+        e_so = rel_linf(_host(xa), xs, neq)
+        msg = f"[Shu-Osher SSP-RK3 {kind} indicator {ind}] 3 steps vs the oracle's Shu-Osher steps {e_so:.2e}"
+        assert e_so <= TOL and abs(la - l_ref) <= 1e-12 * l_ref
+        if ind == S.IND_NOWHERE:
+            xb = x.copy()
+            for _ in range(3):
+                xb, lb = ref.step(xb, dt, S.TABLEAUS["ssprk3"])
+            e_bt = rel_linf(_host(xa), xb, neq)
+            msg += f", vs the oracle's Butcher-form steps {e_bt:.2e}"
+            assert e_bt <= TOL
+        print(msg, flush=True)
+        ctx.close()
