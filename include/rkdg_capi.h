@@ -173,6 +173,19 @@ int rkdg_step_noalloc(rkdg_ctx* ctx, double* x, double t, double dt, int s, cons
 /* Element means, layout [v*NE + e] (DEVICE pointer). */
 int rkdg_means(rkdg_ctx* ctx, const double* x, double* means);
 int rkdg_hmin(rkdg_ctx* ctx, double* hmin);
+/* Post-step diagnostics. rkdg_totals: HOST totals[num_equations], the integral
+ * over the mesh of each conserved variable (check_total_energy.cpp:3-63 forms
+ * the energy's with an order-(p+1) rule at the nodes, exact for Q1 only; here
+ * sum_e |e| mean_e, exact for every order). rkdg_primitives: the nodal fields
+ * rho, u (dim components), p, T of main.cpp:388-451 into DEVICE
+ * prim[(dim+3)][num_dofs] (nullable) and acc += weight * the same (nullable;
+ * the time integrals UMean, pMean, TMean, rhoMean of main.cpp:705-730 with
+ * weight dt). rkdg_interpolate: the state at npts reference points of every
+ * element (HOST ref_pts[npts][dim]), DEVICE out[num_equations][NE][npts]
+ * (ParaView output at the vertices, probes). */
+int rkdg_totals(rkdg_ctx* ctx, const double* x, double* totals);
+int rkdg_primitives(rkdg_ctx* ctx, const double* x, double* prim, double* acc, double weight);
+int rkdg_interpolate(rkdg_ctx* ctx, const double* x, int npts, const double* ref_pts, double* out);
 /* Physical coordinates of the nodes, HOST [NE][nd][dim] (for initial conditions). */
 int rkdg_node_coords(rkdg_ctx* ctx, double* xyz);
 /* Synchronize and report a deferred device error (RKDG_E_UNPHYSICAL, or

@@ -63,7 +63,7 @@ EXPORTS = [
     "rkdg_batch_advance", "rkdg_batch_status", "rkdg_batch_set_options", "rkdg_tune_grid",
     "rkdg_set_bc_params", "rkdg_rhs_adjoint", "rkdg_limit_adjoint", "rkdg_objective",
     "rkdg_step_adjoint", "rkdg_forward_solve", "rkdg_adjoint_solve", "rkdg_step_noalloc",
-    "rkdg_step_ssprk3",
+    "rkdg_step_ssprk3", "rkdg_totals", "rkdg_primitives", "rkdg_interpolate",
 ]
 
 # include/rkdg_selftest.h (host-only)
@@ -99,6 +99,9 @@ def lib():
     L.rkdg_limit.argtypes = [P, dp, dp]
     L.rkdg_step.argtypes = [P, dp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double),
                             C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.rkdg_totals.argtypes = [P, dp, C.POINTER(C.c_double)]
+    L.rkdg_primitives.argtypes = [P, dp, dp, dp, C.c_double]
+    L.rkdg_interpolate.argtypes = [P, dp, C.c_int, C.POINTER(C.c_double), dp]
     L.rkdg_step_ssprk3.argtypes = [P, dp, C.c_double, C.c_double, C.POINTER(C.c_double)]
     L.rkdg_step_noalloc.argtypes = [P, dp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double),
                                     C.POINTER(C.c_double), C.POINTER(C.c_double), dp]
@@ -350,6 +353,29 @@ class Context:
         check_buffer(lam_dev, 1, "lam_dev", nullable=True)
         s, a, b, c = _tableau_arrays(tableau)
         _check(lib().rkdg_step_noalloc(self.h, _ptr(x), t, dt, s, a, b, c, _ptr(lam_dev)))
+
+    def totals(self, x):
+        """Integral over the mesh of each conserved variable (rkdg_totals)."""
+        self._st(x, "x")
+        out = (C.c_double * self.info.num_equations)()
+        _check(lib().rkdg_totals(self.h, _ptr(x), out))
+        return np.array(out[:])
+
+    def primitives(self, x, prim=None, acc=None, weight=1.0):
+        """rho, u, p, T at the nodes into prim [(dim+3)*num_dofs] and/or acc += weight * them."""
+        self._st(x, "x")
+        n = (self.info.dim + 3) * self.info.num_dofs
+        check_buffer(prim, n, "prim", nullable=True)
+        check_buffer(acc, n, "acc", nullable=True)
+        _check(lib().rkdg_primitives(self.h, _ptr(x), _ptr(prim), _ptr(acc), float(weight)))
+
+    def interpolate(self, x, ref_pts, out):
+        """The state at reference points of every element: out [neq*NE*npts] (rkdg_interpolate)."""
+        self._st(x, "x")
+        pts = np.ascontiguousarray(ref_pts, np.float64).reshape(-1, self.info.dim)
+        check_buffer(out, self.info.num_equations * self.info.num_elements * len(pts), "out")
+        _check(lib().rkdg_interpolate(self.h, _ptr(x), len(pts), pts.ctypes.data_as(C.POINTER(C.c_double)),
+                                      _ptr(out)))
 
     def hmin(self):
         v = C.c_double(0.0)

@@ -167,6 +167,13 @@ struct rkdg_ctx {
   std::vector<double> ref_nodes; // reference node coordinates [nd][dim] (dense path)
   double* d_stage[4] = {nullptr, nullptr, nullptr, nullptr};
   double* scratch = nullptr;     // this device's static scratch array (rkdg_step_noalloc)
+  // diagnostics: element volumes, the means and per-block partial sums of the
+  // totals, the basis table of the last interpolation
+  double* d_vol = nullptr;
+  double* d_dmeans = nullptr;
+  double* d_partial = nullptr;
+  double* d_phi = nullptr;
+  int phi_cap = 0;
   std::vector<double> coords;  // [NE][nv][dim]
   std::vector<double> geo_h;   // [g][NE]
   int64_t n_int = 0, n_bdr = 0;
@@ -260,6 +267,10 @@ void free_ctx(rkdg_ctx* c) {
   for (auto p : c->d_rec) dfree(p);
   for (auto p : c->d_cp) dfree(p);
   dfree(c->d_obj_elems);
+  dfree(c->d_vol);
+  dfree(c->d_dmeans);
+  dfree(c->d_partial);
+  dfree(c->d_phi);
   delete c;
 }
 
@@ -1673,6 +1684,110 @@ int rkdg_step_noalloc(rkdg_ctx* c, double* x, double t, double dt, int s, const 
   int rc = step_impl(c, x, dt, s, a, b, W, lambda_dev != nullptr);
   if (rc) return rc;
   if (lambda_dev) HIP_TRY(launch_copy(lambda_dev, c->d_lam + (int64_t)(s - 1) * c->count, 1, c->stream));
+  return RKDG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Post-step diagnostics
+
+namespace {
+constexpr int kTotalBlocks = 512;
+
+double element_volume(const rkdg_ctx* c, int64_t e) {
+  const double* X = &c->coords[(size_t)e * c->nv * c->dim];
+  if (c->geom == RKDG_GEOM_TRI)
+    return 0.5 * std::fabs((X[2] - X[0]) * (X[5] - X[1]) - (X[3] - X[1]) * (X[4] - X[0]));
+  if (c->geom == RKDG_GEOM_QUAD) {  // straight-sided quadrilateral: the polygon's area
+    double a = 0.0;
+    for (int v = 0; v < 4; v++) {
+      const int w = (v + 1) % 4;
+      a += X[2 * v] * X[2 * w + 1] - X[2 * w] * X[2 * v + 1];
+    }
+    return 0.5 * std::fabs(a);
+  }
+  double J[9];  // affine hexahedron: |det J|, columns X1 - X0, X3 - X0, X4 - X0
+  const int cols[3] = {1, 3, 4};
+  for (int i = 0; i < 3; i++)
+    for (int r = 0; r < 3; r++) J[i * 3 + r] = X[cols[r] * 3 + i] - X[i];
+  return std::fabs(J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+                   J[2] * (J[3] * J[7] - J[4] * J[6]));
+}
+}  // namespace
+
+// Conservation totals: integral over the mesh of every conserved variable,
+// sum_e |e| mean_e (the element means are exact quadratures, so this is
+// sum_e sum_k w_k u_k of any order; check_total_energy.cpp:3-63 forms the last
+// variable's total with the order-(p+1) rule at the nodes, exact for Q1 only).
+int rkdg_totals(rkdg_ctx* c, const double* x, double* totals) {
+  if (!c || !x || !totals) return fail(RKDG_E_ARG, "null argument");
+  const int64_t NE = c->NE;
+  if (!c->d_vol) {
+    std::vector<double> vol((size_t)NE);
+    for (int64_t e = 0; e < NE; e++) vol[e] = element_volume(c, e);
+    HIP_TRY(hipMalloc(&c->d_vol, sizeof(double) * NE));
+    HIP_TRY(hipMemcpy(c->d_vol, vol.data(), sizeof(double) * NE, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&c->d_dmeans, sizeof(double) * c->neq * NE));
+    HIP_TRY(hipMalloc(&c->d_partial, sizeof(double) * c->neq * kTotalBlocks));
+  }
+  MeansArgs ma{x, c->d_dmeans, c->d_geo, NE};
+  HIP_TRY(c->dense ? launch_dense_means(c->geom, c->p, ma, c->d_dtab, c->stream)
+                   : launch_tensor_means(c->dim, c->p, ma, c->d_tb, c->stream));
+  HIP_TRY(launch_totals(c->d_dmeans, c->d_vol, NE, c->neq, c->d_partial, kTotalBlocks, c->stream));
+  std::vector<double> part((size_t)c->neq * kTotalBlocks);
+  HIP_TRY(hipMemcpyAsync(part.data(), c->d_partial, sizeof(double) * part.size(), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int v = 0; v < c->neq; v++) {
+    double s = 0.0;
+    for (int b = 0; b < kTotalBlocks; b++) s += part[(size_t)b * c->neq + v];
+    totals[v] = s;
+  }
+  return RKDG_OK;
+}
+
+// Primitive fields at the nodes (main.cpp:388-451): prim[q*NE*nd + i] for
+// q = rho, u_0..u_{dim-1}, p, T; acc (nullable) += weight * the same (the time
+// integrals UMean, pMean, TMean, rhoMean of main.cpp:705-730 use weight dt).
+int rkdg_primitives(rkdg_ctx* c, const double* x, double* prim, double* acc, double weight) {
+  if (!c || !x || (!prim && !acc)) return fail(RKDG_E_ARG, "null argument");
+  HIP_TRY(launch_primitives(x, c->ndofs, c->dim, c->sp.gamma, c->sp.gas_constant, c->sp.covolume, prim, acc, weight,
+                            c->stream));
+  return RKDG_OK;
+}
+
+// The state at npts reference points of every element (host ref_pts [npts][dim]
+// in the reference element: [0,1]^dim, or the unit triangle); out (DEVICE)
+// [v][NE][npts]. Used for output at the vertices (ParaView) and probes.
+int rkdg_interpolate(rkdg_ctx* c, const double* x, int npts, const double* ref_pts, double* out) {
+  if (!c || !x || !ref_pts || !out) return fail(RKDG_E_ARG, "null argument");
+  if (npts < 1 || npts > 4096) return fail(RKDG_E_ARG, "1 <= npts <= 4096");
+  const int nd = c->nd, dim = c->dim;
+  std::vector<double> phi((size_t)npts * nd);
+  if (c->dense) {
+    RefBasis rb;
+    rb.init(c->geom == RKDG_GEOM_TRI ? 1 : 0, c->p);
+    for (int q = 0; q < npts; q++) rb.eval(ref_pts[q * 2], ref_pts[q * 2 + 1], &phi[(size_t)q * nd], nullptr, nullptr);
+  } else {
+    const int N = c->p + 1;
+    for (int q = 0; q < npts; q++)
+      for (int k = 0; k < nd; k++) {
+        double v = 1.0;
+        int kk = k;
+        for (int d = 0; d < dim; d++) {
+          v *= lag(c->tb.x, N, kk % N, ref_pts[q * dim + d]);
+          kk /= N;
+        }
+        phi[(size_t)q * nd + k] = v;
+      }
+  }
+  if (c->phi_cap < npts * nd) {
+    dfree(c->d_phi);
+    c->d_phi = nullptr;
+    HIP_TRY(hipMalloc(&c->d_phi, sizeof(double) * npts * nd));
+    c->phi_cap = npts * nd;
+  }
+  HIP_TRY(hipMemcpyAsync(c->d_phi, phi.data(), sizeof(double) * phi.size(), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(launch_interp(x, c->NE, nd, c->neq, c->d_phi, npts, out, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));  // the host table must outlive the copy
   return RKDG_OK;
 }
 

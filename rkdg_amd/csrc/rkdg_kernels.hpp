@@ -220,6 +220,15 @@ hipError_t launch_tensor_limit(int dim, int p, const LimitArgs& a, const SchemeP
 hipError_t launch_tensor_means(int dim, int p, const MeansArgs& a, const TensorTables* t,
                                hipStream_t s);
 hipError_t launch_fill(double* p, double v, int64_t n, hipStream_t s);
+// diagnostics (rkdg_io.hip): partial[b*neq + v] over nblocks fixed element
+// ranges; primitive fields (and their weighted accumulation); evaluation at
+// reference points with a basis table phi[npts][nd]
+hipError_t launch_totals(const double* means, const double* vol, int64_t NE, int neq, double* partial, int nblocks,
+                         hipStream_t s);
+hipError_t launch_primitives(const double* x, int64_t ndofs, int dim, double gamma, double R, double b, double* prim,
+                             double* acc, double w, hipStream_t s);
+hipError_t launch_interp(const double* x, int64_t NE, int nd, int neq, const double* phi, int npts, double* out,
+                         hipStream_t s);
 // the static scratch array of rkdg_step_noalloc (rkdg_noalloc.hip): its
 // address on the current device, looked up once per device
 constexpr int64_t kScratchDoubles = int64_t(1) << 24;
