@@ -354,6 +354,17 @@ struct Trans {
   const real* X;  // nv*dim vertex coordinates
   void jacobian(const real* xi, real* J) const {
     int d = fe->dim;
+    if (fe->geom == REF_GEOM_QUAD) {
+      // the bilinear map's derivatives written on edge vectors,
+      // dx/dxi = (1-eta)(X1-X0) + eta(X2-X3), dx/deta = (1-xi)(X3-X0) + xi(X2-X1):
+      // the same map as MFEM's PointMat * dshape, without the rounding that
+      // sum leaves when |X| is much larger than the element (|X|/h ulps in J)
+      for (int i = 0; i < 2; i++) {
+        J[i * 2 + 0] = (1 - xi[1]) * (X[2 + i] - X[i]) + xi[1] * (X[4 + i] - X[6 + i]);
+        J[i * 2 + 1] = (1 - xi[0]) * (X[6 + i] - X[i]) + xi[0] * (X[4 + i] - X[2 + i]);
+      }
+      return;
+    }
     std::fill(J, J + d * d, 0.0);
     for (int v = 0; v < fe->nv; v++) {
       real dN[3];
@@ -850,18 +861,29 @@ struct Ref {
   void detect_affine() {
     affine.assign(NE, 1);
     if (fe.geom == REF_GEOM_TRI) return;
+    // quadrilaterals are mapped bilinearly everywhere, as MFEM maps them; the
+    // edge-vector Jacobian above is exact on parallelograms already
+    if (fe.geom == REF_GEOM_QUAD) {
+      affine.assign(NE, 0);
+      return;
+    }
     for (int64_t e = 0; e < NE; e++) {
       real J[9];
       affine_jacobian(e, J);
       const real* Xe = X.data() + e * nv * dim;
-      real scale = 0;
+      real scale = 0, big = 0;
       for (int i = 0; i < dim * dim; i++) scale = std::max(scale, std::fabs(J[i]));
+      for (int k = 0; k < nv * dim; k++) big = std::max(big, std::fabs(Xe[k]));
+      // affine to within rounding of the coordinates (1e-13 of the size or 16
+      // ulps of the largest double coordinate); beyond that the element is
+      // mapped exactly, as MFEM maps every element
+      const real tol = std::max(real(1e-13) * scale, real(16.0 * 2.220446049250313e-16) * big);
       for (int v = 0; v < nv; v++) {
         const real* R = fe.vert(v);
         for (int i = 0; i < dim; i++) {
           real xv = Xe[i];
           for (int r = 0; r < dim; r++) xv += J[i * dim + r] * R[r];
-          if (std::fabs(xv - Xe[v * dim + i]) > 1e-10 * scale) affine[e] = 0;
+          if (std::fabs(xv - Xe[v * dim + i]) > tol) affine[e] = 0;
         }
       }
     }

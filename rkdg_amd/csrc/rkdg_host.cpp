@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -306,6 +307,18 @@ double min_singular(int d, const double* J) {
 }
 
 // ---------------------------------------------------------------------------
+// An element is taken as affine (parallelogram, parallelepiped) when its last
+// vertices sit where the affine map through the first ones puts them, to
+// within rounding of the coordinates: 1e-13 of the element's size or 16 ulps
+// of its largest coordinate. A larger defect is geometry the reference maps
+// exactly (bilinear/trilinear Jacobians per point); treating it as affine
+// would move the right-hand side by about defect / size.
+static double affine_tol(double size, const double* X, int n) {
+  double big = 0.0;
+  for (int k = 0; k < n; k++) big = std::max(big, std::fabs(X[k]));
+  return std::max(1e-13 * size, 16.0 * big * std::numeric_limits<double>::epsilon());
+}
+
 // Dense small-element path: affine triangles P_p and bilinear quadrilaterals Q_p.
 
 bool invert_small(int n, const double* A, double* Ai) {
@@ -436,17 +449,18 @@ const int kQuadEdge[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
 const double kTriV[3][2] = {{0, 0}, {1, 0}, {0, 1}};
 const double kQuadV[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
 
-// bilinear quad map: Jacobian at (x, y) from vertex coordinates X[4][2]; J row-major
+// bilinear quad map: Jacobian at (x, y) from vertex coordinates X[4][2]; J row-major.
+// Written on the edge vectors, dX/dx = (1-y)(X1-X0) + y(X2-X3) and
+// dX/dy = (1-x)(X3-X0) + x(X2-X1): the differences of neighbouring vertices are
+// exact, so J keeps full relative precision however far the element sits from
+// the origin. The vertex-weighted sum sum_v X_v dN_v (MFEM's PointMat * dshape)
+// loses the ratio |X| / size in relative precision (1e-14 on a mesh of
+// 5e-4 elements at x = 0.05), which the cancelling volume and face terms of
+// the right-hand side amplify.
 void bilinear_jac(const double* X, double x, double y, double* J) {
-  const double dNx[4] = {-(1 - y), (1 - y), y, -y};
-  const double dNy[4] = {-(1 - x), -x, x, (1 - x)};
   for (int i = 0; i < 2; i++) {
-    J[i * 2 + 0] = 0.0;
-    J[i * 2 + 1] = 0.0;
-    for (int v = 0; v < 4; v++) {
-      J[i * 2 + 0] += X[v * 2 + i] * dNx[v];
-      J[i * 2 + 1] += X[v * 2 + i] * dNy[v];
-    }
+    J[i * 2 + 0] = (1 - y) * (X[2 + i] - X[i]) + y * (X[4 + i] - X[6 + i]);
+    J[i * 2 + 1] = (1 - x) * (X[6 + i] - X[i]) + x * (X[4 + i] - X[2 + i]);
   }
 }
 
@@ -821,8 +835,12 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
     for (int64_t e = 0; e < NE && !dense; e++) {
       const double* X = &c->coords[e * 8];
       const double sc = std::fabs(X[2] - X[0]) + std::fabs(X[3] - X[1]) + std::fabs(X[6] - X[0]) + std::fabs(X[7] - X[1]);
+      // the tensor kernels take the element as a parallelogram; the reference
+      // maps it bilinearly, and the right-hand side moves by about 40x the
+      // relative geometry defect (the cancelling volume and face terms), so
+      // only defects of 1e-14 of the element size stay on the tensor path
       for (int i = 0; i < 2; i++)
-        if (std::fabs(X[4 + i] - (X[2 + i] + X[6 + i] - X[i])) > 1e-10 * sc) dense = true;
+        if (std::fabs(X[4 + i] - (X[2 + i] + X[6 + i] - X[i])) > 1e-14 * sc) dense = true;
     }
   }
   if (dense) return build_dense(c, m);
@@ -888,7 +906,7 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
       for (int i = 0; i < dim; i++) {
         double xv = X[0 * dim + i];
         for (int r = 0; r < dim; r++) xv += J[i * dim + r] * ref[r];
-        if (std::fabs(xv - X[v * dim + i]) > 1e-10 * scale) c->affine = 0;
+        if (std::fabs(xv - X[v * dim + i]) > affine_tol(scale, X, nv * dim)) c->affine = 0;
       }
     }
     double A[9], det;

@@ -128,22 +128,25 @@ class RunStats:
 
 def run(f: FE_Evolution, solver: ExplicitRKLimitedSolver, limiter: Limiter, sol, t_final: float,
         cfl: float, t0: float = 0.0, dt_const: float = None, max_steps: int = None,
-        callback=None) -> RunStats:
+        callback=None, dt0: float = None) -> RunStats:
     """The time loop of main.cpp: limits the projected state (main.cpp:265), takes
     the first dt from face wave speeds (:467-484), then steps with
     dt_real = min(dt, t_end - t) (:512), recomputes dt from the last RHS
     (:538-553) and advances t by that dt (:701) until t >= t_end - 1e-8 dt (:733).
-    ``dt_const`` replaces the dynamic dt (time.type: constant)."""
+    ``dt_const`` replaces the dynamic dt (time.type: constant); ``dt0`` is the
+    first dt of a restarted run (the frame's saved dt, main.cpp:485-488)."""
     limiter.update(sol)
     solver.Init(f)
     p = f.order
     hmin = f.hmin()
     t = t0
-    if dt_const is None:
+    if dt_const is not None:
+        dt = dt_const
+    elif dt0 is not None:
+        dt = dt0
+    else:
         lam = f.FaceMaxCharSpeed(sol)
         dt = cfl * hmin / lam / (2 * p + 1)
-    else:
-        dt = dt_const
     steps = 0
     done = False
     while not done:
