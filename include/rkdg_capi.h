@@ -186,12 +186,40 @@ int rkdg_hmin(rkdg_ctx* ctx, double* hmin);
 int rkdg_totals(rkdg_ctx* ctx, const double* x, double* totals);
 int rkdg_primitives(rkdg_ctx* ctx, const double* x, double* prim, double* acc, double weight);
 int rkdg_interpolate(rkdg_ctx* ctx, const double* x, int npts, const double* ref_pts, double* out);
+/* Host-pointer variants of rkdg_rhs, rkdg_limit and rkdg_step (x, y and
+ * troubled are HOST arrays): the call copies the state to device buffers of its
+ * own, runs the operator, copies the result back and synchronizes, reporting
+ * a deferred device error as rkdg_check_error does. For tests and small
+ * callers; they allocate per call. */
+int rkdg_rhs_host(rkdg_ctx* ctx, const double* x, double* y, double* lambda_max);
+int rkdg_limit_host(rkdg_ctx* ctx, double* x, int8_t* troubled);
+int rkdg_step_host(rkdg_ctx* ctx, double* x, double t, double dt, int s, const double* a,
+                   const double* b, const double* c, double* lambda_max);
+/* The physics plug-ins on the host, the same code the kernels run.
+ * rkdg_riemann_eval: RiemannSolver::Eval (rs_basic.hpp:49-50) -- the numerical
+ * flux per unit normal through nor, which is normalized here (the reference's
+ * face integrator normalizes CalcOrtho's normal before Eval,
+ * face_integrator.cpp:152-162); max_speed = the solver's wave speed or -1 for
+ * an unphysical input (then RKDG_E_UNPHYSICAL). rkdg_bc_right_state:
+ * BoundaryIntegrator::computeRightState (boundary_integrator.hpp:63) of the
+ * scheme's boundary slot, nor the outward normal (normalized here). States hold
+ * dim+2 conserved variables. */
+int rkdg_riemann_eval(const rkdg_scheme_desc* scheme, int dim, const double* state1, const double* state2,
+                      const double* nor, double* flux, double* max_speed);
+int rkdg_bc_right_state(const rkdg_scheme_desc* scheme, int dim, int slot, const double* state1,
+                        const double* nor, double* state2);
 /* Physical coordinates of the nodes, HOST [NE][nd][dim] (for initial conditions). */
 int rkdg_node_coords(rkdg_ctx* ctx, double* xyz);
 /* Synchronize and report a deferred device error (RKDG_E_UNPHYSICAL, or
  * RKDG_E_NAN when an input of the failing flux was not finite, with the
  * element pair in info[0], info[1]); clears it. */
 int rkdg_check_error(rkdg_ctx* ctx, int64_t* info);
+/* The element pair of the last device error reported by any entry point of
+ * this context ({-1, -1} before the first): an entry point that returns
+ * RKDG_E_UNPHYSICAL or RKDG_E_NAN itself has already cleared the error, and
+ * this is where its pair stays (the reference prints it,
+ * face_integrator.cpp:214-218). */
+int rkdg_error_elements(rkdg_ctx* ctx, int64_t* pair);
 
 /* Instrumentation, no counterpart in the reference (which only has a wall-clock
  * StopWatch per step, main.cpp:458-459): when enabled, HIP events on the

@@ -32,7 +32,7 @@ def _run(cmd, verbose):
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed: " + " ".join(cmd[-3:]))
+        raise RuntimeError(f"{os.path.basename(cmd[0])} failed: " + " ".join(cmd[-3:]))
 
 
 def build(verbose: bool = False) -> str:
@@ -48,7 +48,24 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp], verbose)
     os.replace(tmp, LIB)
+    build_callers(verbose)
     return LIB
+
+
+def build_callers(verbose: bool = False) -> None:
+    """The compiled callers of the C-ABI that the tests run: a C program on the
+    plain header (gcc) and a C++ program on the class mirror include/rkdg.hpp
+    (hipcc for the HIP runtime's allocation calls); both link librkdg.so."""
+    root = os.path.dirname(HERE)
+    capi = os.path.join(root, "tests", "capi")
+    if not os.path.isdir(capi):
+        return
+    inc = ["-I", os.path.join(root, "include"), "-I", capi]
+    link = ["-L", HERE, "-lrkdg", "-Wl,-rpath,$ORIGIN/../../rkdg_amd"]
+    _run(["gcc", "-std=c11", "-O2", "-Wall", *inc, os.path.join(capi, "capi_smoke.c"), *link, "-lm",
+          "-o", os.path.join(capi, "capi_smoke")], verbose)
+    _run([HIPCC, *FLAGS, "-Wall", *inc, os.path.join(capi, "cpp_smoke.cpp"), *link,
+          "-o", os.path.join(capi, "cpp_smoke")], verbose)
 
 
 if __name__ == "__main__":

@@ -64,6 +64,8 @@ EXPORTS = [
     "rkdg_set_bc_params", "rkdg_rhs_adjoint", "rkdg_limit_adjoint", "rkdg_objective",
     "rkdg_step_adjoint", "rkdg_forward_solve", "rkdg_adjoint_solve", "rkdg_step_noalloc",
     "rkdg_step_ssprk3", "rkdg_totals", "rkdg_primitives", "rkdg_interpolate",
+    "rkdg_rhs_host", "rkdg_limit_host", "rkdg_step_host", "rkdg_riemann_eval", "rkdg_bc_right_state",
+    "rkdg_error_elements",
 ]
 
 # include/rkdg_selftest.h (host-only)
@@ -109,6 +111,7 @@ def lib():
     L.rkdg_hmin.argtypes = [P, C.POINTER(C.c_double)]
     L.rkdg_node_coords.argtypes = [P, C.POINTER(C.c_double)]
     L.rkdg_check_error.argtypes = [P, C.POINTER(C.c_int64)]
+    L.rkdg_error_elements.argtypes = [P, C.POINTER(C.c_int64)]
     L.rkdg_profile_enable.argtypes = [P, C.c_int]
     L.rkdg_profile_read.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                     C.POINTER(C.c_double), C.POINTER(C.c_int64)]
@@ -130,6 +133,11 @@ def lib():
     L.rkdg_adjoint_solve.argtypes = [P, dp, C.c_int64, pd, C.c_int, pd, pd, pd, C.c_int, C.c_int64,
                                      pd, dp, pd, dp]
     SD = C.POINTER(SchemeDesc)
+    L.rkdg_rhs_host.argtypes = [P, pd, pd, pd]
+    L.rkdg_limit_host.argtypes = [P, pd, C.POINTER(C.c_int8)]
+    L.rkdg_step_host.argtypes = [P, pd, C.c_double, C.c_double, C.c_int, pd, pd, pd, pd]
+    L.rkdg_riemann_eval.argtypes = [SD, C.c_int, pd, pd, pd, pd, pd]
+    L.rkdg_bc_right_state.argtypes = [SD, C.c_int, C.c_int, pd, pd, pd]
     L.rkdg_selftest_riemann.restype = C.c_double
     L.rkdg_selftest_riemann.argtypes = [SD, pd, pd, pd, pd]
     L.rkdg_selftest_riemann_vjp.argtypes = [SD, pd, pd, pd, pd, pd, pd]

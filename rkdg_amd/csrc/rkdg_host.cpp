@@ -150,6 +150,7 @@ struct rkdg_ctx {
   int* d_fcode = nullptr;
   double* d_lam = nullptr;    // per-stage wave speeds [8]
   long long* d_err = nullptr;
+  int64_t err_pair[2] = {-1, -1};  // element pair of the last reported device error
   double* d_means = nullptr;
   long long* h_pin = nullptr;    // pinned host words: device error word [4], then the wave speeds [count]
   int* d_order = nullptr;        // tensor contexts: work blocks in Morton order of their centroids
@@ -768,27 +769,11 @@ int build_two_colour(rkdg_ctx* c, const std::vector<int>& nbr) {
   return RKDG_OK;
 }
 
-int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
-  if (m->geom != RKDG_GEOM_QUAD && m->geom != RKDG_GEOM_HEX && m->geom != RKDG_GEOM_TRI)
-    return fail(RKDG_E_ARG, "unknown geometry");
-  c->geom = m->geom;
-  c->p = m->order;
-  c->dim = m->geom == RKDG_GEOM_HEX ? 3 : 2;
-  if (m->geom != RKDG_GEOM_TRI && !tensor_supported(c->dim, c->p))
-    return fail(RKDG_E_UNSUPPORTED, "polynomial order not instantiated");
-  c->nv = c->dim == 3 ? 8 : 4;
-  c->nvf = c->dim == 3 ? 4 : 2;
-  c->nf = 2 * c->dim;
-  const int N = c->p + 1;
-  c->nd = c->dim == 3 ? N * N * N : N * N;
-  c->neq = c->dim + 2;
-  c->NE = m->num_elements;
-  c->ndofs = c->NE * c->nd;
-  if (c->NE <= 0) return fail(RKDG_E_ARG, "empty mesh");
-  if (c->NE > (int64_t)2147483647 / 8) return fail(RKDG_E_ARG, "too many elements for 32-bit neighbour ids");
+// rkdg_scheme_desc -> SchemeParams, with the checks of the scheme's plug-in
+// choices (the reference's case manager stops on an unknown name,
+// case_manager.cpp:523-612, 614-820)
+static int scheme_params(const rkdg_scheme_desc* s, SchemeParams& g) {
   if (s->num_bc < 0 || s->num_bc > RKDG_MAX_BC) return fail(RKDG_E_ARG, "num_bc out of range");
-  // the scheme's plug-in choices (the reference's case manager stops on an
-  // unknown name, case_manager.cpp:523-612, 614-820)
   if (s->riemann < RKDG_RS_RUSANOV || s->riemann > RKDG_RS_HLLC) return fail(RKDG_E_ARG, "unknown Riemann solver");
   if (s->wave_speeds != RKDG_WS_TORO && s->wave_speeds != RKDG_WS_EINFELDT)
     return fail(RKDG_E_ARG, "unknown wave-speed estimate");
@@ -800,13 +785,6 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
   for (int i = 0; i < s->num_bc; i++)
     if (s->bc_type[i] < RKDG_BC_SLIP || s->bc_type[i] > RKDG_BC_CHAR_OUTLET)
       return fail(RKDG_E_ARG, "unknown boundary condition type in slot " + std::to_string(i));
-  const int dim = c->dim, nv = c->nv, NE_ = (int)c->NE;
-  const int64_t NE = c->NE;
-  if (m->geom == RKDG_GEOM_TRI) c->nv = 3;
-  c->coords.assign(m->elem_coords, m->elem_coords + NE * c->nv * dim);
-
-  // scheme
-  SchemeParams& g = c->sp;
   g.riemann = s->riemann;
   g.wave_speeds = s->wave_speeds;
   g.hllc_abs = s->hllc_abs;
@@ -825,6 +803,36 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
   g.limiter = s->limiter;
   g.last_hope = s->last_hope;
   g.pow_nd = s->pow_nd;
+  return RKDG_OK;
+}
+
+int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
+  if (m->geom != RKDG_GEOM_QUAD && m->geom != RKDG_GEOM_HEX && m->geom != RKDG_GEOM_TRI)
+    return fail(RKDG_E_ARG, "unknown geometry");
+  c->geom = m->geom;
+  c->p = m->order;
+  c->dim = m->geom == RKDG_GEOM_HEX ? 3 : 2;
+  if (m->geom != RKDG_GEOM_TRI && !tensor_supported(c->dim, c->p))
+    return fail(RKDG_E_UNSUPPORTED, "polynomial order not instantiated");
+  c->nv = c->dim == 3 ? 8 : 4;
+  c->nvf = c->dim == 3 ? 4 : 2;
+  c->nf = 2 * c->dim;
+  const int N = c->p + 1;
+  c->nd = c->dim == 3 ? N * N * N : N * N;
+  c->neq = c->dim + 2;
+  c->NE = m->num_elements;
+  c->ndofs = c->NE * c->nd;
+  if (c->NE <= 0) return fail(RKDG_E_ARG, "empty mesh");
+  if (c->NE > (int64_t)2147483647 / 8) return fail(RKDG_E_ARG, "too many elements for 32-bit neighbour ids");
+  {
+    const int rc = scheme_params(s, c->sp);
+    if (rc) return rc;
+  }
+  SchemeParams& g = c->sp;
+  const int dim = c->dim, nv = c->nv, NE_ = (int)c->NE;
+  const int64_t NE = c->NE;
+  if (m->geom == RKDG_GEOM_TRI) c->nv = 3;
+  c->coords.assign(m->elem_coords, m->elem_coords + NE * c->nv * dim);
 
   // triangles and non-parallelogram quadrilaterals take the dense path (any
   // quadrilateral mesh when asked for)
@@ -1192,6 +1200,8 @@ int check_device_error(rkdg_ctx* c, int64_t* info) {
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (h[0]) {
     const long long kind = h[0], e1 = h[1], e2 = h[2];
+    c->err_pair[0] = e1;
+    c->err_pair[1] = e2;
     if (info) {
       info[0] = e1;
       info[1] = e2;
@@ -1887,6 +1897,13 @@ int rkdg_check_error(rkdg_ctx* c, int64_t* info) {
   return check_device_error(c, info);
 }
 
+int rkdg_error_elements(rkdg_ctx* c, int64_t* pair) {
+  if (!c || !pair) return fail(RKDG_E_ARG, "null argument");
+  pair[0] = c->err_pair[0];
+  pair[1] = c->err_pair[1];
+  return RKDG_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Discrete adjoint
 
@@ -2320,6 +2337,119 @@ int rkdg_tune_grid(rkdg_ctx* c, const double* x, const int* candidates, int ncan
   if (rc) return rc;
   if (best) *best = best_cap;
   return rkdg_batch_set_options(c, c->use_graph, best_cap);
+}
+
+// ---------------------------------------------------------------- host-pointer variants
+// rkdg_rhs / rkdg_limit / rkdg_step on HOST arrays: the state is copied to
+// device buffers of the call, the operator runs on the context's stream and
+// the result is copied back before returning (for tests and small callers;
+// the device-pointer entry points are the fast path).
+namespace {
+struct DevBuf {
+  double* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+}  // namespace
+
+int rkdg_rhs_host(rkdg_ctx* c, const double* x, double* y, double* lambda_max) {
+  if (!c || !x || !y) return fail(RKDG_E_ARG, "null argument");
+  const size_t n = (size_t)c->neq * c->ndofs;
+  DevBuf dx, dy;
+  HIP_TRY(hipMalloc(&dx.p, sizeof(double) * n));
+  HIP_TRY(hipMalloc(&dy.p, sizeof(double) * n));
+  HIP_TRY(hipMemcpyAsync(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  double lam = 0.0;
+  int rc = rkdg_rhs(c, dx.p, dy.p, &lam);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(y, dy.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  rc = check_device_error(c, nullptr);
+  if (rc) return rc;
+  if (lambda_max) *lambda_max = lam;
+  return RKDG_OK;
+}
+
+int rkdg_limit_host(rkdg_ctx* c, double* x, int8_t* troubled) {
+  if (!c || !x) return fail(RKDG_E_ARG, "null argument");
+  const size_t n = (size_t)c->neq * c->ndofs;
+  DevBuf dx;
+  int8_t* dt = nullptr;
+  HIP_TRY(hipMalloc(&dx.p, sizeof(double) * n));
+  if (troubled) HIP_TRY(hipMalloc(&dt, (size_t)c->NE));
+  struct FreeFlags {
+    int8_t* q;
+    ~FreeFlags() {
+      if (q) (void)hipFree(q);
+    }
+  } keep{dt};
+  HIP_TRY(hipMemcpyAsync(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  if (dt) HIP_TRY(hipMemsetAsync(dt, 0, (size_t)c->NE, c->stream));
+  int rc = rkdg_limit(c, dx.p, dt);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  if (dt) HIP_TRY(hipMemcpyAsync(troubled, dt, (size_t)c->NE, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return check_device_error(c, nullptr);
+}
+
+int rkdg_step_host(rkdg_ctx* c, double* x, double t, double dt, int s, const double* a, const double* b,
+                   const double* cc, double* lambda_max) {
+  if (!c || !x) return fail(RKDG_E_ARG, "null argument");
+  const size_t n = (size_t)c->neq * c->ndofs;
+  DevBuf dx;
+  HIP_TRY(hipMalloc(&dx.p, sizeof(double) * n));
+  HIP_TRY(hipMemcpyAsync(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  double lam = 0.0;
+  int rc = rkdg_step(c, dx.p, t, dt, s, a, b, cc, &lam);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  rc = check_device_error(c, nullptr);
+  if (rc) return rc;
+  if (lambda_max) *lambda_max = lam;
+  return RKDG_OK;
+}
+
+// ---------------------------------------------------------------- physics plug-ins on the host
+// The device code's Riemann solvers and boundary states, compiled for the
+// host: RiemannSolver::Eval (rs_basic.hpp:49-50) and
+// BoundaryIntegrator::computeRightState (boundary_integrator.hpp:63).
+int rkdg_riemann_eval(const rkdg_scheme_desc* s, int dim, const double* state1, const double* state2,
+                      const double* nor, double* flux, double* max_speed) {
+  if (!s || !state1 || !state2 || !nor || !flux) return fail(RKDG_E_ARG, "null argument");
+  if (dim != 2 && dim != 3) return fail(RKDG_E_ARG, "dim must be 2 or 3");
+  SchemeParams g;
+  std::memset(&g, 0, sizeof(g));
+  int rc = scheme_params(s, g);
+  if (rc) return rc;
+  double n[3] = {nor[0], nor[1], dim == 3 ? nor[2] : 0.0};
+  const double len = dim == 3 ? norml2_scaled<3>(n) : norml2_scaled<2>(n);
+  if (!(len > 0.0)) return fail(RKDG_E_ARG, "zero normal");
+  for (int i = 0; i < dim; i++) n[i] /= len;
+  bool ok = true;
+  const double lam = dim == 3 ? riemann<3>(g, state1, state2, n, flux, ok) : riemann<2>(g, state1, state2, n, flux, ok);
+  if (max_speed) *max_speed = ok ? lam : -1.0;
+  return ok && lam >= 0.0 ? RKDG_OK : fail(RKDG_E_UNPHYSICAL, "unphysical state in the Riemann solver");
+}
+
+int rkdg_bc_right_state(const rkdg_scheme_desc* s, int dim, int slot, const double* state1, const double* nor,
+                        double* state2) {
+  if (!s || !state1 || !nor || !state2) return fail(RKDG_E_ARG, "null argument");
+  if (dim != 2 && dim != 3) return fail(RKDG_E_ARG, "dim must be 2 or 3");
+  SchemeParams g;
+  std::memset(&g, 0, sizeof(g));
+  int rc = scheme_params(s, g);
+  if (rc) return rc;
+  if (slot < 0 || slot >= g.num_bc) return fail(RKDG_E_ARG, "boundary slot out of range");
+  double n[3] = {nor[0], nor[1], dim == 3 ? nor[2] : 0.0};
+  const double len = dim == 3 ? norml2_scaled<3>(n) : norml2_scaled<2>(n);
+  if (!(len > 0.0)) return fail(RKDG_E_ARG, "zero normal");
+  for (int i = 0; i < dim; i++) n[i] /= len;
+  if (dim == 3) bc_state<3>(g, slot, state1, n, state2);
+  else bc_state<2>(g, slot, state1, n, state2);
+  return RKDG_OK;
 }
 
 }  // extern "C"
