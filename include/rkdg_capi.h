@@ -81,6 +81,9 @@ extern "C" {
 #define RKDG_IND_NOWHERE 0
 #define RKDG_IND_EVERYWHERE 1
 #define RKDG_IND_BJ 2
+/* Shu's indicator on neighbour-extrapolated means (indicator_shu.cpp:13-74,
+ * averager.cpp:152-273); it sets no multiplier, so it pairs with FinDiff only */
+#define RKDG_IND_SHU 3
 
 #define RKDG_PHI_MICHALAK 0
 #define RKDG_PHI_BJ 1
@@ -124,6 +127,8 @@ typedef struct {
   int limiter;      /* RKDG_LIM_* (case_manager.cpp:596-609) */
   int last_hope;    /* haveLastHope */
   int pow_nd;       /* 1: multiplier applied nd times (limiter_multiplier.cpp:24-38) */
+  int linearize;    /* Multi: replace the element by its linear reconstruction first
+                       (limiter_multiplier.cpp:101-178); p = 1 quadrilaterals and hexahedra */
 } rkdg_scheme_desc;
 
 typedef struct {

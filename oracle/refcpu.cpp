@@ -405,6 +405,18 @@ real det(int d, const real* J) {
   return J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
          J[2] * (J[3] * J[7] - J[4] * J[6]);
 }
+void adjugate(int d, const real* J, real* A);
+// x = J^-1 b for d = 2, 3 (adjugate over determinant)
+void solve_small(int d, const real* J, const real* b, real* x) {
+  real A[9];
+  adjugate(d, J, A);
+  const real dj = det(d, J);
+  for (int i = 0; i < d; i++) {
+    real s = 0;
+    for (int k = 0; k < d; k++) s += A[i * d + k] * b[k];
+    x[i] = s / dj;
+  }
+}
 // adj(J): J * adj(J) = det(J) I   (MFEM CalcAdjugate)
 void adjugate(int d, const real* J, real* A) {
   if (d == 2) {
@@ -963,6 +975,8 @@ struct Ref {
   int mult(const real* x, real* y, real& mcs, int64_t* bad);
   void compute_means(const real* x);
   void limiter_update(real* x, real* ind_values, real* min_values);
+  void transform_back(int64_t k, const real* x, real* xi) const;
+  void extrap_averages(int64_t t, const real* x, const std::vector<int64_t>& cells, real* out) const;
   real hmin() const;
 };
 
@@ -1257,6 +1271,82 @@ void Ref::compute_means(const real* x) {
   }
 }
 
+// Reference coordinates in element k of the physical point x (TransformBack):
+// the affine inverse, or Newton's method on the bilinear map. The point may lie
+// outside element k (the Shu indicator extrapolates the neighbour's polynomial
+// over the troubled cell); the iteration is not clipped to the element.
+void Ref::transform_back(int64_t k, const real* x, real* xi) const {
+  real J[9];
+  if (affine[k] || fe.geom == REF_GEOM_TRI) {
+    affine_jacobian(k, J);
+    const real* X0 = X.data() + k * nv * dim;
+    real b[3];
+    for (int i = 0; i < dim; i++) b[i] = x[i] - X0[i];
+    solve_small(dim, J, b, xi);
+    return;
+  }
+  Trans T = trans(k);
+  for (int d = 0; d < dim; d++) xi[d] = 0.5;
+  for (int it = 0; it < 60; it++) {
+    real xc[3], r[3], dxi[3];
+    T.transform(xi, xc);
+    for (int i = 0; i < dim; i++) r[i] = x[i] - xc[i];
+    T.jacobian(xi, J);
+    solve_small(dim, J, r, dxi);
+    real step = 0;
+    for (int d = 0; d < dim; d++) {
+      xi[d] += dxi[d];
+      step = std::max(step, std::fabs(dxi[d]));
+    }
+    if (step < 1e-15) break;
+  }
+}
+
+// Averager::computeStencilExtrapAverages for the stencil of cell t
+// (averager.cpp:152-273): for every cell k of the stencil (t itself first, then
+// its face neighbours), the neighbour's polynomial integrated with the weights
+// of the troubled cell's reference rule (order p + OrderW) at the points
+// T_k^-1(T_t(xi_q)), times Trans_k.Weight() at the last point of the mass rule
+// (assembleShiftedElementMatrix reads the weight without setting a point,
+// :262-266), divided by |k| (the integral of the constant test function).
+// out[m * NEQ + v] for the m-th stencil cell.
+void Ref::extrap_averages(int64_t t, const real* x, const std::vector<int64_t>& cells, real* out) const {
+  const int NEQ = ne();
+  Rule ir = geom_rule(fe.geom, fe.p + fe.order_w());
+  std::vector<real> shape(nd);
+  Trans Tt = trans(t);
+  for (size_t m = 0; m < cells.size(); m++) {
+    const int64_t k = cells[m];
+    Trans Tk = trans(k);
+    real vol = 0, wlast = 0;
+    for (int q = 0; q < ir.n; q++) {
+      real J[9];
+      Tk.jacobian(&ir.pts[q * dim], J);
+      const real dJ = det(dim, J);
+      vol += dJ * ir.w[q];
+      wlast = dJ;
+    }
+    real acc[5] = {};
+    for (int q = 0; q < ir.n; q++) {
+      real xp[3], xi[3];
+      Tt.transform(&ir.pts[q * dim], xp);
+      if (k == t) {
+        for (int d = 0; d < dim; d++) xi[d] = ir.pts[q * dim + d];
+      } else {
+        transform_back(k, xp, xi);
+      }
+      fe.shape(xi, shape.data());
+      const real w = wlast * ir.w[q];
+      for (int v = 0; v < NEQ; v++) {
+        real u = 0;
+        for (int i = 0; i < nd; i++) u += shape[i] * x[v * NDofs + k * nd + i];
+        acc[v] += w * u;
+      }
+    }
+    for (int v = 0; v < NEQ; v++) out[m * NEQ + v] = acc[v] / vol;
+  }
+}
+
 // Limiter::update (limiter.cpp:22-94), IndicatorBJ::checkDiscontinuity
 // (indicator_bj.cpp:88-200) with updateYMin (:15-86), LimiterMultiplier::limit
 // (limiter_multiplier.cpp:12-99), LimiterFinDiff::limit (limiter_findiff.cpp:3-33).
@@ -1277,6 +1367,23 @@ void Ref::limiter_update(real* x, real* ind_values, real* min_values) {
     if (sc.s.indicator == REF_IND_NOWHERE || sc.s.indicator == REF_IND_EVERYWHERE) {
       real val = sc.s.indicator == REF_IND_NOWHERE ? 1.0 : 0.0;
       for (int v = 0; v < NEQ; v++) yMin[v] = val;
+    } else if (sc.s.indicator == REF_IND_SHU) {
+      // IndicatorShu::checkDiscontinuity (indicator_shu.cpp:13-74): maxFabsPj
+      // takes the troubled cell's own mean at every stencil cell (:40-48)
+      std::vector<int64_t> cells;
+      cells.push_back(c);
+      for (int64_t nb : el2el[c]) cells.push_back(nb);
+      std::vector<real> ex(cells.size() * NEQ);
+      extrap_averages(c, x, cells, ex.data());
+      const real Ck = 0.03, eps = 1e-6;
+      for (int v = 0; v < NEQ; v++) {
+        real mx = -1e9, sum = 0;
+        for (size_t m = 0; m < cells.size(); m++) {
+          mx = std::fabs(mean[v]) > mx ? std::fabs(mean[v]) : mx;
+          sum += std::fabs(mean[v] - ex[m * NEQ + v]);
+        }
+        yMin[v] = (std::fabs(mx) < eps && std::fabs(sum) < eps) ? real(1) : std::min(real(1), Ck * mx / (sum + eps));
+      }
     } else {
       for (int v = 0; v < NEQ; v++) { mI[v] = 1e9; MI[v] = -1e9; yMin[v] = 1e9; }
       std::vector<int64_t> cells;
@@ -1349,6 +1456,28 @@ void Ref::limiter_update(real* x, real* ind_values, real* min_values) {
     if (min_values) min_values[c] = minV;
     // --- limit
     if (sc.s.limiter == REF_LIM_MULTI) {
+      if (sc.s.linearize && fe.geom != REF_GEOM_TRI) {  // triangles are skipped (limiter_multiplier.cpp:29)
+        // LimiterMultiplier::linearize (limiter_multiplier.cpp:101-178) for p = 1:
+        // u_i = mean - dF . (xi_c - xi_i) with dF the reference gradient at the
+        // centre xi_c (the image of the element centre, transformed back) and
+        // xi_i the points of the order-2 rule, which are the nodes
+        real xc[3] = {0.5, 0.5, 0.5};
+        std::vector<real> dsh(nd * dim);
+        fe.dshape(xc, dsh.data());
+        Rule lr = geom_rule(fe.geom, fe.p + 1);
+        for (int k = 0; k < NEQ; k++) {
+          real dF[3] = {};
+          for (int d = 0; d < dim; d++)
+            for (int i = 0; i < nd; i++) dF[d] += el[k * nd + i] * dsh[i * dim + d];
+          std::vector<real> nv_(nd);
+          for (int i = 0; i < nd; i++) {
+            real u = mean[k];
+            for (int d = 0; d < dim; d++) u -= dF[d] * (xc[d] - lr.pts[i * dim + d]);
+            nv_[i] = u;
+          }
+          for (int i = 0; i < nd; i++) el[k * nd + i] = nv_[i];
+        }
+      }
       int reps = sc.s.pow_nd ? nd : 1;
       for (int k = 0; k < NEQ; k++)
         for (int r = 0; r < reps; r++)
@@ -1491,7 +1620,11 @@ void* ref_create(int geom, int order, int64_t num_elements, const double* elem_c
   r->X.assign(elem_coords, elem_coords + num_elements * r->nv * r->dim);  // widened element-wise
   r->topo.assign(elem_verts, elem_verts + num_elements * r->nv);
   std::string e;
-  if (!r->build(e)) {
+  if (scheme->indicator == REF_IND_SHU && scheme->limiter == REF_LIM_MULTI)
+    e = "the Shu indicator writes no minValues, which the multiplier reads: pair it with FinDiff";
+  else if (scheme->linearize && scheme->limiter == REF_LIM_MULTI && order != 1 && geom != REF_GEOM_TRI)
+    e = "linearize reads nd points of an order p+1 rule: p = 1 quadrilaterals and hexahedra only";
+  if (!e.empty() || !r->build(e)) {
     if (err) snprintf(err, errlen, "%s", e.c_str());
     delete r;
     return nullptr;

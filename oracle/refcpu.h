@@ -73,6 +73,7 @@ extern "C" {
 #define REF_IND_NOWHERE 0
 #define REF_IND_EVERYWHERE 1
 #define REF_IND_BJ 2
+#define REF_IND_SHU 3
 
 #define REF_PHI_MICHALAK 0
 #define REF_PHI_BJ 1
@@ -99,6 +100,7 @@ typedef struct {
   int limiter;      /* REF_LIM_* */
   int last_hope;
   int pow_nd;       /* 1: multiplier applied nd times (limiter_multiplier.cpp:24-38) */
+  int linearize;    /* multiplier: linear reconstruction first (limiter_multiplier.cpp:101-178; p = 1) */
 } ref_scheme;
 
 void* ref_create(int geom, int order, int64_t num_elements, const double* elem_coords,

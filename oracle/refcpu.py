@@ -28,7 +28,7 @@ class RefScheme(C.Structure):
         ("num_bc", C.c_int), ("bc_attr", C.c_int * MAX_BC), ("bc_type", C.c_int * MAX_BC),
         ("bc_param", (C.c_double * 8) * MAX_BC),
         ("indicator", C.c_int), ("phi", C.c_int), ("venkat_k", C.c_double),
-        ("limiter", C.c_int), ("last_hope", C.c_int), ("pow_nd", C.c_int),
+        ("limiter", C.c_int), ("last_hope", C.c_int), ("pow_nd", C.c_int), ("linearize", C.c_int),
     ]
 
 
@@ -104,6 +104,7 @@ def ref_scheme(s) -> RefScheme:
     r.limiter = s.limiter
     r.last_hope = s.last_hope
     r.pow_nd = s.pow_nd
+    r.linearize = s.linearize
     return r
 
 

@@ -41,7 +41,7 @@ class SchemeDesc(C.Structure):
         ("num_bc", C.c_int), ("bc_attr", C.c_int * S.MAX_BC), ("bc_type", C.c_int * S.MAX_BC),
         ("bc_param", (C.c_double * 8) * S.MAX_BC),
         ("indicator", C.c_int), ("phi", C.c_int), ("venkat_k", C.c_double),
-        ("limiter", C.c_int), ("last_hope", C.c_int), ("pow_nd", C.c_int),
+        ("limiter", C.c_int), ("last_hope", C.c_int), ("pow_nd", C.c_int), ("linearize", C.c_int),
     ]
 
 
@@ -197,6 +197,7 @@ def scheme_desc(s: S.Scheme) -> SchemeDesc:
     d.limiter = s.limiter
     d.last_hope = s.last_hope
     d.pow_nd = s.pow_nd
+    d.linearize = s.linearize
     return d
 
 

@@ -19,8 +19,8 @@ time settings and output settings are the ones the reference would run:
   case_manager.cpp:699-815; boundary faces without a patch get no flux;
 * ``spatial``: ``polyOrder``, ``riemannSolver`` (Rusanov, LLF, HLL, HLLC),
   ``limiter.type`` (Multi, FinDiff), ``limiter.indicator`` (Nowhere,
-  Everywhere, BJ), ``limiter.haveLastHope`` (default 1),
-  ``limiter.linearize`` (only 0) -- case_manager.cpp:523-612;
+  Everywhere, BJ, Shu -- with FinDiff), ``limiter.haveLastHope`` (default 1),
+  ``limiter.linearize`` (default 0; p = 1) -- case_manager.cpp:523-612;
 * ``time``: ``order`` (2 or 3, the reference's literal tableaus,
   case_manager.cpp:822-891), ``tEnd``, ``dt``, ``type`` (``constant`` keeps dt,
   anything else takes dt from ``CoMax``; :943-958), ``restart``,
@@ -243,7 +243,7 @@ def _bc_params(typ, node):
     raise CaseError(f"wrong boundary type {typ!r}")
 
 
-_INDICATORS = {"Nowhere": S.IND_NOWHERE, "Everywhere": S.IND_EVERYWHERE, "BJ": S.IND_BJ}
+_INDICATORS = {"Nowhere": S.IND_NOWHERE, "Everywhere": S.IND_EVERYWHERE, "BJ": S.IND_BJ, "Shu": S.IND_SHU}
 _LIMITERS = {"FinDiff": S.LIM_FINDIFF, "Multi": S.LIM_MULTI}
 _PHI = {"michalak": S.PHI_MICHALAK, "bj": S.PHI_BJ, "venkat": S.PHI_VENKAT}
 _SPEEDS = {"Toro": S.TORO, "Einfeldt": S.EINFELDT}
@@ -314,8 +314,10 @@ def load_case(path) -> FileCase:
     lim_name = lim.get("type")
     if lim_name not in _LIMITERS:
         raise CaseError(f"unknown limiter type {lim_name!r} (FinDiff, Multi)")
-    if _flag(lim.get("linearize"), False):
-        raise CaseError("spatial.limiter.linearize is not supported on the RKDG path")
+    if ind_name == "Shu" and lim_name != "FinDiff":
+        # IndicatorShu writes no minValues, which the multiplier reads
+        # (indicator_shu.cpp:68-73, limiter_multiplier.cpp:35)
+        raise CaseError("the Shu indicator sets no multiplier: pair it with the FinDiff limiter")
     phi = _PHI[str(lim.get("phi", "michalak")).lower()]
     speeds = _SPEEDS[_get(st, "spatial", "waveSpeeds", default="Toro")]
 
@@ -335,7 +337,8 @@ def load_case(path) -> FileCase:
     sch = S.Scheme(riemann=S.RIEMANN_NAMES[rs], wave_speeds=speeds, gamma=gamma, gas_constant=R,
                    covolume=covolume, bcs=bcs, indicator=_INDICATORS[ind_name], phi=phi,
                    venkat_k=float(lim.get("venkatK", 0.0)), limiter=_LIMITERS[lim_name],
-                   last_hope=int(_flag(lim.get("haveLastHope"), True)))
+                   last_hope=int(_flag(lim.get("haveLastHope"), True)),
+                   linearize=int(_flag(lim.get("linearize"), False)))
 
     # time (case_manager.cpp:822-891, 943-958)
     tname = _get(st, "time", "tableau")

@@ -461,6 +461,24 @@ __global__ __launch_bounds__(64, 4) void dense_limit_kernel(const LimitArgs A, c
   double yr = ylane;
   if (g.indicator == RKDG_IND_NOWHERE) yr = 1.0;
   else if (g.indicator == RKDG_IND_EVERYWHERE) yr = 0.0;
+  if constexpr (P == 1 && !C::TRI) {
+    // linearize: the multiplier and the last hope work on the element's linear
+    // reconstruction (triangles are skipped, limiter_multiplier.cpp:29)
+    if (g.linearize && g.limiter == RKDG_LIM_MULTI) {
+      double lin[NEQ];
+      const bool nl = active && lane < ND;
+      if (nl) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) lin[v] = linearize_q1<2>(&s_u[team][v][0], mean[v], lane);
+      }
+      __syncthreads();
+      if (nl) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) s_u[team][v][lane] = lin[v];
+      }
+      __syncthreads();
+    }
+  }
   const int j = lane;
   const bool node_lane = active && lane < ND;
   double u[NEQ];

@@ -166,6 +166,8 @@ struct rkdg_ctx {
   TensorTables* d_tb = nullptr;
   int dense = 0;                 // 1: dense small-element kernels (triangles, bilinear quads)
   double* d_dtab = nullptr;      // dense reference tables
+  double* d_shu_coef = nullptr;  // Shu indicator: extrapolation weights [(nf+1)*nd][NE]
+  int8_t* d_shu_flag = nullptr;  // Shu indicator: troubled elements of the last pass
   std::vector<double> ref_nodes; // reference node coordinates [nd][dim] (dense path)
   double* d_stage[4] = {nullptr, nullptr, nullptr, nullptr};
   double* scratch = nullptr;     // this device's static scratch array (rkdg_step_noalloc)
@@ -245,6 +247,8 @@ void free_ctx(rkdg_ctx* c) {
   dfree(c->d_fbuf);
   dfree(c->d_tb);
   dfree(c->d_dtab);
+  dfree(c->d_shu_coef);
+  if (c->d_shu_flag) (void)hipFree(c->d_shu_flag);
   for (auto& p : c->d_stage) dfree(p);
   for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
   dfree(c->d_inst);
@@ -777,7 +781,7 @@ static int scheme_params(const rkdg_scheme_desc* s, SchemeParams& g) {
   if (s->riemann < RKDG_RS_RUSANOV || s->riemann > RKDG_RS_HLLC) return fail(RKDG_E_ARG, "unknown Riemann solver");
   if (s->wave_speeds != RKDG_WS_TORO && s->wave_speeds != RKDG_WS_EINFELDT)
     return fail(RKDG_E_ARG, "unknown wave-speed estimate");
-  if (s->indicator < RKDG_IND_NOWHERE || s->indicator > RKDG_IND_BJ) return fail(RKDG_E_ARG, "unknown indicator");
+  if (s->indicator < RKDG_IND_NOWHERE || s->indicator > RKDG_IND_SHU) return fail(RKDG_E_ARG, "unknown indicator");
   if (s->phi < RKDG_PHI_MICHALAK || s->phi > RKDG_PHI_VENKAT) return fail(RKDG_E_ARG, "unknown limiter function");
   if (s->limiter < RKDG_LIM_NONE || s->limiter > RKDG_LIM_MULTI) return fail(RKDG_E_ARG, "unknown limiter");
   if (!(s->gamma > 1.0)) return fail(RKDG_E_ARG, "gamma must exceed 1");
@@ -803,6 +807,7 @@ static int scheme_params(const rkdg_scheme_desc* s, SchemeParams& g) {
   g.limiter = s->limiter;
   g.last_hope = s->last_hope;
   g.pow_nd = s->pow_nd;
+  g.linearize = s->linearize ? 1 : 0;
   return RKDG_OK;
 }
 
@@ -1234,6 +1239,15 @@ int limit_impl(rkdg_ctx* c, double* x, int8_t* troubled, bool have_means, const 
     HIP_TRY(c->dense ? launch_dense_means(c->geom, c->p, ma, c->d_dtab, c->stream)
                      : launch_tensor_means(c->dim, c->p, ma, c->d_tb, c->stream));
   }
+  if (c->sp.indicator == RKDG_IND_SHU) {
+    auto pr = prof_begin(c, c->ev_lim);
+    hipError_t le = launch_shu_indicator(x, c->d_means, c->d_shu_coef, c->d_nbr, c->NE, c->nd, c->neq, c->nf,
+                                         c->d_shu_flag, troubled, c->stream);
+    if (le == hipSuccess) le = launch_shu_apply(x, c->d_means, c->d_shu_flag, c->NE, c->nd, c->neq, c->stream);
+    prof_end(c, pr);
+    HIP_TRY(le);
+    return RKDG_OK;
+  }
   LimitArgs la{x, c->d_means, c->d_geo, c->d_nbr, c->d_fcode, c->NE, troubled, c->d_inst, active, nullptr};
   auto pr = prof_begin(c, c->ev_lim);
   hipError_t le = c->dense ? launch_dense_limit(c->geom, c->p, la, c->sp, c->d_dtab, c->stream)
@@ -1544,6 +1558,207 @@ int objective_launch(rkdg_ctx* c, const double* x, int slot, double coef, double
 }  // namespace
 
 // ============================================================================
+// ---------------------------------------------------------------- Shu indicator weights
+// Extrapolated means of the stencil cells over element t
+// (Averager::computeStencilExtrapAverages, averager.cpp:152-215, and
+// assembleShiftedElementMatrix, :218-273): for cell k of t's stencil, the
+// weight of k's node j is
+//   wlast_k / |k| * sum_q w_q phi^k_j(T_k^-1(T_t(xi_q)))
+// with (w_q, xi_q) the rule of order p + OrderW on t's reference element,
+// wlast_k the Jacobian determinant of k at the last point of that rule (the
+// reference reads Trans.Weight() without setting a point), |k| the measure of
+// k. T_k^-1 is the affine inverse (triangles, hexahedra) or Newton's method
+// on the bilinear map (quadrilaterals), not clipped to the element: the
+// neighbour's polynomial is extrapolated over t.
+namespace {
+struct ShuGeom {
+  int geom, dim, nv, p, nd;
+  RefBasis rb;                 // quadrilaterals and triangles
+  std::vector<double> x1;      // 1D Gauss-Legendre nodes
+  void map(const double* X, const double* xi, double* x) const {
+    if (geom == RKDG_GEOM_TRI) {
+      for (int d = 0; d < 2; d++) x[d] = X[d] + (X[2 + d] - X[d]) * xi[0] + (X[4 + d] - X[d]) * xi[1];
+      return;
+    }
+    for (int d = 0; d < dim; d++) x[d] = 0.0;
+    for (int v = 0; v < nv; v++) {
+      const int rx = (v == 1 || v == 2 || v == 5 || v == 6) ? 1 : 0;
+      const int ry = (v == 2 || v == 3 || v == 6 || v == 7) ? 1 : 0;
+      const int rz = v >= 4 ? 1 : 0;
+      double N = (rx ? xi[0] : 1 - xi[0]) * (ry ? xi[1] : 1 - xi[1]);
+      if (dim == 3) N *= rz ? xi[2] : 1 - xi[2];
+      for (int d = 0; d < dim; d++) x[d] += N * X[v * dim + d];
+    }
+  }
+  // J[i*dim + r] = dx_i / dxi_r
+  void jac(const double* X, const double* xi, double* J) const {
+    if (geom == RKDG_GEOM_TRI) {
+      J[0] = X[2] - X[0], J[1] = X[4] - X[0], J[2] = X[3] - X[1], J[3] = X[5] - X[1];
+      return;
+    }
+    if (geom == RKDG_GEOM_QUAD) {
+      bilinear_jac(X, xi[0], xi[1], J);
+      return;
+    }
+    for (int i = 0; i < 9; i++) J[i] = 0.0;
+    for (int v = 0; v < 8; v++) {
+      const int R[3] = {(v == 1 || v == 2 || v == 5 || v == 6) ? 1 : 0, (v == 2 || v == 3 || v == 6 || v == 7) ? 1 : 0,
+                        v >= 4 ? 1 : 0};
+      for (int r = 0; r < 3; r++) {
+        double t = R[r] ? 1.0 : -1.0;
+        for (int q = 0; q < 3; q++)
+          if (q != r) t *= R[q] ? xi[q] : 1.0 - xi[q];
+        for (int i = 0; i < 3; i++) J[i * 3 + r] += X[v * 3 + i] * t;
+      }
+    }
+  }
+  static double det(int d, const double* J) {
+    if (d == 2) return J[0] * J[3] - J[1] * J[2];
+    return J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) + J[2] * (J[3] * J[7] - J[4] * J[6]);
+  }
+  static void solve(int d, const double* J, const double* b, double* x) {
+    double A[9];
+    if (d == 2) {
+      A[0] = J[3], A[1] = -J[1], A[2] = -J[2], A[3] = J[0];
+    } else {
+      A[0] = J[4] * J[8] - J[5] * J[7];
+      A[1] = J[2] * J[7] - J[1] * J[8];
+      A[2] = J[1] * J[5] - J[2] * J[4];
+      A[3] = J[5] * J[6] - J[3] * J[8];
+      A[4] = J[0] * J[8] - J[2] * J[6];
+      A[5] = J[2] * J[3] - J[0] * J[5];
+      A[6] = J[3] * J[7] - J[4] * J[6];
+      A[7] = J[1] * J[6] - J[0] * J[7];
+      A[8] = J[0] * J[4] - J[1] * J[3];
+    }
+    const double dj = det(d, J);
+    for (int i = 0; i < d; i++) {
+      double t = 0.0;
+      for (int k = 0; k < d; k++) t += A[i * d + k] * b[k];
+      x[i] = t / dj;
+    }
+  }
+  void inverse(const double* X, const double* x, double* xi) const {
+    if (geom != RKDG_GEOM_QUAD) {
+      // affine: the vertex differences along the reference axes
+      const int cols[3] = {1, geom == RKDG_GEOM_TRI ? 2 : 3, 4};
+      double J[9], b[3];
+      for (int i = 0; i < dim; i++) {
+        for (int r = 0; r < dim; r++) J[i * dim + r] = X[cols[r] * dim + i] - X[i];
+        b[i] = x[i] - X[i];
+      }
+      solve(dim, J, b, xi);
+      return;
+    }
+    for (int d = 0; d < dim; d++) xi[d] = 0.5;
+    for (int it = 0; it < 60; it++) {
+      double xc[3], r[3], J[9], dxi[3];
+      map(X, xi, xc);
+      for (int i = 0; i < dim; i++) r[i] = x[i] - xc[i];
+      jac(X, xi, J);
+      solve(dim, J, r, dxi);
+      double step = 0.0;
+      for (int d = 0; d < dim; d++) {
+        xi[d] += dxi[d];
+        step = std::max(step, std::fabs(dxi[d]));
+      }
+      if (step < 1e-15) break;
+    }
+  }
+  void shape(const double* xi, double* phi) const {
+    if (geom != RKDG_GEOM_HEX) {
+      rb.eval(xi[0], xi[1], phi, nullptr, nullptr);
+      return;
+    }
+    const int N = p + 1;
+    for (int k = 0; k < N; k++)
+      for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++)
+          phi[i + N * (j + N * k)] = lag(x1.data(), N, i, xi[0]) * lag(x1.data(), N, j, xi[1]) * lag(x1.data(), N, k, xi[2]);
+  }
+};
+}  // namespace
+
+int build_shu(rkdg_ctx* c) {
+  ShuGeom G;
+  G.geom = c->geom;
+  G.dim = c->dim;
+  G.nv = c->geom == RKDG_GEOM_TRI ? 3 : (c->dim == 3 ? 8 : 4);
+  G.p = c->p;
+  G.nd = c->nd;
+  if (c->geom != RKDG_GEOM_HEX) G.rb.init(c->geom == RKDG_GEOM_TRI, c->p);
+  G.x1.assign(c->p + 1, 0.0);
+  {
+    std::vector<double> w(c->p + 1);
+    gl_rule(c->p + 1, G.x1.data(), w.data());
+  }
+  // the rule of order p + OrderW (OrderW: triangle 0, bilinear quad 1, trilinear hex 2)
+  const int order = c->p + (c->geom == RKDG_GEOM_TRI ? 0 : (c->dim == 3 ? 2 : 1));
+  std::vector<double> pts, wts;
+  if (c->geom == RKDG_GEOM_TRI) {
+    tri_rule((order + 3) / 2, pts, wts);
+  } else {
+    const int n = (order | 1) / 2 + 1;
+    std::vector<double> x(n), w(n);
+    gl_rule(n, x.data(), w.data());
+    if (c->dim == 2) {
+      quad_rule(n, pts, wts);
+    } else {
+      for (int k = 0; k < n; k++)
+        for (int j = 0; j < n; j++)
+          for (int i = 0; i < n; i++) {
+            pts.push_back(x[i]), pts.push_back(x[j]), pts.push_back(x[k]);
+            wts.push_back(w[i] * w[j] * w[k]);
+          }
+    }
+  }
+  const int nq = (int)wts.size(), dim = c->dim, nd = c->nd, nf = c->nf;
+  const int64_t NE = c->NE;
+  const int stride = G.nv * dim;
+  // wlast_k / |k| per element
+  std::vector<double> scale(NE);
+  for (int64_t k = 0; k < NE; k++) {
+    const double* Xk = &c->coords[k * stride];
+    double vol = 0.0, wl = 0.0;
+    for (int q = 0; q < nq; q++) {
+      double J[9];
+      G.jac(Xk, &pts[q * dim], J);
+      const double dj = ShuGeom::det(dim, J);
+      vol += dj * wts[q];
+      wl = dj;
+    }
+    scale[k] = wl / vol;
+  }
+  std::vector<double> coef((size_t)(nf + 1) * nd * NE, 0.0);
+  std::vector<double> phi(nd);
+  for (int64_t t = 0; t < NE; t++) {
+    const double* Xt = &c->coords[t * stride];
+    for (int sl = 0; sl <= nf; sl++) {
+      const int64_t k = sl == 0 ? t : (int64_t)c->nbr_h[(size_t)(sl - 1) * NE + t];
+      if (k < 0) continue;
+      const double* Xk = &c->coords[k * stride];
+      std::vector<double> acc(nd, 0.0);
+      for (int q = 0; q < nq; q++) {
+        double xi[3];
+        if (k == t) {
+          for (int d = 0; d < dim; d++) xi[d] = pts[q * dim + d];
+        } else {
+          double xp[3];
+          G.map(Xt, &pts[q * dim], xp);
+          G.inverse(Xk, xp, xi);
+        }
+        G.shape(xi, phi.data());
+        for (int j = 0; j < nd; j++) acc[j] += scale[k] * wts[q] * phi[j];
+      }
+      for (int j = 0; j < nd; j++) coef[((size_t)sl * nd + j) * NE + t] = acc[j];
+    }
+  }
+  HIP_TRY(hipMalloc(&c->d_shu_coef, sizeof(double) * coef.size()));
+  HIP_TRY(hipMemcpy(c->d_shu_coef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMalloc(&c->d_shu_flag, (size_t)NE));
+  return RKDG_OK;
+}
+
 extern "C" {
 
 const char* rkdg_last_error(void) { return g_last_error.c_str(); }
@@ -1558,7 +1773,15 @@ int rkdg_create_ex(const rkdg_mesh_desc* mesh, const rkdg_scheme_desc* scheme, i
   rkdg_ctx* c = new rkdg_ctx();
   c->force_dense = (flags & RKDG_CREATE_DENSE) ? 1 : 0;
   c->two_colour = (flags & RKDG_CREATE_TWO_COLOUR) ? 1 : 0;
-  int rc = build_ctx(c, mesh, scheme);
+  int rc = RKDG_OK;
+  if (scheme->indicator == RKDG_IND_SHU && scheme->limiter != RKDG_LIM_FINDIFF)
+    // IndicatorShu writes no minValues, which the multiplier reads
+    // (indicator_shu.cpp:68-73, limiter_multiplier.cpp:35)
+    rc = fail(RKDG_E_ARG, "the Shu indicator sets no multiplier: pair it with the FinDiff limiter");
+  else if (scheme->linearize && scheme->limiter == RKDG_LIM_MULTI && mesh->order != 1 && mesh->geom != RKDG_GEOM_TRI)
+    rc = fail(RKDG_E_ARG, "linearize reads nd points of an order p+1 rule: p = 1 quadrilaterals and hexahedra only");
+  if (rc == RKDG_OK) rc = build_ctx(c, mesh, scheme);
+  if (rc == RKDG_OK && c->sp.indicator == RKDG_IND_SHU) rc = build_shu(c);
   if (rc == RKDG_OK) {
     const hipError_t se = noalloc_scratch(&c->scratch);
     if (se != hipSuccess) rc = fail(RKDG_E_HIP, std::string("scratch array: ") + hipGetErrorString(se));
@@ -1917,6 +2140,8 @@ int rkdg_set_bc_params(rkdg_ctx* c, int slot, const double* params) {
 
 int rkdg_rhs_adjoint(rkdg_ctx* c, const double* x, const double* kbar, double* xbar, const double* addend,
                      double* bc_param_bar) {
+  if (c && (c->sp.indicator == RKDG_IND_SHU || c->sp.linearize))
+    return fail(RKDG_E_UNSUPPORTED, "the discrete adjoint differentiates the BJ-family limiter without linearize");
   if (!c || !x || !kbar || !xbar) return fail(RKDG_E_ARG, "null argument");
   int rc = adjoint_supported(c);
   if (rc) return rc;
@@ -1924,6 +2149,8 @@ int rkdg_rhs_adjoint(rkdg_ctx* c, const double* x, const double* kbar, double* x
 }
 
 int rkdg_limit_adjoint(rkdg_ctx* c, const double* x, const double* ubar, double* xbar) {
+  if (c && (c->sp.indicator == RKDG_IND_SHU || c->sp.linearize))
+    return fail(RKDG_E_UNSUPPORTED, "the discrete adjoint differentiates the BJ-family limiter without linearize");
   if (!c || !x || !ubar || !xbar) return fail(RKDG_E_ARG, "null argument");
   int rc = adjoint_supported(c);
   if (rc) return rc;
@@ -1941,6 +2168,8 @@ int rkdg_objective(rkdg_ctx* c, const double* x, int bc_slot, double coef, doubl
 
 int rkdg_step_adjoint(rkdg_ctx* c, const double* x, double dt, int s, const double* a, const double* b,
                       const double* cc, const double* ybar, double* xbar, double* bc_param_bar) {
+  if (c && (c->sp.indicator == RKDG_IND_SHU || c->sp.linearize))
+    return fail(RKDG_E_UNSUPPORTED, "the discrete adjoint differentiates the BJ-family limiter without linearize");
   (void)cc;
   if (!c || !x || !a || !b || !ybar || !xbar) return fail(RKDG_E_ARG, "null argument");
   if (s < 2 || s > 4) return fail(RKDG_E_ARG, "2 <= s <= 4 stages supported");
@@ -1997,6 +2226,8 @@ int rkdg_forward_solve(rkdg_ctx* c, double* x, int64_t nsteps, const double* dts
 int rkdg_adjoint_solve(rkdg_ctx* c, const double* x0, int64_t N, const double* dts, int s, const double* a,
                        const double* b, const double* cc, int obj_slot, int64_t every, double* J, double* x0bar,
                        double* bc_param_bar, double* xN) {
+  if (c && (c->sp.indicator == RKDG_IND_SHU || c->sp.linearize))
+    return fail(RKDG_E_UNSUPPORTED, "the discrete adjoint differentiates the BJ-family limiter without linearize");
   if (!c || !x0 || !dts || !a || !b || !x0bar) return fail(RKDG_E_ARG, "null argument");
   if (s < 2 || s > 4) return fail(RKDG_E_ARG, "2 <= s <= 4 stages supported");
   if (N < 1) return fail(RKDG_E_ARG, "nsteps must be positive");
@@ -2072,6 +2303,8 @@ int rkdg_batch_create(const rkdg_mesh_desc* meshes, const rkdg_scheme_desc* sche
   if (m0.geom == RKDG_GEOM_HEX) return fail(RKDG_E_UNSUPPORTED, "batched ensembles of hexahedra are not built");
   const int nv = m0.geom == RKDG_GEOM_TRI ? 3 : 4;
   const rkdg_scheme_desc& s0 = schemes[0];
+  if (s0.indicator == RKDG_IND_SHU || s0.linearize)
+    return fail(RKDG_E_UNSUPPORTED, "batched ensembles run the BJ-family indicator without linearize");
   // every instance shares the discretisation; only boundary parameters differ
   for (int i = 1; i < count; i++) {
     const rkdg_scheme_desc& si = schemes[i];

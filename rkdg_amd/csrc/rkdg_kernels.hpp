@@ -209,6 +209,29 @@ __device__ __attribute__((noinline)) bool nonfinite_nodes(const double* x, int64
   return r;
 }
 
+// LimiterMultiplier::linearize (limiter_multiplier.cpp:101-178) for p = 1
+// tensor elements: the node values of the plane through the mean whose
+// reference gradient is the element's at the centre. With two Gauss-Legendre
+// nodes per axis, xi_i - 1/2 = s (x1 - x0)/2 and
+// d phi_j / d xi_d (1/2) = s_j (1/2)^(dim-1) / (x1 - x0), with s = -1 at the
+// lower node and +1 at the upper, so u_i = mean + sum_d s_d(i) D_d / 2, D_d the
+// mean over the 2^(dim-1) node pairs along axis d of their differences. u holds
+// the element's 2^DIM node values, lexicographic with x fastest.
+template <int DIM>
+__device__ __forceinline__ double linearize_q1(const double* u, double mean, int i) {
+  constexpr int ND = 1 << DIM;
+  double acc = mean;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    double D = 0.0;
+#pragma unroll
+    for (int j = 0; j < ND; j++) D += ((j >> d) & 1) ? u[j] : -u[j];
+    D *= 1.0 / (double)(ND / 2);
+    acc += (((i >> d) & 1) ? 0.5 : -0.5) * D;
+  }
+  return acc;
+}
+
 // launchers (rkdg_tensor.hip)
 // t: device copy of the tables, th: the host copy (its packed tables also go to
 // the kernel as an argument, for reads at wave-uniform indices from scalar
@@ -229,6 +252,15 @@ hipError_t launch_primitives(const double* x, int64_t ndofs, int dim, double gam
                              double* acc, double w, hipStream_t s);
 hipError_t launch_interp(const double* x, int64_t NE, int nd, int neq, const double* phi, int npts, double* out,
                          hipStream_t s);
+// Shu's indicator with the FinDiff limiter (rkdg_shu.hip): flag[e] = 1 where
+// the last variable's indicator is below 0.999, from the element means and the
+// neighbour-extrapolated means coef . u_k; then troubled elements take their
+// means. coef[(s*nd + j)*NE + e]: s = 0 the element itself, s = 1 + lf its
+// neighbour across local face lf.
+hipError_t launch_shu_indicator(const double* x, const double* means, const double* coef, const int* nbr, int64_t NE,
+                                int nd, int neq, int nf, int8_t* flag, int8_t* troubled, hipStream_t s);
+hipError_t launch_shu_apply(double* x, const double* means, const int8_t* flag, int64_t NE, int nd, int neq,
+                            hipStream_t s);
 // the static scratch array of rkdg_step_noalloc (rkdg_noalloc.hip): its
 // address on the current device, looked up once per device
 constexpr int64_t kScratchDoubles = int64_t(1) << 24;

@@ -22,6 +22,7 @@ struct SchemeParams {
   int indicator, phi;
   double venkat_k;
   int limiter, last_hope, pow_nd;
+  int linearize;  // multiplier: linear reconstruction first (p = 1 tensor elements)
 };
 
 // 1/x on the device: v_rcp_f64 refined by two Newton steps (correct to about

@@ -805,6 +805,24 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
       s_tr[team][f][fn] = t;
     }
   }
+  if constexpr (P == 1) {
+    // linearize: the multiplier and the last hope work on the element's linear
+    // reconstruction (the indicator above reads the state as it is)
+    if (g.linearize && g.limiter == RKDG_LIM_MULTI) {
+      double lin[NEQ];
+      const bool nl = active && lane < ND;
+      if (nl) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) lin[v] = linearize_q1<DIM>(&s_u[team][v][0], mean[v], lane);
+      }
+      __syncthreads();
+      if (nl) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) s_u[team][v][lane] = lin[v];
+      }
+      __syncthreads();
+    }
+  }
   if (g.limiter == RKDG_LIM_MULTI && g.last_hope) {
     // sum-factorised vertex values, first two directions: one lane per
     // (x side cx, variable, z node b) forms the x-face traces of its row and

@@ -28,7 +28,7 @@ BC_FIXED_PRESSURE_OUTLET = 5
 BC_TOTAL_PRESSURE_OUTLET = 6
 BC_CHAR_OUTLET = 7
 
-IND_NOWHERE, IND_EVERYWHERE, IND_BJ = 0, 1, 2
+IND_NOWHERE, IND_EVERYWHERE, IND_BJ, IND_SHU = 0, 1, 2, 3
 PHI_MICHALAK, PHI_BJ, PHI_VENKAT = 0, 1, 2
 LIM_NONE, LIM_FINDIFF, LIM_MULTI = 0, 1, 2
 
@@ -77,6 +77,7 @@ class Scheme:
     limiter: int = LIM_NONE
     last_hope: int = 1
     pow_nd: int = 1
+    linearize: int = 0
 
 
 # RK tableaus consumed by the packed a[l++] rule (rk_explicit_limited.cpp:51-58).

@@ -79,6 +79,9 @@ def test_create_rejects_bad_arguments_before_touching_the_device():
         (dict(gamma=1.0), "gamma"),
         (dict(covolume=-0.1), "covolume"),
         (dict(bcs=[S.BC(1, 9)]), "boundary condition type"),
+        # Shu's indicator sets no multiplier; linearize is defined for p = 1
+        (dict(indicator=S.IND_SHU, limiter=S.LIM_MULTI), "FinDiff"),
+        (dict(indicator=S.IND_BJ, limiter=S.LIM_MULTI, linearize=1), "linearize"),
     ]
     for kw, word in cases:
         rc, msg, out = _create_rc(m, 2, capi.scheme_desc(S.Scheme(**kw)))

@@ -767,3 +767,110 @@ def test_last_hope_resets_a_cell_with_a_negative_vertex_to_its_mean(geom):
             np.testing.assert_allclose(ref.means(xl).reshape(neq, -1), mean0, rtol=0, atol=1e-15)
         else:
             np.testing.assert_allclose(u1, u0, rtol=0, atol=1e-14)
+
+
+# ---------------------------------------------------------------- Shu indicator, linearize
+def _lag1(x, i, s):
+    """1D Lagrange basis on the nodes x, and its integral over [a, b]."""
+    return np.prod([(s - x[m]) / (x[i] - x[m]) for m in range(len(x)) if m != i], axis=0)
+
+
+def _lag1_integral(x, i, a, b):
+    c = np.poly1d([1.0])
+    for m in range(len(x)):
+        if m != i:
+            c = c * np.poly1d([1.0, -x[m]]) / (x[i] - x[m])
+    C = c.integ()
+    return C(b) - C(a)
+
+
+def test_shu_indicator_piecewise_constant_closed_form():
+    """A state constant on every element: the extrapolated means are the
+    neighbours' constants, so value = min(1, 0.03 |c_t| / (sum_k |c_t - c_k| + 1e-6))
+    over the face neighbours (boundary faces are not in the stencil), and
+    FinDiff replaces a cell by its mean when the last variable's value < 0.999."""
+    nx, ny = 5, 4
+    m = M.structured_quad(nx, ny, periodic=(False, True))
+    sch = S.Scheme(riemann=S.HLLC, indicator=S.IND_SHU, limiter=S.LIM_FINDIFF, bcs=all_outlet(m))
+    ref = R.RefSolver(m, 1, sch)
+    rng = np.random.default_rng(3)
+    c = rng.uniform(1.0, 1.2, (4, m.num_elements))
+    c[3] = 2.5 + 0.001 * rng.uniform(-1, 1, m.num_elements)
+    c[3, 7] = 2.8  # one cell with a large energy step
+    x = np.repeat(c[:, :, None], 4, axis=2).reshape(-1)
+    xl, ind, _ = ref.limit(x)
+    ind = ind.reshape(4, -1)
+    for e in range(m.num_elements):
+        i, j = e % nx, e // nx
+        nbrs = [((i + di) % nx) + nx * ((j + dj) % ny) for di, dj in ((-1, 0), (1, 0), (0, -1), (0, 1))
+                if 0 <= i + di < nx]
+        for v in range(4):
+            want = min(1.0, 0.03 * abs(c[v, e]) / (sum(abs(c[v, e] - c[v, k]) for k in nbrs) + 1e-6))
+            assert abs(ind[v, e] - want) <= 1e-13, (e, v)
+    flagged = ind[3] < 0.999
+    assert flagged[7] and not flagged.all()
+    # piecewise constants are their own means: FinDiff leaves them as they are
+    np.testing.assert_allclose(xl, x, rtol=1e-15, atol=0)
+
+
+def test_shu_indicator_extrapolates_neighbour_polynomials():
+    """A quadratic field on a stretched affine mesh, Q1 and Q2: the extrapolated
+    mean of neighbour k over cell t is the integral over t of k's interpolant,
+    computed here from 1D Lagrange integrals over the shifted interval."""
+    nx, ny = 6, 4
+    for p in (1, 2):
+        m = M.structured_quad(nx, ny, 0.0, 1.2, 0.0, 0.6, periodic=(False, False))
+        sch = S.Scheme(riemann=S.HLLC, indicator=S.IND_SHU, limiter=S.LIM_FINDIFF, bcs=all_outlet(m))
+        ref = R.RefSolver(m, p, sch)
+        xyz = ref.node_coords()
+        X, Y = xyz[..., 0], xyz[..., 1]
+        f = 1.0 + 0.3 * X + 0.2 * Y + 0.5 * X * X - 0.4 * Y * Y + 0.7 * X * Y
+        x = np.concatenate([(f * (1 + 0.1 * v)).ravel() for v in range(4)])
+        _, ind, _ = ref.limit(x)
+        ind = ind.reshape(4, -1)
+        g = np.array(R.gl_rule(p + 1)[0])
+        N = p + 1
+        U = x.reshape(4, -1, N * N)
+        means = (U * np.outer(R.gl_rule(N)[1], R.gl_rule(N)[1]).reshape(-1)).sum(axis=2)
+        for e in range(m.num_elements):
+            i, j = e % nx, e // nx
+            total = np.zeros(4)
+            for di, dj in ((-1, 0), (1, 0), (0, -1), (0, 1)):
+                if not (0 <= i + di < nx and 0 <= j + dj < ny):
+                    continue
+                k = (i + di) + nx * (j + dj)
+                # cell t is k's reference square shifted by (-di, -dj)
+                wx = np.array([_lag1_integral(g, a, -di, 1 - di) for a in range(N)])
+                wy = np.array([_lag1_integral(g, b, -dj, 1 - dj) for b in range(N)])
+                ex = (U[:, k] * np.outer(wy, wx).reshape(-1)).sum(axis=1)
+                total += np.abs(means[:, e] - ex)
+            want = np.minimum(1.0, 0.03 * np.abs(means[:, e]) / (total + 1e-6))
+            np.testing.assert_allclose(ind[:, e], want, rtol=1e-11, atol=1e-13)
+
+
+def test_linearize_replaces_q1_by_its_centre_gradient_plane():
+    """linearize (p = 1): u_i = mean + grad u(centre) . (xi_i - 1/2) at the nodes,
+    i.e. the bilinear term is dropped; linear fields are left as they are."""
+    m = M.structured_quad(4, 3, periodic=(False, False))
+    bcs = all_outlet(m)
+    lin = S.Scheme(riemann=S.HLLC, indicator=S.IND_NOWHERE, limiter=S.LIM_MULTI, linearize=1, bcs=bcs)
+    ref = R.RefSolver(m, 1, lin)
+    xyz = ref.node_coords()
+    X, Y = xyz[..., 0], xyz[..., 1]
+    for f, bil in ((1.0 + 0.3 * X - 0.2 * Y, 0.0), (1.0 + 0.3 * X - 0.2 * Y + 2.0 * X * Y, 2.0)):
+        # a physical state (the last hope would reset an unphysical vertex)
+        x = np.concatenate([f.ravel(), 0.1 * f.ravel(), 0.05 * f.ravel(), (3.0 + f).ravel()])
+        xl, _, _ = ref.limit(x)
+        U = xl.reshape(4, -1, 4)
+        hx, hy = 1.0 / 4, 1.0 / 3
+        for e in range(m.num_elements):
+            cx, cy = xyz[e, :, 0].mean(), xyz[e, :, 1].mean()
+            gx, gy = 0.3 + bil * cy, -0.2 + bil * cx  # physical gradient at the centre
+            want = f[e].mean() + gx * (X[e] - cx) + gy * (Y[e] - cy)
+            np.testing.assert_allclose(U[0, e], want, rtol=0, atol=1e-14)
+        if bil == 0.0:
+            np.testing.assert_allclose(xl, x, rtol=0, atol=1e-14)
+    with pytest.raises(RuntimeError, match="linearize"):
+        R.RefSolver(m, 2, lin)
+    with pytest.raises(RuntimeError, match="Shu"):
+        R.RefSolver(m, 1, S.Scheme(indicator=S.IND_SHU, limiter=S.LIM_MULTI, bcs=bcs))
