@@ -194,7 +194,7 @@ def cpu_baseline(n, steps, threads, warmup=1, mults=5):
     return dofs * 3 * steps / el, el, m.num_elements
 
 
-def ensemble_bench(dev, count=256, steps=20, warmup=3, seed=12345):
+def ensemble_bench(dev, count=256, steps=20, warmup=3, seed=12345, sequential=True):
     """The batched ensemble: ``count`` converging-diverging nozzles (P1 quads,
     HLL, Michalak multiplier limiter, SSP-RK3, per-instance dynamic dt), element
     counts log-uniform in [2000, 40000]. Every instance takes warmup + steps
@@ -233,6 +233,14 @@ def ensemble_bench(dev, count=256, steps=20, warmup=3, seed=12345):
     del x
     ens.ctx.close()
 
+    if not sequential:
+        # the batched leg alone (a kernel profile of the batched launches)
+        out = {"instances": count, "elements": ne_total, "dof": dof, "steps_timed": steps,
+               "batched_dof_stage_per_s": dof * 3 * steps / batch_s, "batched_ms_per_step": batch_s * 1e3 / steps,
+               "grid_cap": best}
+        print(f"[ensemble] batched {out['batched_ms_per_step']:.2f} ms/step (sequential leg skipped), grid {best}",
+              flush=True)
+        return out
     seq_s = 0.0
     identical = True
     for i, cs in enumerate(css):
@@ -470,6 +478,8 @@ def main():
                          "the reference for scaling FETCH_SIZE/WRITE_SIZE counters")
     ap.add_argument("--ensemble-only", action="store_true")
     ap.add_argument("--ensemble-count", type=int, default=256)
+    ap.add_argument("--ensemble-batched-only", action="store_true",
+                    help="with --ensemble-only: skip the sequential solves (profiles of the batched launches)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-dmr", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true")
@@ -499,7 +509,8 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     if args.ensemble_only:
-        print(json.dumps({"ensemble": ensemble_bench(dev, args.ensemble_count)}), flush=True)
+        print(json.dumps({"ensemble": ensemble_bench(dev, args.ensemble_count,
+                                                     sequential=not args.ensemble_batched_only)}), flush=True)
         return
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_bench(dev, args.adjoint_steps, oracle_steps=args.oracle_steps)}),
