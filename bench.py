@@ -701,8 +701,8 @@ def main():
                 "peak_tflops": FP64_PEAK_TFLOPS,
                 "frac": RHS_FLOP_PER_ELEMENT * info.num_elements / (rhs_avg_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
             },
-            # the plain rkdg_rhs (FE_Evolution::Mult) the north star's 60 % target
-            # names: 16 B per DOF (read x, write y) + geometry 80 B and
+            # the plain rkdg_rhs (FE_Evolution::Mult) against its HBM bound:
+            # 16 B per DOF (read x, write y) + geometry 80 B and
             # connectivity 48 B per element, timed with HIP events
             "rhs": {
                 "kernel": "tensor_rhs_kernel<3,2> as rkdg_rhs (y = RHS(x))",

@@ -14,8 +14,10 @@
  *   src/bc/boundary_integrator*.cpp         boundary faces and ghost states
  *   src/physics/physics.cpp                 EoS, fluxes, wave speeds
  *   src/riemannsolvers/rs_*.cpp             Rusanov, LLF, HLL, HLLC
- *   src/limiter/limiter*.cpp, src/indicator/indicator*  limiter driver, multiplier, FinDiff, BJ-family
+ *   src/limiter/limiter*.cpp, src/indicator/indicator*  limiter driver, multiplier (with linearize),
+ *                                           FinDiff, BJ-family, Shu
  *   src/averager/averager.cpp:84-119        cell means
+ *   src/averager/averager.cpp:152-273       neighbour-extrapolated means (Shu)
  *   src/ode_solver/rk_explicit_limited.cpp  Butcher RK with limiting
  * and the MFEM behaviour those files rely on (Gauss-Legendre nodal bases,
  * IntRules point counts, face Loc1/Loc2 maps, CalcOrtho normals, MassIntegrator,
@@ -37,7 +39,16 @@
  *   - the face and volume quadrature point counts derived from OrderW;
  *   - Vector::Norml2 rounding beyond vectors with one non-zero entry;
  *   - MFEM's face numbering and Elem1 choice (results depend on them only
- *     through rounding, which the orientation-invariance pin bounds).
+ *     through rounding, which the orientation-invariance pin bounds);
+ *   - ElementTransformation::TransformBack for points outside the element
+ *     (the Shu indicator's extrapolation): restated as the exact inverse map;
+ *     MFEM's inverse-transformation solver may project such points onto the
+ *     element instead, which only MFEM itself could settle;
+ *   - the order of the points of IntRules.Get(geom, p + 1) that linearize
+ *     pairs with the nodes (restated as x fastest, as the tensor rules are).
+ * Quadrilaterals are mapped bilinearly everywhere, their Jacobian written on
+ * edge vectors: the same map as MFEM's PointMat * dshape, without the
+ * rounding that form leaves on elements small against their coordinates.
  */
 #ifndef RKDG_REFCPU_H
 #define RKDG_REFCPU_H
