@@ -152,6 +152,10 @@ struct rkdg_ctx {
   long long* d_err = nullptr;
   int64_t err_pair[2] = {-1, -1};  // element pair of the last reported device error
   double* d_means = nullptr;
+  // tensor contexts: the second means buffer and the per-element limiter
+  // factors of the steps that apply the limiter on load
+  double* d_means2 = nullptr;
+  double* d_fac = nullptr;
   long long* h_pin = nullptr;    // pinned host words: device error word [4], then the wave speeds [count]
   int* d_order = nullptr;        // tensor contexts: work blocks in Morton order of their centroids
   // two-coloured tensor meshes (bipartite face graph, no boundary faces): the
@@ -240,6 +244,8 @@ void free_ctx(rkdg_ctx* c) {
   dfree(c->d_err);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
   dfree(c->d_means);
+  dfree(c->d_means2);
+  dfree(c->d_fac);
   dfree(c->d_order);
   dfree(c->d_elist_solve);
   dfree(c->d_elist_recv);
@@ -1055,6 +1061,8 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
   HIP_TRY(hipMalloc(&c->d_err, sizeof(long long) * 4));
   HIP_TRY(hipMemset(c->d_err, 0, sizeof(long long) * 4));
   HIP_TRY(hipMalloc(&c->d_means, sizeof(double) * c->neq * NE));
+  HIP_TRY(hipMalloc(&c->d_means2, sizeof(double) * c->neq * NE));
+  HIP_TRY(hipMalloc(&c->d_fac, sizeof(double) * NE));
   {
     // Traversal order of the RHS work blocks (EPB consecutive elements each):
     // Morton order of the block centroids on a 2^10 (3D) / 2^16 (2D) grid of
@@ -1264,6 +1272,32 @@ int ensure_stage_buffers(rkdg_ctx* c, int n) {
   return RKDG_OK;
 }
 
+// The limiter can run as a factor pass followed by limiting on load in the
+// next stage kernel when it maps each element to mean + f (u - mean) (the
+// multiplier without linearize, FinDiff) and the one-kernel tensor RHS runs.
+bool limit_on_load(const rkdg_ctx* c) {
+  return !c->dense && !c->rb && c->d_fac && c->d_means2 && c->sp.indicator != RKDG_IND_SHU && !c->sp.linearize &&
+         c->sp.limiter != RKDG_LIM_NONE;
+}
+
+// factor pass of the limiter over x (left as it is): every element's factor
+// into c->d_fac, from the means of x in `means`
+int limit_factor(rkdg_ctx* c, const double* x, const double* means) {
+  LimitArgs la{const_cast<double*>(x), means, c->d_geo, c->d_nbr, c->d_fcode, c->NE, nullptr, nullptr, nullptr,
+               nullptr, c->d_fac};
+  auto pr = prof_begin(c, c->ev_lim);
+  const hipError_t le = launch_tensor_limit(c->dim, c->p, la, c->sp, c->d_tb, c->stream);
+  prof_end(c, pr);
+  HIP_TRY(le);
+  return RKDG_OK;
+}
+
+// the three-stage SSP-RK3 tableau exactly: a = (1; 1/4, 1/4), b = (1/6, 1/6, 2/3)
+bool is_ssprk3(int s, const double* a, const double* b) {
+  return s == 3 && a[0] == 1.0 && a[1] == 0.25 && a[2] == 0.25 && b[0] == 1.0 / 6.0 && b[1] == 1.0 / 6.0 &&
+         b[2] == 2.0 / 3.0;
+}
+
 // ExplicitRKLimitedSolver::Step (rk_explicit_limited.cpp:30-82). Stage i reads
 // its (limited) input and produces, in the epilogue of the RHS kernel, every
 // pending combination that needs k_i: W_j += (a_ji dt) k_i for later stages j
@@ -1273,36 +1307,64 @@ int ensure_stage_buffers(rkdg_ctx* c, int n) {
 // W_1..W_{s-1}, buf[3] the accumulator; every one is written before it is read
 // (stage 0 has no bases), so nothing needs clearing. The last stage's wave
 // speed lands in d_lam[s-1] when want_lambda.
+//
+// Where limit_on_load holds, the limiter of a stage input W_i is a factor pass
+// (it writes one factor per element, not W_i) and the stage kernel limits W_i
+// as it loads it; the final x is limited in place as before. The stage inputs
+// then stay unlimited in memory, which for SSP-RK3 allows the stage sums of
+// the Shu-Osher form, an exact rearrangement of the tableau (W_1 = x + dt k_0
+// gives x + dt (k_0 + k_1)/4 = 3/4 x + 1/4 (W_1 + dt k_1) = W_2 and
+// x + dt (k_0/6 + k_1/6 + 2 k_2/3) = 1/3 x + 2/3 (W_2 + dt k_2)): each stage
+// reads its input and x and writes one vector, 8 vectors per step against 12.
+// The means alternate between two buffers (a stage reads the means of its
+// input while it writes those of its output); the last stage's are in d_means.
 int step_impl(rkdg_ctx* c, double* x, double dt, int s, const double* a, const double* b, double* const* buf,
               bool want_lambda) {
   double* W[4] = {nullptr, buf[0], buf[1], buf[2]};
   double* acc = buf[3];
+  const bool onload = limit_on_load(c);
+  const bool so = onload && is_ssprk3(s, a, b);
   HIP_TRY(launch_fill(c->d_lam, 0.0, 8 * c->count, c->stream));
   auto A_ = [&](int i, int j) { return a[i * (i - 1) / 2 + j]; };
+  auto mbuf = [&](int i) { return (!onload || (s - 1 - i) % 2 == 0) ? c->d_means : c->d_means2; };
   for (int i = 0; i < s; i++) {
     const double* in = i == 0 ? x : W[i];
     RhsArgs ra = base_args(c, in);
     ra.lam = c->d_lam + (int64_t)i * c->count;
-    int n = 0;
-    for (int j = i + 1; j < s; j++) {
-      ra.outs[n].out = W[j];
-      ra.outs[n].base = i == 0 ? nullptr : W[j];
-      ra.outs[n].coef = A_(j, i) * dt;
-      n++;
-    }
     const bool last = i == s - 1;
     ra.no_lambda = (!last || !want_lambda) ? 1 : 0;  // only the last stage's wave speed is read
-    ra.outs[n].out = last ? (s == 1 ? acc : x) : acc;
-    ra.outs[n].base = i == 0 ? nullptr : acc;
-    ra.outs[n].coef = b[i] * dt;
-    n++;
+    if (onload && i > 0) {
+      ra.lim_f = c->d_fac;
+      ra.lim_mean = mbuf(i - 1);
+    }
+    int n = 0;
+    if (so) {
+      const double wb[3] = {0.0, 0.75, 1.0 / 3.0}, wi[3] = {0.0, 0.25, 2.0 / 3.0};
+      ra.outs[0].out = last ? x : W[i + 1];
+      ra.outs[0].base = i == 0 ? nullptr : x;
+      ra.outs[0].coef = dt;
+      ra.outs[0].wb = wb[i];
+      ra.outs[0].wi = wi[i];
+      n = 1;
+    } else {
+      for (int j = i + 1; j < s; j++) {
+        ra.outs[n].out = W[j];
+        ra.outs[n].base = i == 0 ? nullptr : W[j];
+        ra.outs[n].coef = A_(j, i) * dt;
+        n++;
+      }
+      ra.outs[n].out = last ? (s == 1 ? acc : x) : acc;
+      ra.outs[n].base = i == 0 ? nullptr : acc;
+      ra.outs[n].coef = b[i] * dt;
+      n++;
+    }
     ra.nout = n;
-    ra.means = c->d_means;
+    ra.means = mbuf(i);
     ra.means_of = last ? n - 1 : 0;  // next stage input W_{i+1} is outs[0]; final is the last
     HIP_TRY(rhs_launch(c, ra));
     if (last && s == 1) HIP_TRY(launch_copy(x, acc, (int64_t)c->neq * c->ndofs, c->stream));
-    double* lim = last ? x : W[i + 1];
-    int rc = limit_impl(c, lim, nullptr, true);
+    const int rc = (last || !onload) ? limit_impl(c, last ? x : W[i + 1], nullptr, true)
+                                     : limit_factor(c, W[i + 1], mbuf(i));
     if (rc) return rc;
   }
   return RKDG_OK;

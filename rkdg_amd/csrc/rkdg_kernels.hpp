@@ -76,6 +76,14 @@ struct RhsArgs {
   const int* fslot;    // [NE]: receiver slot of each element (its index in the receivers' elist)
   double* fbuf;        // [receivers][NF][NEQ][face nodes]: lifted face fluxes in the receiver's face frame
   int aligned;         // tensor kernel: every interior face joins faces (r, c) and (r, 1-c), identity orientation
+  // Limiter applied on load (tensor kernel, role 0; both null otherwise): x
+  // holds the unlimited stage value and the kernel works on the limited one,
+  // mean + f (u - mean) per element and variable with f = lim_f[e] and
+  // mean = lim_mean[v*NE + e] (from the limiter kernel's factor pass). The
+  // stage input of the epilogue's Shu-Osher combinations is then the
+  // unlimited value, read again from x.
+  const double* lim_f;
+  const double* lim_mean;
 };
 
 struct LimitArgs {
@@ -89,6 +97,10 @@ struct LimitArgs {
   const int* inst;    // batched ensembles: element -> instance (nullable)
   const int* active;  // per-instance active flag (nullable)
   double* out;        // dense kernels: write the limited state here instead of in place (nullable)
+  // tensor kernel, factor pass (nullable): write each element's limiter factor
+  // f here and leave x as it is; the limited state is mean + f (u - mean)
+  // (f = y^nd or y, 0 after a reset to the mean, 1 where nothing is limited)
+  double* factor;
 };
 
 struct MeansArgs {

@@ -202,6 +202,16 @@ __global__ __launch_bounds__(64, ROLE == 1 ? 4 : 1) void tensor_rhs_kernel(const
   StateRegs<EPB, NEQ, ND, BLOCK> str;
   if constexpr (ROLE == 0) str.load(A.x, NDOFS, e0, NE, tid);
   else str.load_list(A.x, NDOFS, A.elist, blk0 * EPB, A.nlist, tid);
+  // limiter on load: factor and means of the element whose node this lane holds
+  const bool lim_load = ROLE == 0 && A.lim_f != nullptr;
+  double lf_own = 1.0, lm_own[NEQ];
+  if (lim_load) {
+    const int64_t el = e0 + tid / ND;
+    const bool ok = tid < EPB * ND && el < NE;
+    lf_own = ok ? A.lim_f[el] : 1.0;
+#pragma unroll
+    for (int q = 0; q < NEQ; q++) lm_own[q] = ok ? A.lim_mean[q * NE + el] : 0.0;
+  }
   // receivers: the block's lifted face fluxes, contiguous in A.fbuf
   constexpr int FBP = ROLE == 2 ? (EPB * FB + BLOCK - 1) / BLOCK : 1;
   double fbr[FBP];
@@ -263,6 +273,10 @@ __global__ __launch_bounds__(64, ROLE == 1 ? 4 : 1) void tensor_rhs_kernel(const
     s_nrm[team][f][DIM + 1] = 0.5 * recip(sqrt_ws(nu2));
   }
   tbr.store(reinterpret_cast<double*>(&tb), tid);
+  if (lim_load) {
+#pragma unroll
+    for (int q = 0; q < NEQ; q++) str.v[q] = lm_own[q] + lf_own * (str.v[q] - lm_own[q]);
+  }
   str.store(s_u, tid);
 #pragma unroll
   for (int p = 0; p < GDP; p++)
@@ -378,6 +392,7 @@ __global__ __launch_bounds__(64, ROLE == 1 ? 4 : 1) void tensor_rhs_kernel(const
         const int nb = s_nb[team][lf];
         double vals[N][N];
         int cn = 0;
+        double nf_f = 1.0, nf_m = 0.0;  // limiter on load: the neighbour's factor and mean
         if (nb >= 0) {
           int rn, sk, sa, sb;
           if constexpr (AL) {
@@ -392,6 +407,10 @@ __global__ __launch_bounds__(64, ROLE == 1 ? 4 : 1) void tensor_rhs_kernel(const
           for (int a = 0; a < N; a++)
   #pragma unroll
             for (int k = 0; k < N; k++) vals[a][k] = xn[a * sa + k * sk];
+          if (lim_load) {
+            nf_f = A.lim_f[nb];
+            nf_m = A.lim_mean[v * NE + nb];
+          }
         }
         double tr[N];
   #pragma unroll
@@ -414,7 +433,8 @@ __global__ __launch_bounds__(64, ROLE == 1 ? 4 : 1) void tensor_rhs_kernel(const
             double t = 0.0;
   #pragma unroll
             for (int k = 0; k < N; k++) t += vals[a][k] * tb.E[cn][k];
-            tr[a] = t;
+            // the trace of the limited state (the trace of a constant is the constant)
+            tr[a] = lim_load ? nf_m + nf_f * (t - nf_m) : t;
           }
   #pragma unroll
           for (int qi = 0; qi < NQ; qi++) {
@@ -603,8 +623,13 @@ __global__ __launch_bounds__(64, ROLE == 1 ? 4 : 1) void tensor_rhs_kernel(const
       const int64_t eb = ROLE == 0 ? e0 * ND : 0;
       const unsigned lo = ROLE == 0 ? (unsigned)(team * ND + j) : (unsigned)e * (unsigned)ND + (unsigned)j;
       double u0[NEQ];  // the stage input at this node (the base of outputs without one)
+      if (lim_load) {  // the unlimited stage value
 #pragma unroll
-      for (int v = 0; v < NEQ; v++) u0[v] = s_u[team][v][j];
+        for (int v = 0; v < NEQ; v++) u0[v] = sbase(A.x + (v * NDOFS + eb))[lo];
+      } else {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) u0[v] = s_u[team][v][j];
+      }
       double base[4][NEQ];
 #pragma unroll
       for (int m = 0; m < 4; m++) {
@@ -737,6 +762,7 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
   const bool active = e < NE;
   if (g.limiter == RKDG_LIM_NONE) {
     if (A.troubled && active && lane == 0) A.troubled[e] = 0;
+    if (A.factor && active && lane == 0) A.factor[e] = 1.0;
     return;
   }
   const int vsel = g.limiter == RKDG_LIM_FINDIFF ? NEQ - 1 : 0;
@@ -914,6 +940,7 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
     for (int v = 0; v < NEQ; v++) u[v] = s_u[team][v][j];
   }
   int trouble = 0;
+  double feff = 1.0;  // the factor of the limited state mean + feff (u - mean)
   if (g.limiter == RKDG_LIM_MULTI) {
     if (yr < 1.0) trouble = 1;
     // u <- mean + y (u - mean) applied nd times (limiter_multiplier.cpp:24-38)
@@ -927,6 +954,7 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
       }
       f = acc;
     }
+    feff = f;
     if (node_lane) {
 #pragma unroll
       for (int v = 0; v < NEQ; v++) u[v] = mean[v] + f * (u[v] - mean[v]);
@@ -955,6 +983,7 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
       const unsigned long long tm = T == 64 ? ~0ull : (((1ull << T) - 1ull) << (team * T));
       if (active && (bm & tm)) {
         trouble |= 2;
+        feff = 0.0;
         if (node_lane) {
 #pragma unroll
           for (int v = 0; v < NEQ; v++) u[v] = mean[v];
@@ -964,13 +993,16 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
   } else if (g.limiter == RKDG_LIM_FINDIFF) {
     if (yr < 0.999) {
       trouble = 1;
+      feff = 0.0;
       if (node_lane) {
 #pragma unroll
         for (int v = 0; v < NEQ; v++) u[v] = mean[v];
       }
     }
   }
-  if (node_lane) {
+  if (A.factor) {
+    if (active && lane == 0) A.factor[e] = feff;
+  } else if (node_lane) {
 #pragma unroll
     for (int v = 0; v < NEQ; v++) A.x[v * NDOFS + e * ND + j] = u[v];
   }
