@@ -248,6 +248,32 @@ int rkdg_profile_read(rkdg_ctx* ctx, double* rhs_ms, int64_t* rhs_launches, doub
 #define RKDG_CREATE_TWO_COLOUR 2
 int rkdg_create_ex(const rkdg_mesh_desc* mesh, const rkdg_scheme_desc* scheme, int flags, rkdg_ctx** out);
 
+/* Non-conforming faces of an adaptively refined mesh: MFEM NCMesh's slave
+ * faces, which the reference's face loop integrates as interior faces with
+ * Elem1 = the fine (slave) element and Elem2 = the coarser (master) element
+ * (face_integrator.cpp:11-276), and which its limiter puts in the stencil
+ * (limiter.cpp:166-217, the infoNC branch). Record k: face slave_face[k]
+ * (MFEM local numbering) of element slave[k] covers part of face
+ * master_face[k] of element master[k]; param[(k*nvf + c)*(dim-1) + j] is
+ * coordinate j of the slave face's corner c (the slave's local face order) in
+ * the master face's parametrisation (its local face vertex 0 at the origin,
+ * vertex 1 at 1 (2D) or (1,0), vertex 3 at (0,1) (3D)): the slave's point
+ * matrix. The element kernels take affine quadrilaterals and hexahedra. */
+typedef struct {
+  int64_t count;
+  const int64_t* slave;
+  const int32_t* slave_face;
+  const int64_t* master;
+  const int32_t* master_face;
+  const double* param;
+} rkdg_nc_desc;
+/* rkdg_create_ex on a mesh with non-conforming faces (after each refinement:
+ * the reference's UpdateAndRebalance + FE_Evolution updates, amr.cpp:3-133,
+ * main.cpp:280-311). The BJ-family limiters take the non-conforming
+ * neighbours into their stencil; the Shu indicator is not built for them. */
+int rkdg_create_nc(const rkdg_mesh_desc* mesh, const rkdg_scheme_desc* scheme, int flags, const rkdg_nc_desc* nc,
+                   rkdg_ctx** out);
+
 /* Batched ensembles. No counterpart in the reference, which runs one case per
  * process through main.cpp's time loop (main.cpp:467-553, 701, 733); a batch
  * runs that loop for `count` independent instances at once. Instances share

@@ -829,6 +829,13 @@ struct Face {
   int attr = 0;       // boundary attribute (bdr faces)
   int fv[4];          // topological vertices in Elem1's local face order
   int64_t fvt[4];
+  // non-conforming faces (MFEM NCMesh): nc = 1 on a slave face, whose Elem1 is
+  // the fine element and Elem2 the coarse (master) element; pm[m] holds the
+  // master's reference coordinates of the slave face's corner m (the point
+  // matrix of Loc2). ncm = 1 on a master face: not integrated (its slave faces
+  // are), but an interior face of the master for the limiter.
+  int nc = 0, ncm = 0;
+  real pm[4][3];
 };
 
 struct CSR {
@@ -909,6 +916,11 @@ struct Ref {
     int nvf = fe.nvf;
     std::fill(xi, xi + dim, 0.0);
     int64_t e = side == 0 ? f.e1 : f.e2;
+    if (side == 1 && f.nc) {
+      for (int m = 0; m < nvf; m++)
+        for (int d = 0; d < dim; d++) xi[d] += N[m] * f.pm[m][d];
+      return;
+    }
     for (int m = 0; m < nvf; m++) {
       int lv;
       if (side == 0) lv = fe.face_vert(f.lf1, m);
@@ -969,6 +981,8 @@ struct Ref {
   }
 
   bool build(std::string& err);
+  bool attach_nc(int64_t n, const int64_t* slave, const int* slave_face, const int64_t* master,
+                 const int* master_face, const double* param, std::string& err);
   void assemble_aflux();
   void assemble_mass();
   int face_mult(const real* x, real* z, real& mcs, int64_t* bad);
@@ -1010,6 +1024,74 @@ bool Ref::build(std::string& err) {
       }
     }
   }
+  for (int64_t f = 0; f < (int64_t)faces.size(); f++)
+    if (faces[f].e2 >= 0) interior.push_back(f);
+  el2el.assign(NE, {});
+  for (int64_t f : interior) {
+    el2el[faces[f].e1].push_back(faces[f].e2);
+    el2el[faces[f].e2].push_back(faces[f].e1);
+  }
+  return true;
+}
+
+// Non-conforming faces (MFEM's NCMesh master/slave lists): face slave_face of
+// the fine element `slave` covers part of face master_face of the coarser
+// element `master`; param[(k * nvf + c) * (dim - 1) + j] is coordinate j, in
+// the master face's own parametrisation (face vertex 0 at the origin, as
+// face_to_elem parametrises faces), of the slave face's corner c (in the
+// slave's local face order). The slave face becomes an interior face with
+// Elem1 = the slave and Elem2 = the master, whose Loc2 point matrix holds the
+// master's reference coordinates of those corners (MFEM's slave point
+// matrix). The master face is integrated through its slaves only
+// (face_integrator.cpp runs over interior faces) and enters the limiter's
+// stencil as an interior face (limiter.cpp:166-217, the infoNC branch); the
+// element-to-element table links master and slave.
+bool Ref::attach_nc(int64_t n, const int64_t* slave, const int* slave_face, const int64_t* master,
+                    const int* master_face, const double* param, std::string& err) {
+  if (n <= 0) return true;
+  const int fd = dim - 1;
+  for (int64_t k = 0; k < n; k++) {
+    const int64_t s = slave[k], m = master[k];
+    const int ls = slave_face[k], lm = master_face[k];
+    if (s < 0 || s >= NE || m < 0 || m >= NE || s == m || ls < 0 || ls >= fe.nf || lm < 0 || lm >= fe.nf) {
+      err = "non-conforming face record " + std::to_string(k) + " out of range";
+      return false;
+    }
+    Face& fs = faces[elem_faces[s][ls]];
+    Face& fm = faces[elem_faces[m][lm]];
+    if (fs.e2 >= 0 || fs.e1 != s || fs.nc) {
+      err = "non-conforming face record " + std::to_string(k) + ": the slave face is not an open face";
+      return false;
+    }
+    if (fm.e2 >= 0 || fm.e1 != m) {
+      err = "non-conforming face record " + std::to_string(k) + ": the master face is not an open face";
+      return false;
+    }
+    fs.e2 = m;
+    fs.lf2 = lm;
+    fs.nc = 1;
+    fm.ncm = 1;
+    for (int c = 0; c < fe.nvf; c++) {
+      real sp[2] = {0, 0};
+      for (int j = 0; j < fd; j++) {
+        sp[j] = param[(k * fe.nvf + c) * fd + j];
+        if (!(sp[j] >= 0.0 && sp[j] <= 1.0)) {
+          err = "non-conforming face record " + std::to_string(k) + ": face parameter outside [0, 1]";
+          return false;
+        }
+      }
+      // the master face's bilinear (linear) map of the parameter
+      real N[4];
+      if (dim == 2) { N[0] = 1 - sp[0]; N[1] = sp[0]; }
+      else { N[0] = (1 - sp[0]) * (1 - sp[1]); N[1] = sp[0] * (1 - sp[1]); N[2] = sp[0] * sp[1]; N[3] = (1 - sp[0]) * sp[1]; }
+      for (int d = 0; d < dim; d++) fs.pm[c][d] = 0;
+      for (int q = 0; q < fe.nvf; q++) {
+        const real* R = fe.vert(fe.face_vert(lm, q));
+        for (int d = 0; d < dim; d++) fs.pm[c][d] += N[q] * R[d];
+      }
+    }
+  }
+  interior.clear();
   for (int64_t f = 0; f < (int64_t)faces.size(); f++)
     if (faces[f].e2 >= 0) interior.push_back(f);
   el2el.assign(NE, {});
@@ -1276,7 +1358,7 @@ void Ref::compute_means(const real* x) {
 // outside element k (the Shu indicator extrapolates the neighbour's polynomial
 // over the troubled cell); the iteration is not clipped to the element.
 void Ref::transform_back(int64_t k, const real* x, real* xi) const {
-  real J[9];
+  real J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (affine[k] || fe.geom == REF_GEOM_TRI) {
     affine_jacobian(k, J);
     const real* X0 = X.data() + k * nv * dim;
@@ -1411,7 +1493,7 @@ void Ref::limiter_update(real* x, real* ind_values, real* min_values) {
       for (int lf = 0; lf < fe.nf; lf++) {
         int64_t fid = elem_faces[c][lf];
         const Face& f = faces[fid];
-        if (f.e2 < 0) continue;  // boundary faces are not in the stencil
+        if (f.e2 < 0 && !f.ncm) continue;  // boundary faces are not in the stencil
         int side = f.e1 == c ? 0 : 1;
         for (int q = 0; q < fr.n; q++) {
           real xi[3], u[5];
@@ -1606,9 +1688,11 @@ void narrow(const std::vector<real>& tmp, double* y) {
 
 extern "C" {
 
-void* ref_create(int geom, int order, int64_t num_elements, const double* elem_coords,
-                 const int64_t* elem_verts, int64_t num_bdr, const int64_t* bdr_verts,
-                 const int* bdr_attr, const ref_scheme* scheme, char* err, int errlen) {
+void* ref_create_nc(int geom, int order, int64_t num_elements, const double* elem_coords,
+                    const int64_t* elem_verts, int64_t num_bdr, const int64_t* bdr_verts,
+                    const int* bdr_attr, const ref_scheme* scheme, int64_t num_nc,
+                    const int64_t* nc_slave, const int* nc_slave_face, const int64_t* nc_master,
+                    const int* nc_master_face, const double* nc_param, char* err, int errlen) {
   Ref* r = new Ref();
   r->fe.init(geom, order);
   r->sc.s = *scheme;
@@ -1624,7 +1708,8 @@ void* ref_create(int geom, int order, int64_t num_elements, const double* elem_c
     e = "the Shu indicator writes no minValues, which the multiplier reads: pair it with FinDiff";
   else if (scheme->linearize && scheme->limiter == REF_LIM_MULTI && order != 1 && geom != REF_GEOM_TRI)
     e = "linearize reads nd points of an order p+1 rule: p = 1 quadrilaterals and hexahedra only";
-  if (!e.empty() || !r->build(e)) {
+  if (!e.empty() || !r->build(e) ||
+      !r->attach_nc(num_nc, nc_slave, nc_slave_face, nc_master, nc_master_face, nc_param, e)) {
     if (err) snprintf(err, errlen, "%s", e.c_str());
     delete r;
     return nullptr;
@@ -1666,6 +1751,13 @@ void* ref_create(int geom, int order, int64_t num_elements, const double* elem_c
   r->assemble_aflux();
   r->assemble_mass();
   return r;
+}
+
+void* ref_create(int geom, int order, int64_t num_elements, const double* elem_coords,
+                 const int64_t* elem_verts, int64_t num_bdr, const int64_t* bdr_verts,
+                 const int* bdr_attr, const ref_scheme* scheme, char* err, int errlen) {
+  return ref_create_nc(geom, order, num_elements, elem_coords, elem_verts, num_bdr, bdr_verts, bdr_attr, scheme, 0,
+                       nullptr, nullptr, nullptr, nullptr, nullptr, err, errlen);
 }
 
 void ref_destroy(void* h) { delete (Ref*)h; }

@@ -117,6 +117,15 @@ typedef struct {
 void* ref_create(int geom, int order, int64_t num_elements, const double* elem_coords,
                  const int64_t* elem_verts, int64_t num_bdr, const int64_t* bdr_verts,
                  const int* bdr_attr, const ref_scheme* scheme, char* err, int errlen);
+/* ref_create with non-conforming faces (MFEM NCMesh slaves): face nc_slave_face[k]
+ * of element nc_slave[k] covers part of face nc_master_face[k] of the coarser
+ * element nc_master[k]; nc_param[(k*nvf + c)*(dim-1) + j] is coordinate j of the
+ * slave face's corner c in the master face's parametrisation (the point matrix). */
+void* ref_create_nc(int geom, int order, int64_t num_elements, const double* elem_coords,
+                    const int64_t* elem_verts, int64_t num_bdr, const int64_t* bdr_verts,
+                    const int* bdr_attr, const ref_scheme* scheme, int64_t num_nc,
+                    const int64_t* nc_slave, const int* nc_slave_face, const int64_t* nc_master,
+                    const int* nc_master_face, const double* nc_param, char* err, int errlen);
 void ref_destroy(void* h);
 int64_t ref_num_dofs(void* h);
 int ref_nodes_per_element(void* h);

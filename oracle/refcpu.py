@@ -51,6 +51,10 @@ def lib(precision="double"):
         L.ref_create.restype = P
         L.ref_create.argtypes = [C.c_int, C.c_int, C.c_int64, dp, ip, C.c_int64, ip,
                                  C.POINTER(C.c_int), C.POINTER(RefScheme), C.c_char_p, C.c_int]
+        L.ref_create_nc.restype = P
+        L.ref_create_nc.argtypes = [C.c_int, C.c_int, C.c_int64, dp, ip, C.c_int64, ip,
+                                    C.POINTER(C.c_int), C.POINTER(RefScheme), C.c_int64, ip,
+                                    C.POINTER(C.c_int), ip, C.POINTER(C.c_int), dp, C.c_char_p, C.c_int]
         L.ref_destroy.argtypes = [P]
         L.ref_num_dofs.restype = C.c_int64
         L.ref_num_dofs.argtypes = [P]
@@ -125,9 +129,21 @@ class RefSolver:
         err = C.create_string_buffer(256)
         ba = np.ascontiguousarray(mesh.bdr_attr, dtype=np.int32)
         self._ba = ba
-        self.h = L.ref_create(mesh.geom, order, mesh.num_elements, _dp(mesh.elem_coords),
-                              _ip(mesh.elem_verts), len(mesh.bdr_attr), _ip(mesh.bdr_verts),
-                              ba.ctypes.data_as(C.POINTER(C.c_int)), C.byref(self._rs), err, 256)
+        nc = getattr(mesh, "nc", None)
+        if nc is not None and len(nc):
+            self._nc = [np.ascontiguousarray(nc[:, 0], dtype=np.int64), np.ascontiguousarray(nc[:, 1], dtype=np.int32),
+                        np.ascontiguousarray(nc[:, 2], dtype=np.int64), np.ascontiguousarray(nc[:, 3], dtype=np.int32),
+                        np.ascontiguousarray(mesh.nc_param, dtype=np.float64)]
+            ci = C.POINTER(C.c_int)
+            self.h = L.ref_create_nc(mesh.geom, order, mesh.num_elements, _dp(mesh.elem_coords),
+                                     _ip(mesh.elem_verts), len(mesh.bdr_attr), _ip(mesh.bdr_verts),
+                                     ba.ctypes.data_as(ci), C.byref(self._rs), len(nc), _ip(self._nc[0]),
+                                     self._nc[1].ctypes.data_as(ci), _ip(self._nc[2]),
+                                     self._nc[3].ctypes.data_as(ci), _dp(self._nc[4]), err, 256)
+        else:
+            self.h = L.ref_create(mesh.geom, order, mesh.num_elements, _dp(mesh.elem_coords),
+                                  _ip(mesh.elem_verts), len(mesh.bdr_attr), _ip(mesh.bdr_verts),
+                                  ba.ctypes.data_as(C.POINTER(C.c_int)), C.byref(self._rs), err, 256)
         if not self.h:
             raise RuntimeError("ref_create failed: " + err.value.decode())
         self.nd = L.ref_nodes_per_element(self.h)

@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librkdg.so")
 SOURCES = ["rkdg_tensor.hip", "rkdg_dense.hip", "rkdg_batch.hip", "rkdg_adjoint.hip", "rkdg_noalloc.hip", "rkdg_io.hip", "rkdg_shu.hip",
-           "rkdg_host.cpp", "rkdg_selftest.cpp"]
+           "rkdg_nc.hip", "rkdg_host.cpp", "rkdg_selftest.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

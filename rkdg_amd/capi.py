@@ -34,6 +34,13 @@ class MeshDesc(C.Structure):
     ]
 
 
+class NcDesc(C.Structure):
+    _fields_ = [
+        ("count", C.c_int64), ("slave", C.POINTER(C.c_int64)), ("slave_face", C.POINTER(C.c_int32)),
+        ("master", C.POINTER(C.c_int64)), ("master_face", C.POINTER(C.c_int32)), ("param", C.POINTER(C.c_double)),
+    ]
+
+
 class SchemeDesc(C.Structure):
     _fields_ = [
         ("riemann", C.c_int), ("wave_speeds", C.c_int), ("hllc_abs", C.c_int),
@@ -65,7 +72,7 @@ EXPORTS = [
     "rkdg_step_adjoint", "rkdg_forward_solve", "rkdg_adjoint_solve", "rkdg_step_noalloc",
     "rkdg_step_ssprk3", "rkdg_totals", "rkdg_primitives", "rkdg_interpolate",
     "rkdg_rhs_host", "rkdg_limit_host", "rkdg_step_host", "rkdg_riemann_eval", "rkdg_bc_right_state",
-    "rkdg_error_elements",
+    "rkdg_error_elements", "rkdg_create_nc",
 ]
 
 # include/rkdg_selftest.h (host-only)
@@ -117,6 +124,8 @@ def lib():
                                     C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     pd = C.POINTER(C.c_double)
     L.rkdg_create_ex.argtypes = [C.POINTER(MeshDesc), C.POINTER(SchemeDesc), C.c_int, C.POINTER(C.c_void_p)]
+    L.rkdg_create_nc.argtypes = [C.POINTER(MeshDesc), C.POINTER(SchemeDesc), C.c_int, C.POINTER(NcDesc),
+                                 C.POINTER(C.c_void_p)]
     L.rkdg_batch_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(SchemeDesc), C.c_int, C.POINTER(C.c_void_p)]
     L.rkdg_batch_offsets.argtypes = [P, C.POINTER(C.c_int64)]
     L.rkdg_batch_init.argtypes = [P, dp, pd, pd, C.c_double, C.c_int64]
@@ -262,6 +271,25 @@ def _mesh_desc(mesh, order, md=None):
     return md, keep
 
 
+def _nc_desc(mesh):
+    """NcDesc of a mesh's non-conforming faces (None when it has none), with
+    the arrays it points into."""
+    nc = getattr(mesh, "nc", None)
+    if nc is None or len(nc) == 0:
+        return None, ()
+    keep = (np.ascontiguousarray(nc[:, 0], np.int64), np.ascontiguousarray(nc[:, 1], np.int32),
+            np.ascontiguousarray(nc[:, 2], np.int64), np.ascontiguousarray(nc[:, 3], np.int32),
+            np.ascontiguousarray(mesh.nc_param, np.float64))
+    d = NcDesc()
+    d.count = len(nc)
+    d.slave = keep[0].ctypes.data_as(C.POINTER(C.c_int64))
+    d.slave_face = keep[1].ctypes.data_as(C.POINTER(C.c_int32))
+    d.master = keep[2].ctypes.data_as(C.POINTER(C.c_int64))
+    d.master_face = keep[3].ctypes.data_as(C.POINTER(C.c_int32))
+    d.param = keep[4].ctypes.data_as(C.POINTER(C.c_double))
+    return d, keep
+
+
 def _tableau_arrays(tableau):
     s, a, b, c = check_tableau(tableau)
     a = (C.c_double * max(1, len(a)))(*a) if a else (C.c_double * 1)(0.0)
@@ -284,7 +312,11 @@ class Context:
             self._sd = scheme_desc(scheme)
             h = C.c_void_p()
             flags = (RKDG_CREATE_DENSE if dense else 0) | (RKDG_CREATE_TWO_COLOUR if two_colour else 0)
-            _check(L.rkdg_create_ex(C.byref(md), C.byref(self._sd), flags, C.byref(h)))
+            nd, nkeep = _nc_desc(mesh)
+            if nd is None:
+                _check(L.rkdg_create_ex(C.byref(md), C.byref(self._sd), flags, C.byref(h)))
+            else:
+                _check(L.rkdg_create_nc(C.byref(md), C.byref(self._sd), flags, C.byref(nd), C.byref(h)))
         else:
             h = _handle
         self.h = h

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <limits>
+#include <map>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -195,6 +196,20 @@ struct rkdg_ctx {
   int count = 1;
   int force_dense = 0;
   int two_colour = 0;            // the solver/receiver form where the mesh allows it
+  // non-conforming faces (tensor contexts): the descriptor while the context
+  // is built, the element faces coded kNcCode0 - k (slave and master sides)
+  // with their neighbours, the slave-face records, the elements with such
+  // faces (their items: record * 2 + side) and their flux buffer
+  const rkdg_nc_desc* nc_in = nullptr;
+  int64_t n_nc = 0, n_ncf = 0, n_nce = 0;
+  int* d_ncnb = nullptr;
+  int* d_ncslot = nullptr;
+  double* d_ncbuf = nullptr;
+  int* d_ncrec = nullptr;
+  double* d_ncgeo = nullptr;
+  int* d_nce = nullptr;
+  int* d_ncptr = nullptr;
+  int* d_ncitem = nullptr;
   int aligned = 0;               // tensor mesh: every interior face joins (r, c) to (r, 1-c), identity orientation
   int grid_cap = 0;              // persistent grid of the dense right-hand-side kernel
   std::vector<int64_t> inst_off; // [count+1] element offsets
@@ -246,6 +261,14 @@ void free_ctx(rkdg_ctx* c) {
   dfree(c->d_means);
   dfree(c->d_means2);
   dfree(c->d_fac);
+  dfree(c->d_ncnb);
+  dfree(c->d_ncslot);
+  dfree(c->d_ncbuf);
+  dfree(c->d_ncrec);
+  dfree(c->d_ncgeo);
+  dfree(c->d_nce);
+  dfree(c->d_ncptr);
+  dfree(c->d_ncitem);
   dfree(c->d_order);
   dfree(c->d_elist_solve);
   dfree(c->d_elist_recv);
@@ -817,6 +840,172 @@ static int scheme_params(const rkdg_scheme_desc* s, SchemeParams& g) {
   return RKDG_OK;
 }
 
+// MFEM local face vertices of hexahedra (quadrilateral edges: kQuadEdge)
+const int kHexFace[6][4] = {{3, 2, 1, 0}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}, {4, 5, 6, 7}};
+int mfem_face_vertex(int dim, int lf, int q) { return dim == 2 ? kQuadEdge[lf][q] : kHexFace[lf][q]; }
+void ref_vertex(int dim, int v, double* xi) {
+  const int b = v & 3;
+  xi[0] = (b == 1 || b == 2) ? 1.0 : 0.0;
+  xi[1] = (b == 2 || b == 3) ? 1.0 : 0.0;
+  if (dim == 3) xi[2] = v >= 4 ? 1.0 : 0.0;
+}
+// bilinear (linear) weights of the corners of a face at parameter t
+void face_shape(int dim, const double* t, double* Nw) {
+  if (dim == 2) {
+    Nw[0] = 1.0 - t[0];
+    Nw[1] = t[0];
+  } else {
+    Nw[0] = (1.0 - t[0]) * (1.0 - t[1]);
+    Nw[1] = t[0] * (1.0 - t[1]);
+    Nw[2] = t[0] * t[1];
+    Nw[3] = (1.0 - t[0]) * t[1];
+  }
+}
+
+// Non-conforming faces of a tensor mesh (rkdg_nc_desc): codes for the slave
+// and master element faces (kNcCode0 - k, with the neighbours across each),
+// and, per slave face, the points of its Gauss-Legendre rule (the rule of the
+// conforming faces) in the slave's and the master's reference coordinates
+// (through the point matrix: slave face parameter -> master face parameter ->
+// master reference coordinates), the weights times |n| and the slave's unit
+// outward normal, as the reference's face loop takes them on a slave face
+// (Loc1, Loc2, CalcOrtho of the slave face, face_integrator.cpp:44-150).
+int mark_nc_faces(rkdg_ctx* c, const rkdg_mesh_desc* m,
+                  const std::vector<std::pair<std::array<int64_t, 4>, std::pair<int64_t, int>>>& open,
+                  std::map<std::pair<int64_t, int>, int>& ncface) {
+  const rkdg_nc_desc* d = c->nc_in;
+  const int dim = c->dim, nf = c->nf, nvf = c->nvf, fd = dim - 1;
+  const int64_t NE = c->NE;
+  if (!d->slave || !d->slave_face || !d->master || !d->master_face || !d->param)
+    return fail(RKDG_E_ARG, "null array in the non-conforming face descriptor");
+  std::map<std::pair<int64_t, int>, int> is_open;
+  for (auto& of : open) is_open[{of.second.first, of.second.second}] = 1;
+  std::vector<std::array<int, 4>> nbs;
+  auto code_of = [&](int64_t e, int lf) -> int {
+    auto it = ncface.find({e, lf});
+    if (it != ncface.end()) return it->second;
+    const int k = (int)nbs.size();
+    ncface[{e, lf}] = k;
+    nbs.push_back({-1, -1, -1, -1});
+    return k;
+  };
+  auto add_nb = [&](int k, int64_t n) -> bool {
+    for (int q = 0; q < 4; q++) {
+      if (nbs[k][q] == (int)n) return true;
+      if (nbs[k][q] < 0) {
+        nbs[k][q] = (int)n;
+        return true;
+      }
+    }
+    return false;
+  };
+  const int NQ = dim == 3 ? c->p + 2 : c->p + 1;
+  const int NP = dim == 3 ? NQ * NQ : NQ;
+  const int RG = NP * (2 * dim + 1) + dim;
+  double sq[kMaxQ], wq[kMaxQ];
+  gl_rule(NQ, sq, wq);
+  std::vector<double> geo((size_t)d->count * RG);
+  std::vector<int> rec((size_t)d->count * 3);
+  for (int64_t k = 0; k < d->count; k++) {
+    const int64_t es = d->slave[k], em = d->master[k];
+    const int ls = d->slave_face[k], lm = d->master_face[k];
+    const std::string at = "non-conforming face record " + std::to_string(k);
+    if (es < 0 || es >= NE || em < 0 || em >= NE || es == em || ls < 0 || ls >= nf || lm < 0 || lm >= nf)
+      return fail(RKDG_E_ARG, at + " out of range");
+    if (!is_open.count({es, ls})) return fail(RKDG_E_ARG, at + ": the slave face has a conforming neighbour");
+    if (!is_open.count({em, lm})) return fail(RKDG_E_ARG, at + ": the master face has a conforming neighbour");
+    const int ks = code_of(es, ls), km = code_of(em, lm);
+    if (nbs[ks][0] >= 0 && nbs[ks][0] != (int)em) return fail(RKDG_E_ARG, at + ": a slave face with two masters");
+    add_nb(ks, em);
+    if (!add_nb(km, es)) return fail(RKDG_E_ARG, at + ": more than four slaves on a master face");
+    double par[4][2];
+    for (int q = 0; q < nvf; q++)
+      for (int j = 0; j < fd; j++) {
+        par[q][j] = d->param[(k * nvf + q) * fd + j];
+        if (!(par[q][j] >= 0.0 && par[q][j] <= 1.0)) return fail(RKDG_E_ARG, at + ": face parameter outside [0, 1]");
+      }
+    rec[3 * k] = (int)es;
+    rec[3 * k + 1] = (int)em;
+    rec[3 * k + 2] = ls;
+    int rs, cs;
+    rc_of_lf(dim, ls, rs, cs);
+    const int t1 = rs == 0 ? 1 : 0, t2 = rs == 2 ? 1 : 2;
+    double R0[3], R1[3], R3[3], Rm[4][3];
+    ref_vertex(dim, mfem_face_vertex(dim, ls, 0), R0);
+    ref_vertex(dim, mfem_face_vertex(dim, ls, 1), R1);
+    if (dim == 3) ref_vertex(dim, mfem_face_vertex(dim, ls, 3), R3);
+    for (int q = 0; q < nvf; q++) ref_vertex(dim, mfem_face_vertex(dim, lm, q), Rm[q]);
+    // the slave face's area normal: sign * row rs of adj(J)
+    double an[3], nn = 0.0;
+    for (int dd = 0; dd < dim; dd++) {
+      an[dd] = (cs ? 1.0 : -1.0) * c->geo_h[(size_t)(rs * dim + dd) * NE + es];
+      nn += an[dd] * an[dd];
+    }
+    nn = std::sqrt(nn);
+    double* G = &geo[(size_t)k * RG];
+    for (int qj = 0; qj < (dim == 3 ? NQ : 1); qj++)
+      for (int qi = 0; qi < NQ; qi++) {
+        const int q = qi + NQ * qj;
+        double xs[3] = {0, 0, 0};
+        xs[rs] = cs;
+        xs[t1] = sq[qi];
+        if (dim == 3) xs[t2] = sq[qj];
+        double sg[2] = {0, 0}, Nw[4], pi[2] = {0, 0}, xm[3] = {0, 0, 0};
+        for (int dd = 0; dd < dim; dd++) {
+          sg[0] += (xs[dd] - R0[dd]) * (R1[dd] - R0[dd]);
+          if (dim == 3) sg[1] += (xs[dd] - R0[dd]) * (R3[dd] - R0[dd]);
+        }
+        face_shape(dim, sg, Nw);
+        for (int cc = 0; cc < nvf; cc++)
+          for (int j = 0; j < fd; j++) pi[j] += Nw[cc] * par[cc][j];
+        face_shape(dim, pi, Nw);
+        for (int cc = 0; cc < nvf; cc++)
+          for (int dd = 0; dd < dim; dd++) xm[dd] += Nw[cc] * Rm[cc][dd];
+        double* gp = G + q * (2 * dim + 1);
+        for (int dd = 0; dd < dim; dd++) {
+          gp[dd] = xs[dd];
+          gp[dim + dd] = xm[dd];
+        }
+        gp[2 * dim] = (dim == 3 ? wq[qi] * wq[qj] : wq[qi]) * nn;
+      }
+    for (int dd = 0; dd < dim; dd++) G[NP * (2 * dim + 1) + dd] = an[dd] / nn;
+  }
+  // elements with non-conforming faces, their items in record order
+  std::map<int64_t, std::vector<int>> items;
+  for (int64_t k = 0; k < d->count; k++) {
+    items[d->slave[k]].push_back((int)(2 * k));
+    items[d->master[k]].push_back((int)(2 * k + 1));
+  }
+  std::vector<int> elem, ptr(1, 0), item, slot((size_t)NE, -1);
+  for (auto& kv : items) {
+    slot[kv.first] = (int)elem.size();
+    elem.push_back((int)kv.first);
+    for (int it : kv.second) item.push_back(it);
+    ptr.push_back((int)item.size());
+  }
+  std::vector<int> nbv;
+  for (auto& a : nbs)
+    for (int q = 0; q < 4; q++) nbv.push_back(a[q]);
+  c->n_nc = d->count;
+  c->n_ncf = (int64_t)nbs.size();
+  c->n_nce = (int64_t)elem.size();
+  auto up = [&](auto** dst, const auto& v) -> hipError_t {
+    using T = typename std::decay<decltype(v[0])>::type;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(dst), sizeof(T) * std::max<size_t>(1, v.size()));
+    if (e == hipSuccess && !v.empty()) e = hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+    return e;
+  };
+  HIP_TRY(up(&c->d_ncnb, nbv));
+  HIP_TRY(up(&c->d_ncrec, rec));
+  HIP_TRY(up(&c->d_ncgeo, geo));
+  HIP_TRY(up(&c->d_nce, elem));
+  HIP_TRY(up(&c->d_ncptr, ptr));
+  HIP_TRY(up(&c->d_ncitem, item));
+  HIP_TRY(up(&c->d_ncslot, slot));
+  HIP_TRY(hipMalloc(&c->d_ncbuf, sizeof(double) * std::max<int64_t>(1, c->n_nce * c->neq * c->nd)));
+  return RKDG_OK;
+}
+
 int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
   if (m->geom != RKDG_GEOM_QUAD && m->geom != RKDG_GEOM_HEX && m->geom != RKDG_GEOM_TRI)
     return fail(RKDG_E_ARG, "unknown geometry");
@@ -862,6 +1051,8 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
         if (std::fabs(X[4 + i] - (X[2 + i] + X[6 + i] - X[i])) > 1e-14 * sc) dense = true;
     }
   }
+  if (dense && c->nc_in && c->nc_in->count > 0)
+    return fail(RKDG_E_UNSUPPORTED, "non-conforming faces are built for affine quadrilaterals and hexahedra");
   if (dense) return build_dense(c, m);
 
   // 1D tables
@@ -1021,7 +1212,17 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
     bk[b] = {k, m->bdr_attr[b]};
   }
   std::sort(bk.begin(), bk.end());
+  std::map<std::pair<int64_t, int>, int> ncface;  // (element, local face) -> k of the code kNcCode0 - k
+  if (c->nc_in && c->nc_in->count > 0) {
+    const int rc = mark_nc_faces(c, m, open, ncface);
+    if (rc) return rc;
+  }
   for (auto& of : open) {
+    const auto nf_it = ncface.find({of.second.first, of.second.second});
+    if (nf_it != ncface.end()) {
+      nbr[of.second.second * NE + of.second.first] = kNcCode0 - nf_it->second;
+      continue;
+    }
     auto it = std::lower_bound(bk.begin(), bk.end(), std::make_pair(of.first, -2147483647));
     int slot = RKDG_MAX_BC;  // no boundary integrator: no flux (NonlinearForm skips it)
     if (it != bk.end() && it->first == of.first) {
@@ -1103,7 +1304,7 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
   }
   HIP_TRY(hipMalloc(&c->d_tb, sizeof(TensorTables)));
   HIP_TRY(hipMemcpy(c->d_tb, &c->tb, sizeof(TensorTables), hipMemcpyHostToDevice));
-  if (c->n_bdr == 0 && c->two_colour) {
+  if (c->n_bdr == 0 && c->n_ncf == 0 && c->two_colour) {
     int rc = build_two_colour(c, nbr);
     if (rc) return rc;
   }
@@ -1170,7 +1371,27 @@ hipError_t rhs_launch(rkdg_ctx* c, const RhsArgs& a0) {
       e = launch_tensor_rhs(c->dim, c->p, a, c->sp, c->d_tb, c->tb, false, c->stream);
     }
   } else {
-    e = launch_tensor_rhs(c->dim, c->p, a, c->sp, c->d_tb, c->tb, c->n_bdr > 0, c->stream);
+    e = hipSuccess;
+    if (c->n_nce > 0) {
+      NcArgs na{};
+      na.x = a.x;
+      na.NE = c->NE;
+      na.n_elem = c->n_nce;
+      na.elem = c->d_nce;
+      na.ptr = c->d_ncptr;
+      na.item = c->d_ncitem;
+      na.rec = c->d_ncrec;
+      na.geo = c->d_ncgeo;
+      na.out = c->d_ncbuf;
+      na.lam = a.lam;
+      na.err = a.err;
+      na.no_lambda = a.no_lambda && !a.face_only;
+      e = launch_nc_faces(c->dim, c->p, na, c->sp, c->d_tb, c->stream);
+      a.ncslot = c->d_ncslot;
+      a.ncbuf = c->d_ncbuf;
+    }
+    if (e == hipSuccess)
+      e = launch_tensor_rhs(c->dim, c->p, a, c->sp, c->d_tb, c->tb, c->n_bdr + c->n_ncf > 0, c->stream);
   }
   prof_end(c, pr);
   return e;
@@ -1256,7 +1477,8 @@ int limit_impl(rkdg_ctx* c, double* x, int8_t* troubled, bool have_means, const 
     HIP_TRY(le);
     return RKDG_OK;
   }
-  LimitArgs la{x, c->d_means, c->d_geo, c->d_nbr, c->d_fcode, c->NE, troubled, c->d_inst, active, nullptr};
+  LimitArgs la{x, c->d_means, c->d_geo, c->d_nbr, c->d_fcode, c->NE, troubled, c->d_inst, active, nullptr,
+               nullptr, c->d_ncnb};
   auto pr = prof_begin(c, c->ev_lim);
   hipError_t le = c->dense ? launch_dense_limit(c->geom, c->p, la, c->sp, c->d_dtab, c->stream)
                            : launch_tensor_limit(c->dim, c->p, la, c->sp, c->d_tb, c->stream);
@@ -1276,15 +1498,15 @@ int ensure_stage_buffers(rkdg_ctx* c, int n) {
 // next stage kernel when it maps each element to mean + f (u - mean) (the
 // multiplier without linearize, FinDiff) and the one-kernel tensor RHS runs.
 bool limit_on_load(const rkdg_ctx* c) {
-  return !c->dense && !c->rb && c->d_fac && c->d_means2 && c->sp.indicator != RKDG_IND_SHU && !c->sp.linearize &&
-         c->sp.limiter != RKDG_LIM_NONE;
+  return !c->dense && !c->rb && c->n_nc == 0 && c->d_fac && c->d_means2 && c->sp.indicator != RKDG_IND_SHU &&
+         !c->sp.linearize && c->sp.limiter != RKDG_LIM_NONE;
 }
 
 // factor pass of the limiter over x (left as it is): every element's factor
 // into c->d_fac, from the means of x in `means`
 int limit_factor(rkdg_ctx* c, const double* x, const double* means) {
   LimitArgs la{const_cast<double*>(x), means, c->d_geo, c->d_nbr, c->d_fcode, c->NE, nullptr, nullptr, nullptr,
-               nullptr, c->d_fac};
+               nullptr, c->d_fac, c->d_ncnb};
   auto pr = prof_begin(c, c->ev_lim);
   const hipError_t le = launch_tensor_limit(c->dim, c->p, la, c->sp, c->d_tb, c->stream);
   prof_end(c, pr);
@@ -1830,9 +2052,18 @@ int rkdg_create(const rkdg_mesh_desc* mesh, const rkdg_scheme_desc* scheme, rkdg
 }
 
 int rkdg_create_ex(const rkdg_mesh_desc* mesh, const rkdg_scheme_desc* scheme, int flags, rkdg_ctx** out) {
+  return rkdg_create_nc(mesh, scheme, flags, nullptr, out);
+}
+
+int rkdg_create_nc(const rkdg_mesh_desc* mesh, const rkdg_scheme_desc* scheme, int flags, const rkdg_nc_desc* nc,
+                   rkdg_ctx** out) {
   if (!mesh || !scheme || !out) return fail(RKDG_E_ARG, "null argument");
   *out = nullptr;
+  if (nc && nc->count < 0) return fail(RKDG_E_ARG, "negative count of non-conforming faces");
+  if (nc && nc->count > 0 && scheme->indicator == RKDG_IND_SHU)
+    return fail(RKDG_E_UNSUPPORTED, "the Shu indicator is not built for non-conforming faces");
   rkdg_ctx* c = new rkdg_ctx();
+  c->nc_in = nc;
   c->force_dense = (flags & RKDG_CREATE_DENSE) ? 1 : 0;
   c->two_colour = (flags & RKDG_CREATE_TWO_COLOUR) ? 1 : 0;
   int rc = RKDG_OK;
@@ -1843,6 +2074,7 @@ int rkdg_create_ex(const rkdg_mesh_desc* mesh, const rkdg_scheme_desc* scheme, i
   else if (scheme->linearize && scheme->limiter == RKDG_LIM_MULTI && mesh->order != 1 && mesh->geom != RKDG_GEOM_TRI)
     rc = fail(RKDG_E_ARG, "linearize reads nd points of an order p+1 rule: p = 1 quadrilaterals and hexahedra only");
   if (rc == RKDG_OK) rc = build_ctx(c, mesh, scheme);
+  c->nc_in = nullptr;
   if (rc == RKDG_OK && c->sp.indicator == RKDG_IND_SHU) rc = build_shu(c);
   if (rc == RKDG_OK) {
     const hipError_t se = noalloc_scratch(&c->scratch);

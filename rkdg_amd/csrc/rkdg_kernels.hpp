@@ -84,6 +84,11 @@ struct RhsArgs {
   // unlimited value, read again from x.
   const double* lim_f;
   const double* lim_mean;
+  // non-conforming faces (tensor kernel, nullable): the element's share of
+  // the fluxes over its non-conforming faces, before the inverse mass, is
+  // ncbuf[(ncslot[e] * NEQ + v) * ND + node] where ncslot[e] >= 0
+  const int* ncslot;
+  const double* ncbuf;
 };
 
 struct LimitArgs {
@@ -101,6 +106,9 @@ struct LimitArgs {
   // f here and leave x as it is; the limited state is mean + f (u - mean)
   // (f = y^nd or y, 0 after a reset to the mean, 1 where nothing is limited)
   double* factor;
+  // non-conforming faces (tensor kernel, nullable): a face coded
+  // kNcCode0 - k in nbr has the neighbours ncnb[4 k .. 4 k + 3] (-1 padded)
+  const int* ncnb;
 };
 
 struct MeansArgs {
@@ -273,6 +281,31 @@ hipError_t launch_shu_indicator(const double* x, const double* means, const doub
                                 int nd, int neq, int nf, int8_t* flag, int8_t* troubled, hipStream_t s);
 hipError_t launch_shu_apply(double* x, const double* means, const int8_t* flag, int64_t NE, int nd, int neq,
                             hipStream_t s);
+// Non-conforming faces (MFEM NCMesh master/slave faces) of tensor meshes. In
+// nbr such a face is coded kNcCode0 - k (k >= 0): below every boundary code,
+// so the element kernels give it no flux, as a face without an integrator.
+// Its flux comes from nc_face_kernel: for every element with such faces, the
+// slave faces touching it are integrated at the slave face's points (Elem1 =
+// the slave, Elem2 = the master, at the master's reference points given by
+// the slave's point matrix) and the element's share is lifted to its nodes.
+constexpr int kNcCode0 = -(RKDG_MAX_BC + 2);
+struct NcArgs {
+  const double* x;        // stage input
+  int64_t NE;
+  int64_t n_elem;         // elements with non-conforming faces
+  const int* elem;        // [n_elem] their ids
+  const int* ptr;         // [n_elem + 1] ranges in item
+  const int* item;        // slave-face record * 2 + side (0: the element is the slave, 1: the master)
+  const int* rec;         // [records][3]: slave element, master element, slave local face
+  const double* geo;      // [records][NP * (2 DIM + 1) + DIM]: per point slave and master reference
+                          // coordinates and weight * |n|, then the slave's outward unit normal
+  double* out;            // [n_elem][NEQ][ND]
+  double* lam;            // wave speed slot
+  long long* err;
+  int no_lambda;
+};
+hipError_t launch_nc_faces(int dim, int p, const NcArgs& a, const SchemeParams& g, const TensorTables* t,
+                           hipStream_t s);
 // the static scratch array of rkdg_step_noalloc (rkdg_noalloc.hip): its
 // address on the current device, looked up once per device
 constexpr int64_t kScratchDoubles = int64_t(1) << 24;

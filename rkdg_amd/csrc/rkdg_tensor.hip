@@ -611,6 +611,14 @@ __global__ __launch_bounds__(64, ROLE == 1 ? 4 : 1) void tensor_rhs_kernel(const
   if (!A.face_only) {
     double outv[NEQ];
     double detJ = 1.0;
+    if (node_lane && A.ncslot) {
+      // the element's share of its non-conforming faces (nc_face_kernel)
+      const int ns = A.ncslot[e];
+      if (ns >= 0) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) z[v] += A.ncbuf[((int64_t)ns * NEQ + v) * ND + j];
+      }
+    }
     if (node_lane) {
       detJ = s_geo[team][GD - 1];
       double iw = tb.invw[jj[0]] * tb.invw[jj[1]];
@@ -751,7 +759,8 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
   // last hope: every variable on the x faces interpolated to y = 0, 1: [cx][cy][v][z node]
   __shared__ double s_ty[EPB][2][2][NEQ][NB];
   __shared__ double s_mean[EPB][NEQ];  // the element's means
-  __shared__ double s_nbm[EPB][NF];    // deciding variable's means of the face neighbours
+  __shared__ double s_nbm[EPB][NF];    // deciding variable's means of the face neighbours (lowest)
+  __shared__ double s_nbh[EPB][NF];    // ... highest (several neighbours across a non-conforming face)
 
   const int tid = threadIdx.x;
   const int team = tid / T, lane = tid % T;
@@ -780,8 +789,25 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
   const bool ind_bj = g.indicator == RKDG_IND_BJ;
   const double mown = (active && lane < NEQ) ? A.means[lane * NE + e] : 0.0;
   const int nbl = (active && ind_bj && lane < NF) ? A.nbr[lane * NE + e] : -1;
-  const double nml = nbl >= 0 ? A.means[vsel * NE + nbl] : 0.0;
-  const unsigned long long nbb = __ballot(nbl >= 0);
+  double nml = nbl >= 0 ? A.means[vsel * NE + nbl] : 0.0;
+  double nmh = nml;  // a non-conforming face: the bounds over its neighbours
+  bool has_nb = nbl >= 0;
+  if (A.ncnb && nbl <= kNcCode0) {
+    const int k = kNcCode0 - nbl;
+    nml = 1e300;
+    nmh = -1e300;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int n = A.ncnb[4 * k + q];
+      if (n >= 0) {
+        const double a = A.means[vsel * NE + n];
+        nml = a < nml ? a : nml;
+        nmh = a > nmh ? a : nmh;
+      }
+    }
+    has_nb = true;
+  }
+  const unsigned long long nbb = __ballot(has_nb);
   const int nbmask = (int)((nbb >> (team * T)) & ((1ull << NF) - 1ull));  // bit lf: face lf has a neighbour
   // Venkatakrishnan eps^2 = (K h)^3 with h = |element|^(1/dim): h^3 = det J
   // (3D) or det J^(3/2) (2D) for affine elements
@@ -794,7 +820,10 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
   tbr.store(reinterpret_cast<double*>(&tb), tid);
   str.store(s_u, tid);
   if (lane < NEQ) s_mean[team][lane] = mown;
-  if (lane < NF) s_nbm[team][lane] = nml;
+  if (lane < NF) {
+    s_nbm[team][lane] = nml;
+    s_nbh[team][lane] = nmh;
+  }
   __syncthreads();
   double mean[NEQ];
 #pragma unroll
@@ -808,9 +837,9 @@ __global__ __launch_bounds__(64) void tensor_limit_kernel(const LimitArgs A, con
 #pragma unroll
     for (int lf = 0; lf < NF; lf++) {
       if ((nbmask >> lf) & 1) {
-        const double a = s_nbm[team][lf];
+        const double a = s_nbm[team][lf], b = s_nbh[team][lf];
         mI = a < mI ? a : mI;
-        MI = a > MI ? a : MI;
+        MI = b > MI ? b : MI;
       }
     }
   }

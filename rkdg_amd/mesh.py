@@ -38,6 +38,13 @@ class Mesh:
     bdr_verts: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.int64))
     bdr_attr: np.ndarray = field(default_factory=lambda: np.zeros((0,), np.int32))
     name: str = ""
+    # non-conforming faces (rows: slave element, its local face, master element,
+    # its local face): the slave's face covers part of the coarser master's face
+    nc: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.int64))
+    # per record, the slave face's corners (in the slave's local face order) in
+    # the master face's parametrisation (face vertex 0 at 0; 2D one, 3D two
+    # coordinates per corner): MFEM's slave point matrix
+    nc_param: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.float64))
 
     def __post_init__(self):
         self.elem_coords = np.ascontiguousarray(self.elem_coords, dtype=np.float64)
@@ -45,6 +52,10 @@ class Mesh:
         nvf = _NVF[self.geom]
         self.bdr_verts = np.ascontiguousarray(np.asarray(self.bdr_verts, dtype=np.int64).reshape(-1, nvf))
         self.bdr_attr = np.ascontiguousarray(self.bdr_attr, dtype=np.int32)
+        self.nc = np.ascontiguousarray(np.asarray(self.nc, dtype=np.int64).reshape(-1, 4))
+        npar = nvf * (_DIM[self.geom] - 1)
+        self.nc_param = np.ascontiguousarray(np.asarray(self.nc_param, dtype=np.float64).reshape(-1, npar))
+        assert len(self.nc_param) == len(self.nc)
         assert self.elem_coords.shape == (self.num_elements, _NV[self.geom], _DIM[self.geom])
         assert self.elem_verts.shape == (self.num_elements, _NV[self.geom])
 
@@ -248,7 +259,7 @@ def relabel_boundary(m: Mesh, rule) -> Mesh:
     for k in range(len(ba)):
         mid = np.mean([vx[int(v)] for v in m.bdr_verts[k]], axis=0)
         ba[k] = rule(int(ba[k]), mid)
-    out = Mesh(m.geom, m.elem_coords, m.elem_verts, m.bdr_verts, ba, name=m.name)
+    out = Mesh(m.geom, m.elem_coords, m.elem_verts, m.bdr_verts, ba, name=m.name, nc=m.nc, nc_param=m.nc_param)
     return out
 
 

@@ -1,0 +1,219 @@
+"""Non-conforming (hanging-node) meshes on the GPU against the oracle: the
+right-hand side with its slave-face fluxes (face_integrator.cpp:11-276 on an
+NCMesh slave face), the limiter with the non-conforming neighbours in its
+stencil (limiter.cpp:166-217), and RK steps, on quadrilateral and hexahedral
+meshes refined with one and two levels of hanging nodes, periodic and with
+boundaries, with scrambled element orientations; free-stream preservation and
+discrete conservation on the GPU; the C-ABI's checks of the descriptor."""
+import numpy as np
+import pytest
+
+from oracle.refcpu import RefSolver
+from rkdg_amd import amr
+from rkdg_amd import mesh as M
+from rkdg_amd import scheme as S
+from tests.helpers import all_outlet, rel_linf, scramble_orientations, smooth_state
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _nc(base, *marks):
+    t = amr.NCMesh(base)
+    for mk in marks:
+        t.refine(mk)
+    return t.mesh()
+
+
+MESHES = {
+    "quad_q1_periodic_2lvl": (lambda: _nc(M.structured_quad(6, 6, periodic=(True, True)), [14, 15], [0]), 1, S.HLLC),
+    "quad_q2_periodic_2lvl": (lambda: _nc(M.structured_quad(6, 6, periodic=(True, True)), [14, 15], [0]), 2, S.RUSANOV),
+    "quad_q3_boundary": (lambda: _nc(M.structured_quad(6, 4), [0, 7, 23]), 3, S.LLF),
+    "quad_q2_rotated": (lambda: _nc(scramble_orientations(M.structured_quad(5, 5, periodic=(True, True)), seed=3),
+                                    [6, 12, 13]), 2, S.HLL),
+    "hex_q1_boundary": (lambda: _nc(M.structured_hex(3, 3, 3, periodic=(False, False, False)), [13, 4]), 1, S.HLL),
+    "hex_q2_periodic": (lambda: _nc(M.structured_hex(4, 4, 4), [21, 22]), 2, S.RUSANOV),
+    "hex_q2_rotated": (lambda: _nc(scramble_orientations(M.structured_hex(3, 3, 3, periodic=(False, False, False)),
+                                                         seed=5), [13]), 2, S.HLLC),
+}
+
+
+def _scheme(m, rs, **kw):
+    return S.Scheme(riemann=rs, bcs=all_outlet(m), **kw)
+
+
+def _dev(x, cuda):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(x)).to(cuda)
+
+
+@pytest.mark.parametrize("name", list(MESHES))
+def test_nc_rhs_matches_oracle(name, cuda):
+    import torch
+
+    from rkdg_amd.capi import Context
+
+    mk, p, rs = MESHES[name]
+    m = mk()
+    assert len(m.nc) > 0
+    sch = _scheme(m, rs)
+    ref = RefSolver(m, p, sch)
+    x = smooth_state(ref.node_coords(), seed=4, amp=1e-3)
+    y_ref, lam_ref = ref.mult(x)
+    ctx = Context(m, p, sch)
+    xd = _dev(x, cuda)
+    yd = torch.empty_like(xd)
+    lam = ctx.rhs(xd, yd)
+    err = rel_linf(yd.cpu().numpy(), y_ref, m.dim + 2)
+    lf = ctx.face_lambda(xd)
+    _, lf_ref = ref.face_mult(x)
+    print(f"[nc {name}] NE={m.num_elements} slave faces={len(m.nc)} RHS rel Linf {err:.2e}, "
+          f"lambda {lam:.15f} / {lam_ref:.15f}", flush=True)
+    assert err <= TOL
+    assert abs(lam - lam_ref) <= 1e-13 * lam_ref
+    assert abs(lf - lf_ref) <= 1e-13 * lf_ref
+    ctx.close()
+
+
+def _jump_state(xyz):
+    dim = xyz.shape[-1]
+    lo = xyz.reshape(-1, dim).min(axis=0)
+    hi = xyz.reshape(-1, dim).max(axis=0)
+    u = (xyz - lo) / (hi - lo)
+    x = smooth_state(u, seed=2, amp=0.0)
+    U = x.reshape(dim + 2, -1)
+    j = (u[..., 0].ravel() + 0.3 * u[..., 1].ravel() > 0.55).astype(float)
+    U[0] *= 1.0 + 1.5 * j
+    U[-1] *= 1.0 + 2.5 * j
+    return np.ascontiguousarray(U.reshape(-1))
+
+
+LIMITED = {
+    "quad_q2_multi_venkat": ("quad_q2_periodic_2lvl", dict(indicator=S.IND_BJ, phi=S.PHI_VENKAT, venkat_k=5.0,
+                                                           limiter=S.LIM_MULTI)),
+    "quad_q1_multi_michalak": ("quad_q1_periodic_2lvl", dict(indicator=S.IND_BJ, phi=S.PHI_MICHALAK,
+                                                             limiter=S.LIM_MULTI)),
+    "quad_q3_findiff": ("quad_q3_boundary", dict(indicator=S.IND_BJ, phi=S.PHI_BJ, limiter=S.LIM_FINDIFF)),
+    "hex_q1_multi_bj": ("hex_q1_boundary", dict(indicator=S.IND_BJ, phi=S.PHI_BJ, limiter=S.LIM_MULTI)),
+    "hex_q2_rotated_venkat": ("hex_q2_rotated", dict(indicator=S.IND_BJ, phi=S.PHI_VENKAT, venkat_k=5.0,
+                                                     limiter=S.LIM_MULTI)),
+}
+
+
+@pytest.mark.parametrize("name", list(LIMITED))
+def test_nc_limiter_and_steps_match_oracle(name, cuda):
+    import torch
+
+    from rkdg_amd.capi import Context
+
+    mesh_name, kw = LIMITED[name]
+    mk, p, rs = MESHES[mesh_name]
+    m = mk()
+    sch = _scheme(m, rs, **kw)
+    ref = RefSolver(m, p, sch)
+    x = _jump_state(ref.node_coords())
+    x_ref, ind, mv = ref.limit(x)
+    ctx = Context(m, p, sch)
+    xd = _dev(x, cuda)
+    tr = torch.zeros(m.num_elements, dtype=torch.int8, device=cuda)
+    ctx.limit(xd, tr)
+    neq = m.dim + 2
+    err = rel_linf(xd.cpu().numpy(), x_ref, neq)
+    flags = tr.cpu().numpy()
+    if sch.limiter == S.LIM_MULTI:
+        want = (mv < 1.0).astype(np.int8)
+        near = np.abs(mv - 1.0) < 1e-12
+    else:
+        last = ind.reshape(neq, -1)[-1]
+        want = (last < 0.999).astype(np.int8)
+        near = np.abs(last - 0.999) < 1e-12
+    # elements next to a non-conforming face, and their troubled count
+    nce = np.unique(np.concatenate([m.nc[:, 0], m.nc[:, 2]]))
+    print(f"[nc limit {name}] rel Linf {err:.2e}, troubled {int((flags & 1).sum())} (oracle {int(want.sum())}), "
+          f"at non-conforming faces {int((flags[nce] & 1).sum())} of {len(nce)}", flush=True)
+    assert err <= TOL
+    assert np.all(((flags & 1) == want) | near)
+    assert int(want.sum()) > 0
+    xs = x_ref.copy()
+    xg = torch.from_numpy(x_ref.copy()).to(cuda)
+    dt = 0.05 * ref.hmin() / ref.mult(x_ref)[1] / (2 * p + 1)
+    for _ in range(2):
+        xs, _ = ref.step(xs, dt, S.TABLEAUS["ssprk3"])
+        ctx.step(xg, dt, S.TABLEAUS["ssprk3"])
+    e2 = rel_linf(xg.cpu().numpy(), xs, neq)
+    print(f"[nc step {name}] 2 SSP-RK3 steps rel Linf {e2:.2e}", flush=True)
+    assert e2 <= TOL
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", ["quad_q2_periodic_2lvl", "hex_q2_periodic"])
+def test_nc_free_stream_and_conservation(name, cuda):
+    import torch
+
+    from rkdg_amd.capi import Context
+
+    mk, p, rs = MESHES[name]
+    m = mk()
+    sch = _scheme(m, rs)
+    ctx = Context(m, p, sch)
+    dim, N = m.dim, p + 1
+    nd = N ** dim
+    n = m.num_elements * nd
+    rho, pr = 1.3, 0.8
+    vel = [0.2, -0.15, 0.1][:dim]
+    E = pr / 0.4 + 0.5 * rho * sum(v * v for v in vel)
+    x = np.concatenate([np.full(n, rho)] + [np.full(n, rho * v) for v in vel] + [np.full(n, E)])
+    xd = _dev(x, cuda)
+    yd = torch.empty_like(xd)
+    ctx.rhs(xd, yd)
+    h = float(np.min(np.linalg.norm(m.elem_coords[:, 1] - m.elem_coords[:, 0], axis=-1)))
+    scale = (E + pr) * max(abs(v) for v in vel) / h
+    fs = float(yd.abs().max())
+    # conservation: sum over elements of the weighted RHS (periodic mesh: zero)
+    xs = smooth_state(ctx.node_coords(), seed=7)
+    xd = _dev(xs, cuda)
+    ctx.rhs(xd, yd)
+    w = np.polynomial.legendre.leggauss(N)[1] / 2.0
+    W = w
+    for _ in range(dim - 1):
+        W = np.multiply.outer(w, W)
+    X = m.elem_coords
+    J = np.stack([X[:, 1] - X[:, 0], X[:, 3] - X[:, 0]] + ([X[:, 4] - X[:, 0]] if dim == 3 else []), axis=-1)
+    det = np.abs(np.linalg.det(J))
+    Y = yd.cpu().numpy().reshape(dim + 2, m.num_elements, nd)
+    tot = (Y * W.ravel()[None, None, :] * det[None, :, None]).sum(axis=(1, 2))
+    mag = (np.abs(Y) * W.ravel()[None, None, :] * det[None, :, None]).sum(axis=(1, 2))
+    print(f"[nc {name}] free stream max |RHS| {fs:.2e} (term scale {scale:.2f}); conservation {tot / mag}", flush=True)
+    assert fs <= 1e-13 * scale
+    assert np.abs(tot / mag).max() <= 1e-14
+    ctx.close()
+
+
+def test_nc_descriptor_checks(cuda):
+    from rkdg_amd import capi
+    from rkdg_amd.capi import Context
+
+    m = MESHES["quad_q2_periodic_2lvl"][0]()
+    sch = _scheme(m, S.HLLC)
+    with pytest.raises(capi.RKDGError) as ei:
+        Context(m, 2, S.Scheme(riemann=S.HLLC, indicator=S.IND_SHU, limiter=S.LIM_FINDIFF))
+    assert ei.value.code == capi.RKDG_E_UNSUPPORTED
+    with pytest.raises(capi.RKDGError) as ei:
+        Context(m, 2, sch, dense=True)
+    assert ei.value.code == capi.RKDG_E_UNSUPPORTED
+    bad = M.Mesh(m.geom, m.elem_coords, m.elem_verts, m.bdr_verts, m.bdr_attr, nc=m.nc, nc_param=m.nc_param * 3.0)
+    with pytest.raises(capi.RKDGError) as ei:
+        Context(bad, 2, sch)
+    assert ei.value.code == capi.RKDG_E_ARG and "outside" in str(ei.value)
+    # a record naming a conforming face as the master
+    nc = m.nc.copy()
+    e = int(nc[0, 2])
+    for lf in range(4):
+        if not np.any((m.nc[:, 2] == e) & (m.nc[:, 3] == lf)) and not np.any((m.nc[:, 0] == e) & (m.nc[:, 1] == lf)):
+            nc[0, 3] = lf
+            break
+    bad = M.Mesh(m.geom, m.elem_coords, m.elem_verts, m.bdr_verts, m.bdr_attr, nc=nc, nc_param=m.nc_param)
+    with pytest.raises(capi.RKDGError) as ei:
+        Context(bad, 2, sch)
+    assert ei.value.code == capi.RKDG_E_ARG and "conforming neighbour" in str(ei.value)
