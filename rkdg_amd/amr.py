@@ -237,7 +237,104 @@ class NCMesh:
                     out.append((old_index[anc], scale, off))
                     break
             else:
-                out.append((-1, 1.0, np.zeros(self.dim)))
+                # a parent of old leaves (derefinement): its children's old indices
+                kids = [old_index.get((root, path + (c,)), -1) for c in range(1 << self.dim)]
+                if min(kids) < 0:
+                    raise ValueError("a leaf without old ancestor or old children")
+                out.append((-1, kids, None))
+        return out
+
+    def derefine(self, parents):
+        """Coarsen the sibling groups whose parents are given as (root, path)
+        keys, when all their children are leaves and the coarsened parent keeps
+        the level limit with its face neighbours (MFEM's DerefineByError with
+        nc_limit 1). Returns (transfer, the parents coarsened)."""
+        old = self.ordered_keys()
+        old_index = {k: i for i, k in enumerate(old)}
+        nch = 1 << self.dim
+        R = _REF_CORNERS[self.geom]
+        done = []
+        for root, path in parents:
+            path = tuple(path)
+            kids = [(root, path + (c,)) for c in range(nch)]
+            if not all(k in self.leaves for k in kids):
+                continue
+            ch = [self.leaves[k] for k in kids]
+            X = np.zeros((self.nv, self.dim))
+            V = []
+            for k in range(self.nv):
+                child = sum(int(R[k][d]) << d for d in range(self.dim))  # the child at corner k
+                X[k] = ch[child].X[k]
+                V.append(ch[child].V[k])
+            parent = _Leaf(root, path, X, tuple(V))
+            for k in kids:
+                self.leaves.pop(k)
+            self.leaves[(root, path)] = parent
+            saved = self._prune_vertices()
+            if self._needs_refinement(parent):
+                self.leaves.pop((root, path))
+                for k, c in zip(kids, ch):
+                    self.leaves[k] = c
+                self._new.update(saved[0])
+                self._built_from.update(saved[1])
+                continue
+            for lf in range(len(self.faces)):
+                fk = self._face_key(parent, lf)
+                subs = self._subfaces_of_leaf_face(ch, lf)
+                if subs and all(sk in self.bdr for sk in subs):
+                    attr = self.bdr[subs[0]]
+                    for sk in subs:
+                        self.bdr.pop(sk)
+                    self.bdr[fk] = attr
+            done.append((root, path))
+        return self._transfer(old_index), done
+
+    def _subfaces_of_leaf_face(self, children, lf):
+        """Keys of the children's faces that lie on local face lf of their parent."""
+        r, c = self._face_axis(lf)
+        out = []
+        for i, lv in enumerate(children):
+            if ((i >> r) & 1) == c:
+                out.append(self._face_key(lv, lf))
+        return out
+
+    def _face_axis(self, lf):
+        """(normal axis, side) of local face lf in the reference element."""
+        R = _REF_CORNERS[self.geom]
+        pts = R[list(self.faces[lf])]
+        for d in range(self.dim):
+            if np.all(pts[:, d] == pts[0, d]):
+                return d, int(pts[0, d])
+        raise ValueError("face not on a coordinate plane")
+
+    def _prune_vertices(self):
+        """Forget built vertices that no leaf uses any more (so a face whose
+        neighbour was coarsened reads as conforming again); returns what was
+        removed, to restore it."""
+        used = set()
+        for lv in self.leaves.values():
+            used.update(lv.V)
+        gone_new = {k: v for k, v in self._new.items() if v not in used}
+        gone_bf = {v: self._built_from[v] for v in gone_new.values() if v in self._built_from}
+        for k in gone_new:
+            self._new.pop(k)
+        for v in gone_bf:
+            self._built_from.pop(v)
+        return gone_new, gone_bf
+
+    def derefinable(self):
+        """(root, path) of every sibling group whose children are all leaves."""
+        nch = 1 << self.dim
+        out, seen = [], set()
+        for root, path in self.ordered_keys():
+            if not path:
+                continue
+            par = (root, path[:-1])
+            if par in seen:
+                continue
+            seen.add(par)
+            if all((root, path[:-1] + (c,)) in self.leaves for c in range(nch)):
+                out.append(par)
         return out
 
     # -- output ----------------------------------------------------------
@@ -308,28 +405,321 @@ class NCMesh:
         return np.array([len(k[1]) for k in self.ordered_keys()], np.int64)
 
 
-def prolong_state(x_old, transfer, order, dim, neq, ne_old):
-    """States of the new leaves from the old ones (refinement: the old
-    polynomial at the new nodes). x is byNODES: [v][e][node], nodes x fastest."""
+def _apply_axes(mats, u, dim):
+    """Apply the 1D matrix mats[d] along reference axis d of nodal values
+    u[v][z][y][x] (x fastest)."""
+    u = np.einsum("ia,v...a->v...i", mats[0], u)
+    if dim == 2:
+        return np.einsum("jb,vba->vja", mats[1], u)
+    u = np.einsum("jb,vcba->vcja", mats[1], u)
+    return np.einsum("kc,vcba->vkba", mats[2], u)
+
+
+def transfer_state(x_old, transfer, order, dim, neq, ne_old):
+    """States of the new leaves from the old ones. x is byNODES: [v][e][node],
+    nodes x fastest. A refined element's children take its polynomial at their
+    nodes (exact). A coarsened parent takes the L2 projection of its children
+    (the parent's Gauss-Legendre mass matrix is diagonal and exact, and the
+    children's integrals are exact with their own rule), which keeps every
+    element mean and so the conserved totals."""
     N = order + 1
     nd = N ** dim
     nodes = _gl_nodes(N)
+    w = np.polynomial.legendre.leggauss(N)[1] / 2.0
     xo = np.asarray(x_old).reshape(neq, ne_old, nd)
     out = np.zeros((neq, len(transfer), nd))
     for i, (src, scale, off) in enumerate(transfer):
         if src < 0:
-            raise ValueError("a leaf without an old ancestor")
+            kids = scale
+            acc = np.zeros((neq,) + (N,) * dim)
+            for c, k in enumerate(kids):
+                mats = []
+                for d in range(dim):
+                    o = 0.5 * ((c >> d) & 1)
+                    # R[i][k] = (1/2) w_k psi_i(o + x_k / 2) / w_i
+                    mats.append(0.5 * _lagrange_matrix(nodes, o + 0.5 * nodes).T * w[None, :] / w[:, None])
+                acc += _apply_axes(mats, xo[:, k].reshape((neq,) + (N,) * dim), dim)
+            out[:, i] = acc.reshape(neq, nd)
+            continue
         if scale == 1.0:
             out[:, i] = xo[:, src]
             continue
         mats = [_lagrange_matrix(nodes, off[d] + scale * nodes) for d in range(dim)]
-        u = xo[:, src].reshape((neq,) + (N,) * dim)  # [v][z][y][x]
-        # apply per axis: x is the last array axis
-        u = np.einsum("ia,v...a->v...i", mats[0], u)
-        if dim == 2:
-            u = np.einsum("jb,vba->vja", mats[1], u)
-        else:
-            u = np.einsum("jb,vcba->vcja", mats[1], u)
-            u = np.einsum("kc,vcba->vkba", mats[2], u)
-        out[:, i] = u.reshape(neq, nd)
+        out[:, i] = _apply_axes(mats, xo[:, src].reshape((neq,) + (N,) * dim), dim).reshape(neq, nd)
     return out.reshape(-1)
+
+
+prolong_state = transfer_state
+
+
+# ---------------------------------------------------------------------------
+# Error indicator and the threshold refinement loop
+
+
+def zz_indicator(mesh, order, x, var=0):
+    """Zienkiewicz-Zhu-style error indicator of one conserved variable
+    (density by default) per element: the L2 norm over the element of the
+    difference between its DG gradient and a recovered continuous gradient
+    (the element-mean gradients averaged at each vertex, interpolated
+    multilinearly). This stands in for MFEM's L2ZienkiewiczZhuEstimator on the
+    density (main.cpp:237-251), whose recovery is a global L2 projection onto
+    a Raviart-Thomas space that is not rebuilt here: the two estimate the same
+    quantity, not with the same numbers. Affine quadrilaterals and hexahedra."""
+    dim = mesh.dim
+    N = order + 1
+    nd = N ** dim
+    NE = mesh.num_elements
+    nodes = _gl_nodes(N)
+    w1 = np.polynomial.legendre.leggauss(N)[1] / 2.0
+    D = np.zeros((N, N))  # D[a, b] = psi_b'(x_a)
+    for b in range(N):
+        for a in range(N):
+            s = 0.0
+            for m in range(N):
+                if m == b:
+                    continue
+                t = 1.0 / (nodes[b] - nodes[m])
+                for j in range(N):
+                    if j not in (b, m):
+                        t *= (nodes[a] - nodes[j]) / (nodes[b] - nodes[j])
+                s += t
+            D[a, b] = s
+    u = np.asarray(x).reshape(dim + 2, NE, nd)[var].reshape((NE,) + (N,) * dim)  # [e][z][y][x]
+    gref = []
+    for d in range(dim):
+        ax = u.ndim - 1 - d  # x is the last axis
+        gref.append(np.moveaxis(np.tensordot(u, D, axes=([ax], [1])), -1, ax).reshape(NE, nd))
+    gref = np.stack(gref, axis=-1)  # [e][node][d] reference gradient
+    X = mesh.elem_coords
+    cols = [1, 3, 4][:dim]
+    J = np.stack([X[:, c] - X[:, 0] for c in cols], axis=-1)  # [e][i][r]
+    Jinv = np.linalg.inv(J)
+    grad = np.einsum("enr,eri->eni", gref, Jinv)  # J^-T applied: g_x = sum_r dxi_r/dx_i g_r
+    det = np.abs(np.linalg.det(J))
+    W = w1
+    for _ in range(dim - 1):
+        W = np.multiply.outer(w1, W)
+    W = W.ravel()  # node weights, x fastest
+    gmean = np.einsum("n,eni->ei", W, grad)
+    V = mesh.elem_verts
+    nv = V.shape[1]
+    ids, inv = np.unique(V.ravel(), return_inverse=True)
+    acc = np.zeros((len(ids), dim))
+    cnt = np.zeros(len(ids))
+    np.add.at(acc, inv, np.repeat(gmean, nv, axis=0))
+    np.add.at(cnt, inv, 1.0)
+    Gv = (acc / cnt[:, None])[inv].reshape(NE, nv, dim)
+    # multilinear interpolation of the corner values at the nodes
+    R = _REF_CORNERS[mesh.geom]
+    grids = np.meshgrid(*([nodes] * dim), indexing="ij")  # [x][y][z] ordering
+    xi = np.stack([g.transpose(*range(dim)[::-1]).ravel() for g in grids], axis=-1)  # nodes x fastest
+    Nk = np.ones((nd, nv))
+    for k in range(nv):
+        for d in range(dim):
+            Nk[:, k] *= xi[:, d] if R[k][d] else 1.0 - xi[:, d]
+    Gn = np.einsum("nk,eki->eni", Nk, Gv)
+    err2 = np.einsum("n,eni->e", W, (Gn - grad) ** 2) * det
+    return np.sqrt(err2)
+
+
+def refine_marks(eta, max_elem_error, total_error_fraction=0.0, total_norm_p=np.inf):
+    """ThresholdRefiner's marking (the reference sets the local goal
+    maxElemError and the total fraction totalErrorFraction,
+    case_manager.cpp:450-466): elements whose indicator exceeds
+    max(total_error_fraction * total * NE^(-1/p), max_elem_error)."""
+    ne = len(eta)
+    if np.isinf(total_norm_p):
+        total = float(np.max(eta)) if ne else 0.0
+        scale = 1.0
+    else:
+        total = float(np.sum(eta ** total_norm_p) ** (1.0 / total_norm_p))
+        scale = ne ** (-1.0 / total_norm_p)
+    thr = max(total_error_fraction * total * scale, max_elem_error)
+    return np.flatnonzero(eta > thr)
+
+
+def derefine_groups(tree, eta, threshold):
+    """ThresholdDerefiner's choice: sibling groups whose children's indicators
+    sum below the threshold (hysteresis * maxElemError)."""
+    keys = tree.ordered_keys()
+    index = {k: i for i, k in enumerate(keys)}
+    nch = 1 << tree.dim
+    out = []
+    for root, path in tree.derefinable():
+        s = sum(eta[index[(root, tuple(path) + (c,))]] for c in range(nch))
+        if s < threshold:
+            out.append((root, path))
+    return out
+
+
+class _GpuSolver:
+    """The GPU path on one mesh of the adaptive run (rkdg_create_nc after each
+    change of the mesh, the state handed over through host memory)."""
+
+    def __init__(self, mesh, order, scheme, device=0):
+        import torch
+
+        from .capi import Context
+
+        self.torch = torch
+        self.dev = torch.device("cuda", device)
+        self.ctx = Context(mesh, order, scheme)
+
+    def node_coords(self):
+        return self.ctx.node_coords()
+
+    def hmin(self):
+        return self.ctx.hmin()
+
+    def _d(self, x):
+        return self.torch.from_numpy(np.ascontiguousarray(x)).to(self.dev)
+
+    def limit(self, x):
+        xd = self._d(x)
+        self.ctx.limit(xd)
+        return xd.cpu().numpy()
+
+    def face_lambda(self, x):
+        return self.ctx.face_lambda(self._d(x))
+
+    def step(self, x, dt, tableau):
+        xd = self._d(x)
+        lam = self.ctx.step(xd, dt, tableau)
+        return xd.cpu().numpy(), lam
+
+    def close(self):
+        self.ctx.close()
+
+
+class _OracleSolver:
+    """The oracle on one mesh of the adaptive run (the checker of the GPU run)."""
+
+    def __init__(self, mesh, order, scheme):
+        from oracle.refcpu import RefSolver
+
+        self.ref = RefSolver(mesh, order, scheme)
+
+    def node_coords(self):
+        return self.ref.node_coords()
+
+    def hmin(self):
+        return self.ref.hmin()
+
+    def limit(self, x):
+        return self.ref.limit(x)[0]
+
+    def face_lambda(self, x):
+        return self.ref.face_mult(x)[1]
+
+    def step(self, x, dt, tableau):
+        return self.ref.step(x, dt, tableau)
+
+    def close(self):
+        pass
+
+
+@dataclass
+class AdaptiveResult:
+    x: np.ndarray
+    mesh: M.Mesh
+    tree: NCMesh
+    t: float
+    dt: float
+    steps: int
+    elements: list      # element count after each step
+    refinements: int    # refinement iterations over the run
+
+
+def run_adaptive(base, order, scheme, initial_state, t_final, cfl, tableau, max_elem_error, hysteresis=0.5,
+                 total_error_fraction=0.0, max_steps=None, solver="gpu", advance_time_per_retry=True,
+                 hmin_from_base=True, max_level=None, dt_const=None, callback=None):
+    """The reference's adaptive time loop (main.cpp:253-386 and :505-700) on
+    the host, with the right-hand side, limiter and RK step on the chosen
+    solver ("gpu": rkdg_create_nc and the HIP kernels; "oracle": the CPU
+    restatement, as the checker). Per time step: step, estimate on the new
+    state, refine the marked elements and repeat the step from the old state
+    on the refined mesh until nothing is marked, then coarsen the groups below
+    hysteresis * max_elem_error and limit. The reference advances t on every
+    repeat of the step (Step takes t by reference inside the refinement loop,
+    main.cpp:514); advance_time_per_retry=False advances it once per step. It
+    also takes hmin of the mesh it read, before any refinement
+    (main.cpp:191-201), for every dt; hmin_from_base=False takes the current
+    mesh's. max_level caps the refinement level (an indicator that does not
+    fall with h, as at a shock, would refine without end; MFEM caps it through
+    its element budget). dt_const keeps dt fixed (cfl <= 0 in the
+    reference). callback(steps, t, dt, mesh, x) after every step.
+    initial_state(xyz) -> state on the nodes xyz [NE][nd][dim]."""
+    make = (lambda m: _GpuSolver(m, order, scheme)) if solver == "gpu" else (lambda m: _OracleSolver(m, order, scheme))
+
+    def marks_of(mesh, x):
+        mk = refine_marks(zz_indicator(mesh, order, x), max_elem_error, total_error_fraction)
+        if max_level is not None:
+            mk = mk[tree.levels()[mk] < max_level]
+        return mk
+
+    dim = base.dim
+    neq = dim + 2
+    tree = NCMesh(base)
+    refinements = 0
+    hmin0 = None
+    # initial refinement on the projected initial condition (main.cpp:253-320)
+    while True:
+        mesh = tree.mesh()
+        S_ = make(mesh)
+        if hmin0 is None:
+            hmin0 = S_.hmin()
+        x = S_.limit(initial_state(S_.node_coords()))
+        marks = marks_of(mesh, x)
+        if len(marks) == 0:
+            break
+        S_.close()
+        tree.refine(marks)
+        refinements += 1
+
+    def coarsen(x, S_, mesh):
+        groups = derefine_groups(tree, zz_indicator(mesh, order, x), hysteresis * max_elem_error)
+        if not groups:
+            return x, S_, mesh
+        ne_old = mesh.num_elements
+        tr, done = tree.derefine(groups)
+        if not done:
+            return x, S_, mesh
+        x = transfer_state(x, tr, order, dim, neq, ne_old)
+        S_.close()
+        mesh = tree.mesh()
+        return x, make(mesh), mesh
+
+    x, S_, mesh = coarsen(x, S_, mesh)
+    x = S_.limit(x)
+    hmin = (lambda: hmin0) if hmin_from_base else (lambda: S_.hmin())
+    lam = S_.face_lambda(x)
+    dt = dt_const if dt_const is not None else cfl * hmin() / lam / (2 * order + 1)
+    t, steps, counts = 0.0, 0, []
+    while t < t_final - 1e-8 * dt and (max_steps is None or steps < max_steps):
+        dt_real = min(dt, t_final - t)
+        x_old = x
+        t_start = t
+        while True:
+            x_new, lam = S_.step(x_old, dt_real, tableau)
+            t = t + dt_real if advance_time_per_retry else t_start + dt_real
+            if dt_const is None:
+                dt = cfl * hmin() / lam / (2 * order + 1)
+            marks = marks_of(mesh, x_new)
+            if len(marks) == 0:
+                x = x_new
+                break
+            ne_old = mesh.num_elements
+            tr = tree.refine(marks)
+            refinements += 1
+            x_old = transfer_state(x_old, tr, order, dim, neq, ne_old)
+            S_.close()
+            mesh = tree.mesh()
+            S_ = make(mesh)
+        x, S_, mesh = coarsen(x, S_, mesh)
+        x = S_.limit(x)
+        steps += 1
+        counts.append(mesh.num_elements)
+        if callback is not None:
+            callback(steps, t, dt, mesh, x)
+    S_.close()
+    return AdaptiveResult(x, mesh, tree, t, dt, steps, counts, refinements)

@@ -8,7 +8,10 @@ time settings and output settings are the ones the reference would run:
 
 * ``mesh``: ``file`` (Gmsh, relative to the case directory),
   ``serialRefLevels`` + ``parallelRefLevels`` uniform refinements,
-  ``adaptive`` (only false: the path has no adaptive refinement);
+  ``adaptive`` with ``maxElemError``, ``totalErrorFraction``, ``hysteresis``,
+  ``nonConformingLimit`` (1) and ``preferConformingRefinement`` (false) --
+  case_manager.cpp:450-466: non-conforming refinement of quadrilateral and
+  hexahedral meshes (``amr.run_adaptive``);
 * ``physics``: ``type`` (``covolume`` reads ``covolume``, anything else is a
   perfect gas), ``gamma``, ``R`` (default 1) -- case_manager.cpp:83-101;
 * ``internalField``: ``constant``, ``planeBreakup``, ``sphericalBreakup``,
@@ -34,7 +37,9 @@ reference's, for workloads it cannot express:
 * ``spatial.limiter.phi`` (michalak | bj | venkat) and ``venkatK``: the
   BJ-family limiter function (the reference compiles Michalak in);
 * ``spatial.waveSpeeds`` (Toro | Einfeldt);
-* ``time.tableau`` (a name in ``scheme.TABLEAUS``) in place of ``order``.
+* ``time.tableau`` (a name in ``scheme.TABLEAUS``) in place of ``order``;
+* ``mesh.maxRefLevel``: a cap on the refinement level of adaptive runs (the
+  reference has none; its ZZ estimator does not fall with h at a shock).
 """
 from __future__ import annotations
 
@@ -46,6 +51,7 @@ import numpy as np
 import yaml
 
 from . import gmsh
+from . import mesh as M
 from . import scheme as S
 from .cases import Case
 
@@ -74,6 +80,12 @@ class FileCase(Case):
     case_dir: str = ""
     bdr_names: dict = field(default_factory=dict)
     skipped_patches: list = field(default_factory=list)
+    # adaptive refinement (mesh.adaptive, case_manager.cpp:450-466)
+    adaptive: bool = False
+    max_elem_error: float = 0.0
+    total_error_fraction: float = 0.0
+    hysteresis: float = 0.0
+    max_ref_level: int = None
 
     @property
     def dt_const(self):
@@ -293,8 +305,19 @@ def load_case(path) -> FileCase:
 
     # mesh (case_manager.cpp:217-297)
     mfile = _get(st, "mesh", "file", required=True)
-    if _flag(_get(st, "mesh", "adaptive"), False):
-        raise CaseError("mesh.adaptive: adaptive refinement is not part of the RKDG path")
+    adaptive = _flag(_get(st, "mesh", "adaptive"), False)
+    amr_kw = {}
+    if adaptive:
+        ncl = int(_get(st, "mesh", "nonConformingLimit", default=1))
+        if ncl != 1:
+            raise CaseError("mesh.nonConformingLimit: one level of hanging nodes is built")
+        if _flag(_get(st, "mesh", "preferConformingRefinement"), False):
+            raise CaseError("mesh.preferConformingRefinement: quadrilaterals and hexahedra refine non-conformingly")
+        mre = _get(st, "mesh", "maxRefLevel")
+        amr_kw = dict(adaptive=True, max_elem_error=float(_get(st, "mesh", "maxElemError", required=True)),
+                      total_error_fraction=float(_get(st, "mesh", "totalErrorFraction", default=0.0)),
+                      hysteresis=float(_get(st, "mesh", "hysteresis", default=0.0)),
+                      max_ref_level=None if mre is None else int(mre))
     gm = gmsh.read_gmsh(os.path.join(case_dir, mfile))
     levels = int(_get(st, "mesh", "serialRefLevels", default=0)) + int(_get(st, "mesh", "parallelRefLevels", default=0))
     mesh = gmsh.uniform_refinement(gm.mesh, levels) if levels > 0 else gm.mesh
@@ -355,7 +378,13 @@ def load_case(path) -> FileCase:
     cfl = -1.0 if ttype == "constant" else float(_get(st, "time", "CoMax", required=True))
 
     name = os.path.basename(case_dir)
-    return FileCase(name, mesh, order, sch, t_end, cfl, tableau=tname, params={"ic": ic},
+    if adaptive and mesh.geom == M.GEOM_TRI:
+        raise CaseError("mesh.adaptive: non-conforming refinement is built for quadrilaterals and hexahedra")
+    if adaptive and _flag(_get(st, "time", "restart"), False):
+        raise CaseError("time.restart with mesh.adaptive: restart frames hold no refinement tree")
+    if adaptive and ind_name == "Shu":
+        raise CaseError("mesh.adaptive: the Shu indicator is not built for non-conforming faces")
+    return FileCase(name, mesh, order, sch, t_end, cfl, tableau=tname, params={"ic": ic}, **amr_kw,
                     dt=dt, time_type=ttype,
                     restart=_flag(_get(st, "time", "restart"), False),
                     restart_cycle=int(_get(st, "time", "restartCycle", default=0)),

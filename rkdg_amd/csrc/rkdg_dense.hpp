@@ -8,7 +8,7 @@
 
 // element types and orders instantiated
 #define RKDG_DENSE_CASES(X) X(RKDG_GEOM_TRI, 1) X(RKDG_GEOM_TRI, 2) X(RKDG_GEOM_TRI, 3) \
-  X(RKDG_GEOM_QUAD, 1) X(RKDG_GEOM_QUAD, 2)
+  X(RKDG_GEOM_QUAD, 1) X(RKDG_GEOM_QUAD, 2) X(RKDG_GEOM_QUAD, 3)
 
 namespace rkdg {
 

@@ -149,7 +149,7 @@ hipError_t launch_nc_faces(int dim, int p, const NcArgs& a, const SchemeParams& 
     nc_face_kernel<D, PP><<<(unsigned)a.n_elem, 64, 0, s>>>(a, g, t);            \
     return hipGetLastError();                                                    \
   }
-  X(2, 1) X(2, 2) X(2, 3) X(3, 1) X(3, 2) X(3, 3)
+  X(2, 1) X(2, 2) X(2, 3) X(2, 4) X(2, 5) X(3, 1) X(3, 2) X(3, 3)
 #undef X
   return hipErrorInvalidValue;
 }

@@ -1059,7 +1059,7 @@ hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s)
   return hipGetLastError();
 }
 
-#define RKDG_TENSOR_CASES(X) X(2, 1) X(2, 2) X(2, 3) X(3, 1) X(3, 2) X(3, 3)
+#define RKDG_TENSOR_CASES(X) X(2, 1) X(2, 2) X(2, 3) X(2, 4) X(2, 5) X(3, 1) X(3, 2) X(3, 3)
 
 bool tensor_supported(int dim, int p) {
 #define X(D, PP) \

@@ -33,6 +33,8 @@ def run_case(case_dir: str, out_dir: str = None, max_steps: int = None, lod: int
     out_dir = out_dir or cs.case_dir
     cuda = torch.device("cuda", device)
     torch.cuda.set_device(cuda)
+    if cs.adaptive:
+        return _run_adaptive_case(cs, out_dir, max_steps, lod, quiet, cuda)
     f = FE_Evolution(cs.mesh, cs.order, cs.scheme)
     lim = Limiter(f)
     sol = ExplicitRKLimitedSolver(*S.TABLEAUS[cs.tableau], limiter=lim)
@@ -69,6 +71,39 @@ def run_case(case_dir: str, out_dir: str = None, max_steps: int = None, lod: int
     last_step = step0 + stats.steps
     if not stats.t >= cs.t_final - 1e-8 * stats.dt and last_step % pp.vis_steps != 0:
         pp.write(last_step, stats.t, stats.dt, x)
+    return cs, stats, pp, x
+
+
+def _run_adaptive_case(cs, out_dir, max_steps, lod, quiet, cuda):
+    """mesh.adaptive: the reference's refinement loop around every step
+    (main.cpp:253-386, 505-700) through amr.run_adaptive, the GPU context
+    re-created after every change of the mesh; one log line per step with the
+    element count, and the final state written on the final mesh."""
+    import types
+
+    import torch
+
+    from . import amr
+    from .capi import Context
+
+    clock = time.perf_counter()
+
+    def callback(steps, t, dt, mesh, x):
+        if not quiet:
+            print(f"Time step: {steps}\tdt: {dt:.6g}\tPhys time: {t:.6g} \tExecution time: "
+                  f"{time.perf_counter() - clock:.6g} \tElements: {mesh.num_elements} "
+                  f"(non-conforming faces {len(mesh.nc)})", flush=True)
+
+    r = amr.run_adaptive(cs.mesh, cs.order, cs.scheme, cs.initial_state, cs.t_final, cs.cfl,
+                         S.TABLEAUS[cs.tableau], cs.max_elem_error, hysteresis=cs.hysteresis,
+                         total_error_fraction=cs.total_error_fraction, max_steps=max_steps, solver="gpu",
+                         max_level=cs.max_ref_level, dt_const=cs.dt_const, callback=callback)
+    ctx = Context(r.mesh, cs.order, cs.scheme)
+    pp = io.PostProcessor(ctx, out_dir, vis_steps=1, n_saved=max(cs.n_saved_frames, 1), lod=lod,
+                          check_totals=cs.check_total_energy)
+    x = torch.from_numpy(np.ascontiguousarray(r.x)).to(cuda)
+    pp.write(r.steps, r.t, r.dt, x)
+    stats = types.SimpleNamespace(steps=r.steps, t=r.t, dt=r.dt, elements=r.elements, mesh=r.mesh)
     return cs, stats, pp, x
 
 

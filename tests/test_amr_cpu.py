@@ -226,3 +226,102 @@ def test_oracle_nc_stencil_includes_the_fine_neighbours():
     x = np.concatenate([rho, z, z, 2.5 + 0.0 * rho])
     _, _, mv = ref.limit(x)
     assert abs(mv[0] - 0.2) < 1e-12
+
+
+def test_indicator_vanishes_on_linear_fields_and_sees_a_kink():
+    base = M.structured_quad(6, 6)
+    t = amr.NCMesh(base)
+    t.refine([14])
+    m = t.mesh()
+    X = RefSolver(m, 2, _outlet(m)).node_coords()
+    x0, y0 = X[..., 0].ravel(), X[..., 1].ravel()
+    one = np.ones_like(x0)
+    lin = np.concatenate([1 + 0.3 * x0 - 0.2 * y0, one, one, one])
+    assert amr.zz_indicator(m, 2, lin).max() < 1e-13
+    kink = np.concatenate([1 + 0.5 * np.abs(x0 - 0.5), one, one, one])
+    eta = amr.zz_indicator(m, 2, kink)
+    # elements touching the line x = 0.5 (through the recovered vertex
+    # gradients) see the kink; the others have a constant gradient and zero
+    xs = m.elem_coords[:, :, 0]
+    near = (xs.max(axis=1) >= 0.5 - 1e-12) & (xs.min(axis=1) <= 0.5 + 1e-12)
+    assert eta[near].min() > 1e-3 and eta[~near].max() < 1e-13
+
+
+def test_derefine_round_trip_is_exact_and_restores_the_mesh():
+    base = M.structured_quad(4, 3)
+    p = 2
+    ref0 = RefSolver(base, p, _outlet(base))
+    X = ref0.node_coords()
+    x0 = X[..., 0].ravel()
+    y0 = X[..., 1].ravel()
+    field = np.concatenate([1 + 0.3 * x0 * x0 - 0.2 * x0 * y0, 0.1 + x0 * y0, -0.2 + y0 * y0, 2.0 + 0.5 * x0])
+    t = amr.NCMesh(base)
+    tr = t.refine([5, 6])
+    m1 = t.mesh()
+    x1 = amr.transfer_state(field, tr, p, 2, 4, base.num_elements)
+    # derefinement: the L2 projection of the prolonged polynomial is the polynomial
+    tr2, done = t.derefine(t.derefinable())
+    assert sorted(done) == [(5, ()), (6, ())]
+    m2 = t.mesh()
+    assert m2.num_elements == base.num_elements and len(m2.nc) == 0
+    assert len(m2.bdr_attr) == len(base.bdr_attr)
+    x2 = amr.transfer_state(x1, tr2, p, 2, 4, m1.num_elements)
+    assert np.abs(x2 - field).max() < 1e-14
+
+
+def test_restriction_keeps_the_means():
+    base = M.structured_quad(3, 3)
+    p = 2
+    t = amr.NCMesh(base)
+    t.refine([4])
+    m1 = t.mesh()
+    rng = np.random.default_rng(3)
+    x1 = 1.0 + rng.uniform(-0.3, 0.3, 4 * m1.num_elements * 9)
+    tr, done = t.derefine(t.derefinable())
+    m2 = t.mesh()
+    x2 = amr.transfer_state(x1, tr, p, 2, 4, m1.num_elements)
+    r1, r2 = RefSolver(m1, p, _outlet(m1)), RefSolver(m2, p, _outlet(m2))
+    mean1 = r1.means(x1).reshape(4, -1)
+    mean2 = r2.means(x2).reshape(4, -1)
+    # the coarsened element (index 4 in the base order) averages its four children (4..7 before)
+    assert np.abs(mean2[:, 4] - mean1[:, 4:8].mean(axis=1)).max() < 1e-14
+
+
+def test_derefine_keeps_the_level_limit():
+    base = M.structured_quad(4, 4)
+    t = amr.NCMesh(base)
+    t.refine([5])
+    child = t.ordered_keys().index((5, (0,)))
+    t.refine([child])
+    # element 1 was refined to keep the limit next to the level-2 children of
+    # element 5's child 0; coarsening it again would break the limit
+    tr, done = t.derefine([(1, ())])
+    assert done == [] and (1, (0,)) in t.leaves
+    t.mesh()
+    # element 5's child 0 itself can be coarsened
+    tr, done = t.derefine([(5, (0,))])
+    assert done == [(5, (0,))]
+    t.mesh()
+
+
+def test_adaptive_loop_with_the_oracle_refines_at_the_jump():
+    from tests.helpers import conservative
+
+    base = M.structured_quad(16, 4, 0.0, 1.0, 0.0, 0.25)
+    sch = S.Scheme(riemann=S.HLLC, indicator=S.IND_BJ, phi=S.PHI_BJ, limiter=S.LIM_MULTI,
+                   bcs=[S.BC(1, S.BC_SLIP), S.BC(3, S.BC_SLIP), S.BC(2, S.BC_OUTLET), S.BC(4, S.BC_OUTLET)])
+
+    def ic(xyz):
+        x = xyz[..., 0].ravel()
+        rho = np.where(x < 0.5, 1.0, 0.125)
+        p = np.where(x < 0.5, 1.0, 0.1)
+        return np.concatenate(conservative(rho, [0 * x, 0 * x], p))
+
+    r = amr.run_adaptive(base, 1, sch, ic, 0.05, 0.2, S.TABLEAUS["ssprk3"], max_elem_error=0.004,
+                         max_steps=6, solver="oracle", max_level=2)
+    lv = r.tree.levels()
+    cen = r.mesh.elem_coords.mean(axis=1)
+    assert lv.max() == 2 and len(r.mesh.nc) > 0
+    # refined elements sit around the jump, none at the ends of the tube
+    assert np.all(lv[(cen[:, 0] < 0.2) | (cen[:, 0] > 0.9)] == 0)
+    assert r.steps == 6 and r.refinements >= 1

@@ -378,11 +378,24 @@ def test_case_file_settings(tmp_path):
     (("order:  3", "order:  4"), "wrong RK scheme order"),
     (("type:   fixedPressureOutlet", "type:   inlet"), "wrong boundary type"),
     (("notInMesh:", "Patch1:"), "appears twice"),
-    (("adaptive: false", "adaptive: true"), "adaptive"),
+    (("adaptive: false", "adaptive: true"), "maxElemError"),
+    (("adaptive: false", "adaptive: true\n    maxElemError: 1e-3\n    nonConformingLimit: 2"), "nonConformingLimit"),
+    (("adaptive: false", "adaptive: true\n    maxElemError: 1e-3\n    preferConformingRefinement: true"),
+     "preferConformingRefinement"),
 ])
 def test_case_file_errors(tmp_path, edit, msg):
     with pytest.raises(CF.CaseError, match=msg):
         CF.load_case(_case_dir(tmp_path, "sod1d_covolume_sod2d.msh", SETTINGS.replace(*edit)))
+
+
+def test_case_file_adaptive_settings(tmp_path):
+    # the keys of the reference's adaptive cases (cases/sod2d/settings.yml:1-14)
+    s = SETTINGS.replace("adaptive: false", "adaptive: 1\n    totalErrorFraction: 0.0\n    maxElemError: 10e-4\n"
+                         "    hysteresis: 0.5\n    preferConformingRefinement: false\n    nonConformingLimit: 1\n"
+                         "    maxRefLevel: 3")
+    cs = CF.load_case(_case_dir(tmp_path, "sod1d_quad.msh", s))
+    assert cs.adaptive and cs.max_elem_error == 1e-3 and cs.hysteresis == 0.5 and cs.total_error_fraction == 0.0
+    assert cs.max_ref_level == 3
 
 
 def test_case_file_constant_time_and_refinement(tmp_path):

@@ -285,3 +285,31 @@ def test_run_case_driver_with_restart(tmp_path, cuda):
     err = rel_linf(a, b, 4)
     print(f"[run_case] restart vs straight rel Linf {err:.2e}, t {s6.t:.9e}", flush=True)
     assert err <= 1e-10
+
+
+def test_run_case_adaptive_matches_oracle_loop(tmp_path, cuda):
+    """mesh.adaptive through python -m rkdg_amd.run_case (the reference's keys,
+    cases/sod2d/settings.yml:1-14, plus a level cap) on the shipped Sod quad
+    mesh refined twice: the refine / repeat / coarsen loop on the GPU against
+    the same loop on the oracle."""
+    from rkdg_amd import amr
+    from rkdg_amd import case_file as CF
+    from rkdg_amd import run_case as RC
+
+    text = COVOLUME.replace("file: sod1d_covolume_sod2d.msh", "file: sod1d_quad.msh")
+    text = text.replace("serialRefLevels: 1", "serialRefLevels: 2\n    adaptive: 1\n    totalErrorFraction: 0.0\n"
+                        "    maxElemError: 0.1\n    hysteresis: 0.5\n    preferConformingRefinement: false\n"
+                        "    nonConformingLimit: 1\n    maxRefLevel: 2")
+    d = _copy_case(tmp_path, "sod1d_quad.msh", text)
+    cs, stats, pp, x = RC.run_case(d, max_steps=8, quiet=True)
+    ro = amr.run_adaptive(cs.mesh, cs.order, cs.scheme, cs.initial_state, cs.t_final, cs.cfl,
+                          S.TABLEAUS[cs.tableau], cs.max_elem_error, hysteresis=cs.hysteresis, max_steps=8,
+                          solver="oracle", max_level=cs.max_ref_level)
+    errs = _errs(x.cpu().numpy(), ro.x, 4)
+    print(f"[case-run adaptive] elements per step {stats.elements} (oracle {ro.elements}), t {stats.t:.9e} / "
+          f"{ro.t:.9e}, final {['%.1e' % e for e in errs]}", flush=True)
+    assert stats.elements == ro.elements and max(stats.elements) > cs.mesh.num_elements
+    assert len(stats.mesh.nc) > 0
+    assert abs(stats.t - ro.t) <= 1e-12 * ro.t
+    assert max(errs) <= 1e-9
+    assert len(pp.series) == 1 and os.path.exists(pp.series[0][1])

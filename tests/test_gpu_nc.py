@@ -217,3 +217,40 @@ def test_nc_descriptor_checks(cuda):
     with pytest.raises(capi.RKDGError) as ei:
         Context(bad, 2, sch)
     assert ei.value.code == capi.RKDG_E_ARG and "conforming neighbour" in str(ei.value)
+
+
+def test_adaptive_run_matches_the_oracle_loop(cuda):
+    """The reference's adaptive loop (main.cpp:253-386, 505-700: step,
+    estimate, refine and repeat the step, coarsen, limit) driven by the GPU
+    path, re-created through rkdg_create_nc after every change of the mesh,
+    against the same loop driven by the oracle: same meshes, times and steps,
+    states to 1e-12."""
+    from tests.helpers import conservative
+
+    base = M.structured_quad(16, 4, 0.0, 1.0, 0.0, 0.25)
+    sch = S.Scheme(riemann=S.HLLC, indicator=S.IND_BJ, phi=S.PHI_BJ, limiter=S.LIM_MULTI,
+                   bcs=[S.BC(1, S.BC_SLIP), S.BC(3, S.BC_SLIP), S.BC(2, S.BC_OUTLET), S.BC(4, S.BC_OUTLET)])
+
+    def ic(xyz):
+        x = xyz[..., 0].ravel()
+        rho = np.where(x < 0.5, 1.0, 0.125)
+        p = np.where(x < 0.5, 1.0, 0.1)
+        return np.concatenate(conservative(rho, [0 * x, 0 * x], p))
+
+    kw = dict(max_elem_error=0.004, max_steps=8, max_level=2, hysteresis=0.5)
+    rg = amr.run_adaptive(base, 1, sch, ic, 0.05, 0.2, S.TABLEAUS["ssprk3"], solver="gpu", **kw)
+    ro = amr.run_adaptive(base, 1, sch, ic, 0.05, 0.2, S.TABLEAUS["ssprk3"], solver="oracle", **kw)
+    print(f"[amr] elements per step {rg.elements} (oracle {ro.elements}), refinement iterations {rg.refinements}, "
+          f"t {rg.t:.15f} / {ro.t:.15f}", flush=True)
+    assert rg.elements == ro.elements and rg.steps == ro.steps
+    assert rg.tree.ordered_keys() == ro.tree.ordered_keys()
+    assert len(rg.mesh.nc) > 0 and rg.refinements > 0
+    assert abs(rg.t - ro.t) <= 1e-12 * ro.t and abs(rg.dt - ro.dt) <= 1e-12 * ro.dt
+    # per variable, relative to the variable's largest magnitude; the
+    # transverse momentum of this one-dimensional flow is rounding noise
+    # (~1e-16), so its error is taken relative to the state's magnitude
+    a, b = rg.x.reshape(4, -1), ro.x.reshape(4, -1)
+    big = np.abs(b).max()
+    errs = [np.abs(a[v] - b[v]).max() / max(np.abs(b[v]).max(), 1e-3 * big) for v in range(4)]
+    print(f"[amr] final state errors per variable {errs}", flush=True)
+    assert max(errs) <= TOL

@@ -531,3 +531,45 @@ def test_ssprk3_shu_osher_step_matches_oracle(kind, cuda):
             assert e_bt <= TOL
         print(msg, flush=True)
         ctx.close()
+
+
+@pytest.mark.parametrize("p", [4, 5])
+def test_high_order_quads(p, cuda):
+    """Quadrilaterals of order 4 and 5 on the tensor path (the reference takes
+    any polyOrder, fe_evolution.cpp:42-69): the right-hand side with every
+    Riemann solver on a periodic mesh and with boundaries, the limiter with its
+    last hope on a state with a jump, and two limited SSP-RK3 steps."""
+    import torch
+
+    m = M.structured_quad(6, 5, periodic=(True, True))
+    for rs in (S.RUSANOV, S.LLF, S.HLL, S.HLLC):
+        y, y_ref, lam, lam_ref, neq = _rhs_pair(m, p, S.Scheme(riemann=rs), cuda, seed=p)
+        err = rel_linf(y, y_ref, neq)
+        print(f"[quad q{p} rs={rs}] rel Linf {err:.2e}", flush=True)
+        assert err <= TOL and abs(lam - lam_ref) <= 1e-13 * lam_ref
+    mb = M.structured_quad(7, 4)
+    sch = S.Scheme(riemann=S.HLLC, indicator=S.IND_BJ, phi=S.PHI_VENKAT, venkat_k=5.0, limiter=S.LIM_MULTI,
+                   bcs=all_outlet(mb))
+    ref = RefSolver(mb, p, sch)
+    x = _shocky_state(ref.node_coords(), seed=p)
+    y_ref, _ = ref.mult(x)
+    ctx = _ctx(mb, p, sch)
+    xd = _dev(x, cuda)
+    yd = torch.empty_like(xd)
+    ctx.rhs(xd, yd)
+    assert rel_linf(_host(yd), y_ref, 4) <= TOL
+    x_ref, ind, mv = ref.limit(x)
+    tr = torch.zeros(mb.num_elements, dtype=torch.int8, device=cuda)
+    ctx.limit(xd, tr)
+    err = rel_linf(_host(xd), x_ref, 4)
+    flags = tr.cpu().numpy()
+    near = np.abs(mv - 1.0) < 1e-12
+    print(f"[quad q{p} limiter] rel Linf {err:.2e} troubled {int((flags & 1).sum())}", flush=True)
+    assert err <= TOL and np.all(((flags & 1) == (mv < 1.0)) | near)
+    xs = x_ref.copy()
+    xg = _dev(x_ref, cuda)
+    dt = 0.05 * ref.hmin() / ref.mult(x_ref)[1] / (2 * p + 1)
+    for _ in range(2):
+        xs, _ = ref.step(xs, dt, S.TABLEAUS["ssprk3"])
+        ctx.step(xg, dt, S.TABLEAUS["ssprk3"])
+    assert rel_linf(_host(xg), xs, 4) <= TOL

@@ -87,7 +87,7 @@ def test_triangle_rhs_einfeldt_periodic_free(cuda):
     assert err <= TOL
 
 
-@pytest.mark.parametrize("p", [1, 2])
+@pytest.mark.parametrize("p", [1, 2, 3])
 @pytest.mark.parametrize("rs", [S.RUSANOV, S.HLL, S.HLLC])
 def test_bilinear_quad_rhs_parity(p, rs, cuda):
     m = _nozzle()
