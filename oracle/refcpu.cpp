@@ -365,6 +365,29 @@ struct Trans {
       }
       return;
     }
+    if (fe->geom == REF_GEOM_HEX) {
+      // the trilinear map's derivatives on edge vectors: dx/dxi_r is the
+      // bilinear blend (in the other two coordinates) of the four edges along
+      // axis r; same map as PointMat * dshape, without its rounding on
+      // elements small against their coordinates
+      for (int r = 0; r < 3; r++) {
+        const int s1 = r == 0 ? 1 : 0, s2 = r == 2 ? 1 : 2;
+        for (int i = 0; i < 3; i++) J[i * 3 + r] = 0.0;
+        for (int b2 = 0; b2 < 2; b2++)
+          for (int b1 = 0; b1 < 2; b1++) {
+            const real w = (b1 ? xi[s1] : 1 - xi[s1]) * (b2 ? xi[s2] : 1 - xi[s2]);
+            int i0[3] = {0, 0, 0}, i1[3];
+            i0[s1] = b1;
+            i0[s2] = b2;
+            for (int q = 0; q < 3; q++) i1[q] = i0[q];
+            i1[r] = 1;
+            const int v0 = (i0[2] ? 4 : 0) + (i0[1] ? (i0[0] ? 2 : 3) : (i0[0] ? 1 : 0));
+            const int v1 = (i1[2] ? 4 : 0) + (i1[1] ? (i1[0] ? 2 : 3) : (i1[0] ? 1 : 0));
+            for (int i = 0; i < 3; i++) J[i * 3 + r] += w * (X[v1 * 3 + i] - X[v0 * 3 + i]);
+          }
+      }
+      return;
+    }
     std::fill(J, J + d * d, 0.0);
     for (int v = 0; v < fe->nv; v++) {
       real dN[3];
@@ -944,12 +967,41 @@ struct Ref {
   // Exactness matters: the reference 3D rotation (rs_basic.cpp:21-28) loses
   // orthogonality when n_z = 1 - ulp, so last-bit noise in an axis-aligned
   // unit normal would perturb rotated fluxes at the 1e-8 level.
+  // Non-affine elements: the Jacobian of the face's own vertex map (MFEM's
+  // FaceElementTransformations: the face's linear element over its vertices in
+  // Elem1's local order, PointMat * dshape), so a planar axis-aligned face has
+  // exact zero tangent components, as in the reference; the element Jacobian at
+  // the embedded point would carry the rounding of the embedding (a coordinate
+  // of 1 - ulp for 1) into them.
   void face_normal(const Face& f, const real* s, real* nor) const {
+    if (!affine[f.e1]) {
+      const real* Xe = X.data() + f.e1 * nv * dim;
+      const real* P[4];
+      for (int m = 0; m < fe.nvf; m++) P[m] = Xe + fe.face_vert(f.lf1, m) * dim;
+      if (dim == 2) {
+        // segment: dshape = (-1, 1)
+        nor[0] = -P[0][1] + P[1][1];
+        nor[1] = -(-P[0][0] + P[1][0]);
+      } else {
+        // bilinear square, vertices (0,0), (1,0), (1,1), (0,1)
+        const real ds[4] = {-(1 - s[1]), 1 - s[1], s[1], -s[1]};
+        const real dt[4] = {-(1 - s[0]), -s[0], s[0], 1 - s[0]};
+        real a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+        for (int m = 0; m < 4; m++)
+          for (int i = 0; i < 3; i++) {
+            a[i] += P[m][i] * ds[m];
+            b[i] += P[m][i] * dt[m];
+          }
+        nor[0] = a[1] * b[2] - a[2] * b[1];
+        nor[1] = a[2] * b[0] - a[0] * b[2];
+        nor[2] = a[0] * b[1] - a[1] * b[0];
+      }
+      return;
+    }
     real xi[3];
     face_to_elem(f, 0, s, xi);
     real J[9];
-    if (affine[f.e1]) affine_jacobian(f.e1, J);
-    else trans(f.e1).jacobian(xi, J);
+    affine_jacobian(f.e1, J);
     const real* R[4];
     for (int m = 0; m < fe.nvf; m++) R[m] = fe.vert(fe.face_vert(f.lf1, m));
     if (dim == 2) {

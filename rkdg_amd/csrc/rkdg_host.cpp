@@ -200,6 +200,10 @@ struct rkdg_ctx {
   // is built, the element faces coded kNcCode0 - k (slave and master sides)
   // with their neighbours, the slave-face records, the elements with such
   // faces (their items: record * 2 + side) and their flux buffer
+  // curved (trilinear) hexahedra: the CurvedLayout records [e][size] and the volumes
+  int curved = 0;
+  double* d_cgeo = nullptr;
+  std::vector<double> vol_h;
   const rkdg_nc_desc* nc_in = nullptr;
   int64_t n_nc = 0, n_ncf = 0, n_nce = 0;
   int* d_ncnb = nullptr;
@@ -261,6 +265,7 @@ void free_ctx(rkdg_ctx* c) {
   dfree(c->d_means);
   dfree(c->d_means2);
   dfree(c->d_fac);
+  dfree(c->d_cgeo);
   dfree(c->d_ncnb);
   dfree(c->d_ncslot);
   dfree(c->d_ncbuf);
@@ -840,6 +845,158 @@ static int scheme_params(const rkdg_scheme_desc* s, SchemeParams& g) {
   return RKDG_OK;
 }
 
+// Jacobian of the trilinear map of hexahedron X (MFEM vertex order) at xi:
+// dx/dxi_r as the bilinear blend, in the other two coordinates, of the four
+// edges along axis r (the same map as PointMat * dshape, without its rounding
+// on elements small against their coordinates)
+void trilinear_jac(const double* X, const double* xi, double* J) {
+  for (int r = 0; r < 3; r++) {
+    const int s1 = r == 0 ? 1 : 0, s2 = r == 2 ? 1 : 2;
+    for (int i = 0; i < 3; i++) J[i * 3 + r] = 0.0;
+    for (int b2 = 0; b2 < 2; b2++)
+      for (int b1 = 0; b1 < 2; b1++) {
+        const double w = (b1 ? xi[s1] : 1.0 - xi[s1]) * (b2 ? xi[s2] : 1.0 - xi[s2]);
+        int i0[3] = {0, 0, 0};
+        i0[s1] = b1;
+        i0[s2] = b2;
+        int i1[3] = {i0[0], i0[1], i0[2]};
+        i1[r] = 1;
+        const int v0 = hex_vertex(i0[0], i0[1], i0[2]), v1 = hex_vertex(i1[0], i1[1], i1[2]);
+        for (int i = 0; i < 3; i++) J[i * 3 + r] += w * (X[v1 * 3 + i] - X[v0 * 3 + i]);
+      }
+  }
+}
+void adj3(const double* J, double* A) {
+  A[0] = J[4] * J[8] - J[5] * J[7];
+  A[1] = J[2] * J[7] - J[1] * J[8];
+  A[2] = J[1] * J[5] - J[2] * J[4];
+  A[3] = J[5] * J[6] - J[3] * J[8];
+  A[4] = J[0] * J[8] - J[2] * J[6];
+  A[5] = J[2] * J[3] - J[0] * J[5];
+  A[6] = J[3] * J[7] - J[4] * J[6];
+  A[7] = J[1] * J[6] - J[0] * J[7];
+  A[8] = J[0] * J[4] - J[1] * J[3];
+}
+
+// CurvedLayout records of curved (trilinear) hexahedra of order p, as the
+// reference builds them on any element: the collocated volume term takes
+// adj(J) at the nodes (DomainIntegrator's rule of order 2p has the p + 1
+// Gauss-Legendre points per axis, domain_integrator.cpp:22-24), the mass
+// matrix MassIntegrator's rule of order 2p + OrderW (OrderW = 2,
+// fe_evolution.cpp:42-69), the means Averager's order p + OrderW
+// (averager.cpp:84-109), the face normals CalcOrtho of the face Jacobian at
+// each point of the face rule (face_integrator.cpp:144-152). Returns the
+// records [e][size]; vol receives the volumes.
+template <int P>
+bool curved_records(const rkdg_ctx* c, std::vector<double>& rec, std::vector<double>& vol) {
+  using CL = CurvedLayout<P>;
+  constexpr int N = CL::N, ND = CL::ND, NQ = CL::NQ, NFP = CL::NFP;
+  const int64_t NE = c->NE;
+  rec.assign((size_t)NE * CL::size, 0.0);
+  vol.assign((size_t)NE, 0.0);
+  double xn[kMaxN], wn[kMaxN];
+  gl_rule(N, xn, wn);
+  auto rule = [](int order, std::vector<double>& x, std::vector<double>& w) {
+    const int n = (order | 1) / 2 + 1;
+    x.resize(n);
+    w.resize(n);
+    gl_rule(n, x.data(), w.data());
+  };
+  std::vector<double> xm, wm, xa, wa;
+  rule(2 * P + 2, xm, wm);
+  rule(P + 2, xa, wa);
+  double sq[kMaxQ], wq[kMaxQ];
+  gl_rule(NQ, sq, wq);
+  auto shape = [&](const double* xi, double* phi) {
+    double L[3][kMaxN];
+    for (int d = 0; d < 3; d++)
+      for (int a = 0; a < N; a++) L[d][a] = lag(xn, N, a, xi[d]);
+    for (int k = 0; k < ND; k++) phi[k] = L[0][k % N] * L[1][(k / N) % N] * L[2][k / (N * N)];
+  };
+  std::vector<double> M((size_t)ND * ND), Mi((size_t)ND * ND), phi(ND);
+  for (int64_t e = 0; e < NE; e++) {
+    const double* X = &c->coords[(size_t)e * 24];
+    double* R = &rec[(size_t)e * CL::size];
+    double J[9], A[9];
+    for (int k = 0; k < ND; k++) {
+      const double xi[3] = {xn[k % N], xn[(k / N) % N], xn[k / (N * N)]};
+      trilinear_jac(X, xi, J);
+      adj3(J, A);
+      for (int q = 0; q < 9; q++) R[CL::adj + k * 9 + q] = A[q];
+    }
+    std::fill(M.begin(), M.end(), 0.0);
+    double v = 0.0;
+    const int nm = (int)xm.size();
+    for (int q3 = 0; q3 < nm; q3++)
+      for (int q2 = 0; q2 < nm; q2++)
+        for (int q1 = 0; q1 < nm; q1++) {
+          const double xi[3] = {xm[q1], xm[q2], xm[q3]};
+          trilinear_jac(X, xi, J);
+          const double dj = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+                            J[2] * (J[3] * J[7] - J[4] * J[6]);
+          if (!(dj > 0.0)) return false;
+          const double w = dj * wm[q1] * wm[q2] * wm[q3];
+          v += w;
+          shape(xi, phi.data());
+          for (int i = 0; i < ND; i++)
+            for (int j = 0; j < ND; j++) M[(size_t)i * ND + j] += w * phi[i] * phi[j];
+        }
+    if (!invert_small(ND, M.data(), Mi.data())) return false;
+    // transposed: the kernel's lane j reads entry (j, k) at k * ND + j
+    for (int i = 0; i < ND; i++)
+      for (int k = 0; k < ND; k++) R[CL::minv + k * ND + i] = Mi[(size_t)i * ND + k];
+    vol[e] = v;
+    R[CL::vol] = v;
+    const int na = (int)xa.size();
+    for (int q3 = 0; q3 < na; q3++)
+      for (int q2 = 0; q2 < na; q2++)
+        for (int q1 = 0; q1 < na; q1++) {
+          const double xi[3] = {xa[q1], xa[q2], xa[q3]};
+          trilinear_jac(X, xi, J);
+          const double dj = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+                            J[2] * (J[3] * J[7] - J[4] * J[6]);
+          const double w = dj * wa[q1] * wa[q2] * wa[q3];
+          shape(xi, phi.data());
+          for (int k = 0; k < ND; k++) R[CL::mw + k] += w * phi[k];
+        }
+    for (int lf = 0; lf < 6; lf++) {
+      int r, cc;
+      rc_of_lf(3, lf, r, cc);
+      const int t1 = r == 0 ? 1 : 0, t2 = r == 2 ? 1 : 2;
+      for (int qj = 0; qj < NQ; qj++)
+        for (int qi = 0; qi < NQ; qi++) {
+          double xi[3];
+          xi[r] = cc;
+          xi[t1] = sq[qi];
+          xi[t2] = sq[qj];
+          trilinear_jac(X, xi, J);
+          adj3(J, A);
+          double an[3], n2 = 0.0;
+          for (int d = 0; d < 3; d++) {
+            an[d] = (cc ? 1.0 : -1.0) * A[r * 3 + d];
+            n2 += an[d] * an[d];
+          }
+          const double nm2 = std::sqrt(n2);
+          double* F = R + CL::fn + ((size_t)lf * NFP + qi + NQ * qj) * 5;
+          // the reference's unit normal: nor * (1 / Norml2(nor))
+          // (face_integrator.cpp:165), as the affine kernels form it; an
+          // axis-aligned normal may come out as 1 - ulp, which the 3D rotation
+          // of the Riemann solvers (rs_basic.cpp:21-28) turns into a 1e-8-level
+          // term: taking the same steps keeps that term the reference's
+          const double inv = 1.0 / norml2_scaled<3>(an);
+          double u2 = 0.0;
+          for (int d = 0; d < 3; d++) {
+            F[d] = an[d] * inv;
+            u2 += F[d] * F[d];
+          }
+          F[3] = nm2;
+          F[4] = 0.5 / std::sqrt(u2);
+        }
+    }
+  }
+  return true;
+}
+
 // MFEM local face vertices of hexahedra (quadrilateral edges: kQuadEdge)
 const int kHexFace[6][4] = {{3, 2, 1, 0}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}, {4, 5, 6, 7}};
 int mfem_face_vertex(int dim, int lf, int q) { return dim == 2 ? kQuadEdge[lf][q] : kHexFace[lf][q]; }
@@ -1141,7 +1298,37 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
     c->h_elem[e] = min_singular(dim, J);
     hmin = std::min(hmin, c->h_elem[e]);
   }
-  if (!c->affine) return fail(RKDG_E_UNSUPPORTED, "non-affine tensor elements are not built into this library");
+  if (!c->affine) {
+    // curved (trilinear) hexahedra: the per-element records of the CV kernels;
+    // geo keeps adj(J) at the centre and the volume in the det(J) slot (the
+    // limiter's Venkatakrishnan eps^2 = (K h)^3 with h^3 the volume), h the
+    // smallest singular value of J at the centre (GetElementSize)
+    if (dim != 3) return fail(RKDG_E_UNSUPPORTED, "non-affine quadrilaterals take the dense path");
+    if (c->nc_in && c->nc_in->count > 0)
+      return fail(RKDG_E_UNSUPPORTED, "non-conforming faces are built for affine elements");
+    if (g.indicator == RKDG_IND_SHU || g.linearize)
+      return fail(RKDG_E_UNSUPPORTED, "the Shu indicator and linearize are built for affine hexahedra");
+    std::vector<double> rec;
+    bool ok = false;
+    if (c->p == 1) ok = curved_records<1>(c, rec, c->vol_h);
+    else if (c->p == 2) ok = curved_records<2>(c, rec, c->vol_h);
+    else if (c->p == 3) ok = curved_records<3>(c, rec, c->vol_h);
+    if (!ok) return fail(RKDG_E_ARG, "a curved hexahedron with a non-positive Jacobian");
+    hmin = 1e300;
+    for (int64_t e = 0; e < NE; e++) {
+      const double xi[3] = {0.5, 0.5, 0.5};
+      double J[9], A[9];
+      trilinear_jac(&c->coords[(size_t)e * 24], xi, J);
+      adj3(J, A);
+      for (int i = 0; i < 9; i++) c->geo_h[(size_t)i * NE + e] = A[i];
+      c->geo_h[(size_t)9 * NE + e] = c->vol_h[e];
+      c->h_elem[e] = min_singular(3, J);
+      hmin = std::min(hmin, c->h_elem[e]);
+    }
+    HIP_TRY(hipMalloc(&c->d_cgeo, sizeof(double) * rec.size()));
+    HIP_TRY(hipMemcpy(c->d_cgeo, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice));
+    c->curved = 1;
+  }
   c->hmin = hmin;
 
   // faces: sort element faces by their sorted topological vertex sets
@@ -1235,7 +1422,7 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
   (void)NE_;
   // aligned (structured) meshes: every interior face is the neighbour's
   // opposite face of the same axis with the identity orientation
-  c->aligned = 1;
+  c->aligned = c->curved ? 0 : 1;  // the curved kernels take any orientation
   for (int64_t e = 0; e < NE && c->aligned; e++)
     for (int lf = 0; lf < nf; lf++) {
       if (nbr[(size_t)lf * NE + e] < 0) continue;
@@ -1304,7 +1491,7 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
   }
   HIP_TRY(hipMalloc(&c->d_tb, sizeof(TensorTables)));
   HIP_TRY(hipMemcpy(c->d_tb, &c->tb, sizeof(TensorTables), hipMemcpyHostToDevice));
-  if (c->n_bdr == 0 && c->n_ncf == 0 && c->two_colour) {
+  if (c->n_bdr == 0 && c->n_ncf == 0 && !c->curved && c->two_colour) {
     int rc = build_two_colour(c, nbr);
     if (rc) return rc;
   }
@@ -1416,6 +1603,7 @@ RhsArgs base_args(rkdg_ctx* c, const double* x) {
   a.grid_cap = c->grid_cap;
   a.order = c->dense ? nullptr : c->d_order;
   a.aligned = c->dense ? 0 : c->aligned;
+  a.cgeo = c->d_cgeo;
   return a;
 }
 
@@ -1464,7 +1652,7 @@ int read_lambda(rkdg_ctx* c, int slot, double* lam) {
 
 int limit_impl(rkdg_ctx* c, double* x, int8_t* troubled, bool have_means, const int* active = nullptr) {
   if (!have_means) {
-    MeansArgs ma{x, c->d_means, c->d_geo, c->NE};
+    MeansArgs ma{x, c->d_means, c->d_geo, c->NE, c->d_cgeo};
     HIP_TRY(c->dense ? launch_dense_means(c->geom, c->p, ma, c->d_dtab, c->stream)
                      : launch_tensor_means(c->dim, c->p, ma, c->d_tb, c->stream));
   }
@@ -2146,7 +2334,7 @@ int rkdg_limit(rkdg_ctx* c, double* x, int8_t* troubled) {
 
 int rkdg_means(rkdg_ctx* c, const double* x, double* means) {
   if (!c || !x || !means) return fail(RKDG_E_ARG, "null argument");
-  MeansArgs ma{x, means, c->d_geo, c->NE};
+  MeansArgs ma{x, means, c->d_geo, c->NE, c->d_cgeo};
   HIP_TRY(c->dense ? launch_dense_means(c->geom, c->p, ma, c->d_dtab, c->stream)
                    : launch_tensor_means(c->dim, c->p, ma, c->d_tb, c->stream));
   return RKDG_OK;
@@ -2239,6 +2427,7 @@ namespace {
 constexpr int kTotalBlocks = 512;
 
 double element_volume(const rkdg_ctx* c, int64_t e) {
+  if (c->curved) return c->vol_h[(size_t)e];
   const double* X = &c->coords[(size_t)e * c->nv * c->dim];
   if (c->geom == RKDG_GEOM_TRI)
     return 0.5 * std::fabs((X[2] - X[0]) * (X[5] - X[1]) - (X[3] - X[1]) * (X[4] - X[0]));
@@ -2274,7 +2463,7 @@ int rkdg_totals(rkdg_ctx* c, const double* x, double* totals) {
     HIP_TRY(hipMalloc(&c->d_dmeans, sizeof(double) * c->neq * NE));
     HIP_TRY(hipMalloc(&c->d_partial, sizeof(double) * c->neq * kTotalBlocks));
   }
-  MeansArgs ma{x, c->d_dmeans, c->d_geo, NE};
+  MeansArgs ma{x, c->d_dmeans, c->d_geo, NE, c->d_cgeo};
   HIP_TRY(c->dense ? launch_dense_means(c->geom, c->p, ma, c->d_dtab, c->stream)
                    : launch_tensor_means(c->dim, c->p, ma, c->d_tb, c->stream));
   HIP_TRY(launch_totals(c->d_dmeans, c->d_vol, NE, c->neq, c->d_partial, kTotalBlocks, c->stream));

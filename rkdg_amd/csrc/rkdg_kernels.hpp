@@ -89,6 +89,9 @@ struct RhsArgs {
   // ncbuf[(ncslot[e] * NEQ + v) * ND + node] where ncslot[e] >= 0
   const int* ncslot;
   const double* ncbuf;
+  // curved (non-affine) hexahedra (tensor kernel, CV instantiation): the
+  // per-element record of CurvedLayout, AoS [e][CurvedLayout::size]
+  const double* cgeo;
 };
 
 struct LimitArgs {
@@ -116,6 +119,25 @@ struct MeansArgs {
   double* means;
   const double* geo;
   int64_t NE;
+  const double* cgeo;  // curved hexahedra (nullable): the mean weights of CurvedLayout
+};
+
+// Geometry record of a curved (non-affine, trilinear) hexahedron of order P,
+// AoS per element: adj(J) at every node [ND][3][3] (row-major) for the
+// collocated volume term; the inverse of the exact mass matrix [ND][ND],
+// transposed (MassIntegrator's rule, order 2p + OrderW); the mean weights [ND] (the
+// integrals of the basis, Averager's rule); per local face (MFEM numbering)
+// and face point (qi + NQ qj in the face frame of the kernels) the element's
+// outward unit normal, |n| and 0.5 / |unit normal| [NF][NFP][5]; the volume.
+template <int P>
+struct CurvedLayout {
+  static constexpr int N = P + 1, ND = N * N * N, NQ = P + 2, NFP = NQ * NQ, NF = 6;
+  static constexpr int adj = 0;
+  static constexpr int minv = adj + ND * 9;
+  static constexpr int mw = minv + ND * ND;
+  static constexpr int fn = mw + ND;
+  static constexpr int vol = fn + NF * NFP * 5;
+  static constexpr int size = vol + 1;
 };
 
 // LDS staging of a kernel's setup data in two steps, so that a workgroup issues

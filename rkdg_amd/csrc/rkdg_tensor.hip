@@ -142,7 +142,10 @@ struct TArg {
 // AL: every interior face joins local face (r, c) to the neighbour's (r, 1-c)
 // with the identity orientation (structured meshes), so the neighbour's face,
 // node strides and point map are known at compile time.
-template <int DIM, int P, int RS, bool BC, int ROLE, bool AL>
+// CV: curved (trilinear) hexahedra, role 0 only: adj(J) per node in the volume
+// term, the normal and |n| per face point, the dense inverse mass and the mass
+// weights of the means, all from the element's CurvedLayout record.
+template <int DIM, int P, int RS, bool BC, int ROLE, bool AL, bool CV = false>
 __global__ __launch_bounds__(64, ROLE == 1 ? 4 : 1) void tensor_rhs_kernel(const RhsArgs A, const SchemeParams g,
                                                         const TensorTables* __restrict__ gtb,
                                                         const TArg<DIM, P + 1, TC<DIM, P>::NQ> ka) {
@@ -164,6 +167,8 @@ __global__ __launch_bounds__(64, ROLE == 1 ? 4 : 1) void tensor_rhs_kernel(const
   constexpr bool LIFT_IN_WORK = 2 * NEQ * N <= T && 2 * NFP <= T;
   constexpr int LIFT = 2 * NEQ * NFN;
 
+  static_assert(!CV || (DIM == 3 && ROLE == 0 && !AL), "curved elements: 3D, one-kernel form, any orientation");
+  using CL = CurvedLayout<P>;
   constexpr bool SOLVE = ROLE != 2;    // this kernel solves face fluxes
   constexpr int ES = SOLVE ? EPB : 1;  // extent of the arrays only the face solves use
   constexpr int FB = NF * NEQ * NFN;   // lifted face fluxes of one receiver element
@@ -315,6 +320,12 @@ __global__ __launch_bounds__(64, ROLE == 1 ? 4 : 1) void tensor_rhs_kernel(const
       for (int v = 0; v < NEQ; v++) st[v] = s_u[team][v][j];
       const SidePrim<DIM> q = side_prim<DIM>(st, g);
       if (!A.no_lambda) lmax = sqrt_ws(q.m2) * q.inv + side_sound<DIM>(st, q, g);
+      double aj[CV ? DIM * DIM : 1];  // curved: adj(J) at this node
+      if constexpr (CV) {
+        const double* cg = A.cgeo + e * CL::size + CL::adj + j * DIM * DIM;
+#pragma unroll
+        for (int k = 0; k < DIM * DIM; k++) aj[k] = cg[k];
+      }
       double wj = tb.w[jj[0]] * tb.w[jj[1]];
       if (DIM == 3) wj *= tb.w[jj[2]];
       // contravariant fluxes G^r = w_j adj(J)(r,:) . F(u): with m.a_r = sum_d a_rd m_d,
@@ -323,7 +334,7 @@ __global__ __launch_bounds__(64, ROLE == 1 ? 4 : 1) void tensor_rhs_kernel(const
       for (int r = 0; r < DIM; r++) {
         double ar[DIM];
 #pragma unroll
-        for (int d = 0; d < DIM; d++) ar[d] = wj * s_geo[team][r * DIM + d];
+        for (int d = 0; d < DIM; d++) ar[d] = wj * (CV ? aj[r * DIM + d] : s_geo[team][r * DIM + d]);
         double ma = 0.0;
 #pragma unroll
         for (int d = 0; d < DIM; d++) ma += st[1 + d] * ar[d];
@@ -471,10 +482,26 @@ __global__ __launch_bounds__(64, ROLE == 1 ? 4 : 1) void tensor_rhs_kernel(const
           um[v] = t;
         }
         double nrm[DIM];
+        double snm, inm;  // -|n| on Elem1's side, +|n| on Elem2's; 0.5 / |unit normal|
+        if constexpr (CV) {
+          // Elem1's normal at this point: the own record, or the neighbour's
+          // at the point in its face frame
+          const int code = nb >= 0 ? s_code[team][lf] : 0;
+          const bool nb_first = nb >= 0 && !((code >> 6) & 1);
+          const int64_t eo = nb_first ? (int64_t)nb : e;
+          const int lfo = nb_first ? (code & 7) : lf;
+          const int qo = nb_first ? map_point<DIM, NQ>(code, qi, qj) : q;
+          const double* fn = A.cgeo + eo * CL::size + CL::fn + (lfo * NFP + qo) * 5;
   #pragma unroll
-        for (int d = 0; d < DIM; d++) nrm[d] = s_nrm[team][f][d];
-        const double snm = s_nrm[team][f][DIM];  // -|n| on Elem1's side, +|n| on Elem2's
-        const double inm = s_nrm[team][f][DIM + 1];
+          for (int d = 0; d < DIM; d++) nrm[d] = fn[d];
+          snm = nb_first ? fn[DIM] : -fn[DIM];
+          inm = fn[DIM + 1];
+        } else {
+  #pragma unroll
+          for (int d = 0; d < DIM; d++) nrm[d] = s_nrm[team][f][d];
+          snm = s_nrm[team][f][DIM];
+          inm = s_nrm[team][f][DIM + 1];
+        }
         double fl[NEQ];
         double factor = -1.0;
         bool ok = true;
@@ -619,11 +646,36 @@ __global__ __launch_bounds__(64, ROLE == 1 ? 4 : 1) void tensor_rhs_kernel(const
         for (int v = 0; v < NEQ; v++) z[v] += A.ncbuf[((int64_t)ns * NEQ + v) * ND + j];
       }
     }
+    if constexpr (CV) {
+      // curved element: z <- M^-1 z with the element's dense inverse mass
+      __syncthreads();  // the face phases' reads of work are done
+      if (node_lane) {
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) work[v * ND + j] = z[v];
+      }
+      __syncthreads();
+      if (node_lane) {
+        // column j of the record's M^-1 (stored transposed): the lanes' loads
+        // of one k are consecutive; a partial unroll keeps the row out of VGPRs
+        const double* mi = A.cgeo + e * CL::size + CL::minv + j;
+        double y[NEQ];
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) y[v] = 0.0;
+#pragma unroll 4
+        for (int k = 0; k < ND; k++) {
+          const double m = mi[k * ND];
+#pragma unroll
+          for (int v = 0; v < NEQ; v++) y[v] += m * work[v * ND + k];
+        }
+#pragma unroll
+        for (int v = 0; v < NEQ; v++) z[v] = y[v];
+      }
+    }
     if (node_lane) {
       detJ = s_geo[team][GD - 1];
       double iw = tb.invw[jj[0]] * tb.invw[jj[1]];
       if (DIM == 3) iw *= tb.invw[jj[2]];
-      const double im = iw * recip(detJ);
+      const double im = CV ? 1.0 : iw * recip(detJ);
       // every base is loaded before the first store (an output may be a base)
       // addresses: the workgroup's block of each variable (uniform, scalar
       // registers) plus a small 32-bit lane offset; with an element list the
@@ -682,9 +734,18 @@ __global__ __launch_bounds__(64, ROLE == 1 ? 4 : 1) void tensor_rhs_kernel(const
       __syncthreads();
       if (active && lane < NEQ) {
         double s = 0.0;
+        if constexpr (CV) {
+          // curved element: the integrals of the basis as weights (Averager)
+          const double* mw = A.cgeo + e * CL::size + CL::mw;
+          double ws = 0.0;
+          for (int k = 0; k < ND; k++) ws += mw[k];
+          for (int k = 0; k < ND; k++) s += mw[k] * work[lane * ND + k];
+          A.means[lane * NE + e] = s / ws;
+        } else {
 #pragma unroll
-        for (int k = 0; k < ND; k++) s += ka.wk[k] * work[lane * ND + k];
-        A.means[lane * NE + e] = s / ka.wsum;
+          for (int k = 0; k < ND; k++) s += ka.wk[k] * work[lane * ND + k];
+          A.means[lane * NE + e] = s / ka.wsum;
+        }
       }
     }
   }
@@ -718,8 +779,13 @@ __global__ __launch_bounds__(64) void tensor_means_kernel(const MeansArgs A,
 #pragma unroll
   for (int v = 0; v < NEQ; v++) s[v] = on ? A.x[v * NDOFS + e * ND + lane] : 0.0;
   if (on) {
-    wk = gtb->w[lane % N] * gtb->w[(lane / N) % N];
-    if (DIM == 3) wk *= gtb->w[lane / (N * N)];
+    if (DIM == 3 && A.cgeo) {
+      // curved hexahedra: the integrals of the basis (Averager)
+      wk = A.cgeo[e * CurvedLayout<P>::size + CurvedLayout<P>::mw + lane];
+    } else {
+      wk = gtb->w[lane % N] * gtb->w[(lane / N) % N];
+      if (DIM == 3) wk *= gtb->w[lane / (N * N)];
+    }
   }
 #pragma unroll
   for (int v = 0; v < NEQ; v++) s[v] *= wk;
@@ -1100,6 +1166,13 @@ hipError_t launch_rhs_rs(const RhsArgs& a, const SchemeParams& g, const TensorTa
     if (a.role == 1) tensor_rhs_kernel<D, PP, RS, false, 1, false><<<(unsigned)nl, C::BLOCK, 0, s>>>(a, g, t, ka);
     else tensor_rhs_kernel<D, PP, RS, false, 2, false><<<(unsigned)nl, C::BLOCK, 0, s>>>(a, g, t, ka);
     return hipGetLastError();
+  }
+  if constexpr (D == 3) {
+    if (a.cgeo) {
+      if (bc) tensor_rhs_kernel<D, PP, RS, true, 0, false, true><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t, ka);
+      else tensor_rhs_kernel<D, PP, RS, false, 0, false, true><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t, ka);
+      return hipGetLastError();
+    }
   }
   if (a.aligned) {
     if (bc) tensor_rhs_kernel<D, PP, RS, true, 0, true><<<(unsigned)nb, C::BLOCK, 0, s>>>(a, g, t, ka);

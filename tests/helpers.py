@@ -87,3 +87,24 @@ def scramble_orientations(mesh, seed=1):
 
 def all_outlet(mesh):
     return [S.BC(int(a), S.BC_OUTLET) for a in sorted(set(mesh.bdr_attr.tolist()))]
+
+
+def curved_hex(nx, ny, nz, amp=0.15, seed=0, periodic=(True, True, True)):
+    """structured_hex with each topological vertex moved by U(-amp, amp) times
+    the spacing per coordinate; vertices on a periodic seam move together."""
+    m = M.structured_hex(nx, ny, nz, periodic=periodic)
+    rng = np.random.default_rng(seed)
+    nv = int(m.elem_verts.max()) + 1
+    h = np.array([1.0 / nx, 1.0 / ny, 1.0 / nz])
+    d = rng.uniform(-amp, amp, size=(nv, 3)) * h
+    if not all(periodic):
+        # boundary vertices stay on their planes (the boundary attributes keep their meaning)
+        X = m.elem_coords.reshape(-1, 3)
+        V = m.elem_verts.reshape(-1)
+        for ax in range(3):
+            if not periodic[ax]:
+                on = (np.abs(X[:, ax]) < 1e-12) | (np.abs(X[:, ax] - 1.0) < 1e-12)
+                d[V[on], ax] = 0.0
+    m.elem_coords = np.ascontiguousarray(m.elem_coords + d[m.elem_verts])
+    m.name += "_curved"
+    return m

@@ -11,7 +11,7 @@ from oracle import refcpu as R
 from rkdg_amd import mesh as M
 from rkdg_amd import scheme as S
 from tests.exact_riemann import sample, star_state
-from tests.helpers import all_outlet, conservative, rel_linf, scramble_orientations, smooth_state
+from tests.helpers import all_outlet, conservative, curved_hex, rel_linf, scramble_orientations, smooth_state
 
 try:
     import mpmath
@@ -299,6 +299,83 @@ def test_mass_inverse_affine_is_diagonal_weights():
     detJ = 1.0 / m.num_elements
     Mi = ref.mass_inverse(0)
     np.testing.assert_allclose(Mi, np.diag(1.0 / (W * detJ)), rtol=1e-13, atol=1e-10)
+
+
+# ---------------------------------------------------------------- curved hexahedra
+# Trilinear elements with randomly moved corners: the metric terms vary inside
+# each element. The scheme's own rules integrate the metric identities exactly
+# there, so the pins below hold to round-off.
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_curved_hex_free_stream(p):
+    for m in (curved_hex(4, 3, 3, seed=p), scramble_orientations(curved_hex(3, 4, 3, seed=10 + p), seed=p)):
+        for rs in (S.RUSANOV, S.HLLC):
+            ref = R.RefSolver(m, p, S.Scheme(riemann=rs))
+            y, _ = ref.mult(_const(m, ref.nd, [1.2, 0.3, -0.2, 0.1, 2.5]))
+            assert np.abs(y).max() < 2e-11
+
+
+def _elem_mass(ref, e):
+    return np.linalg.inv(ref.mass_inverse(e))
+
+
+def test_curved_hex_mass_and_conservation():
+    """The element masses tile the periodic box (sum of 1^T M 1 is its volume),
+    each M is symmetric positive definite, and the integral of the RHS of a
+    smooth state vanishes."""
+    m = curved_hex(4, 4, 3, seed=4)
+    ref = R.RefSolver(m, 2, S.Scheme(riemann=S.HLLC))
+    y, _ = ref.mult(smooth_state(ref.node_coords(), seed=3, amp=1e-2))
+    Y = y.reshape(5, m.num_elements, ref.nd)
+    vol, tot = 0.0, np.zeros(5)
+    for e in range(m.num_elements):
+        Me = _elem_mass(ref, e)
+        np.testing.assert_allclose(Me, Me.T, rtol=0, atol=1e-15)
+        assert np.linalg.eigvalsh(Me).min() > 0
+        one = Me.sum(axis=0)
+        vol += one.sum()
+        tot += Y[:, e, :] @ one
+    assert abs(vol - 1.0) < 1e-13
+    assert np.all(np.abs(tot) < 1e-13 * np.abs(Y).reshape(5, -1).max(axis=1))
+    # means weight the nodes by the integrals of the basis
+    mu = ref.means(y).reshape(5, -1)
+    e = 7
+    one = _elem_mass(ref, e).sum(axis=0)
+    np.testing.assert_allclose(mu[:, e], Y[:, e, :] @ one / one.sum(), rtol=1e-12, atol=1e-14)
+
+
+def test_curved_hex_rhs_converges_to_minus_divergence():
+    """On curved meshes of spacing 1/n the Q2 operator approaches -div F(u) at
+    the nodes; a transposed or misplaced metric term leaves an O(1) error."""
+    g = 1.4
+
+    def flux(x, y, z):
+        rho = 1 + 0.2 * np.sin(2 * np.pi * x) + 0.1 * np.cos(2 * np.pi * z)
+        u, v = 0.5 + 0.1 * np.cos(2 * np.pi * y), 0.2 + 0.05 * np.sin(2 * np.pi * z)
+        w = -0.1 + 0.05 * np.cos(2 * np.pi * x)
+        p = 1 + 0.1 * np.sin(2 * np.pi * (x + y))
+        E = p / (g - 1) + 0.5 * rho * (u * u + v * v + w * w)
+        U = np.stack(conservative(rho, [u, v, w], p))
+        F = [np.stack([rho * a, rho * a * u + p * (k == 0), rho * a * v + p * (k == 1), rho * a * w + p * (k == 2),
+                       (E + p) * a]) for k, a in enumerate((u, v, w))]
+        return U, F
+
+    errs = []
+    ns = (4, 8, 16)
+    for n in ns:
+        m = curved_hex(n, n, n, seed=3)
+        ref = R.RefSolver(m, 2, S.Scheme(riemann=S.RUSANOV))
+        X = ref.node_coords().reshape(-1, 3).T
+        h = 1e-5
+        div = 0.0
+        for k in range(3):
+            dx = np.zeros((3, 1))
+            dx[k] = h
+            div = div + (flux(*(X + dx))[1][k] - flux(*(X - dx))[1][k]) / (2 * h)
+        y, _ = ref.mult(np.ascontiguousarray(flux(*X)[0].reshape(-1)))
+        errs.append(np.abs(y.reshape(5, -1) + div).max())
+    rates = [math.log(errs[i] / errs[i + 1]) / math.log(ns[i + 1] / ns[i]) for i in range(2)]
+    assert min(rates) > 1.4, (errs, rates)
 
 
 # ---------------------------------------------------------------- limiter
