@@ -2,7 +2,8 @@
 (src/case_manager/case_manager.cpp:217-297) with readPhysicalNames (:299-352)
 and minimizeAttributes (:354-448).
 
-Reads the ASCII MSH 2.2 and MSH 4.1 formats into the C-ABI's mesh description
+Reads MSH 2.2 and MSH 4.1, ASCII or binary (MFEM's reader, which the
+reference calls, takes both encodings), into the C-ABI's mesh description
 (``mesh.Mesh``): elements of the highest dimension present become cells,
 elements one dimension lower become boundary elements, points and anything
 else are ignored. A cell's attribute is its physical tag (MSH 2.2: the first
@@ -112,21 +113,25 @@ def _read_v2(secs):
         tags = v[3:3 + nt]
         phys = tags[0] if nt > 0 and tags[0] > 0 else 1
         elems.append((et, phys, v[3 + nt:]))
-    links = []
-    if "Periodic" in secs:
-        pl = secs["Periodic"]
-        k = 1
-        for _ in range(int(pl[0])):
-            k += 1  # dim slave master
-            if pl[k].strip().startswith("Affine"):
-                k += 1
-            cnt = int(pl[k])
-            k += 1
-            for ln in pl[k:k + cnt]:
-                a, b = ln.split()[:2]
-                links.append((int(a), int(b)))
-            k += cnt
+    links = _periodic_v2(secs["Periodic"]) if "Periodic" in secs else []
     return ids, xyz, elems, links
+
+
+def _periodic_v2(pl):
+    """$Periodic of MSH 2.2 (ASCII, also in binary files): (slave, master) node pairs."""
+    links = []
+    k = 1
+    for _ in range(int(pl[0])):
+        k += 1  # dim slave master
+        if pl[k].strip().startswith("Affine"):
+            k += 1
+        cnt = int(pl[k])
+        k += 1
+        for ln in pl[k:k + cnt]:
+            a, b = ln.split()[:2]
+            links.append((int(a), int(b)))
+        k += cnt
+    return links
 
 
 def _read_v41(secs):
@@ -185,6 +190,176 @@ def _read_v41(secs):
     return np.asarray(ids, np.int64), np.asarray(xyz, np.float64).reshape(-1, 3), elems, links
 
 
+class _Cursor:
+    """Position in the bytes of a binary MSH file."""
+
+    def __init__(self, data: bytes, pos: int = 0):
+        self.data = data
+        self.pos = pos
+
+    def line(self) -> str:
+        end = self.data.index(b"\n", self.pos)
+        out = self.data[self.pos:end].decode("ascii", "replace").strip()
+        self.pos = end + 1
+        return out
+
+    def take(self, dtype, count):
+        dt = np.dtype(dtype)
+        n = dt.itemsize * count
+        if self.pos + n > len(self.data):
+            raise GmshError("binary section ends early")
+        out = np.frombuffer(self.data, dtype=dt, count=count, offset=self.pos)
+        self.pos += n
+        return out
+
+    def skip_to_end(self, name: str):
+        """Moves past the line $End<name> that follows the section's data."""
+        tag = ("$End" + name).encode()
+        at = self.data.find(tag, self.pos)
+        if at < 0:
+            raise GmshError(f"section ${name} has no $End{name}")
+        self.pos = at + len(tag)
+        nl = self.data.find(b"\n", self.pos)
+        self.pos = len(self.data) if nl < 0 else nl + 1
+
+
+def _read_binary(data: bytes, path):
+    """Binary MSH 2.2 or 4.1 (the Gmsh reference manual's layouts; numbers in
+    the file's byte order, which the integer 1 after the format line gives).
+    Returns (version, ids, xyz, elems, links, names)."""
+    cur = _Cursor(data)
+    version = None
+    bo = "<"
+    ids = xyz = None
+    elems, links, names = [], [], []
+    ent_phys = {}
+
+    def i4(n):
+        return cur.take(bo + "i4", n)
+
+    def sz(n):
+        return cur.take(bo + "u8", n).astype(np.int64)
+
+    while True:
+        try:
+            ln = cur.line()
+        except ValueError:
+            break
+        if not ln.startswith("$") or ln.startswith("$End"):
+            continue
+        name = ln[1:]
+        if name == "MeshFormat":
+            fmt = cur.line().split()
+            version, dsize = fmt[0], int(fmt[2])
+            if dsize != 8:
+                raise GmshError(f"{path}: binary MSH with data size {dsize} (8 is read)")
+            one = cur.data[cur.pos:cur.pos + 4]
+            if np.frombuffer(one, "<i4")[0] == 1:
+                bo = "<"
+            elif np.frombuffer(one, ">i4")[0] == 1:
+                bo = ">"
+            else:
+                raise GmshError(f"{path}: binary MSH without the integer 1 after the format line")
+            cur.pos += 4
+            cur.skip_to_end(name)
+        elif name == "PhysicalNames":
+            n = int(cur.line().split()[0])
+            names = _physical_names([str(n)] + [cur.line() for _ in range(n)])
+            cur.skip_to_end(name)
+        elif version is None:
+            raise GmshError(f"{path}: ${name} before $MeshFormat")
+        elif version.startswith("2"):
+            if name == "Nodes":
+                n = int(cur.line().split()[0])
+                rec = cur.take(np.dtype([("id", bo + "i4"), ("x", bo + "f8", (3,))]), n)
+                ids = rec["id"].astype(np.int64)
+                xyz = np.array(rec["x"], np.float64)
+                cur.skip_to_end(name)
+            elif name == "Elements":
+                ne = int(cur.line().split()[0])
+                got = 0
+                while got < ne:
+                    et, nfollow, ntags = (int(t) for t in i4(3))
+                    if et not in _ELEM:
+                        raise GmshError(f"{path}: unknown element type {et}")
+                    nn = _ELEM[et][1]
+                    blk = i4(nfollow * (1 + ntags + nn)).reshape(nfollow, 1 + ntags + nn)
+                    for row in blk:
+                        phys = int(row[1]) if ntags > 0 and row[1] > 0 else 1
+                        elems.append((et, phys, [int(t) for t in row[1 + ntags:]]))
+                    got += nfollow
+                cur.skip_to_end(name)
+            elif name == "Periodic":
+                # the MSH 2 writer keeps this section in ASCII in binary files
+                start = cur.pos
+                cur.skip_to_end(name)
+                body = data[start:cur.pos].decode("ascii").splitlines()
+                links = _periodic_v2([b for b in body if b.strip() and not b.startswith("$End")])
+            else:
+                cur.skip_to_end(name)
+        elif version.startswith("4.1"):
+            if name == "Entities":
+                counts = sz(4)
+                for dim, cnt in enumerate(counts):
+                    for _ in range(int(cnt)):
+                        tag = int(i4(1)[0])
+                        cur.take(bo + "f8", 3 if dim == 0 else 6)
+                        nph = int(sz(1)[0])
+                        ent_phys[(dim, tag)] = [int(t) for t in i4(nph)]
+                        if dim > 0:
+                            nb = int(sz(1)[0])
+                            i4(nb)
+                cur.skip_to_end(name)
+            elif name == "Nodes":
+                nblocks, nnodes = (int(t) for t in sz(4)[:2])
+                id_parts, x_parts = [], []
+                for _ in range(nblocks):
+                    _, _, param = (int(t) for t in i4(3))
+                    cnt = int(sz(1)[0])
+                    if param:
+                        raise GmshError("parametric node coordinates are not supported")
+                    id_parts.append(sz(cnt))
+                    x_parts.append(cur.take(bo + "f8", 3 * cnt).reshape(cnt, 3))
+                ids = np.concatenate(id_parts) if id_parts else np.zeros(0, np.int64)
+                xyz = np.concatenate(x_parts).astype(np.float64) if x_parts else np.zeros((0, 3))
+                if len(ids) != nnodes:
+                    raise GmshError(f"{path}: $Nodes holds {len(ids)} nodes, its header {nnodes}")
+                cur.skip_to_end(name)
+            elif name == "Elements":
+                nblocks = int(sz(4)[0])
+                for _ in range(nblocks):
+                    edim, etag, et = (int(t) for t in i4(3))
+                    cnt = int(sz(1)[0])
+                    if et not in _ELEM:
+                        raise GmshError(f"{path}: unknown element type {et}")
+                    nn = _ELEM[et][1]
+                    blk = sz(cnt * (1 + nn)).reshape(cnt, 1 + nn)
+                    ph = ent_phys.get((edim, etag), [])
+                    phys = ph[0] if ph and ph[0] > 0 else 1
+                    for row in blk:
+                        elems.append((et, phys, [int(t) for t in row[1:]]))
+                cur.skip_to_end(name)
+            elif name == "Periodic":
+                nlinks = int(sz(1)[0])
+                for _ in range(nlinks):
+                    i4(3)
+                    naff = int(sz(1)[0])
+                    cur.take(bo + "f8", naff)
+                    cnt = int(sz(1)[0])
+                    pairs = sz(2 * cnt).reshape(cnt, 2)
+                    links.extend((int(a), int(b)) for a, b in pairs)
+                cur.skip_to_end(name)
+            else:
+                cur.skip_to_end(name)
+        else:
+            raise GmshError(f"{path}: MSH version {version} (2.2 and 4.1 are read)")
+    if version is None:
+        raise GmshError(f"{path}: no $MeshFormat")
+    if ids is None or not elems:
+        raise GmshError(f"{path}: MSH file without $Nodes or $Elements")
+    return version, ids, xyz, elems, links, names
+
+
 def _orient_2d(geom, verts, X):
     """CheckElementOrientation(fix_it = true) for 2D cells: the Jacobian at the
     cell centre decides; inverted cells get two vertices exchanged."""
@@ -231,27 +406,31 @@ def _hex_jacobians_ok(X):
 
 
 def read_gmsh(path, fix_orientation: bool = True, mark_triangles: bool = True) -> GmshMesh:
-    """Reads an ASCII MSH 2.2 or 4.1 file, gzip-compressed when its name ends
-    in .gz (case_manager.cpp:255-263, 298-448)."""
+    """Reads an MSH 2.2 or 4.1 file, ASCII or binary, gzip-compressed when its
+    name ends in .gz (case_manager.cpp:255-263, 298-448)."""
     opener = gzip.open if str(path).endswith(".gz") else open
-    with opener(path, "rt") as fh:
-        text = fh.read()
-    secs = {}
-    for name, lines in _sections(text):
-        secs.setdefault(name, lines)
-    if "MeshFormat" not in secs:
+    with opener(path, "rb") as fh:
+        data = fh.read()
+    at = data.find(b"$MeshFormat")
+    if at < 0:
         raise GmshError(f"{path}: no $MeshFormat")
-    fmt = secs["MeshFormat"][0].split()
-    version, ftype = fmt[0], int(fmt[1])
-    if ftype != 0:
-        raise GmshError(f"{path}: binary MSH files are not supported")
-    if version.startswith("2"):
-        ids, xyz, elems, links = _read_v2(secs)
-    elif version.startswith("4.1"):
-        ids, xyz, elems, links = _read_v41(secs)
+    fmt_line = data[at:].split(b"\n", 2)[1].split()
+    if len(fmt_line) < 2:
+        raise GmshError(f"{path}: malformed $MeshFormat")
+    if int(fmt_line[1]) == 1:
+        version, ids, xyz, elems, links, names = _read_binary(data, path)
     else:
-        raise GmshError(f"{path}: MSH version {version} (2.2 and 4.1 are read)")
-    names = _physical_names(secs["PhysicalNames"]) if "PhysicalNames" in secs else []
+        secs = {}
+        for name, lines in _sections(data.decode("utf-8", "replace")):
+            secs.setdefault(name, lines)
+        version = secs["MeshFormat"][0].split()[0]
+        if version.startswith("2"):
+            ids, xyz, elems, links = _read_v2(secs)
+        elif version.startswith("4.1"):
+            ids, xyz, elems, links = _read_v41(secs)
+        else:
+            raise GmshError(f"{path}: MSH version {version} (2.2 and 4.1 are read)")
+        names = _physical_names(secs["PhysicalNames"]) if "PhysicalNames" in secs else []
 
     for et, _, nodes in elems:
         if et not in _ELEM:

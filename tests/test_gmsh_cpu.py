@@ -170,6 +170,147 @@ def test_msh22_round_trip(tmp_path):
         np.testing.assert_array_equal(r.bdr_attr, [tags.index(a) + 2 for a in m.bdr_attr])
 
 
+# ---------------------------------------------------------------- binary MSH
+# Encoded here by hand from the Gmsh reference manual's layouts, independently
+# of the reader: MSH 2.2 (int32 records, $PhysicalNames and $Periodic in ASCII)
+# and MSH 4.1 (int32 tags and flags, size_t counts and node/element tags).
+
+def _msh22_binary(path, m, links=(), bo="<"):
+    import struct
+
+    dim = m.dim
+    nv = int(m.elem_verts.max()) + 1
+    coords = np.zeros((nv, 3))
+    coords[m.elem_verts.ravel(), :dim] = m.elem_coords.reshape(-1, dim)
+    et = {M.GEOM_TRI: 2, M.GEOM_QUAD: 3, M.GEOM_HEX: 5}[m.geom]
+    ft = {M.GEOM_TRI: 1, M.GEOM_QUAD: 1, M.GEOM_HEX: 3}[m.geom]
+    battrs = sorted(set(int(a) for a in m.bdr_attr))
+    cell_tag = max(battrs, default=0) + 1
+    out = bytearray(b"$MeshFormat\n2.2 1 8\n" + struct.pack(bo + "i", 1) + b"\n$EndMeshFormat\n")
+    out += b"$PhysicalNames\n%d\n" % (len(battrs) + 1)
+    for a in battrs:
+        out += b'%d %d "Patch%d"\n' % (dim - 1, a, a)
+    out += b'%d %d "Volume"\n$EndPhysicalNames\n' % (dim, cell_tag)
+    out += b"$Nodes\n%d\n" % nv
+    for i in range(nv):
+        out += struct.pack(bo + "i3d", i + 1, *coords[i])
+    out += b"\n$EndNodes\n$Elements\n%d\n" % (len(m.bdr_attr) + m.num_elements)
+    nb = len(m.bdr_attr)
+    if nb:
+        out += struct.pack(bo + "3i", ft, nb, 2)
+        for k, (b, a) in enumerate(zip(m.bdr_verts, m.bdr_attr)):
+            out += struct.pack(bo + "%di" % (3 + len(b)), k + 1, int(a), int(a), *(int(v) + 1 for v in b))
+    out += struct.pack(bo + "3i", et, m.num_elements, 2)
+    for e in range(m.num_elements):
+        vs = m.elem_verts[e]
+        out += struct.pack(bo + "%di" % (3 + len(vs)), nb + e + 1, cell_tag, cell_tag, *(int(v) + 1 for v in vs))
+    out += b"\n$EndElements\n"
+    if links:
+        out += b"$Periodic\n1\n1 1 2\n%d\n" % len(links)
+        for a, b in links:
+            out += b"%d %d\n" % (a + 1, b + 1)
+        out += b"$EndPeriodic\n"
+    with open(path, "wb") as fh:
+        fh.write(bytes(out))
+
+
+@pytest.mark.parametrize("bo", ["<", ">"])
+def test_msh22_binary_matches_ascii(tmp_path, bo):
+    for mk in (lambda: M.structured_quad(5, 3, 0.0, 2.0, 0.0, 1.0),
+               lambda: M.jittered_delaunay_tri(5, 4, 0.0, 1.0, 0.0, 1.0, seed=3),
+               lambda: M.structured_hex(3, 2, 2, periodic=(False, False, False))):
+        m = mk()
+        pa, pb = tmp_path / "a.msh", tmp_path / "b.msh"
+        gmsh.write_gmsh22(str(pa), m)
+        _msh22_binary(str(pb), m, bo=bo)
+        ga, gb = gmsh.read_gmsh(str(pa)), gmsh.read_gmsh(str(pb))
+        for f in ("elem_coords", "elem_verts", "bdr_verts", "bdr_attr"):
+            np.testing.assert_array_equal(getattr(gb.mesh, f), getattr(ga.mesh, f))
+        np.testing.assert_array_equal(gb.elem_attr, ga.elem_attr)
+        assert gb.bdr_names == ga.bdr_names and gb.cell_names == ga.cell_names
+
+
+def test_msh22_binary_periodic(tmp_path):
+    n = 4
+    plain = M.structured_quad(n, n)
+    per = M.structured_quad(n, n, periodic=(True, True))
+    vid = lambda i, j: j * (n + 1) + i  # noqa: E731
+    links = [(vid(n, j), vid(0, j)) for j in range(n + 1)] + [(vid(i, n), vid(i, 0)) for i in range(n + 1)]
+    p = tmp_path / "per.msh"
+    _msh22_binary(str(p), plain, links)
+    g = gmsh.read_gmsh(str(p))
+    assert g.periodic
+    np.testing.assert_array_equal(g.mesh.elem_coords, per.elem_coords)
+    assert not _open_faces(g.mesh)[0]
+
+
+def _msh41_binary(path, periodic=False):
+    """The two-quad mesh of MSH41 above in binary MSH 4.1, optionally with the
+    right edge's nodes linked to the left edge's."""
+    import struct
+
+    out = bytearray(b"$MeshFormat\n4.1 1 8\n" + struct.pack("<i", 1) + b"\n$EndMeshFormat\n")
+    out += b'$PhysicalNames\n3\n1 7 "wall"\n1 9 "side"\n2 4 "fluid"\n$EndPhysicalNames\n'
+    out += b"$Entities\n" + struct.pack("<4Q", 0, 2, 1, 0)
+    # curves 1 (phys 7) and 2 (phys 9): tag, bbox, phys tags, bounding points
+    out += struct.pack("<i6dQiQ", 1, 0, 0, 0, 2, 0, 0, 1, 7, 0)
+    out += struct.pack("<i6dQiQ", 2, 0, 0, 0, 0, 1, 0, 1, 9, 0)
+    # surface 1 (phys 4) bounded by curves 1, 2
+    out += struct.pack("<i6dQiQ2i", 1, 0, 0, 0, 2, 1, 0, 1, 4, 2, 1, 2)
+    out += b"\n$EndEntities\n$Nodes\n" + struct.pack("<4Q", 1, 6, 1, 6)
+    out += struct.pack("<3iQ", 2, 1, 0, 6) + struct.pack("<6Q", 1, 2, 3, 4, 5, 6)
+    out += struct.pack("<18d", 0, 0, 0, 1, 0, 0, 2, 0, 0, 0, 1, 0, 1, 1, 0, 2, 1, 0)
+    out += b"\n$EndNodes\n$Elements\n" + struct.pack("<4Q", 3, 6, 1, 6)
+    out += struct.pack("<3iQ", 1, 1, 1, 2) + struct.pack("<6Q", 1, 1, 2, 2, 2, 3)
+    out += struct.pack("<3iQ", 1, 2, 1, 2) + struct.pack("<6Q", 3, 3, 6, 4, 4, 1)
+    out += struct.pack("<3iQ", 2, 1, 3, 2) + struct.pack("<10Q", 5, 1, 2, 5, 4, 6, 2, 3, 6, 5)
+    out += b"\n$EndElements\n"
+    if periodic:
+        out += b"$Periodic\n" + struct.pack("<Q", 1)
+        out += struct.pack("<3iQ", 1, 2, 2, 0)  # curve 2 -> itself as master, no affine map
+        out += struct.pack("<Q4Q", 2, 3, 1, 6, 4)
+        out += b"\n$EndPeriodic\n"
+    with open(path, "wb") as fh:
+        fh.write(bytes(out))
+
+
+def test_msh41_binary_matches_ascii(tmp_path):
+    pa, pb = tmp_path / "a.msh", tmp_path / "b.msh"
+    pa.write_text(MSH41)
+    _msh41_binary(str(pb))
+    ga, gb = gmsh.read_gmsh(str(pa)), gmsh.read_gmsh(str(pb))
+    assert gb.version == "4.1"
+    for f in ("elem_coords", "elem_verts", "bdr_verts", "bdr_attr"):
+        np.testing.assert_array_equal(getattr(gb.mesh, f), getattr(ga.mesh, f))
+    assert gb.bdr_names == {"wall": 2, "side": 3} and gb.cell_names == {"fluid": 1}
+    # node 3 -> 1 and 6 -> 4: the right edge is the left edge, the cells join there
+    gp = gmsh.read_gmsh(str(_write_tmp(tmp_path, periodic=True)))
+    assert gp.periodic
+    np.testing.assert_array_equal(gp.mesh.elem_verts[1], [1, 0, 3, 4])
+
+
+def _write_tmp(tmp_path, periodic):
+    p = tmp_path / "p.msh"
+    _msh41_binary(str(p), periodic=periodic)
+    return p
+
+
+def test_binary_msh_errors(tmp_path):
+    import struct
+
+    p = tmp_path / "bad.msh"
+    p.write_bytes(b"$MeshFormat\n2.2 1 4\n" + struct.pack("<i", 1) + b"\n$EndMeshFormat\n")
+    with pytest.raises(gmsh.GmshError, match="data size"):
+        gmsh.read_gmsh(str(p))
+    p.write_bytes(b"$MeshFormat\n2.2 1 8\n" + struct.pack("<i", 7) + b"\n$EndMeshFormat\n")
+    with pytest.raises(gmsh.GmshError, match="integer 1"):
+        gmsh.read_gmsh(str(p))
+    p.write_bytes(b"$MeshFormat\n2.2 1 8\n" + struct.pack("<i", 1) + b"\n$EndMeshFormat\n$Nodes\n5\n"
+                  + struct.pack("<i3d", 1, 0, 0, 0) + b"\n$EndNodes\n")
+    with pytest.raises(gmsh.GmshError, match="ends early"):
+        gmsh.read_gmsh(str(p))
+
+
 def test_orientation_fix(tmp_path):
     m = M.structured_quad(3, 2)
     t = M.jittered_delaunay_tri(3, 3, 0.0, 1.0, 0.0, 1.0, seed=1)
