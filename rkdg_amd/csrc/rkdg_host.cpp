@@ -1306,8 +1306,6 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
     if (dim != 3) return fail(RKDG_E_UNSUPPORTED, "non-affine quadrilaterals take the dense path");
     if (c->nc_in && c->nc_in->count > 0)
       return fail(RKDG_E_UNSUPPORTED, "non-conforming faces are built for affine elements");
-    if (g.indicator == RKDG_IND_SHU || g.linearize)
-      return fail(RKDG_E_UNSUPPORTED, "the Shu indicator and linearize are built for affine hexahedra");
     std::vector<double> rec;
     bool ok = false;
     if (c->p == 1) ok = curved_records<1>(c, rec, c->vol_h);
@@ -2039,12 +2037,13 @@ int objective_launch(rkdg_ctx* c, const double* x, int slot, double coef, double
 // with (w_q, xi_q) the rule of order p + OrderW on t's reference element,
 // wlast_k the Jacobian determinant of k at the last point of that rule (the
 // reference reads Trans.Weight() without setting a point), |k| the measure of
-// k. T_k^-1 is the affine inverse (triangles, hexahedra) or Newton's method
-// on the bilinear map (quadrilaterals), not clipped to the element: the
-// neighbour's polynomial is extrapolated over t.
+// k. T_k^-1 is the affine inverse (triangles, affine hexahedra) or Newton's
+// method on the bilinear or trilinear map (quadrilaterals, curved hexahedra),
+// not clipped to the element: the neighbour's polynomial is extrapolated over t.
 namespace {
 struct ShuGeom {
   int geom, dim, nv, p, nd;
+  bool curved = false;         // trilinear hexahedra: Newton's method for T^-1
   RefBasis rb;                 // quadrilaterals and triangles
   std::vector<double> x1;      // 1D Gauss-Legendre nodes
   void map(const double* X, const double* xi, double* x) const {
@@ -2111,7 +2110,7 @@ struct ShuGeom {
     }
   }
   void inverse(const double* X, const double* x, double* xi) const {
-    if (geom != RKDG_GEOM_QUAD) {
+    if (geom == RKDG_GEOM_TRI || (geom == RKDG_GEOM_HEX && !curved)) {
       // affine: the vertex differences along the reference axes
       const int cols[3] = {1, geom == RKDG_GEOM_TRI ? 2 : 3, 4};
       double J[9], b[3];
@@ -2158,6 +2157,7 @@ int build_shu(rkdg_ctx* c) {
   G.nv = c->geom == RKDG_GEOM_TRI ? 3 : (c->dim == 3 ? 8 : 4);
   G.p = c->p;
   G.nd = c->nd;
+  G.curved = c->curved != 0;
   if (c->geom != RKDG_GEOM_HEX) G.rb.init(c->geom == RKDG_GEOM_TRI, c->p);
   G.x1.assign(c->p + 1, 0.0);
   {
