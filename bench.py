@@ -360,6 +360,77 @@ def dmr_bench(dev, reps=20, steps=5):
     return out
 
 
+def curved_bench(dev, n=48, reps=20, steps=3):
+    """Curved (trilinear) hexahedra: the n^3 periodic box with every vertex
+    moved by a seeded U(-0.15, 0.15) of the spacing per coordinate, Q2, the
+    headline's scheme (Rusanov, Venkatakrishnan K = 5, SSP-RK3, CFL 0.2). The
+    plain right-hand side timed with HIP events on the context's stream against
+    its roofline (16 B per DOF, the element's CurvedLayout record and the face
+    tables), and steps of the reference dt loop."""
+    import torch
+
+    from rkdg_amd import mesh as M
+    from rkdg_amd import scheme as S
+    from rkdg_amd.capi import Context
+
+    t0 = time.perf_counter()
+    m = M.structured_hex(n, n, n)
+    rng = np.random.default_rng(0)
+    nv = int(m.elem_verts.max()) + 1
+    m.elem_coords = np.ascontiguousarray(m.elem_coords + rng.uniform(-0.15, 0.15, (nv, 3))[m.elem_verts] / n)
+    sch = S.Scheme(riemann=S.RUSANOV, indicator=S.IND_BJ, phi=S.PHI_VENKAT, venkat_k=5.0, limiter=S.LIM_MULTI)
+    ctx = Context(m, 2, sch)
+    if ctx.info.affine:
+        raise RuntimeError("the curved box came out affine")
+    ne = m.num_elements
+    dof = ctx.info.state_size
+    xyz = ctx.node_coords()
+    r2 = ((xyz - 0.5) ** 2).sum(axis=-1)
+    rho = 1.0 + 0.1 * np.exp(-r2 / 0.01)
+    p = 1.0 + 0.1 * np.exp(-r2 / 0.01)
+    z = np.zeros_like(rho)
+    x = torch.from_numpy(np.concatenate([rho.ravel(), z.ravel(), z.ravel(), z.ravel(),
+                                         (p / 0.4).ravel()])).to(dev)
+    y = torch.empty_like(x)
+    ctx.rhs(x, y, want_lambda=False)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    ctx.profile(True)
+    for _ in range(reps):
+        ctx.rhs(x, y, want_lambda=False)
+    ms, cnt, _, _ = ctx.profile_read()
+    ctx.profile(False)
+    avg = ms / max(1, cnt)
+    # CurvedLayout<2>: adj(J) at 27 nodes, the 27x27 inverse mass, 27 mean
+    # weights, 6 x 16 face points x 5, the volume: 1480 doubles; with the
+    # affine block (10 doubles) and nbr/fcode (6 x 2 int32)
+    nbytes = 16 * dof + (1480 * 8 + 80 + 48) * ne
+    tab = S.TABLEAUS["ssprk3"]
+    hmin = ctx.hmin()
+    lam = ctx.face_lambda(x)
+    dt = 0.2 * hmin / lam / 5.0
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        lam = ctx.step(x, dt, tab)
+        dt = 0.2 * hmin / lam / 5.0
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t1
+    ctx.check_error()
+    ctx.close()
+    out = {"workload": f"{n}^3 periodic box of curved (trilinear) Q2 hexahedra, corners moved by up to 15 % "
+                       f"of the spacing; Rusanov, Venkatakrishnan K=5, SSP-RK3",
+           "elements": ne, "dof": dof, "rhs_avg_launch_ms": avg, "rhs_launches": cnt,
+           "rhs_algorithmic_bytes": nbytes, "rhs_achieved_gbs": nbytes / (avg * 1e-3) / 1e9,
+           "rhs_frac_of_hbm": nbytes / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "steps": steps, "ms_per_step": el * 1e3 / steps, "dof_stage_per_s": dof * 3 * steps / el,
+           "setup_s": setup}
+    print(f"[curved] {ne} curved Q2 hexes: RHS {avg:.3f} ms ({out['rhs_achieved_gbs']:.0f} GB/s, "
+          f"{out['rhs_frac_of_hbm'] * 100:.1f} % of HBM), step {out['ms_per_step']:.3f} ms, "
+          f"setup {setup:.1f} s", flush=True)
+    return out
+
+
 def adjoint_bench(dev, nsteps=2000, nx=240, ny=80, oracle_steps=200, fd_steps=1000, threads=None):
     """The discrete adjoint on the Mach 3 forward-facing step (240x80 minus the
     step, P1 quads, HLLC, Venkatakrishnan K=5, SSP-RK3): objective = pressure
@@ -497,6 +568,7 @@ def main():
                     help="with --ensemble-only: skip the sequential solves (profiles of the batched launches)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-dmr", action="store_true")
+    ap.add_argument("--no-curved", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--oracle-steps", type=int, default=200)
@@ -666,13 +738,16 @@ def main():
               f"1 thread {cv1:.3e} ({cel1:.1f}s)", flush=True)
 
     ens = None
-    if not (args.no_ensemble and args.no_adjoint and args.no_dmr):
+    if not (args.no_ensemble and args.no_adjoint and args.no_dmr and args.no_curved):
         del x
         ctx.close()
         torch.cuda.empty_cache()
     dmr = None
     if not args.no_dmr:
         dmr = dmr_bench(dev)
+    curved = None
+    if not args.no_curved:
+        curved = curved_bench(dev)
     if not args.no_ensemble:
         ens = ensemble_bench(dev, args.ensemble_count)
     adj = None
@@ -744,6 +819,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "dmr": dmr,
+        "curved": curved,
         "ensemble": ens,
         "adjoint": adj,
     }
