@@ -1125,7 +1125,9 @@ int mark_nc_faces(rkdg_ctx* c, const rkdg_mesh_desc* m,
         }
         gp[2 * dim] = (dim == 3 ? wq[qi] * wq[qj] : wq[qi]) * nn;
       }
-    for (int dd = 0; dd < dim; dd++) G[NP * (2 * dim + 1) + dd] = an[dd] / nn;
+    // the reference's unit normal nor * (1 / Norml2(nor)), as the element kernels form it
+    const double inv = 1.0 / (dim == 3 ? norml2_scaled<3>(an) : norml2_scaled<2>(an));
+    for (int dd = 0; dd < dim; dd++) G[NP * (2 * dim + 1) + dd] = an[dd] * inv;
   }
   // elements with non-conforming faces, their items in record order
   std::map<int64_t, std::vector<int>> items;
