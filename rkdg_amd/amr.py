@@ -85,6 +85,14 @@ class NCMesh:
         for bv, ba in zip(base.bdr_verts, base.bdr_attr):
             self.bdr[tuple(sorted(int(v) for v in bv))] = int(ba)
         self.name = base.name
+        # parallelograms / parallelepipeds: X(v) = X0 + sum_r R(v)_r (X(e_r) - X0)
+        X = base.elem_coords
+        R = _REF_CORNERS[self.geom]
+        cols = [1, 3, 4][:self.dim]
+        E = np.stack([X[:, c] - X[:, 0] for c in cols], axis=1)  # [e][r][i]
+        aff = X[:, :1] + np.einsum("kr,eri->eki", R.astype(float), E)
+        scale = np.abs(X).max() if X.size else 1.0
+        self.affine = bool(np.all(np.abs(aff - X) <= 1e-12 * max(scale, 1.0)))
 
     # -- topology helpers ------------------------------------------------
     def _vid(self, corners):
@@ -249,6 +257,10 @@ class NCMesh:
         keys, when all their children are leaves and the coarsened parent keeps
         the level limit with its face neighbours (MFEM's DerefineByError with
         nc_limit 1). Returns (transfer, the parents coarsened)."""
+        if not self.affine:
+            # the restriction of transfer_state is the L2 projection of affine
+            # elements (a constant Jacobian); curved parents would weight it by det J
+            raise NotImplementedError("derefinement is built for affine elements")
         old = self.ordered_keys()
         old_index = {k: i for i, k in enumerate(old)}
         nch = 1 << self.dim
@@ -464,7 +476,8 @@ def zz_indicator(mesh, order, x, var=0):
     multilinearly). This stands in for MFEM's L2ZienkiewiczZhuEstimator on the
     density (main.cpp:237-251), whose recovery is a global L2 projection onto
     a Raviart-Thomas space that is not rebuilt here: the two estimate the same
-    quantity, not with the same numbers. Affine quadrilaterals and hexahedra."""
+    quantity, not with the same numbers. Quadrilaterals and hexahedra, the
+    Jacobian of the element's multilinear map taken at every node."""
     dim = mesh.dim
     N = order + 1
     nd = N ** dim
@@ -491,18 +504,27 @@ def zz_indicator(mesh, order, x, var=0):
         gref.append(np.moveaxis(np.tensordot(u, D, axes=([ax], [1])), -1, ax).reshape(NE, nd))
     gref = np.stack(gref, axis=-1)  # [e][node][d] reference gradient
     X = mesh.elem_coords
-    cols = [1, 3, 4][:dim]
-    J = np.stack([X[:, c] - X[:, 0] for c in cols], axis=-1)  # [e][i][r]
+    R = _REF_CORNERS[mesh.geom]
+    grids = np.meshgrid(*([nodes] * dim), indexing="ij")  # [x][y][z] ordering
+    xi = np.stack([g.transpose(*range(dim)[::-1]).ravel() for g in grids], axis=-1)  # nodes x fastest
+    nv = X.shape[1]
+    # the multilinear map's Jacobian at every node (constant on affine elements)
+    dN = np.ones((nd, nv, dim))  # dN[n, k, r] = d N_k / d xi_r at node n
+    for k in range(nv):
+        for r in range(dim):
+            for d in range(dim):
+                f = (1.0 if R[k][d] else -1.0) if d == r else (xi[:, d] if R[k][d] else 1.0 - xi[:, d])
+                dN[:, k, r] *= f
+    J = np.einsum("nkr,eki->enir", dN, X)  # [e][node][i][r]
     Jinv = np.linalg.inv(J)
-    grad = np.einsum("enr,eri->eni", gref, Jinv)  # J^-T applied: g_x = sum_r dxi_r/dx_i g_r
-    det = np.abs(np.linalg.det(J))
+    grad = np.einsum("enr,enri->eni", gref, Jinv)  # J^-T applied: g_x = sum_r dxi_r/dx_i g_r
+    det = np.abs(np.linalg.det(J))  # [e][node]
     W = w1
     for _ in range(dim - 1):
         W = np.multiply.outer(w1, W)
     W = W.ravel()  # node weights, x fastest
-    gmean = np.einsum("n,eni->ei", W, grad)
+    gmean = np.einsum("n,en,eni->ei", W, det, grad) / np.einsum("n,en->e", W, det)[:, None]
     V = mesh.elem_verts
-    nv = V.shape[1]
     ids, inv = np.unique(V.ravel(), return_inverse=True)
     acc = np.zeros((len(ids), dim))
     cnt = np.zeros(len(ids))
@@ -510,15 +532,12 @@ def zz_indicator(mesh, order, x, var=0):
     np.add.at(cnt, inv, 1.0)
     Gv = (acc / cnt[:, None])[inv].reshape(NE, nv, dim)
     # multilinear interpolation of the corner values at the nodes
-    R = _REF_CORNERS[mesh.geom]
-    grids = np.meshgrid(*([nodes] * dim), indexing="ij")  # [x][y][z] ordering
-    xi = np.stack([g.transpose(*range(dim)[::-1]).ravel() for g in grids], axis=-1)  # nodes x fastest
     Nk = np.ones((nd, nv))
     for k in range(nv):
         for d in range(dim):
             Nk[:, k] *= xi[:, d] if R[k][d] else 1.0 - xi[:, d]
     Gn = np.einsum("nk,eki->eni", Nk, Gv)
-    err2 = np.einsum("n,eni->e", W, (Gn - grad) ** 2) * det
+    err2 = np.einsum("n,en,eni->e", W, det, (Gn - grad) ** 2)
     return np.sqrt(err2)
 
 

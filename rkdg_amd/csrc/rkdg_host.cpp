@@ -1058,7 +1058,8 @@ int mark_nc_faces(rkdg_ctx* c, const rkdg_mesh_desc* m,
   };
   const int NQ = dim == 3 ? c->p + 2 : c->p + 1;
   const int NP = dim == 3 ? NQ * NQ : NQ;
-  const int RG = NP * (2 * dim + 1) + dim;
+  const int PR = 3 * dim + 1;  // per point: slave xi, master xi, w |n|, unit normal
+  const int RG = NP * PR;
   double sq[kMaxQ], wq[kMaxQ];
   gl_rule(NQ, sq, wq);
   std::vector<double> geo((size_t)d->count * RG);
@@ -1092,7 +1093,8 @@ int mark_nc_faces(rkdg_ctx* c, const rkdg_mesh_desc* m,
     ref_vertex(dim, mfem_face_vertex(dim, ls, 1), R1);
     if (dim == 3) ref_vertex(dim, mfem_face_vertex(dim, ls, 3), R3);
     for (int q = 0; q < nvf; q++) ref_vertex(dim, mfem_face_vertex(dim, lm, q), Rm[q]);
-    // the slave face's area normal: sign * row rs of adj(J)
+    // the slave face's area normal: sign * row rs of adj(J), constant on an
+    // affine element, per point on a curved one
     double an[3], nn = 0.0;
     for (int dd = 0; dd < dim; dd++) {
       an[dd] = (cs ? 1.0 : -1.0) * c->geo_h[(size_t)(rs * dim + dd) * NE + es];
@@ -1118,16 +1120,28 @@ int mark_nc_faces(rkdg_ctx* c, const rkdg_mesh_desc* m,
         face_shape(dim, pi, Nw);
         for (int cc = 0; cc < nvf; cc++)
           for (int dd = 0; dd < dim; dd++) xm[dd] += Nw[cc] * Rm[cc][dd];
-        double* gp = G + q * (2 * dim + 1);
+        double* gp = G + q * PR;
+        double ap[3] = {an[0], an[1], an[2]}, np_ = nn;
+        if (c->curved) {
+          double J[9], A[9];
+          trilinear_jac(&c->coords[(size_t)es * 24], xs, J);
+          adj3(J, A);
+          np_ = 0.0;
+          for (int dd = 0; dd < 3; dd++) {
+            ap[dd] = (cs ? 1.0 : -1.0) * A[rs * 3 + dd];
+            np_ += ap[dd] * ap[dd];
+          }
+          np_ = std::sqrt(np_);
+        }
         for (int dd = 0; dd < dim; dd++) {
           gp[dd] = xs[dd];
           gp[dim + dd] = xm[dd];
         }
-        gp[2 * dim] = (dim == 3 ? wq[qi] * wq[qj] : wq[qi]) * nn;
+        gp[2 * dim] = (dim == 3 ? wq[qi] * wq[qj] : wq[qi]) * np_;
+        // the reference's unit normal nor * (1 / Norml2(nor)), as the element kernels form it
+        const double inv = 1.0 / (dim == 3 ? norml2_scaled<3>(ap) : norml2_scaled<2>(ap));
+        for (int dd = 0; dd < dim; dd++) gp[2 * dim + 1 + dd] = ap[dd] * inv;
       }
-    // the reference's unit normal nor * (1 / Norml2(nor)), as the element kernels form it
-    const double inv = 1.0 / (dim == 3 ? norml2_scaled<3>(an) : norml2_scaled<2>(an));
-    for (int dd = 0; dd < dim; dd++) G[NP * (2 * dim + 1) + dd] = an[dd] * inv;
   }
   // elements with non-conforming faces, their items in record order
   std::map<int64_t, std::vector<int>> items;
@@ -1306,8 +1320,6 @@ int build_ctx(rkdg_ctx* c, const rkdg_mesh_desc* m, const rkdg_scheme_desc* s) {
     // limiter's Venkatakrishnan eps^2 = (K h)^3 with h^3 the volume), h the
     // smallest singular value of J at the centre (GetElementSize)
     if (dim != 3) return fail(RKDG_E_UNSUPPORTED, "non-affine quadrilaterals take the dense path");
-    if (c->nc_in && c->nc_in->count > 0)
-      return fail(RKDG_E_UNSUPPORTED, "non-conforming faces are built for affine elements");
     std::vector<double> rec;
     bool ok = false;
     if (c->p == 1) ok = curved_records<1>(c, rec, c->vol_h);

@@ -319,8 +319,8 @@ struct NcArgs {
   const int* ptr;         // [n_elem + 1] ranges in item
   const int* item;        // slave-face record * 2 + side (0: the element is the slave, 1: the master)
   const int* rec;         // [records][3]: slave element, master element, slave local face
-  const double* geo;      // [records][NP * (2 DIM + 1) + DIM]: per point slave and master reference
-                          // coordinates and weight * |n|, then the slave's outward unit normal
+  const double* geo;      // [records][NP][3 DIM + 1]: per point the slave and master reference
+                          // coordinates, weight * |n| and the slave's outward unit normal
   double* out;            // [n_elem][NEQ][ND]
   double* lam;            // wave speed slot
   long long* err;

@@ -45,7 +45,8 @@ __global__ __launch_bounds__(64) void nc_face_kernel(const NcArgs A, const Schem
   constexpr int NEQ = DIM + 2;
   constexpr int NQ = DIM == 3 ? P + 2 : P + 1;
   constexpr int NP = DIM == 3 ? NQ * NQ : NQ;
-  constexpr int RG = NP * (2 * DIM + 1) + DIM;
+  constexpr int PR = 3 * DIM + 1;  // per point: slave xi, master xi, w |n|, unit normal
+  constexpr int RG = NP * PR;
   static_assert(ND <= 64 && NP <= 64, "one lane per node and per face point");
 
   __shared__ double s_x[N];
@@ -75,7 +76,7 @@ __global__ __launch_bounds__(64) void nc_face_kernel(const NcArgs A, const Schem
     __syncthreads();
     if (tid < NP) {
       const double* G = A.geo + (int64_t)r * RG;
-      const double* gp = G + tid * (2 * DIM + 1);
+      const double* gp = G + tid * PR;
       double us[NEQ], um[NEQ];
 #pragma unroll
       for (int h = 0; h < 2; h++) {
@@ -96,7 +97,7 @@ __global__ __launch_bounds__(64) void nc_face_kernel(const NcArgs A, const Schem
           u[v] = acc;
         }
       }
-      const double* nrm = G + NP * (2 * DIM + 1);
+      const double* nrm = gp + 2 * DIM + 1;
       double fl[NEQ];
       bool ok = true;
       const double lam = riemann<DIM>(g, us, um, nrm, fl, ok);
