@@ -48,6 +48,16 @@ class _Leaf:
         return len(self.path)
 
 
+def _is_affine(X):
+    """A parallelogram / parallelepiped: X(v) = X0 + sum_r R(v)_r (X(e_r) - X0)
+    to 1e-12 of the coordinates."""
+    dim = X.shape[1]
+    R = _REF_CORNERS[M.GEOM_QUAD if dim == 2 else M.GEOM_HEX].astype(float)
+    E = np.stack([X[c] - X[0] for c in [1, 3, 4][:dim]])  # [r][i]
+    aff = X[0][None, :] + R @ E
+    return bool(np.all(np.abs(aff - X) <= 1e-12 * max(float(np.abs(X).max()), 1.0)))
+
+
 def _gl_nodes(n):
     x, _ = np.polynomial.legendre.leggauss(n)
     return 0.5 * (x + 1.0)
@@ -85,14 +95,6 @@ class NCMesh:
         for bv, ba in zip(base.bdr_verts, base.bdr_attr):
             self.bdr[tuple(sorted(int(v) for v in bv))] = int(ba)
         self.name = base.name
-        # parallelograms / parallelepipeds: X(v) = X0 + sum_r R(v)_r (X(e_r) - X0)
-        X = base.elem_coords
-        R = _REF_CORNERS[self.geom]
-        cols = [1, 3, 4][:self.dim]
-        E = np.stack([X[:, c] - X[:, 0] for c in cols], axis=1)  # [e][r][i]
-        aff = X[:, :1] + np.einsum("kr,eri->eki", R.astype(float), E)
-        scale = np.abs(X).max() if X.size else 1.0
-        self.affine = bool(np.all(np.abs(aff - X) <= 1e-12 * max(scale, 1.0)))
 
     # -- topology helpers ------------------------------------------------
     def _vid(self, corners):
@@ -257,10 +259,6 @@ class NCMesh:
         keys, when all their children are leaves and the coarsened parent keeps
         the level limit with its face neighbours (MFEM's DerefineByError with
         nc_limit 1). Returns (transfer, the parents coarsened)."""
-        if not self.affine:
-            # the restriction of transfer_state is the L2 projection of affine
-            # elements (a constant Jacobian); curved parents would weight it by det J
-            raise NotImplementedError("derefinement is built for affine elements")
         old = self.ordered_keys()
         old_index = {k: i for i, k in enumerate(old)}
         nch = 1 << self.dim
@@ -427,13 +425,69 @@ def _apply_axes(mats, u, dim):
     return np.einsum("kc,vcba->vkba", mats[2], u)
 
 
-def transfer_state(x_old, transfer, order, dim, neq, ne_old):
+def _tensor_eval(mats):
+    """Rows: points of the tensor grid (first axis fastest); columns: the tensor
+    basis (first axis fastest), from the 1D matrices mats[d][point, node]."""
+    out = mats[0]
+    for m in mats[1:]:
+        out = np.einsum("qa,pb->pqba", out, m).reshape(out.shape[0] * m.shape[0], out.shape[1] * m.shape[1])
+    return out
+
+
+def _multilinear_det(X, pts):
+    """det J of the element's multilinear map (vertices X, MFEM order) at the
+    reference points pts [n][dim]."""
+    dim = pts.shape[1]
+    R = _REF_CORNERS[M.GEOM_QUAD if dim == 2 else M.GEOM_HEX]
+    J = np.zeros((len(pts), dim, dim))
+    for k in range(len(R)):
+        for r in range(dim):
+            f = np.ones(len(pts))
+            for d in range(dim):
+                f = f * ((1.0 if R[k][d] else -1.0) if d == r else (pts[:, d] if R[k][d] else 1.0 - pts[:, d]))
+            J[:, :, r] += f[:, None] * X[k][None, :]
+    return np.linalg.det(J)
+
+
+def _curved_restriction(kid_states, Xp, order, dim):
+    """The L2 projection onto the parent of its children's polynomials, weighted
+    by the parent's Jacobian determinant (a child's map is the restriction of
+    the parent's): M_P u_P = sum_c int_c phi_P u_c det J, with the Gauss rule of
+    p + 2 points per axis on every child, exact for the trilinear map."""
+    N = order + 1
+    nodes = _gl_nodes(N)
+    nq = order + 2
+    t = _gl_nodes(nq)
+    wq = np.polynomial.legendre.leggauss(nq)[1] / 2.0
+    Wt = wq
+    for _ in range(dim - 1):
+        Wt = np.multiply.outer(wq, Wt)
+    Wt = Wt.ravel() / (1 << dim)
+    grids = np.meshgrid(*([t] * dim), indexing="ij")
+    grid = np.stack([g.transpose(*range(dim)[::-1]).ravel() for g in grids], axis=-1)  # first axis fastest
+    Psi = _tensor_eval([_lagrange_matrix(nodes, t)] * dim)  # child basis at the child's points
+    nd = N ** dim
+    Mp = np.zeros((nd, nd))
+    rhs = np.zeros((kid_states[0].shape[0], nd))
+    for c, uc in enumerate(kid_states):
+        o = np.array([0.5 * ((c >> d) & 1) for d in range(dim)])
+        pts = o + 0.5 * grid
+        Phi = _tensor_eval([_lagrange_matrix(nodes, o[d] + 0.5 * t) for d in range(dim)])
+        w = Wt * _multilinear_det(Xp, pts)
+        Mp += Phi.T @ (w[:, None] * Phi)
+        rhs += (uc @ Psi.T * w[None, :]) @ Phi
+    return np.linalg.solve(Mp, rhs.T).T
+
+
+def transfer_state(x_old, transfer, order, dim, neq, ne_old, coords_new=None):
     """States of the new leaves from the old ones. x is byNODES: [v][e][node],
     nodes x fastest. A refined element's children take its polynomial at their
-    nodes (exact). A coarsened parent takes the L2 projection of its children
-    (the parent's Gauss-Legendre mass matrix is diagonal and exact, and the
-    children's integrals are exact with their own rule), which keeps every
-    element mean and so the conserved totals."""
+    nodes (exact). A coarsened parent takes the L2 projection of its children,
+    which keeps every element's integral and so the conserved totals: on a
+    parallelogram / parallelepiped the parent's Gauss-Legendre mass matrix is
+    diagonal and exact and the children's integrals are exact with their own
+    rule; on a curved parent (its vertices in coords_new, the new mesh's
+    element coordinates) the projection is weighted by det J."""
     N = order + 1
     nd = N ** dim
     nodes = _gl_nodes(N)
@@ -443,6 +497,9 @@ def transfer_state(x_old, transfer, order, dim, neq, ne_old):
     for i, (src, scale, off) in enumerate(transfer):
         if src < 0:
             kids = scale
+            if coords_new is not None and not _is_affine(coords_new[i]):
+                out[:, i] = _curved_restriction([xo[:, k] for k in kids], coords_new[i], order, dim)
+                continue
             acc = np.zeros((neq,) + (N,) * dim)
             for c, k in enumerate(kids):
                 mats = []
@@ -703,9 +760,9 @@ def run_adaptive(base, order, scheme, initial_state, t_final, cfl, tableau, max_
         tr, done = tree.derefine(groups)
         if not done:
             return x, S_, mesh
-        x = transfer_state(x, tr, order, dim, neq, ne_old)
-        S_.close()
         mesh = tree.mesh()
+        x = transfer_state(x, tr, order, dim, neq, ne_old, mesh.elem_coords)
+        S_.close()
         return x, make(mesh), mesh
 
     x, S_, mesh = coarsen(x, S_, mesh)
@@ -730,9 +787,9 @@ def run_adaptive(base, order, scheme, initial_state, t_final, cfl, tableau, max_
             ne_old = mesh.num_elements
             tr = tree.refine(marks)
             refinements += 1
-            x_old = transfer_state(x_old, tr, order, dim, neq, ne_old)
-            S_.close()
             mesh = tree.mesh()
+            x_old = transfer_state(x_old, tr, order, dim, neq, ne_old, mesh.elem_coords)
+            S_.close()
             S_ = make(mesh)
         x, S_, mesh = coarsen(x, S_, mesh)
         x = S_.limit(x)

@@ -325,3 +325,52 @@ def test_adaptive_loop_with_the_oracle_refines_at_the_jump():
     # refined elements sit around the jump, none at the ends of the tube
     assert np.all(lv[(cen[:, 0] < 0.2) | (cen[:, 0] > 0.9)] == 0)
     assert r.steps == 6 and r.refinements >= 1
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_curved_derefinement_is_the_weighted_l2_projection(p):
+    """On curved (trilinear) hexahedra, coarsening keeps each conserved total
+    (the integrals with the oracle's element mass matrices) and undoes a
+    refinement exactly: a parent's polynomial prolonged to its children and
+    restricted again comes back. At p = 1, det J (degree 2 per axis) is not in
+    the space, and the unweighted restriction loses the totals."""
+    from tests.helpers import curved_hex
+
+    def totals(ref, m, x):
+        Y = x.reshape(5, m.num_elements, -1)
+        out, mag = np.zeros(5), np.zeros(5)
+        for e in range(m.num_elements):
+            ones = np.linalg.inv(ref.mass_inverse(e)).sum(axis=0)
+            out += Y[:, e, :] @ ones
+            mag += np.abs(Y[:, e, :]) @ np.abs(ones)
+        return out, mag
+
+    base = curved_hex(3, 3, 3, seed=80)
+    t = amr.NCMesh(base)
+    t.refine([4, 13])
+    m1 = t.mesh()
+    r1 = RefSolver(m1, p, S.Scheme())
+    x1 = smooth_state(r1.node_coords(), seed=3, amp=1e-2)
+    T1, mag = totals(r1, m1, x1)
+    tr, done = t.derefine([(4, ()), (13, ())])
+    assert len(done) == 2
+    m2 = t.mesh()
+    r2 = RefSolver(m2, p, S.Scheme())
+    x2 = amr.transfer_state(x1, tr, p, 3, 5, m1.num_elements, m2.elem_coords)
+    T2, _ = totals(r2, m2, x2)
+    assert np.all(np.abs(T2 - T1) <= 1e-14 * mag)
+    x2a = amr.transfer_state(x1, tr, p, 3, 5, m1.num_elements)  # unweighted
+    assert np.abs(x2a - x2).max() > 1e-8
+    if p == 1:
+        T2a, _ = totals(r2, m2, x2a)
+        assert np.abs(T2a - T1).max() > 1e-10 * mag.max()
+    t2 = amr.NCMesh(base)
+    r0 = RefSolver(base, p, S.Scheme())
+    x0 = smooth_state(r0.node_coords(), seed=5, amp=1e-2)
+    tr = t2.refine([4])
+    ma = t2.mesh()
+    xa = amr.transfer_state(x0, tr, p, 3, 5, base.num_elements, ma.elem_coords)
+    tr, _ = t2.derefine([(4, ())])
+    mb = t2.mesh()
+    xb = amr.transfer_state(xa, tr, p, 3, 5, ma.num_elements, mb.elem_coords)
+    np.testing.assert_allclose(xb, x0, rtol=0, atol=1e-13)
