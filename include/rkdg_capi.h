@@ -99,7 +99,11 @@ typedef struct rkdg_ctx rkdg_ctx;
 
 /* Mesh after any refinement, flattened by the host (the analogue of an MFEM
  * mesh with discontinuous nodes). All pointers are HOST pointers, copied at
- * create time. */
+ * create time. The element map is the vertices' multilinear map: affine
+ * triangles; quadrilaterals (parallelograms on the tensor kernels, bilinear
+ * ones on the dense kernels); hexahedra, parallelepipeds or trilinear (curved:
+ * per-element records of adj(J) at the nodes, the inverse mass matrix and
+ * per-point face normals, orders 1-3). */
 typedef struct {
   int geom;                  /* RKDG_GEOM_* */
   int order;                 /* polynomial order p (DG_FECollection(p, dim)) */
@@ -258,7 +262,8 @@ int rkdg_create_ex(const rkdg_mesh_desc* mesh, const rkdg_scheme_desc* scheme, i
  * coordinate j of the slave face's corner c (the slave's local face order) in
  * the master face's parametrisation (its local face vertex 0 at the origin,
  * vertex 1 at 1 (2D) or (1,0), vertex 3 at (0,1) (3D)): the slave's point
- * matrix. The element kernels take affine quadrilaterals and hexahedra. */
+ * matrix. The element kernels take affine quadrilaterals and affine or
+ * trilinear hexahedra. */
 typedef struct {
   int64_t count;
   const int64_t* slave;
