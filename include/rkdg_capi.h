@@ -245,7 +245,7 @@ int rkdg_profile_read(rkdg_ctx* ctx, double* rhs_ms, int64_t* rhs_launches, doub
  * batch bit for bit). Same counterpart as rkdg_create. */
 #define RKDG_CREATE_DENSE 1
 /* RKDG_CREATE_TWO_COLOUR: on tensor meshes whose face graph is two-colourable
- * and has no boundary faces (periodic meshes of even extents), run the
+ * and has no boundary faces (affine periodic meshes of even extents), run the
  * right-hand side as a solver/receiver pair of kernels that solves each face
  * once; the receivers' share of the face fluxes goes through a buffer of the
  * size of one state vector. Other meshes keep the one-kernel form. */
