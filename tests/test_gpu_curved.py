@@ -179,42 +179,44 @@ CURVED_LIMITERS = [
 ]
 
 
+@pytest.mark.parametrize("p", [1, 2, 3])
 @pytest.mark.parametrize("li", range(len(CURVED_LIMITERS)))
-def test_curved_hex_limiter_parity(li, cuda):
+def test_curved_hex_limiter_parity(li, p, cuda):
     import torch
 
     sch = CURVED_LIMITERS[li]
     m = curved_hex(6, 4, 3, seed=20 + li)
-    ref = RefSolver(m, 2, sch)
+    ref = RefSolver(m, p, sch)
     x = _shocky_state(ref.node_coords(), seed=li)
     x_ref, _, mv = ref.limit(x)
-    ctx = _ctx(m, 2, sch)
+    ctx = _ctx(m, p, sch)
     xd = _dev(x, cuda)
     tr = torch.zeros(m.num_elements, dtype=torch.int8, device=cuda)
     ctx.limit(xd, tr)
     err = rel_linf(_host(xd), x_ref, 5)
     flags = _host(tr)
-    print(f"[curved limiter {li}] rel Linf {err:.3e} troubled {int((flags & 1).sum())}", flush=True)
+    print(f"[curved limiter {li} p={p}] rel Linf {err:.3e} troubled {int((flags & 1).sum())}", flush=True)
     assert err <= TOL
     if sch.limiter == S.LIM_MULTI:
         near = np.abs(mv - 1.0) < 1e-12
         assert np.all(((flags & 1) == (mv < 1.0)) | near)
 
 
-@pytest.mark.parametrize("tab", ["ssprk3", "ref_literal_rk2"])
-def test_curved_hex_step_parity(tab, cuda):
+@pytest.mark.parametrize("tab,p", [("ssprk3", 1), ("ssprk3", 2), ("ssprk3", 3), ("ref_literal_rk2", 2)])
+def test_curved_hex_step_parity(tab, p, cuda):
     """Limited RK steps (the SSP-RK3 one applies the limiter on load)."""
     m = curved_hex(5, 4, 3, seed=30)
-    sch = S.Scheme(riemann=S.HLLC, indicator=S.IND_BJ, phi=S.PHI_MICHALAK, limiter=S.LIM_MULTI)
-    ref = RefSolver(m, 2, sch)
+    sch = S.Scheme(riemann=S.HLLC if p < 3 else S.RUSANOV, indicator=S.IND_BJ, phi=S.PHI_MICHALAK,
+                   limiter=S.LIM_MULTI)
+    ref = RefSolver(m, p, sch)
     x = _shocky_state(ref.node_coords(), seed=3)
-    ctx = _ctx(m, 2, sch)
+    ctx = _ctx(m, p, sch)
     xd = _dev(x, cuda)
     for _ in range(2):
         x, lam_ref = ref.step(x, 1e-3, S.TABLEAUS[tab])
         lam = ctx.step(xd, 1e-3, S.TABLEAUS[tab])
     err = rel_linf(_host(xd), x, 5)
-    print(f"[curved step {tab}] rel Linf {err:.3e} lam {lam} {lam_ref}", flush=True)
+    print(f"[curved step {tab} p={p}] rel Linf {err:.3e} lam {lam} {lam_ref}", flush=True)
     assert err <= TOL
     assert abs(lam - lam_ref) <= 1e-12 * lam_ref
 
