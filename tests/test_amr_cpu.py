@@ -374,3 +374,25 @@ def test_curved_derefinement_is_the_weighted_l2_projection(p):
     mb = t2.mesh()
     xb = amr.transfer_state(xa, tr, p, 3, 5, ma.num_elements, mb.elem_coords)
     np.testing.assert_allclose(xb, x0, rtol=0, atol=1e-13)
+
+
+def test_zz_indicator_vanishes_for_a_linear_field_on_curved_hexahedra():
+    """A density linear in x is trilinear in the reference coordinates of a
+    trilinear element, so its nodal interpolant is exact: every element's DG
+    gradient, taken with the Jacobian at each node, is the constant gradient,
+    the recovered gradient equals it and the indicator vanishes."""
+    from tests.helpers import curved_hex
+
+    m = curved_hex(4, 3, 3, seed=85, periodic=(False, False, False))
+    p = 2
+    ref = RefSolver(m, p, S.Scheme())
+    X = ref.node_coords()
+    rho = 1.0 + 0.3 * X[..., 0] - 0.2 * X[..., 1] + 0.1 * X[..., 2]
+    x = np.concatenate([rho.ravel()] + [np.zeros(rho.size)] * 3 + [np.full(rho.size, 2.5)])
+    eta = amr.zz_indicator(m, p, x)
+    assert eta.max() < 1e-13
+    # the elements are not parallelepipeds: the Jacobian differs between corners
+    X = m.elem_coords
+    J0 = np.stack([X[:, 1] - X[:, 0], X[:, 3] - X[:, 0], X[:, 4] - X[:, 0]], axis=-1)
+    J7 = np.stack([X[:, 6] - X[:, 7], X[:, 6] - X[:, 5], X[:, 6] - X[:, 2]], axis=-1)
+    assert np.abs(J0 - J7).max() > 1e-3  # the elements are not parallelepipeds
