@@ -20,10 +20,12 @@ mesh of the same workload.
 The same line carries the batched-ensemble figure (256 nozzle instances,
 batched vs sequential solves, bit-identical check) under "ensemble", and the
 adjoint/forward time ratio of the discrete adjoint (Mach 3 forward-facing
-step, 2000 steps) with its finite-difference checks under "adjoint".
+step, 2000 steps) with its finite-difference checks under "adjoint", the
+double Mach reflection mesh's right-hand side and steps under "dmr", and the
+curved (trilinear) Q2 hexahedra of a 48^3 box under "curved".
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 96] [--no-cpu]
-                       [--no-ensemble | --ensemble-only]
+                       [--no-ensemble | --ensemble-only] [--no-dmr] [--no-curved]
 """
 from __future__ import annotations
 
