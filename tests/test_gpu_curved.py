@@ -261,3 +261,29 @@ def test_curved_nc_free_stream_and_conservation(cuda):
     assert fs <= 2e-11
     assert np.all(np.abs(tot) <= 1e-12 * scale)
     assert err <= TOL
+
+
+@pytest.mark.parametrize("p", [1, 3])
+def test_curved_hex_walls_other_orders(p, cuda):
+    """Inlet, outlet and slip walls on the x planes at p = 1 and 3 (the
+    boundary instantiations of the curved kernel at those orders)."""
+    m = curved_hex(4, 3, 3, seed=100 + p, periodic=(False, True, True))
+    bcs = [S.BC(5, S.BC_SUPERSONIC_INLET, (1.1, 0.5, 0.0, 0.1, 1.0)), S.BC(3, S.BC_SLIP)]
+    err, bar, lam, lam_ref = _rhs_check(m, p, S.Scheme(riemann=S.HLL, bcs=bcs), cuda, seed=p)
+    print(f"[curved walls p={p}] rel Linf {err:.3e} bar {bar:.1e}", flush=True)
+    assert err <= bar
+    assert abs(lam - lam_ref) <= 1e-13 * lam_ref
+
+
+@pytest.mark.parametrize("which", ["einfeldt", "covolume"])
+def test_curved_hex_scheme_variants(which, cuda):
+    """Einfeldt's wave speeds with HLLC, and the covolume equation of state
+    with HLL and Rusanov, on the curved mesh."""
+    m = curved_hex(4, 3, 3, seed=110)
+    schemes = ([S.Scheme(riemann=S.HLLC, wave_speeds=S.EINFELDT)] if which == "einfeldt" else
+               [S.Scheme(riemann=S.HLL, covolume=0.05), S.Scheme(riemann=S.RUSANOV, covolume=0.05)])
+    for sch in schemes:
+        err, bar, lam, lam_ref = _rhs_check(m, 2, sch, cuda, seed=9)
+        print(f"[curved {which} rs={sch.riemann}] rel Linf {err:.3e} bar {bar:.1e}", flush=True)
+        assert err <= bar
+        assert abs(lam - lam_ref) <= 1e-13 * lam_ref
